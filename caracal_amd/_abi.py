@@ -1,0 +1,101 @@
+"""ctypes mirror of include/caracal_amd.h.  The shared library is built
+in-tree (caracal_amd/libcaracal_amd.so); loading fails loudly if it is not."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcaracal_amd.so")
+
+CAM_OK = 0
+CAM_SURFACE_H3, CAM_SURFACE_CH4H = 1, 2
+CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
+CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
+CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD = 0, 1, 2
+CAM_MAX_ATOMS, CAM_MAX_BONDS, CAM_MAX_REAC = 16, 4, 4
+
+
+class Reaction(ctypes.Structure):
+    _fields_ = [
+        ("n_form", ctypes.c_int32), ("n_break", ctypes.c_int32),
+        ("bond_form", (ctypes.c_int32 * 2) * CAM_MAX_BONDS),
+        ("bond_break", (ctypes.c_int32 * 2) * CAM_MAX_BONDS),
+        ("form_ref", ctypes.c_double * CAM_MAX_BONDS),
+        ("break_ref", ctypes.c_double * CAM_MAX_BONDS),
+        ("n_reac", ctypes.c_int32),
+        ("reac_natoms", ctypes.c_int32 * CAM_MAX_REAC),
+        ("reac_atoms", (ctypes.c_int32 * CAM_MAX_ATOMS) * CAM_MAX_REAC),
+        ("r_inf", ctypes.c_double), ("xi_ideal", ctypes.c_double),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("surface", ctypes.c_int32), ("natoms", ctypes.c_int32), ("nbeads", ctypes.c_int32),
+        ("ntraj", ctypes.c_int32), ("precision", ctypes.c_int32), ("nm_mode", ctypes.c_int32),
+        ("ensemble", ctypes.c_int32), ("remove_com", ctypes.c_int32), ("andersen_step", ctypes.c_int32),
+        ("steps_per_launch", ctypes.c_int32),
+        ("dt", ctypes.c_double),
+        ("mass", ctypes.c_double * CAM_MAX_ATOMS),
+        ("seed", ctypes.c_uint64),
+        ("reaction", Reaction),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "cam_last_error": (ctypes.c_char_p, []),
+    "cam_version": (ctypes.c_char_p, []),
+    "cam_egrad_h3": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "cam_egrad_ch4h": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "cam_egrad_h3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "cam_egrad_ch4h_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "cam_engine_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "cam_engine_destroy": (ctypes.c_int, [_vp]),
+    "cam_engine_set_state": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "cam_engine_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_engine_get_forces": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_engine_draw_momenta": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int]),
+    "cam_engine_constrain": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "cam_engine_begin_children": (ctypes.c_int, [_vp]),
+    "cam_engine_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_engine_sync": (ctypes.c_int, [_vp]),
+    "cam_engine_kappa": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int]),
+    "cam_engine_observables": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _ip]),
+    "cam_engine_device_state": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "cam_engine_last_timing": (ctypes.c_int, [_vp, _dp, _ip, _ip]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI.  Raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(there is no CPU fallback)")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class CamError(RuntimeError):
+    def __init__(self, fn, code):
+        msg = lib().cam_last_error().decode()
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def check(fn, code):
+    if code != CAM_OK:
+        raise CamError(fn, code)

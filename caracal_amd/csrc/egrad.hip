@@ -1,0 +1,164 @@
+// Drop-in analytic surfaces: the bead-batched plugin contract of the
+// reference (egrad_h3, src/egrad_h3.f:29-77; egrad_ch4h, src/egrad_ch4h.f:74-129).
+// One lane per bead.  A block stages its 256 beads through LDS so the global
+// reads and writes are coalesced although each bead's coordinates are one
+// contiguous 3*natoms-double record.
+#include "common.hpp"
+#include "pes_ch4h.hpp"
+#include "pes_h3.hpp"
+
+namespace cam {
+namespace {
+
+constexpr int kBlock = 256;
+
+// H3: one lane per bead.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void egrad_h3_kernel(const double *__restrict__ q, int nbeads,
+                                                          double *__restrict__ v, double *__restrict__ g)
+{
+    constexpr int NC = 9;
+    __shared__ double tile[kBlock * NC];
+    const long b0 = (long)blockIdx.x * kBlock;
+    const int nb = (int)min((long)kBlock, (long)nbeads - b0);
+    for (int i = threadIdx.x; i < nb * NC; i += kBlock) tile[i] = q[b0 * NC + i];
+    __syncthreads();
+    double x[NC], gr[NC], vb = 0.0;
+    const bool act = (int)threadIdx.x < nb;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) x[c] = tile[threadIdx.x * NC + c];
+        h3::egrad_bead<T>(x, vb, gr);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) tile[threadIdx.x * NC + c] = gr[c];
+        v[b0 + threadIdx.x] = vb;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * NC; i += kBlock) g[b0 * NC + i] = tile[i];
+}
+
+// CH4H: four lanes per bead (see ch4h::egrad_quad); a block holds 64 beads.
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void egrad_ch4h_kernel(const double *__restrict__ q, int nbeads,
+                                                            double *__restrict__ v, double *__restrict__ g)
+{
+    constexpr int NC = 18, BPB = kBlock / 4;
+    __shared__ double tile[BPB * NC];
+    const long b0 = (long)blockIdx.x * BPB;
+    const int nb = (int)min((long)BPB, (long)nbeads - b0);
+    for (int i = threadIdx.x; i < nb * NC; i += kBlock) tile[i] = q[b0 * NC + i];
+    __syncthreads();
+    const int bead = threadIdx.x >> 2, r = threadIdx.x & 3;
+    const bool act = bead < nb;
+    const int bl = act ? bead : 0;                       // idle quads evaluate bead 0 of the block
+    const int ah = ch4h::hidx(r);
+    const int ax = r == 0 ? ch4h::kC : ch4h::kB;
+    double own[6], gown[6], vb;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        own[d] = tile[bl * NC + 3 * ah + d];
+        own[3 + d] = r < 2 ? tile[bl * NC + 3 * ax + d] : 0.0;
+    }
+    ch4h::egrad_quad<T>(own, vb, gown);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            tile[bead * NC + 3 * ah + d] = gown[d];
+            if (r < 2) tile[bead * NC + 3 * ax + d] = gown[3 + d];
+        }
+        if (r == 0) v[b0 + bead] = vb;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * NC; i += kBlock) g[b0 * NC + i] = tile[i];
+}
+
+template <int NA>
+int launch_egrad(const double *q, int nbeads, double *v, double *g, int precision, hipStream_t s)
+{
+    if (nbeads == 0) return CAM_OK;
+    if constexpr (NA == 3) {
+        const int grid = (nbeads + kBlock - 1) / kBlock;
+        if (precision == CAM_PREC_MIXED)
+            hipLaunchKernelGGL(egrad_h3_kernel<float>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+        else
+            hipLaunchKernelGGL(egrad_h3_kernel<double>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+    } else {
+        const int grid = (nbeads + kBlock / 4 - 1) / (kBlock / 4);
+        if (precision == CAM_PREC_MIXED)
+            hipLaunchKernelGGL(egrad_ch4h_kernel<float>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+        else
+            hipLaunchKernelGGL(egrad_ch4h_kernel<double>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+    }
+    CAM_LAUNCH_CHECK();
+    return CAM_OK;
+}
+
+template <int NA>
+int egrad_dev(const double *q, int natoms, int nbeads, double *v, double *g, int precision, void *stream)
+{
+    CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
+    CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
+    CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null device pointer");
+    CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision %d", precision);
+    clear_error();
+    return launch_egrad<NA>(q, nbeads, v, g, precision, (hipStream_t)stream);
+}
+
+// Host-pointer form: stage through device buffers owned by this call.
+template <int NA>
+int egrad_host(const double *q, int natoms, int nbeads, double *v, double *g)
+{
+    CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
+    CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
+    CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null host pointer");
+    clear_error();
+    if (nbeads == 0) return CAM_OK;
+    const size_t nq = (size_t)nbeads * 3 * NA;
+    double *dq = nullptr, *dv = nullptr, *dg = nullptr;
+    int rc = CAM_OK;
+    auto cleanup = [&]() {
+        if (dq) (void)hipFree(dq);
+        if (dv) (void)hipFree(dv);
+        if (dg) (void)hipFree(dg);
+    };
+    auto step = [&]() -> int {
+        CAM_HIP(hipMalloc(&dq, nq * sizeof(double)));
+        CAM_HIP(hipMalloc(&dv, (size_t)nbeads * sizeof(double)));
+        CAM_HIP(hipMalloc(&dg, nq * sizeof(double)));
+        CAM_HIP(hipMemcpy(dq, q, nq * sizeof(double), hipMemcpyHostToDevice));
+        int r = launch_egrad<NA>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
+        if (r != CAM_OK) return r;
+        CAM_HIP(hipMemcpy(v, dv, (size_t)nbeads * sizeof(double), hipMemcpyDeviceToHost));
+        CAM_HIP(hipMemcpy(g, dg, nq * sizeof(double), hipMemcpyDeviceToHost));
+        return CAM_OK;
+    };
+    rc = step();
+    cleanup();
+    return rc;
+}
+
+}  // namespace
+}  // namespace cam
+
+extern "C" {
+int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    return cam::egrad_host<3>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    return cam::egrad_host<6>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
+{
+    return cam::egrad_dev<3>(q, natoms, nbeads, v, dvdq, precision, stream);
+}
+int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
+{
+    return cam::egrad_dev<6>(q, natoms, nbeads, v, dvdq, precision, stream);
+}
+}

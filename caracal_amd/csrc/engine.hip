@@ -1,0 +1,1255 @@
+// Trajectory-batch RPMD engine: replaces the reference's per-step `verlet`
+// (src/verlet.f90:65-1308) with its per-bead force loop, for a batch of
+// independent trajectories that share a surface, an atom count and a bead
+// count.
+//
+// Device layout of the state (q, p, dV/dq): [traj][3*natoms][nbeads] with
+// the bead index fastest, so the lanes of one trajectory read consecutive
+// addresses.  A block of 256 lanes holds 256/(L*nbeads) trajectories, where
+// L is the number of lanes that share one bead (1 for H3, 4 for CH4H: see
+// ch4h::egrad_quad).  One launch advances its trajectories by several steps
+// with the state held in registers; the normal-mode step goes through LDS.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "pes_ch4h.hpp"
+#include "pes_h3.hpp"
+#include "philox.hpp"
+
+namespace cam {
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Surface policies: how the coordinates of one bead are spread over its L lanes.
+// ---------------------------------------------------------------------------
+struct SurfH3 {
+    static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
+    __device__ static int coord(int slot, int) { return slot; }
+    __device__ static bool valid(int, int) { return true; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<T>(own, v, g); }
+};
+
+struct SurfCH4H {
+    static constexpr int NA = 6, NC = 18, L = 4, SLOTS = 6;
+    // slot 0..2: methane H number r; slot 3..5: C on lane 0, Hb on lane 1
+    __device__ static int coord(int slot, int r)
+    {
+        return slot < 3 ? 3 * ch4h::hidx(r) + slot : 3 * (r == 0 ? ch4h::kC : ch4h::kB) + slot - 3;
+    }
+    __device__ static bool valid(int slot, int r) { return slot < 3 || r < 2; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { ch4h::egrad_quad<T>(own, v, g); }
+};
+
+struct Rx {                                  // reaction coordinate, 0-based indices
+    int n_form, n_break, n_reac;
+    int form[CAM_MAX_BONDS][2], brk[CAM_MAX_BONDS][2];
+    double form_ref[CAM_MAX_BONDS], break_ref[CAM_MAX_BONDS];
+    int reac_n[CAM_MAX_REAC], reac_at[CAM_MAX_REAC][CAM_MAX_ATOMS];
+    double reac_mass[CAM_MAX_REAC];
+    double r_inf, xi_ideal;
+};
+
+struct StepArgs {
+    double *q, *p, *g, *vbead;
+    const double *beta;          // [ntraj]
+    const double *tfwd, *tinv;   // [nbeads][nbeads], element (j,k) = T[k][j]
+    uint8_t *xi_flag;            // CHILD: [nsteps_run][ntraj]
+    const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
+    int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
+    int andersen_step;
+    uint64_t seed;
+    double dt;
+    double mass[CAM_MAX_ATOMS];
+    Rx rx;
+};
+
+// xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213),
+// c: centroid (3*natoms, atom-major).  Optionally the gradient dxi (3*natoms).
+template <int NA>
+__device__ double calc_xi(const Rx &rx, const double *mass, const double *c, double *dxi)
+{
+    double s1 = 0.0;
+    double rb[CAM_MAX_BONDS], rf[CAM_MAX_BONDS];
+    for (int i = 0; i < rx.n_break; ++i) {
+        const int a = rx.brk[i][0], b = rx.brk[i][1];
+        const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
+        rb[i] = sqrt(dx * dx + dy * dy + dz * dz);
+        s1 = s1 + (rb[i] - rx.break_ref[i]) / (double)rx.n_break;
+    }
+    for (int i = 0; i < rx.n_form; ++i) {
+        const int a = rx.form[i][0], b = rx.form[i][1];
+        const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
+        rf[i] = sqrt(dx * dx + dy * dy + dz * dz);
+        s1 = s1 - (rf[i] - rx.form_ref[i]) / (double)rx.n_form;
+    }
+    double com[CAM_MAX_REAC][3];
+    for (int k = 0; k < rx.n_reac; ++k) {
+        com[k][0] = com[k][1] = com[k][2] = 0.0;
+        for (int i = 0; i < rx.reac_n[k]; ++i) {
+            const int at = rx.reac_at[k][i];
+            for (int j = 0; j < 3; ++j) com[k][j] = com[k][j] + mass[at] * c[3 * at + j] / rx.reac_mass[k];
+        }
+    }
+    double s0 = 0.0;
+    const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
+    for (int i = 0; i < rx.n_reac; ++i)
+        for (int j = i + 1; j < rx.n_reac; ++j) {
+            const double dx = com[j][0] - com[i][0], dy = com[j][1] - com[i][1], dz = com[j][2] - com[i][2];
+            s0 = s0 + (rx.r_inf - sqrt(dx * dx + dy * dy + dz * dz));
+        }
+    s0 = s0 / (double)terms;
+    const double xi = rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
+    if (dxi) {
+        double ds1[3 * CAM_MAX_ATOMS], ds0[3 * CAM_MAX_ATOMS];
+        for (int i = 0; i < 3 * NA; ++i) ds1[i] = ds0[i] = 0.0;
+        for (int i = 0; i < rx.n_form; ++i) {
+            const int a = rx.form[i][0], b = rx.form[i][1];
+            const double rinv = 1.0 / rf[i];
+            for (int d = 0; d < 3; ++d) {
+                const double r = c[3 * a + d] - c[3 * b + d];
+                ds1[3 * a + d] = ds1[3 * a + d] - r * rinv / (double)rx.n_form;
+                ds1[3 * b + d] = ds1[3 * b + d] + r * rinv / (double)rx.n_form;
+            }
+        }
+        for (int i = 0; i < rx.n_break; ++i) {
+            const int a = rx.brk[i][0], b = rx.brk[i][1];
+            const double rinv = 1.0 / rb[i];
+            for (int d = 0; d < 3; ++d) {
+                const double r = c[3 * a + d] - c[3 * b + d];
+                ds1[3 * a + d] = ds1[3 * a + d] + r * rinv / (double)rx.n_break;
+                ds1[3 * b + d] = ds1[3 * b + d] - r * rinv / (double)rx.n_break;
+            }
+        }
+        for (int i = 0; i < rx.n_reac; ++i)
+            for (int j = i + 1; j < rx.n_reac; ++j) {
+                double red[3];
+                for (int d = 0; d < 3; ++d) red[d] = com[j][d] - com[i][d];
+                const double rinv = 1.0 / sqrt(red[0] * red[0] + red[1] * red[1] + red[2] * red[2]);
+                for (int k = 0; k < rx.reac_n[i]; ++k) {
+                    const int at = rx.reac_at[i][k];
+                    for (int d = 0; d < 3; ++d)
+                        ds0[3 * at + d] = ds0[3 * at + d] + red[d] * rinv * mass[at] / rx.reac_mass[i] / (double)terms;
+                }
+                for (int k = 0; k < rx.reac_n[j]; ++k) {
+                    const int at = rx.reac_at[j][k];
+                    for (int d = 0; d < 3; ++d)
+                        ds0[3 * at + d] = ds0[3 * at + d] - red[d] * rinv * mass[at] / rx.reac_mass[j] / (double)terms;
+                }
+            }
+        for (int i = 0; i < 3 * NA; ++i) dxi[i] = rx.xi_ideal * ds1[i] + (1.0 - rx.xi_ideal) * ds0[i];
+    }
+    return xi;
+}
+
+// ---------------------------------------------------------------------------
+// The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN or CAM_ENS_CHILD.
+// One step: p -= dt/2 g; free ring-polymer propagation in normal-mode space;
+// [CHILD: xi of the centroid]; forces; p -= dt/2 g; [ANDERSEN: resample].
+// ---------------------------------------------------------------------------
+template <class S, int NB, typename T, int ENS, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void step_kernel(StepArgs a)
+{
+    constexpr int NC = S::NC, L = S::L, SL = S::SLOTS;
+    constexpr int BPB = kBlock / L;          // beads per block
+    constexpr int TPB = BPB / NB;            // trajectories per block
+    static_assert(TPB >= 1, "a trajectory must fit in one block");
+    __shared__ double X[TPB * NC * NB];
+    __shared__ double cent[TPB * NC];
+
+    const int tid = threadIdx.x;
+    const int r = tid % L;
+    const int bib = tid / L;                 // bead in block
+    const int tl = bib / NB, k = bib % NB;
+    const int traj_raw = blockIdx.x * TPB + tl;
+    const bool act = traj_raw < a.ntraj;
+    const int traj = act ? traj_raw : a.ntraj - 1;
+    double *xs = X + tl * NC * NB;
+
+    int cc[SL];
+    double m[SL], q[SL], p[SL], g[SL];
+    const size_t base = (size_t)traj * NC * NB;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+        cc[s] = S::coord(s, r);
+        m[s] = a.mass[cc[s] / 3];
+        const bool ok = S::valid(s, r);
+        q[s] = ok ? a.q[base + (size_t)cc[s] * NB + k] : 0.0;
+        p[s] = ok ? a.p[base + (size_t)cc[s] * NB + k] : 0.0;
+        g[s] = ok ? a.g[base + (size_t)cc[s] * NB + k] : 0.0;
+    }
+    // free ring-polymer propagator of mode k (src/verlet.f90:405-432)
+    const double beta = a.beta[traj];
+    double p1 = 1.0, p2[SL], p3[SL];
+    if (NB > 1 && k != 0) {
+        const int kk = k <= NB / 2 ? k : NB - k;
+        const double wk = (2.0 / (beta / NB)) * sin(kk * (3.141592653589793 / NB));
+        const double wt = wk * a.dt;
+        const double cw = cos(wt), sw = sin(wt);
+        p1 = cw;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            const double wm = wk * m[s];
+            p2[s] = -wm * sw;
+            p3[s] = sw / wm;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            p2[s] = 0.0;
+            p3[s] = a.dt / m[s];
+        }
+    }
+    const double hdt = 0.5 * a.dt;
+    double vb = 0.0;
+
+    for (int it = 0; it < a.nsteps; ++it) {
+        // half kick (src/verlet.f90:216-218)
+#pragma unroll
+        for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
+
+        if constexpr (NB == 1) {
+#pragma unroll
+            for (int s = 0; s < SL; ++s) q[s] = p[s] * p3[s] + q[s];
+        } else {
+            // bead -> normal modes, propagate, back (src/verlet.f90:377-463)
+            double yp[SL], yq[SL];
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < SL; ++s)
+                    if (S::valid(s, r)) xs[cc[s] * NB + k] = pass == 0 ? p[s] : q[s];
+                __syncthreads();
+                double y[SL];
+#pragma unroll
+                for (int s = 0; s < SL; ++s) y[s] = 0.0;
+#pragma unroll 1
+                for (int j = 0; j < NB; j += 2) {
+                    const double t0 = a.tfwd[j * NB + k], t1 = a.tfwd[(j + 1) * NB + k];
+#pragma unroll
+                    for (int s = 0; s < SL; ++s) {
+                        const double2 x2 = *reinterpret_cast<const double2 *>(&xs[cc[s] * NB + j]);
+                        y[s] = fma(t0, x2.x, y[s]);
+                        y[s] = fma(t1, x2.y, y[s]);
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    if (pass == 0) yp[s] = y[s];
+                    else yq[s] = y[s];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < SL; ++s) {
+                const double pn = yp[s] * p1 + yq[s] * p2[s];
+                yq[s] = yp[s] * p3[s] + yq[s] * p1;
+                yp[s] = pn;
+            }
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < SL; ++s)
+                    if (S::valid(s, r)) xs[cc[s] * NB + k] = pass == 0 ? yp[s] : yq[s];
+                __syncthreads();
+                double y[SL];
+#pragma unroll
+                for (int s = 0; s < SL; ++s) y[s] = 0.0;
+#pragma unroll 1
+                for (int j = 0; j < NB; j += 2) {
+                    const double t0 = a.tinv[j * NB + k], t1 = a.tinv[(j + 1) * NB + k];
+#pragma unroll
+                    for (int s = 0; s < SL; ++s) {
+                        const double2 x2 = *reinterpret_cast<const double2 *>(&xs[cc[s] * NB + j]);
+                        y[s] = fma(t0, x2.x, y[s]);
+                        y[s] = fma(t1, x2.y, y[s]);
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    if (pass == 0) p[s] = y[s];
+                    else q[s] = y[s];
+                }
+            }
+        }
+
+        if constexpr (ENS == CAM_ENS_CHILD) {
+            // centroid (src/get_centroid.f90:66-85) and xi of the child step
+            // (umbrella mode 1 -> calc_xi mode 2, src/verlet.f90:1049-1050)
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < SL; ++s)
+                if (S::valid(s, r)) xs[cc[s] * NB + k] = q[s];
+            __syncthreads();
+            if (k == 0) {
+#pragma unroll
+                for (int s = 0; s < SL; ++s)
+                    if (S::valid(s, r)) {
+                        double acc = 0.0;
+#pragma unroll 1
+                        for (int j = 0; j < NB; ++j) acc = acc + xs[cc[s] * NB + j];
+                        cent[tl * NC + cc[s]] = acc / NB;
+                    }
+            }
+            __syncthreads();
+            if (k == 0 && r == 0 && act) {
+                const double xi = calc_xi<S::NA>(a.rx, a.mass, cent + tl * NC, nullptr);
+                a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
+            }
+        }
+
+        // forces of every bead (src/verlet.f90:772-777)
+        {
+            double x[SL];
+#pragma unroll
+            for (int s = 0; s < SL; ++s) x[s] = q[s];
+            S::template egrad<T>(x, vb, g);
+        }
+        // second half kick (src/verlet.f90:1060-1062)
+#pragma unroll
+        for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
+
+        if constexpr (ENS == CAM_ENS_ANDERSEN) {
+            // Andersen: all bead momenta redrawn every andersen_step steps
+            // (src/verlet.f90:1159-1165, src/andersen.f90:37-74)
+            const uint64_t istep = *a.step_base + (uint64_t)(a.step_off + it) + 1;
+            if (istep % (uint64_t)a.andersen_step == 0) {
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    const double sd = sqrt(m[s] / (beta / NB));
+                    p[s] = andersen_normal(a.seed, (uint32_t)traj, istep, cc[s], k) * sd;
+                }
+            }
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+            if (S::valid(s, r)) {
+                const size_t o = base + (size_t)cc[s] * NB + k;
+                a.q[o] = q[s];
+                a.p[o] = p[s];
+                a.g[o] = g[s];
+            }
+        if (r == 0) a.vbead[(size_t)traj * NB + k] = vb;
+    }
+}
+
+// Forces and bead energies of the current positions (mdinit, src/mdinit.f90:79-83).
+template <class S, int NB, typename T>
+__global__ __launch_bounds__(kBlock) void force_kernel(const double *__restrict__ qa, double *__restrict__ ga,
+                                                       double *__restrict__ vbead, int ntraj)
+{
+    constexpr int NC = S::NC, L = S::L, SL = S::SLOTS, BPB = kBlock / L, TPB = BPB / NB;
+    const int tid = threadIdx.x, r = tid % L, bib = tid / L, tl = bib / NB, k = bib % NB;
+    const int traj_raw = blockIdx.x * TPB + tl;
+    const bool act = traj_raw < ntraj;
+    const int traj = act ? traj_raw : ntraj - 1;
+    const size_t base = (size_t)traj * NC * NB;
+    double x[SL], g[SL], vb;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) x[s] = S::valid(s, r) ? qa[base + (size_t)S::coord(s, r) * NB + k] : 0.0;
+    S::template egrad<T>(x, vb, g);
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+            if (S::valid(s, r)) ga[base + (size_t)S::coord(s, r) * NB + k] = g[s];
+        if (r == 0) vbead[(size_t)traj * NB + k] = vb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-trajectory helper kernels (one thread per trajectory).
+// ---------------------------------------------------------------------------
+struct TrajArgs {
+    double *q, *p;
+    const double *g, *vbead, *beta;
+    int ntraj, natoms, nbeads;
+    double dt;
+    double mass[CAM_MAX_ATOMS];
+    Rx rx;
+};
+
+// COM translation and rotation removal over all beads, as src/transrot.f90:36-237
+// computes it (including its total mass of nbeads^2 * M).
+__global__ void transrot_kernel(TrajArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntraj) return;
+    const int NA = a.natoms, NB = a.nbeads, NC = 3 * NA;
+    double *q = a.q + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
+    auto Q = [&](int d, int i, int k) -> double & { return q[(size_t)(3 * i + d) * NB + k]; };
+    auto P = [&](int d, int i, int k) -> double & { return p[(size_t)(3 * i + d) * NB + k]; };
+    double totmass = 0.0, vtot[3] = { 0, 0, 0 };
+    for (int i = 0; i < NA; ++i) {
+        const double w = a.mass[i];
+        for (int k = 0; k < NB; ++k) {
+            totmass = totmass + w;
+            for (int j = 0; j < 3; ++j) vtot[j] = vtot[j] + (P(j, i, k) / a.mass[i]) * w;
+        }
+    }
+    totmass = totmass * NB;
+    for (int j = 0; j < 3; ++j) vtot[j] = vtot[j] / totmass;
+    double xt = 0, yt = 0, zt = 0;
+    for (int i = 0; i < NA; ++i) {
+        const double w = a.mass[i];
+        for (int k = 0; k < NB; ++k) {
+            xt = xt + Q(0, i, k) * w;
+            yt = yt + Q(1, i, k) * w;
+            zt = zt + Q(2, i, k) * w;
+        }
+    }
+    xt = xt / totmass;
+    yt = yt / totmass;
+    zt = zt / totmass;
+    double mang[3] = { 0, 0, 0 };
+    for (int i = 0; i < NA; ++i) {
+        const double w = a.mass[i];
+        for (int k = 0; k < NB; ++k) {
+            const double vx = P(0, i, k) / a.mass[i], vy = P(1, i, k) / a.mass[i], vz = P(2, i, k) / a.mass[i];
+            mang[0] = mang[0] + (Q(1, i, k) * vz - Q(2, i, k) * vy) * w;
+            mang[1] = mang[1] + (Q(2, i, k) * vx - Q(0, i, k) * vz) * w;
+            mang[2] = mang[2] + (Q(0, i, k) * vy - Q(1, i, k) * vx) * w;
+        }
+    }
+    mang[0] = mang[0] - (yt * vtot[2] - zt * vtot[1]) * totmass;
+    mang[1] = mang[1] - (zt * vtot[0] - xt * vtot[2]) * totmass;
+    mang[2] = mang[2] - (xt * vtot[1] - yt * vtot[0]) * totmass;
+    double xx = 0, xy = 0, xz = 0, yy = 0, yz = 0, zz = 0;
+    for (int i = 0; i < NA; ++i) {
+        const double w = a.mass[i];
+        for (int k = 0; k < NB; ++k) {
+            const double xd = Q(0, i, k) - xt, yd = Q(1, i, k) - yt, zd = Q(2, i, k) - zt;
+            xx = xx + xd * xd * w;
+            xy = xy + xd * yd * w;
+            xz = xz + xd * zd * w;
+            yy = yy + yd * yd * w;
+            yz = yz + yd * zd * w;
+            zz = zz + zd * zd * w;
+        }
+    }
+    // tensor, column-major a(row,col) as the reference builds it
+    double ten[3][3];
+    ten[0][0] = yy + zz; ten[0][1] = -xy;    ten[0][2] = -xz;
+    ten[1][0] = -xy;     ten[1][1] = xx + zz; ten[1][2] = -yz;
+    ten[2][0] = -xz;     ten[2][1] = -yz;    ten[2][2] = xx + yy;
+    if (NA <= 2) {
+        ten[0][0] += 0.000001;
+        ten[1][1] += 0.000001;
+        ten[2][2] += 0.000001;
+    }
+    // Gauss-Jordan inversion with full pivoting (src/invert.f90)
+    {
+        int ipiv[3] = { 0, 0, 0 }, indxr[3], indxc[3];
+        for (int i = 0; i < 3; ++i) {
+            double big = 0.0;
+            int irow = 0, icol = 0;
+            for (int j = 0; j < 3; ++j)
+                if (ipiv[j] != 1)
+                    for (int kk = 0; kk < 3; ++kk)
+                        if (ipiv[kk] == 0 && fabs(ten[j][kk]) >= big) {
+                            big = fabs(ten[j][kk]);
+                            irow = j;
+                            icol = kk;
+                        }
+            ipiv[icol] += 1;
+            if (irow != icol)
+                for (int j = 0; j < 3; ++j) {
+                    const double tmp = ten[irow][j];
+                    ten[irow][j] = ten[icol][j];
+                    ten[icol][j] = tmp;
+                }
+            indxr[i] = irow;
+            indxc[i] = icol;
+            const double piv = ten[icol][icol];
+            ten[icol][icol] = 1.0;
+            for (int j = 0; j < 3; ++j) ten[icol][j] = ten[icol][j] / piv;
+            for (int j = 0; j < 3; ++j)
+                if (j != icol) {
+                    const double tmp = ten[j][icol];
+                    ten[j][icol] = 0.0;
+                    for (int kk = 0; kk < 3; ++kk) ten[j][kk] = ten[j][kk] - ten[icol][kk] * tmp;
+                }
+        }
+        for (int i = 2; i >= 0; --i)
+            if (indxr[i] != indxc[i])
+                for (int kk = 0; kk < 3; ++kk) {
+                    const double tmp = ten[kk][indxr[i]];
+                    ten[kk][indxr[i]] = ten[kk][indxc[i]];
+                    ten[kk][indxc[i]] = tmp;
+                }
+    }
+    double vang[3];
+    for (int i = 0; i < 3; ++i) {
+        vang[i] = 0.0;
+        for (int j = 0; j < 3; ++j) vang[i] = vang[i] + ten[i][j] * mang[j];
+    }
+    for (int i = 0; i < NA; ++i)
+        for (int k = 0; k < NB; ++k) {
+            double v[3];
+            for (int j = 0; j < 3; ++j) v[j] = P(j, i, k) / a.mass[i] - vtot[j];
+            const double xd = Q(0, i, k) - xt, yd = Q(1, i, k) - yt, zd = Q(2, i, k) - zt;
+            v[0] = v[0] - vang[1] * zd + vang[2] * yd;
+            v[1] = v[1] - vang[2] * xd + vang[0] * zd;
+            v[2] = v[2] - vang[0] * yd + vang[1] * xd;
+            for (int j = 0; j < 3; ++j) P(j, i, k) = v[j] * a.mass[i];
+        }
+}
+
+// Centroid of trajectory t (3*natoms, atom-major).
+__device__ void centroid_of(const TrajArgs &a, int t, double *c)
+{
+    const int NB = a.nbeads, NC = 3 * a.natoms;
+    const double *q = a.q + (size_t)t * NC * NB;
+    for (int i = 0; i < NC; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < NB; ++k) s = s + q[(size_t)i * NB + k];
+        c[i] = s / NB;
+    }
+}
+
+// constrain_q (src/constrain_q.f90:35-113) and optionally constrain_p
+// (src/constrain_p.f90:36-76) on the centroid's dividing surface.
+template <int NA>
+__global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads, NC = 3 * NA;
+    double c[3 * NA], dxi_act[3 * NA], qt[3 * NA], dxn[3 * NA];
+    centroid_of(a, t, c);
+    (void)calc_xi<NA>(a.rx, a.mass, c, dxi_act);
+    double mult = 0.0, coeff = 0.0;
+    int good = 0;
+    for (int iter = 1; iter <= 200; ++iter) {
+        coeff = mult * a.dt * a.dt / NB;
+        for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dxi_act[i] / a.mass[i / 3];
+        const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
+        double dsigma = 0.0;
+        for (int i = 0; i < NC; ++i) dsigma = dsigma + dxn[i] * a.dt * a.dt * dxi_act[i] / (a.mass[i / 3] * NB);
+        const double dx = sigma / dsigma;
+        mult = mult - dx;
+        // single-precision literals 1.0E-8 and 1.0E-10 of the reference
+        if (fabs(dx) < (double)1.0e-8f || fabs(sigma) < (double)1.0e-10f) break;
+        if (iter == 200) {
+            good = 1;
+            break;
+        }
+    }
+    if (status) status[t] = good;
+    if (good) return;
+    double *q = a.q + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
+    // note: the shift uses the coefficient of the last iteration, computed
+    // before its multiplier update, as the reference does
+    for (int i = 0; i < NC; ++i)
+        for (int k = 0; k < NB; ++k) {
+            q[(size_t)i * NB + k] = q[(size_t)i * NB + k] + coeff / a.mass[i / 3] * dxi_act[i];
+            p[(size_t)i * NB + k] = p[(size_t)i * NB + k] + mult * a.dt / NB * dxi_act[i];
+        }
+    if (!project_p) return;
+    double c1 = 0.0, c2 = 0.0;
+    for (int i = 0; i < NC; ++i)
+        for (int k = 0; k < NB; ++k) c1 = c1 + dxi_act[i] * p[(size_t)i * NB + k] / a.mass[i / 3];
+    for (int i = 0; i < NC; ++i) c2 = c2 + dxi_act[i] * dxi_act[i] / a.mass[i / 3];
+    const double lam = -c1 / c2 / NB;
+    for (int i = 0; i < NC; ++i)
+        for (int k = 0; k < NB; ++k) p[(size_t)i * NB + k] = p[(size_t)i * NB + k] + lam * dxi_act[i];
+}
+
+// Flux weight vs/fs of each child and the kappa denominator contribution
+// (src/recross_serial.f90:189-233).
+template <int NA>
+__global__ void child_begin_kernel(TrajArgs a, double *weight, double *den_part)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads, NC = 3 * NA;
+    double c[3 * NA], dxi[3 * NA];
+    centroid_of(a, t, c);
+    (void)calc_xi<NA>(a.rx, a.mass, c, dxi);
+    const double *p = a.p + (size_t)t * NC * NB;
+    double vs = 0.0;
+    for (int i = 0; i < NC; ++i)
+        for (int k = 0; k < NB; ++k) vs = vs + dxi[i] * p[(size_t)i * NB + k] / a.mass[i / 3];
+    vs = vs / NB;
+    double fs = 0.0;
+    for (int i = 0; i < NC; ++i) fs = fs + dxi[i] * dxi[i] / a.mass[i / 3];
+    const double beta = a.beta[t];
+    fs = sqrt(fs / (2.0 * 3.141592653589793 * beta));
+    weight[t] = vs / fs;
+    den_part[t] = vs > 0.0 ? vs / fs : 0.0;
+}
+
+// num(t) += sum_traj weight * flag(t, traj), one block per step, fixed order.
+__global__ void kappa_reduce_kernel(const uint8_t *flag, const double *weight, int ntraj, double *num)
+{
+    __shared__ double part[256];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < ntraj; t += 256) acc += flag[(size_t)s * ntraj + t] ? weight[t] : 0.0;
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) num[s] += part[0];
+}
+
+__global__ void sum_kernel(const double *v, int n, double *out)
+{
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) acc += v[t];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out += part[0];
+}
+
+__global__ void advance_kernel(uint64_t *ctr, int n) { *ctr += (uint64_t)n; }
+
+// Redraw momenta (andersen, src/andersen.f90:37-74).
+__global__ void draw_kernel(double *p, const double *beta, int ntraj, int natoms, int nbeads, uint64_t seed,
+                            uint64_t draw, int pair, TrajArgs a)
+{
+    const size_t NC = 3 * natoms;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ntraj * NC * nbeads) return;
+    const int k = (int)(i % nbeads);
+    const int c = (int)((i / nbeads) % NC);
+    const int t = (int)(i / (NC * nbeads));
+    const int src = pair ? (t & ~1) : t;
+    const double sd = sqrt(a.mass[c / 3] / (beta[src] / nbeads));
+    const double z = andersen_normal(seed, (uint32_t)src, draw, c, k) * sd;
+    p[i] = (pair && (t & 1)) ? -z : z;
+}
+
+// Observables per trajectory.
+__global__ void observe_kernel(TrajArgs a, const uint8_t *, double *epot, double *ekin, double *espring,
+                               double *xi, int *flags, int child)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads, NC = 3 * a.natoms;
+    const double *q = a.q + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
+    double ep = 0.0, ek = 0.0, es = 0.0;
+    int fl = 0;
+    for (int k = 0; k < NB; ++k) ep += a.vbead[(size_t)t * NB + k];
+    const double wn = NB / a.beta[t];
+    for (int i = 0; i < NC; ++i) {
+        const double m = a.mass[i / 3];
+        for (int k = 0; k < NB; ++k) {
+            const double pk = p[(size_t)i * NB + k], qk = q[(size_t)i * NB + k];
+            ek += pk * pk / (2.0 * m);
+            if (!isfinite(qk)) fl |= 1;
+            if (NB > 1) {
+                const double dq = qk - q[(size_t)i * NB + (k + 1) % NB];
+                es += 0.5 * m * wn * wn * dq * dq;
+            }
+        }
+    }
+    epot[t] = ep;
+    ekin[t] = ek;
+    espring[t] = es;
+    flags[t] = fl;
+    if (child) {
+        double c[3 * CAM_MAX_ATOMS];
+        centroid_of(a, t, c);
+        xi[t] = a.natoms == 3 ? calc_xi<3>(a.rx, a.mass, c, nullptr) : calc_xi<6>(a.rx, a.mass, c, nullptr);
+    } else {
+        xi[t] = 0.0;
+    }
+}
+
+// Layout conversions between the ABI blocks (3,natoms,nbeads) and the device
+// layout [3*natoms][nbeads].
+__global__ void to_device_layout(const double *in, double *out, int ntraj, int nc, int nb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ntraj * nc * nb) return;
+    const int c = (int)(i % nc);
+    const int k = (int)((i / nc) % nb);
+    const size_t t = i / ((size_t)nc * nb);
+    out[(t * nc + c) * nb + k] = in[i];
+}
+__global__ void from_device_layout(const double *in, double *out, int ntraj, int nc, int nb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)ntraj * nc * nb) return;
+    const int c = (int)(i % nc);
+    const int k = (int)((i / nc) % nb);
+    const size_t t = i / ((size_t)nc * nb);
+    out[i] = in[(t * nc + c) * nb + k];
+}
+
+}  // namespace
+}  // namespace cam
+
+// ---------------------------------------------------------------------------
+// Host side of the engine.
+// ---------------------------------------------------------------------------
+using namespace cam;
+
+typedef void (*step_fn)(StepArgs);
+typedef void (*force_fn)(const double *, double *, double *, int);
+
+struct cam_engine {
+    cam_config cfg{};
+    Rx rx{};
+    hipStream_t stream = nullptr;
+    double *q = nullptr, *p = nullptr, *g = nullptr, *vbead = nullptr, *beta = nullptr;
+    double *tfwd = nullptr, *tinv = nullptr;
+    double *weight = nullptr, *den_part = nullptr, *den = nullptr, *num = nullptr;
+    uint8_t *xi_flag = nullptr;
+    uint64_t *step_base = nullptr;
+    double *scratch_io = nullptr;     // staging for layout conversion
+    int num_cap = 0;
+    bool have_state = false;
+    int spl = 1;                      // steps per launch
+    int wpe = 1;
+    step_fn step = nullptr;
+    force_fn force = nullptr;
+    std::map<int, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    int tpb = 1;                      // trajectories per block of the step kernel
+};
+
+namespace {
+
+template <class S, int NB, typename T>
+step_fn pick_step(int ens, int wpe)
+{
+    (void)wpe;
+    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1>;
+    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1>;
+    return step_kernel<S, NB, T, CAM_ENS_CHILD, 1>;
+}
+
+template <class S, typename T>
+bool pick_nb(int nb, int ens, int wpe, step_fn &st, force_fn &fo)
+{
+    switch (nb) {
+#define CAM_NB_CASE(N)                               \
+    case N:                                          \
+        st = pick_step<S, N, T>(ens, wpe);           \
+        fo = force_kernel<S, N, T>;                  \
+        return true;
+        CAM_NB_CASE(1)
+        CAM_NB_CASE(2)
+        CAM_NB_CASE(4)
+        CAM_NB_CASE(8)
+        CAM_NB_CASE(16)
+        CAM_NB_CASE(32)
+        CAM_NB_CASE(64)
+#undef CAM_NB_CASE
+    default:
+        return false;
+    }
+}
+
+void build_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double> &inv)
+{
+    // element (j,k) of the stored arrays is T[k][j]
+    fwd.assign((size_t)nb * nb, 0.0);
+    inv.assign((size_t)nb * nb, 0.0);
+    const double pi = 3.141592653589793;
+    if (mode == CAM_NM_REFERENCE) {
+        // rfft/irfft: Re(DFT)/sqrt(N) -- the symmetric cosine matrix, used both ways
+        const double f = std::sqrt(1.0 / nb);
+        for (int kk = 0; kk < nb; ++kk)
+            for (int j = 0; j < nb; ++j) {
+                const long m = ((long)kk * j) % nb;
+                const double c = std::cos(2.0 * pi * (double)m / nb) * f;
+                fwd[(size_t)j * nb + kk] = c;
+                inv[(size_t)j * nb + kk] = c;
+            }
+    } else {
+        // orthogonal real normal-mode basis O[j][k] (bead j, mode k); mode k
+        // and mode nb-k share the frequency 2 (nb/beta) sin(k pi / nb)
+        std::vector<double> O((size_t)nb * nb);
+        for (int j = 0; j < nb; ++j)
+            for (int kk = 0; kk < nb; ++kk) {
+                double v;
+                if (kk == 0) v = std::sqrt(1.0 / nb);
+                else if (2 * kk < nb) v = std::sqrt(2.0 / nb) * std::cos(2.0 * pi * j * kk / nb);
+                else if (2 * kk == nb) v = ((j & 1) ? -1.0 : 1.0) / std::sqrt((double)nb);
+                else v = std::sqrt(2.0 / nb) * std::sin(2.0 * pi * j * kk / nb);
+                O[(size_t)j * nb + kk] = v;
+            }
+        // forward: y_k = sum_j O[j][k] x_j  -> T[k][j] = O[j][k]
+        // inverse: x_k = sum_j O[k][j] y_j  -> Ti[k][j] = O[k][j]
+        for (int kk = 0; kk < nb; ++kk)
+            for (int j = 0; j < nb; ++j) {
+                fwd[(size_t)j * nb + kk] = O[(size_t)j * nb + kk];
+                inv[(size_t)j * nb + kk] = O[(size_t)kk * nb + j];
+            }
+    }
+}
+
+TrajArgs traj_args(cam_engine *e)
+{
+    TrajArgs a{};
+    a.q = e->q;
+    a.p = e->p;
+    a.g = e->g;
+    a.vbead = e->vbead;
+    a.beta = e->beta;
+    a.ntraj = e->cfg.ntraj;
+    a.natoms = e->cfg.natoms;
+    a.nbeads = e->cfg.nbeads;
+    a.dt = e->cfg.dt;
+    std::memcpy(a.mass, e->cfg.mass, sizeof a.mass);
+    a.rx = e->rx;
+    return a;
+}
+
+int run_forces(cam_engine *e)
+{
+    const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
+    hipLaunchKernelGGL(e->force, dim3(grid), dim3(kBlock), 0, e->stream, e->q, e->g, e->vbead, e->cfg.ntraj);
+    CAM_LAUNCH_CHECK();
+    return CAM_OK;
+}
+
+int ensure_num(cam_engine *e, int nsteps)
+{
+    if (nsteps <= e->num_cap) return CAM_OK;
+    if (e->num) CAM_HIP(hipFree(e->num));
+    if (e->xi_flag) CAM_HIP(hipFree(e->xi_flag));
+    e->num = nullptr;
+    e->xi_flag = nullptr;
+    CAM_HIP(hipMalloc(&e->num, (size_t)nsteps * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->xi_flag, (size_t)nsteps * e->cfg.ntraj));
+    CAM_HIP(hipMemsetAsync(e->num, 0, (size_t)nsteps * sizeof(double), e->stream));
+    e->num_cap = nsteps;
+    // captured graphs refer to the old buffers
+    for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    e->graphs.clear();
+    return CAM_OK;
+}
+
+// Enqueue one run of nsteps (used both for capture and eagerly).
+int enqueue_run(cam_engine *e, int nsteps)
+{
+    StepArgs a{};
+    a.q = e->q;
+    a.p = e->p;
+    a.g = e->g;
+    a.vbead = e->vbead;
+    a.beta = e->beta;
+    a.tfwd = e->tfwd;
+    a.tinv = e->tinv;
+    a.xi_flag = e->xi_flag;
+    a.step_base = e->step_base;
+    a.ntraj = e->cfg.ntraj;
+    a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
+    a.seed = e->cfg.seed;
+    a.dt = e->cfg.dt;
+    std::memcpy(a.mass, e->cfg.mass, sizeof a.mass);
+    a.rx = e->rx;
+    const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
+    const TrajArgs ta = traj_args(e);
+    const int tgrid = (e->cfg.ntraj + 127) / 128;
+    int done = 0;
+    while (done < nsteps) {
+        a.nsteps = std::min(e->spl, nsteps - done);
+        a.step_off = done;
+        hipLaunchKernelGGL(e->step, dim3(grid), dim3(kBlock), 0, e->stream, a);
+        CAM_LAUNCH_CHECK();
+        if (e->cfg.remove_com) {
+            hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, e->stream, ta);
+            CAM_LAUNCH_CHECK();
+        }
+        done += a.nsteps;
+    }
+    if (e->cfg.ensemble == CAM_ENS_CHILD) {
+        hipLaunchKernelGGL(kappa_reduce_kernel, dim3(nsteps), dim3(256), 0, e->stream, e->xi_flag, e->weight,
+                           e->cfg.ntraj, e->num);
+        CAM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base, nsteps);
+    CAM_LAUNCH_CHECK();
+    return CAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cam_engine_create(const cam_config *cfg, cam_engine **out)
+{
+    CAM_CHECK_ARG(cfg && out, "null argument");
+    *out = nullptr;
+    const int surf = cfg->surface;
+    CAM_CHECK_ARG(surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_CH4H, "unknown surface %d", surf);
+    const int na = surf == CAM_SURFACE_H3 ? 3 : 6;
+    CAM_CHECK_ARG(cfg->natoms == na, "surface %d needs natoms = %d (got %d)", surf, na, cfg->natoms);
+    const int nb = cfg->nbeads;
+    CAM_CHECK_ARG(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, "nbeads must be a power of two in 1..64 (got %d)", nb);
+    CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
+    CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
+    CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
+    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_CHILD, "bad ensemble");
+    CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
+    for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
+    CAM_CHECK_ARG(!(cfg->remove_com && cfg->ensemble == CAM_ENS_CHILD),
+                  "remove_com applies to dynamic.x trajectories (constrain <= 0) only");
+    const cam_reaction &rr = cfg->reaction;
+    Rx rx{};
+    if (cfg->ensemble == CAM_ENS_CHILD) {
+        CAM_CHECK_ARG(rr.n_form >= 0 && rr.n_form <= CAM_MAX_BONDS && rr.n_break >= 0 && rr.n_break <= CAM_MAX_BONDS &&
+                          rr.n_form + rr.n_break > 0,
+                      "reaction needs 1..%d forming and breaking bonds", CAM_MAX_BONDS);
+        CAM_CHECK_ARG(rr.n_reac >= 2 && rr.n_reac <= CAM_MAX_REAC, "reaction needs 2..%d reactants", CAM_MAX_REAC);
+    }
+    rx.n_form = rr.n_form;
+    rx.n_break = rr.n_break;
+    rx.n_reac = rr.n_reac;
+    rx.r_inf = rr.r_inf;
+    rx.xi_ideal = rr.xi_ideal;
+    for (int i = 0; i < rr.n_form && i < CAM_MAX_BONDS; ++i) {
+        for (int j = 0; j < 2; ++j) {
+            CAM_CHECK_ARG(rr.bond_form[i][j] >= 1 && rr.bond_form[i][j] <= na, "bond_form atom out of range");
+            rx.form[i][j] = rr.bond_form[i][j] - 1;
+        }
+        rx.form_ref[i] = rr.form_ref[i];
+    }
+    for (int i = 0; i < rr.n_break && i < CAM_MAX_BONDS; ++i) {
+        for (int j = 0; j < 2; ++j) {
+            CAM_CHECK_ARG(rr.bond_break[i][j] >= 1 && rr.bond_break[i][j] <= na, "bond_break atom out of range");
+            rx.brk[i][j] = rr.bond_break[i][j] - 1;
+        }
+        rx.break_ref[i] = rr.break_ref[i];
+    }
+    for (int k = 0; k < rr.n_reac && k < CAM_MAX_REAC; ++k) {
+        CAM_CHECK_ARG(rr.reac_natoms[k] >= 1 && rr.reac_natoms[k] <= na, "reactant size out of range");
+        rx.reac_n[k] = rr.reac_natoms[k];
+        rx.reac_mass[k] = 0.0;
+        for (int i = 0; i < rr.reac_natoms[k]; ++i) {
+            CAM_CHECK_ARG(rr.reac_atoms[k][i] >= 1 && rr.reac_atoms[k][i] <= na, "reactant atom out of range");
+            rx.reac_at[k][i] = rr.reac_atoms[k][i] - 1;
+            rx.reac_mass[k] += cfg->mass[rx.reac_at[k][i]];
+        }
+    }
+    clear_error();
+    std::unique_ptr<cam_engine> e(new cam_engine());
+    e->cfg = *cfg;
+    e->rx = rx;
+    const int L = surf == CAM_SURFACE_H3 ? 1 : 4;
+    e->tpb = kBlock / (L * nb);
+    // steps fused per launch: state stays in registers for this many steps
+    e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
+    if (cfg->remove_com) e->spl = 1;
+    e->wpe = 1;
+    bool ok;
+    if (surf == CAM_SURFACE_H3)
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, e->step, e->force)
+                                             : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, e->step, e->force);
+    else
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, e->step, e->force)
+                                             : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, e->step, e->force);
+    if (!ok) {
+        set_error("nbeads %d not supported", nb);
+        return CAM_ERR_UNSUPPORTED;
+    }
+    CAM_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    const size_t nstate = (size_t)cfg->ntraj * 3 * na * nb;
+    CAM_HIP(hipMalloc(&e->q, nstate * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->p, nstate * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->g, nstate * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->scratch_io, nstate * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->vbead, (size_t)cfg->ntraj * nb * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->beta, (size_t)cfg->ntraj * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->weight, (size_t)cfg->ntraj * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->den_part, (size_t)cfg->ntraj * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->den, sizeof(double)));
+    CAM_HIP(hipMalloc(&e->step_base, sizeof(uint64_t)));
+    CAM_HIP(hipMalloc(&e->tfwd, (size_t)nb * nb * sizeof(double)));
+    CAM_HIP(hipMalloc(&e->tinv, (size_t)nb * nb * sizeof(double)));
+    std::vector<double> fwd, inv;
+    build_matrices(nb, cfg->nm_mode, fwd, inv);
+    CAM_HIP(hipMemcpy(e->tfwd, fwd.data(), fwd.size() * sizeof(double), hipMemcpyHostToDevice));
+    CAM_HIP(hipMemcpy(e->tinv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+    CAM_HIP(hipMemset(e->step_base, 0, sizeof(uint64_t)));
+    CAM_HIP(hipMemset(e->den, 0, sizeof(double)));
+    CAM_HIP(hipMemset(e->weight, 0, (size_t)cfg->ntraj * sizeof(double)));
+    CAM_HIP(hipEventCreate(&e->ev0));
+    CAM_HIP(hipEventCreate(&e->ev1));
+    *out = e.release();
+    return CAM_OK;
+}
+
+int cam_engine_destroy(cam_engine *e)
+{
+    if (!e) return CAM_OK;
+    (void)hipStreamSynchronize(e->stream);
+    for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->weight, e->den_part, e->den,
+                       e->num, e->scratch_io };
+    for (double *b : bufs)
+        if (b) (void)hipFree(b);
+    if (e->xi_flag) (void)hipFree(e->xi_flag);
+    if (e->step_base) (void)hipFree(e->step_base);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return CAM_OK;
+}
+
+int cam_engine_set_state(cam_engine *e, const double *q, const double *p, const double *beta)
+{
+    CAM_CHECK_ARG(e && q && p, "null argument");
+    CAM_CHECK_ARG(beta || e->cfg.ensemble == CAM_ENS_NVE, "beta is required for this ensemble");
+    const int nt = e->cfg.ntraj, nc = 3 * e->cfg.natoms, nb = e->cfg.nbeads;
+    const size_t n = (size_t)nt * nc * nb;
+    std::vector<double> hb(nt, 1.0);
+    if (beta) {
+        for (int t = 0; t < nt; ++t) CAM_CHECK_ARG(beta[t] > 0.0, "beta[%d] must be > 0", t);
+        std::copy(beta, beta + nt, hb.begin());
+    }
+    clear_error();
+    const int grid = (int)((n + 255) / 256);
+    CAM_HIP(hipMemcpyAsync(e->scratch_io, q, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(to_device_layout, dim3(grid), dim3(256), 0, e->stream, e->scratch_io, e->q, nt, nc, nb);
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipMemcpyAsync(e->scratch_io, p, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(to_device_layout, dim3(grid), dim3(256), 0, e->stream, e->scratch_io, e->p, nt, nc, nb);
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipMemcpyAsync(e->beta, hb.data(), nt * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    int rc = run_forces(e);
+    if (rc != CAM_OK) return rc;
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    e->have_state = true;
+    return CAM_OK;
+}
+
+int cam_engine_get_state(cam_engine *e, double *q, double *p)
+{
+    CAM_CHECK_ARG(e && (q || p), "null argument");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    const int nt = e->cfg.ntraj, nc = 3 * e->cfg.natoms, nb = e->cfg.nbeads;
+    const size_t n = (size_t)nt * nc * nb;
+    const int grid = (int)((n + 255) / 256);
+    clear_error();
+    if (q) {
+        hipLaunchKernelGGL(from_device_layout, dim3(grid), dim3(256), 0, e->stream, e->q, e->scratch_io, nt, nc, nb);
+        CAM_LAUNCH_CHECK();
+        CAM_HIP(hipMemcpyAsync(q, e->scratch_io, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        CAM_HIP(hipStreamSynchronize(e->stream));
+    }
+    if (p) {
+        hipLaunchKernelGGL(from_device_layout, dim3(grid), dim3(256), 0, e->stream, e->p, e->scratch_io, nt, nc, nb);
+        CAM_LAUNCH_CHECK();
+        CAM_HIP(hipMemcpyAsync(p, e->scratch_io, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        CAM_HIP(hipStreamSynchronize(e->stream));
+    }
+    return CAM_OK;
+}
+
+int cam_engine_get_forces(cam_engine *e, double *dvdq, double *vbead)
+{
+    CAM_CHECK_ARG(e && (dvdq || vbead), "null argument");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    const int nt = e->cfg.ntraj, nc = 3 * e->cfg.natoms, nb = e->cfg.nbeads;
+    const size_t n = (size_t)nt * nc * nb;
+    clear_error();
+    if (dvdq) {
+        const int grid = (int)((n + 255) / 256);
+        hipLaunchKernelGGL(from_device_layout, dim3(grid), dim3(256), 0, e->stream, e->g, e->scratch_io, nt, nc, nb);
+        CAM_LAUNCH_CHECK();
+        CAM_HIP(hipMemcpyAsync(dvdq, e->scratch_io, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    }
+    if (vbead) CAM_HIP(hipMemcpyAsync(vbead, e->vbead, (size_t)nt * nb * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    return CAM_OK;
+}
+
+int cam_engine_draw_momenta(cam_engine *e, uint64_t draw, int pair)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(!pair || (e->cfg.ntraj % 2) == 0, "antithetic pairs need an even ntraj");
+    clear_error();
+    const size_t n = (size_t)e->cfg.ntraj * 3 * e->cfg.natoms * e->cfg.nbeads;
+    const TrajArgs ta = traj_args(e);
+    // draw in the ABI layout order, then convert
+    hipLaunchKernelGGL(draw_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, e->p, e->beta,
+                       e->cfg.ntraj, e->cfg.natoms, e->cfg.nbeads, e->cfg.seed, draw, pair, ta);
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    return CAM_OK;
+}
+
+int cam_engine_constrain(cam_engine *e, int project_p, int *status)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(e->rx.n_form + e->rx.n_break > 0 && e->rx.n_reac >= 2, "no reaction coordinate configured");
+    clear_error();
+    int *dstat = nullptr;
+    CAM_HIP(hipMalloc(&dstat, (size_t)e->cfg.ntraj * sizeof(int)));
+    const TrajArgs ta = traj_args(e);
+    const int grid = (e->cfg.ntraj + 127) / 128;
+    if (e->cfg.natoms == 3)
+        hipLaunchKernelGGL(constrain_kernel<3>, dim3(grid), dim3(128), 0, e->stream, ta, project_p, dstat);
+    else
+        hipLaunchKernelGGL(constrain_kernel<6>, dim3(grid), dim3(128), 0, e->stream, ta, project_p, dstat);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess && status)
+        le = hipMemcpyAsync(status, dstat, (size_t)e->cfg.ntraj * sizeof(int), hipMemcpyDeviceToHost, e->stream);
+    if (le == hipSuccess) le = hipStreamSynchronize(e->stream);
+    (void)hipFree(dstat);
+    CAM_HIP(le);
+    int rc = run_forces(e);
+    if (rc != CAM_OK) return rc;
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    return CAM_OK;
+}
+
+int cam_engine_begin_children(cam_engine *e)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_CHILD, "begin_children needs ensemble CHILD");
+    clear_error();
+    const TrajArgs ta = traj_args(e);
+    const int grid = (e->cfg.ntraj + 127) / 128;
+    if (e->cfg.natoms == 3)
+        hipLaunchKernelGGL(child_begin_kernel<3>, dim3(grid), dim3(128), 0, e->stream, ta, e->weight, e->den_part);
+    else
+        hipLaunchKernelGGL(child_begin_kernel<6>, dim3(grid), dim3(128), 0, e->stream, ta, e->weight, e->den_part);
+    CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, e->stream, e->den_part, e->cfg.ntraj, e->den);
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    return CAM_OK;
+}
+
+int cam_engine_run(cam_engine *e, int nsteps)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(nsteps >= 0, "nsteps must be >= 0");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    clear_error();
+    if (nsteps == 0) return CAM_OK;
+    if (e->cfg.ensemble == CAM_ENS_CHILD) {
+        int rc = ensure_num(e, nsteps);
+        if (rc != CAM_OK) return rc;
+    }
+    auto it = e->graphs.find(nsteps);
+    if (it == e->graphs.end()) {
+        hipGraph_t graph;
+        CAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_run(e, nsteps);
+        hipGraph_t captured = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &captured);
+        if (rc != CAM_OK) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return rc;
+        }
+        CAM_HIP(ce);
+        graph = captured;
+        hipGraphExec_t exec;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CAM_HIP(ie);
+        it = e->graphs.emplace(nsteps, exec).first;
+    }
+    CAM_HIP(hipEventRecord(e->ev0, e->stream));
+    CAM_HIP(hipGraphLaunch(it->second, e->stream));
+    CAM_HIP(hipEventRecord(e->ev1, e->stream));
+    e->last_launches = (nsteps + e->spl - 1) / e->spl;
+    e->last_ms = -1.0;
+    return CAM_OK;
+}
+
+int cam_engine_sync(cam_engine *e)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    clear_error();
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    if (e->last_ms < 0.0) {
+        float ms = 0.f;
+        CAM_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        e->last_ms = ms;
+    }
+    return CAM_OK;
+}
+
+int cam_engine_kappa(cam_engine *e, double *num, double *den, int nsteps)
+{
+    CAM_CHECK_ARG(e && num && den, "null argument");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_CHILD, "kappa needs ensemble CHILD");
+    CAM_CHECK_ARG(nsteps >= 0 && nsteps <= e->num_cap, "nsteps exceeds the last run (%d)", e->num_cap);
+    clear_error();
+    if (nsteps) CAM_HIP(hipMemcpyAsync(num, e->num, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    CAM_HIP(hipMemcpyAsync(den, e->den, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    return CAM_OK;
+}
+
+int cam_engine_observables(cam_engine *e, double *epot, double *ekin, double *espring, double *xi, int *flags)
+{
+    CAM_CHECK_ARG(e && epot && ekin && espring && xi && flags, "null argument");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    clear_error();
+    const int nt = e->cfg.ntraj;
+    double *buf = nullptr;
+    int *fl = nullptr;
+    CAM_HIP(hipMalloc(&buf, (size_t)4 * nt * sizeof(double)));
+    hipError_t err = hipMalloc(&fl, (size_t)nt * sizeof(int));
+    if (err == hipSuccess) {
+        const TrajArgs ta = traj_args(e);
+        hipLaunchKernelGGL(observe_kernel, dim3((nt + 127) / 128), dim3(128), 0, e->stream, ta, (const uint8_t *)nullptr,
+                           buf, buf + nt, buf + 2 * nt, buf + 3 * nt, fl, e->cfg.ensemble == CAM_ENS_CHILD ? 1 : 0);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(epot, buf, nt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(ekin, buf + nt, nt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(espring, buf + 2 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(xi, buf + 3 * nt, nt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(flags, fl, nt * sizeof(int), hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    (void)hipFree(buf);
+    if (fl) (void)hipFree(fl);
+    CAM_HIP(err);
+    return CAM_OK;
+}
+
+int cam_engine_device_state(cam_engine *e, double **q, double **p, double **dvdq, void **stream)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    if (q) *q = e->q;
+    if (p) *p = e->p;
+    if (dvdq) *dvdq = e->g;
+    if (stream) *stream = (void *)e->stream;
+    return CAM_OK;
+}
+
+int cam_engine_last_timing(cam_engine *e, double *ms_per_launch, int *steps_per_launch, int *launches)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->last_ms >= 0.0 && e->last_launches > 0, "no completed run (call cam_engine_sync first)");
+    if (ms_per_launch) *ms_per_launch = e->last_ms / e->last_launches;
+    if (steps_per_launch) *steps_per_launch = e->spl;
+    if (launches) *launches = e->last_launches;
+    return CAM_OK;
+}
+
+}  // extern "C"

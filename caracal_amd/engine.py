@@ -1,0 +1,154 @@
+"""Host mirror of the reference's RPMD step driver (verlet, src/verlet.f90:65)
+over the engine entry points of the C ABI."""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+@dataclass
+class Reaction:
+    """Bimolecular reaction coordinate, keys of the reference's MECHA section
+    (1-based atom indices, bohr)."""
+    bond_form: list = field(default_factory=list)     # [(i, j), ...]
+    bond_break: list = field(default_factory=list)
+    form_ref: list = field(default_factory=list)
+    break_ref: list = field(default_factory=list)
+    reactants: list = field(default_factory=list)     # [[atoms of reactant 1], [atoms of reactant 2]]
+    r_inf: float = 16.0
+    xi_ideal: float = 1.0
+
+    def to_c(self):
+        r = _abi.Reaction()
+        r.n_form, r.n_break = len(self.bond_form), len(self.bond_break)
+        for i, (a, b) in enumerate(self.bond_form):
+            r.bond_form[i][0], r.bond_form[i][1] = a, b
+            r.form_ref[i] = self.form_ref[i]
+        for i, (a, b) in enumerate(self.bond_break):
+            r.bond_break[i][0], r.bond_break[i][1] = a, b
+            r.break_ref[i] = self.break_ref[i]
+        r.n_reac = len(self.reactants)
+        for k, atoms in enumerate(self.reactants):
+            r.reac_natoms[k] = len(atoms)
+            for i, a in enumerate(atoms):
+                r.reac_atoms[k][i] = a
+        r.r_inf, r.xi_ideal = self.r_inf, self.xi_ideal
+        return r
+
+
+class RPMDEngine:
+    """A batch of ntraj ring polymers of nbeads beads on one analytic surface,
+    resident on the GPU.  State arrays have shape (ntraj, nbeads, natoms, 3)."""
+
+    def __init__(self, surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64,
+                 nm_mode=_abi.CAM_NM_REFERENCE, ensemble=_abi.CAM_ENS_NVE, remove_com=False,
+                 andersen_step=0, steps_per_launch=0, seed=0, reaction=None):
+        natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
+        masses = np.asarray(masses, dtype=np.float64)
+        if masses.shape != (natoms,):
+            raise ValueError(f"need {natoms} masses")
+        c = _abi.Config()
+        c.surface, c.natoms, c.nbeads, c.ntraj = surface, natoms, nbeads, ntraj
+        c.precision, c.nm_mode, c.ensemble = precision, nm_mode, ensemble
+        c.remove_com, c.andersen_step, c.steps_per_launch = int(remove_com), andersen_step, steps_per_launch
+        c.dt = dt
+        for i, m in enumerate(masses):
+            c.mass[i] = m
+        c.seed = seed
+        if reaction is not None:
+            c.reaction = reaction.to_c()
+        self.cfg = c
+        self.natoms, self.nbeads, self.ntraj = natoms, nbeads, ntraj
+        h = ctypes.c_void_p()
+        check("cam_engine_create", lib().cam_engine_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cam_engine_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.ntraj, self.nbeads, self.natoms, 3)
+
+    def set_state(self, q, p, beta=None):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(self.shape)
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(self.shape)
+        bp = None
+        if beta is not None:
+            b = np.ascontiguousarray(np.broadcast_to(beta, (self.ntraj,)), dtype=np.float64)
+            self._beta = b
+            bp = _ptr(b)
+        check("cam_engine_set_state", lib().cam_engine_set_state(self._h, _ptr(q), _ptr(p), bp))
+
+    def get_state(self):
+        q = np.empty(self.shape)
+        p = np.empty(self.shape)
+        check("cam_engine_get_state", lib().cam_engine_get_state(self._h, _ptr(q), _ptr(p)))
+        return q, p
+
+    def get_forces(self):
+        g = np.empty(self.shape)
+        v = np.empty((self.ntraj, self.nbeads))
+        check("cam_engine_get_forces", lib().cam_engine_get_forces(self._h, _ptr(g), _ptr(v)))
+        return v, g
+
+    def draw_momenta(self, draw, antithetic_pairs=False):
+        check("cam_engine_draw_momenta", lib().cam_engine_draw_momenta(self._h, draw, int(antithetic_pairs)))
+
+    def constrain(self, project_p=True):
+        st = np.zeros(self.ntraj, dtype=np.int32)
+        check("cam_engine_constrain", lib().cam_engine_constrain(self._h, int(project_p), st.ctypes.data_as(_ip)))
+        return st
+
+    def begin_children(self):
+        check("cam_engine_begin_children", lib().cam_engine_begin_children(self._h))
+
+    def run(self, nsteps, sync=True):
+        check("cam_engine_run", lib().cam_engine_run(self._h, nsteps))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        check("cam_engine_sync", lib().cam_engine_sync(self._h))
+
+    def kappa(self, nsteps):
+        num = np.empty(nsteps)
+        den = np.empty(1)
+        check("cam_engine_kappa", lib().cam_engine_kappa(self._h, _ptr(num), _ptr(den), nsteps))
+        return num, den[0]
+
+    def observables(self):
+        n = self.ntraj
+        ep, ek, es, xi = (np.empty(n) for _ in range(4))
+        fl = np.empty(n, dtype=np.int32)
+        check("cam_engine_observables",
+              lib().cam_engine_observables(self._h, _ptr(ep), _ptr(ek), _ptr(es), _ptr(xi), fl.ctypes.data_as(_ip)))
+        return dict(epot=ep, ekin=ek, espring=es, xi=xi, flags=fl)
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        spl = ctypes.c_int()
+        nl = ctypes.c_int()
+        check("cam_engine_last_timing",
+              lib().cam_engine_last_timing(self._h, ctypes.byref(ms), ctypes.byref(spl), ctypes.byref(nl)))
+        return ms.value, spl.value, nl.value

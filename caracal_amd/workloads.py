@@ -1,0 +1,101 @@
+"""Synthetic inputs of the RPMD workloads (seeded; no data files).
+
+h3_nve        3 H atoms (1.008 amu) on BKMP2, collinear at R = 1.757 bohr
+              (examples/calc_rate/h+h2/ts.xyz), bead positions + N(0, 0.01^2)
+              bohr, momenta Maxwell-Boltzmann at the bead temperature, seed 0.
+ch4h_children CH4 + H recrossing children on the CH5 surface: saddle
+              geometry + N(0, 0.02^2) per trajectory, centroid constrained to
+              xi = 0 (dividing surface xi_ideal = 1), momenta at 500 K.
+ragged        H3 / CH4H half-and-half with nbeads in {4,8,16,32,64} and
+              T in U[200, 1000] K, seed 1 (bucketed by surface and nbeads).
+"""
+import numpy as np
+
+from .engine import Reaction
+from .units import FS_TO_AU, MASS_C_REF, MASS_H_REF, beta_of, mass_au
+
+H3_TS = np.array([[0.0, 0.0, -1.757], [0.0, 0.0, 0.0], [0.0, 0.0, 1.757]])
+
+# First-order saddle of the CH5 surface (bohr; atom order h4, C, h1, h2, h3, Hb),
+# found by a partitioned rational-function Newton search on the oracle surface
+# (tests/golden/make_ch4h_saddle.py): one imaginary mode, |dV/dq| < 1e-12.
+CH4H_TS = np.array([
+    [-1.140933199706927, -1.1409331997068821, 1.2740139038988854],
+    [0.00838657829034074, 0.00838657829033506, 0.0083865782904126],
+    [1.5253054531799726, 1.5253054531798804, 1.525305453180047],
+    [1.2740139038988167, -1.1409331997069112, -1.1409331997068224],
+    [-1.1409331997068484, 1.2740139038988751, -1.1409331997068088],
+    [2.4775365649116083, 2.477536564911461, 2.477536564911683],
+])
+
+
+def h3_reaction():
+    """H_A H_B + H_C -> H_A + H_B H_C (examples/calc_rate/h+h2/rate.key)."""
+    return Reaction(bond_form=[(2, 3)], bond_break=[(1, 2)], form_ref=[1.757], break_ref=[1.757],
+                    reactants=[[1, 2], [3]], r_inf=16.0, xi_ideal=1.0)
+
+
+def ch4h_reaction():
+    """CH4 + H -> CH3 + H2: C-h1 breaks, h1-Hb forms."""
+    rc = CH4H_TS
+    return Reaction(bond_form=[(3, 6)], bond_break=[(2, 3)],
+                    form_ref=[float(np.linalg.norm(rc[2] - rc[5]))],
+                    break_ref=[float(np.linalg.norm(rc[1] - rc[2]))],
+                    reactants=[[1, 2, 3, 4, 5], [6]], r_inf=16.0, xi_ideal=1.0)
+
+
+def mb_momenta(rng, masses, nbeads, beta, shape):
+    """Maxwell-Boltzmann bead momenta: p ~ N(0, m nbeads / beta) (andersen.f90:50-56)."""
+    sd = np.sqrt(np.asarray(masses)[:, None] * nbeads / np.asarray(beta))  # (natoms, 1)
+    return rng.normal(0.0, 1.0, shape) * sd[None, :, :]
+
+
+def h3_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
+    rng = np.random.default_rng(seed)
+    masses = mass_au([1.008] * 3)
+    beta = float(beta_of(kelvin))
+    q = H3_TS[None] + rng.normal(0.0, 0.01, (nbeads, 3, 3))
+    p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 3, 3))
+    return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
+
+
+def ch4h_children(ntraj=4096, nbeads=32, seed=0, kelvin=500.0, dt_fs=0.1):
+    rng = np.random.default_rng(seed)
+    masses = mass_au([MASS_H_REF, MASS_C_REF, MASS_H_REF, MASS_H_REF, MASS_H_REF, MASS_H_REF])
+    beta = float(beta_of(kelvin))
+    pert = rng.normal(0.0, 0.02, (ntraj, 1, 6, 3))
+    q = np.broadcast_to(CH4H_TS[None, None] + pert, (ntraj, nbeads, 6, 3)).copy()
+    q += rng.normal(0.0, 0.02, (ntraj, nbeads, 6, 3)) * 0.25
+    p = np.zeros_like(q)
+    return dict(q=q, p=p, beta=np.full(ntraj, beta), masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads,
+                reaction=ch4h_reaction())
+
+
+def ragged(ntraj=8192, seed=1, dt_fs=0.1):
+    """Bucket list [(surface, nbeads, dict(q, p, beta, masses))] of the ragged batch."""
+    from ._abi import CAM_SURFACE_CH4H, CAM_SURFACE_H3
+    rng = np.random.default_rng(seed)
+    nb_choice = np.array([4, 8, 16, 32, 64])
+    surf = np.where(np.arange(ntraj) < ntraj // 2, CAM_SURFACE_H3, CAM_SURFACE_CH4H)
+    nbs = nb_choice[rng.integers(0, 5, ntraj)]
+    temps = rng.uniform(200.0, 1000.0, ntraj)
+    buckets = []
+    for s in (CAM_SURFACE_H3, CAM_SURFACE_CH4H):
+        for nb in nb_choice:
+            sel = np.nonzero((surf == s) & (nbs == nb))[0]
+            if len(sel) == 0:
+                continue
+            n = len(sel)
+            beta = beta_of(temps[sel])
+            if s == CAM_SURFACE_H3:
+                masses = mass_au([1.008] * 3)
+                base, na = H3_TS, 3
+            else:
+                masses = mass_au([MASS_H_REF, MASS_C_REF] + [MASS_H_REF] * 4)
+                base, na = CH4H_TS, 6
+            q = base[None, None] + rng.normal(0.0, 0.02, (n, int(nb), na, 3))
+            sd = np.sqrt(masses[None, None, :, None] * nb / beta[:, None, None, None])
+            p = rng.normal(0.0, 1.0, (n, int(nb), na, 3)) * sd
+            buckets.append((int(s), int(nb), dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU,
+                                                   traj_index=sel, temps=temps[sel])))
+    return buckets

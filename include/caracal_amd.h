@@ -1,0 +1,188 @@
+/*
+ * caracal_amd.h -- C ABI of the MI355X ring-polymer MD engine.
+ *
+ * Plain C types only (pointers, sizes, POD structs); no HIP or torch types.
+ * Every entry point returns an int status (CAM_OK == 0) and never aborts;
+ * cam_last_error() returns the message of the last failure on the calling
+ * thread.  Arrays use the reference's Fortran layout: a position block
+ * q(3,natoms,nbeads) is column-major, so the Cartesian component runs fastest,
+ * then the atom, then the bead; a batch stacks such blocks trajectory by
+ * trajectory.  Units are atomic units throughout (bohr, hartree, electron
+ * masses, atomic time), as in the reference.
+ *
+ * Two families of entry points:
+ *  (1) drop-in analytic surfaces, the bead-batched plugin contract of the
+ *      reference (src/egrad_h3.f:29-35, src/egrad_ch4h.f:74-79);
+ *  (2) a trajectory-batch engine that replaces the reference's per-step
+ *      `verlet` call (src/verlet.f90:65-1308) and its force loop, keeping
+ *      state resident on the GPU between calls.
+ */
+#ifndef CARACAL_AMD_H
+#define CARACAL_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+    CAM_OK = 0,
+    CAM_ERR_ARG = 1,          /* invalid argument (null pointer, size out of range) */
+    CAM_ERR_DEVICE = 2,       /* a HIP runtime call failed */
+    CAM_ERR_NOMEM = 3,        /* device allocation failed */
+    CAM_ERR_UNSUPPORTED = 4,  /* configuration the engine does not implement */
+    CAM_ERR_STATE = 5         /* call out of order (e.g. step before set_state) */
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char *cam_last_error(void);
+/* Library version string. */
+const char *cam_version(void);
+
+/* ---- (1) analytic surfaces: drop-in for egrad_X(q,Natoms,Nbeads,V,dVdq,info) ----
+ * q(3,natoms,nbeads) in bohr; V(nbeads) in hartree; dVdq(3,natoms,nbeads) in
+ * hartree/bohr.  Host pointers; the call is synchronous (it copies over PCIe).
+ * Replaces: egrad_h3 (src/egrad_h3.f:29-77), called per bead from gradient
+ * (src/gradient.f90:181).  natoms must be 3. */
+int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+/* Replaces: egrad_ch4h (src/egrad_ch4h.f:74-129), initialised by
+ * initialize_ch4h (src/read_pes.f90:551).  natoms must be 6, atom order of the
+ * reference surface: h4, C, h1, h2, h3, H_attacking. */
+int cam_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+
+/* Device-resident variants: q, v, dvdq are device pointers, stream is a
+ * hipStream_t (NULL = default stream); the call is asynchronous on that
+ * stream.  precision selects CAM_PREC_FP64 or CAM_PREC_MIXED (FP32
+ * arithmetic, FP64 inputs/outputs and FP64 accumulation). */
+int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
+                     int precision, void *stream);
+int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
+                       int precision, void *stream);
+
+/* ---- (2) trajectory-batch RPMD engine ---- */
+enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2 };
+enum { CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1 };
+/* Normal-mode transform.  REFERENCE reproduces the reference's rfft/irfft pair
+ * (src/rfft.f90:36-62, src/irfft.f90:36-63): both directions are the
+ * symmetric cosine matrix cos(2 pi k n / N)/sqrt(N), which projects the beads
+ * onto their bead-reversal-symmetric part every step.  ORTHOGONAL is the
+ * textbook orthogonal real-DFT normal-mode basis. */
+enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
+/* What happens around the force evaluation in one step:
+ *  NVE      -- dynamic.x without thermostat (verlet constrain = -1);
+ *  ANDERSEN -- dynamic.x with the Andersen thermostat (thermostat = 1):
+ *              all bead momenta redrawn every andersen_step steps;
+ *  CHILD    -- recrossing child trajectory (verlet constrain = 2): the
+ *              reaction coordinate of the centroid is evaluated each step and
+ *              the recrossing-factor numerator is accumulated. */
+enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
+
+#define CAM_MAX_ATOMS 16
+#define CAM_MAX_BONDS 4
+#define CAM_MAX_REAC 4
+
+/* Bimolecular reaction coordinate (calc_xi BIMOLEC, src/calc_xi.f90:63-523;
+ * keys of the reference's MECHA section, src/calc_rate_read.f90). 1-based atom
+ * indices as in the reference key file. */
+typedef struct {
+    int32_t n_form, n_break;
+    int32_t bond_form[CAM_MAX_BONDS][2];
+    int32_t bond_break[CAM_MAX_BONDS][2];
+    double form_ref[CAM_MAX_BONDS];      /* bond lengths at the transition state, bohr */
+    double break_ref[CAM_MAX_BONDS];
+    int32_t n_reac;                      /* number of reactant fragments (sum_reacs) */
+    int32_t reac_natoms[CAM_MAX_REAC];
+    int32_t reac_atoms[CAM_MAX_REAC][CAM_MAX_ATOMS];
+    double r_inf;                        /* asymptotic reactant distance (dist_inf), bohr */
+    double xi_ideal;                     /* dividing-surface value (recrossing uses mode 2) */
+} cam_reaction;
+
+typedef struct {
+    int32_t surface;          /* CAM_SURFACE_* */
+    int32_t natoms;           /* 3 (H3) or 6 (CH4H) */
+    int32_t nbeads;           /* 1, 2, 4, 8, 16, 32 or 64 */
+    int32_t ntraj;            /* trajectories in the batch */
+    int32_t precision;        /* CAM_PREC_* */
+    int32_t nm_mode;          /* CAM_NM_* */
+    int32_t ensemble;         /* CAM_ENS_* */
+    int32_t remove_com;       /* 1: remove centre-of-mass translation and rotation after every
+                                 step as the reference's transrot does (src/transrot.f90) */
+    int32_t andersen_step;    /* ANDERSEN: resample every andersen_step steps */
+    int32_t steps_per_launch; /* steps fused into one kernel launch; 0 = pick automatically */
+    double dt;                /* time step, atomic time units */
+    double mass[CAM_MAX_ATOMS];   /* electron masses */
+    uint64_t seed;            /* counter-based RNG key for Andersen draws */
+    cam_reaction reaction;    /* used by CAM_ENS_CHILD and cam_engine_constrain */
+} cam_config;
+
+typedef struct cam_engine cam_engine;
+
+int cam_engine_create(const cam_config *cfg, cam_engine **out);
+int cam_engine_destroy(cam_engine *eng);
+
+/* Upload state.  q, p: ntraj blocks of (3,natoms,nbeads), host memory;
+ * beta: ntraj inverse temperatures 1/(k_B T) in 1/hartree (may be NULL only
+ * for NVE).  Computes the forces of every bead (as mdinit does,
+ * src/mdinit.f90:79-83). */
+int cam_engine_set_state(cam_engine *eng, const double *q, const double *p, const double *beta);
+int cam_engine_get_state(cam_engine *eng, double *q, double *p);
+/* Gradient dV/dq of the current state, (3,natoms,nbeads) per trajectory, and
+ * per-bead potential energies (nbeads per trajectory). */
+int cam_engine_get_forces(cam_engine *eng, double *dvdq, double *vbead);
+
+/* Redraw all bead momenta from the Maxwell-Boltzmann distribution at the
+ * bead temperature (Andersen, src/andersen.f90:37-74).  'draw' selects the
+ * counter of the stream (two calls with the same draw give the same
+ * momenta).  If pair_antithetic != 0, trajectory 2m+1 receives minus the
+ * momenta of trajectory 2m (the reference spawns children in +-p pairs,
+ * src/recross_serial.f90:178-188). */
+int cam_engine_draw_momenta(cam_engine *eng, uint64_t draw, int pair_antithetic);
+
+/* Shift every trajectory's beads so its centroid lies on the dividing
+ * surface (constrain_q, src/constrain_q.f90:35-113), then project the
+ * momenta (constrain_p, src/constrain_p.f90:36-76) if project_p != 0.
+ * status (ntraj ints, may be NULL) receives 0 on success, 1 if the Newton
+ * iteration did not converge in 200 steps. */
+int cam_engine_constrain(cam_engine *eng, int project_p, int *status);
+
+/* Start a recrossing child batch: evaluates xi and its gradient at the
+ * centroid and the flux factor vs/fs of every trajectory
+ * (src/recross_serial.f90:189-233), adds vs/fs to the denominator where
+ * vs > 0 and clears the numerator.  The engine's nsteps-long numerator
+ * buffer is sized by the next cam_engine_run call. */
+int cam_engine_begin_children(cam_engine *eng);
+
+/* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
+ * engine's stream; all steps are captured in one HIP graph on first use and
+ * replayed afterwards.  Asynchronous: call cam_engine_sync to wait. */
+int cam_engine_run(cam_engine *eng, int nsteps);
+int cam_engine_sync(cam_engine *eng);
+
+/* Recrossing factor kappa(t) = num(t)/den accumulated since
+ * cam_engine_begin_children, for t = 1..nsteps of the last run (host array of
+ * nsteps doubles); den is returned separately.  Sums run over the batch in a
+ * fixed order, so the result does not depend on scheduling. */
+int cam_engine_kappa(cam_engine *eng, double *num, double *den, int nsteps);
+
+/* Per-trajectory observables of the current state (host arrays of ntraj):
+ * epot = sum over beads of V (as verlet returns it), ekin = sum p^2/2m over
+ * beads, espring = ring-polymer spring energy sum 1/2 m w_n^2 (q_k - q_k+1)^2
+ * with w_n = nbeads/beta, xi = reaction coordinate of the centroid (CHILD
+ * only, else 0), flags = bit 0: non-finite position seen. */
+int cam_engine_observables(cam_engine *eng, double *epot, double *ekin, double *espring,
+                           double *xi, int *flags);
+
+/* Device pointers of the resident state (q, p, dvdq) in the engine's internal
+ * layout [traj][3*natoms][nbeads] and the hipStream_t it runs on. */
+int cam_engine_device_state(cam_engine *eng, double **q, double **p, double **dvdq, void **stream);
+
+/* Mean duration in milliseconds of the step kernel launches of the last
+ * cam_engine_run, measured with HIP events on the engine's stream, and the
+ * number of steps one launch advances. */
+int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_per_launch, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

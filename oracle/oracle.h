@@ -15,11 +15,52 @@
 extern "C" {
 #endif
 
+#define ORACLE_MAX_ATOMS 16
+#define ORACLE_MAX_BEADS 256
+enum { ORACLE_SURF_H3 = 1, ORACLE_SURF_CH4H = 2 };
+
+typedef struct {
+    int surface, natoms, nbeads;
+    double dt;
+    double mass[ORACLE_MAX_ATOMS];
+} oracle_sys;
+
+/* BIMOLEC reaction coordinate definition, 1-based atom indices. */
+typedef struct {
+    int n_form, n_break;
+    int bond_form[4][2], bond_break[4][2];
+    double form_ref[4], break_ref[4];
+    int n_reac;
+    int reac_natoms[4];
+    int reac_atoms[4][ORACLE_MAX_ATOMS];
+    double r_inf, xi_ideal;
+} oracle_rx;
+
 /* analytic surfaces (src/egrad_h3.f, src/egrad_ch4h.f) */
 void oracle_h3_pote(const double r[3], double *pe, double dpe[3]);
 void oracle_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq);
 void oracle_ch4h_pot(const double *x, double *energy, double *g);
 void oracle_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+
+/* RPMD step and helpers (src/verlet.f90 and callees) */
+void oracle_rfft(double *x, int n);
+void oracle_forces(const oracle_sys *s, const double *q, double *v, double *g);
+void oracle_free_rp(const oracle_sys *s, double *q, double *p, double beta);
+void oracle_centroid(const oracle_sys *s, const double *q, double *c);
+void oracle_transrot(const oracle_sys *s, const double *q, double *p);
+double oracle_calc_xi(const oracle_sys *s, const oracle_rx *rx, const double *c, double *dxi, int mode);
+int oracle_constrain_q(const oracle_sys *s, const oracle_rx *rx, const double *centroid, const double *dxi_act,
+                       double *q, double *p);
+void oracle_constrain_p(const oracle_sys *s, const double *dxi_act, double *p);
+double oracle_child_weight(const oracle_sys *s, const oracle_rx *rx, const double *q, const double *p, double beta);
+void oracle_andersen(const oracle_sys *s, double *p, double beta, uint64_t seed, uint32_t traj, uint64_t draw);
+double oracle_verlet(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g, double *vbead,
+                     double beta, int ensemble, int remove_com, long istep, int andersen_step, uint64_t seed,
+                     uint32_t traj);
+
+/* counter-based normals */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double oracle_andersen_normal(uint64_t seed, uint32_t traj, uint64_t draw, int coord, int bead);
 
 #ifdef __cplusplus
 }

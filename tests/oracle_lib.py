@@ -73,3 +73,131 @@ def ref_egrad_h3(q):
 
 def ref_egrad_ch4h(q):
     return _ref_egrad(ref().egrad_ch4h_, q)
+
+
+# ---- RPMD step (oracle/rpmd.c) ----
+class OracleSys(ctypes.Structure):
+    _fields_ = [("surface", ctypes.c_int), ("natoms", ctypes.c_int), ("nbeads", ctypes.c_int),
+                ("dt", ctypes.c_double), ("mass", ctypes.c_double * 16)]
+
+
+class OracleRx(ctypes.Structure):
+    _fields_ = [("n_form", ctypes.c_int), ("n_break", ctypes.c_int),
+                ("bond_form", (ctypes.c_int * 2) * 4), ("bond_break", (ctypes.c_int * 2) * 4),
+                ("form_ref", ctypes.c_double * 4), ("break_ref", ctypes.c_double * 4),
+                ("n_reac", ctypes.c_int), ("reac_natoms", ctypes.c_int * 4),
+                ("reac_atoms", (ctypes.c_int * 16) * 4),
+                ("r_inf", ctypes.c_double), ("xi_ideal", ctypes.c_double)]
+
+
+def make_sys(surface, nbeads, dt, masses):
+    s = OracleSys()
+    s.surface = surface
+    s.natoms = len(masses)
+    s.nbeads = nbeads
+    s.dt = dt
+    for i, m in enumerate(masses):
+        s.mass[i] = m
+    return s
+
+
+def make_rx(reaction):
+    """From a caracal_amd.Reaction (same field meaning)."""
+    r = OracleRx()
+    r.n_form, r.n_break = len(reaction.bond_form), len(reaction.bond_break)
+    for i, (a, b) in enumerate(reaction.bond_form):
+        r.bond_form[i][0], r.bond_form[i][1] = a, b
+        r.form_ref[i] = reaction.form_ref[i]
+    for i, (a, b) in enumerate(reaction.bond_break):
+        r.bond_break[i][0], r.bond_break[i][1] = a, b
+        r.break_ref[i] = reaction.break_ref[i]
+    r.n_reac = len(reaction.reactants)
+    for k, atoms in enumerate(reaction.reactants):
+        r.reac_natoms[k] = len(atoms)
+        for i, a in enumerate(atoms):
+            r.reac_atoms[k][i] = a
+    r.r_inf, r.xi_ideal = reaction.r_inf, reaction.xi_ideal
+    return r
+
+
+def _setup_rpmd():
+    o = oracle()
+    o.oracle_verlet.restype = ctypes.c_double
+    o.oracle_verlet.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.c_int,
+                                ctypes.c_uint64, ctypes.c_uint32]
+    o.oracle_calc_xi.restype = ctypes.c_double
+    o.oracle_calc_xi.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, ctypes.c_int]
+    o.oracle_child_weight.restype = ctypes.c_double
+    o.oracle_child_weight.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, ctypes.c_double]
+    o.oracle_andersen_normal.restype = ctypes.c_double
+    o.oracle_andersen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    o.oracle_andersen.argtypes = [ctypes.POINTER(OracleSys), _dp, ctypes.c_double, ctypes.c_uint64,
+                                  ctypes.c_uint32, ctypes.c_uint64]
+    o.oracle_constrain_q.restype = ctypes.c_int
+    o.oracle_constrain_q.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp]
+    o.oracle_constrain_p.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
+    o.oracle_free_rp.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp, ctypes.c_double]
+    o.oracle_rfft.argtypes = [_dp, ctypes.c_int]
+    o.oracle_transrot.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
+    o.oracle_centroid.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
+    o.oracle_forces.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp, _dp]
+    o.oracle_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.POINTER(ctypes.c_uint32)]
+    return o
+
+
+_rpmd = None
+
+
+def rpmd():
+    global _rpmd
+    if _rpmd is None:
+        _rpmd = _setup_rpmd()
+    return _rpmd
+
+
+def forces(sys, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.zeros(sys.nbeads)
+    g = np.zeros_like(q)
+    rpmd().oracle_forces(ctypes.byref(sys), _ptr(q), _ptr(v), _ptr(g))
+    return v, g
+
+
+class OracleTraj:
+    """One trajectory stepped by the oracle: q, p (nbeads, natoms, 3)."""
+
+    def __init__(self, sys, q, p, beta, rx=None, ensemble=0, remove_com=False, andersen_step=0, seed=0, traj=0):
+        self.sys, self.rx, self.beta = sys, rx if rx is not None else OracleRx(), beta
+        self.q = np.array(q, dtype=np.float64, order="C")
+        self.p = np.array(p, dtype=np.float64, order="C")
+        self.v, self.g = forces(sys, self.q)
+        self.ensemble, self.remove_com = ensemble, remove_com
+        self.andersen_step, self.seed, self.traj = andersen_step, seed, traj
+        self.istep = 0
+
+    def step(self):
+        self.istep += 1
+        return rpmd().oracle_verlet(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.p),
+                                    _ptr(self.g), _ptr(self.v), self.beta, self.ensemble, int(self.remove_com),
+                                    self.istep, self.andersen_step, self.seed, self.traj)
+
+
+def calc_xi(sys, rx, c, mode=2):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    d = np.zeros_like(c)
+    xi = rpmd().oracle_calc_xi(ctypes.byref(sys), ctypes.byref(rx), _ptr(c), _ptr(d), mode)
+    return xi, d
+
+
+def andersen_normal(seed, traj, draw, coord, bead):
+    return rpmd().oracle_andersen_normal(seed, traj, draw, coord, bead)
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    rpmd().oracle_philox4x32_10(c, k, o)
+    return list(o)

@@ -1,0 +1,174 @@
+"""GPU parity of the trajectory-batch engine (C ABI) against the oracle's
+verlet, for the reference normal-mode transform, the child trajectories of
+the recrossing scheme, Andersen draws and the centroid constraint."""
+import numpy as np
+import pytest
+
+import caracal_amd as cam
+import oracle_lib as ol
+from caracal_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _h3_engine(nb, ntraj=1, **kw):
+    w = workloads.h3_nve(nbeads=nb)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_H3, nb, ntraj, w["dt"], w["masses"], **kw)
+    return e, w
+
+
+@pytest.mark.parametrize("nb", [16, 1, 4, 64])
+def test_h3_nve_trajectory_matches_reference_100_steps(nb):
+    """dynamic.x NVE with transrot, reference normal-mode transform: positions
+    within 1e-10 relative of the oracle after 100 steps."""
+    e, w = _h3_engine(nb, remove_com=True)
+    e.set_state(w["q"][None], w["p"][None], w["beta"])
+    s = ol.make_sys(1, nb, w["dt"], w["masses"])
+    t = ol.OracleTraj(s, w["q"], w["p"], w["beta"], ensemble=0, remove_com=True)
+    for block in range(4):
+        e.run(25)
+        for _ in range(25):
+            t.step()
+        q, p = e.get_state()
+        scale = np.abs(t.q).max()
+        err = np.abs(q[0] - t.q).max() / scale
+        print(f"nb={nb} step={25 * (block + 1)} max rel position error {err:.2e}")
+        assert err <= 1e-10
+        assert np.abs(p[0] - t.p).max() <= 1e-9 * np.abs(t.p).max()
+    e.close()
+
+
+def test_steps_per_launch_and_graph_replay_are_bitwise_invariant():
+    w = workloads.ch4h_children(ntraj=24, nbeads=8)
+    outs = []
+    for spl, runs in ((1, [12]), (5, [12]), (50, [6, 6])):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 8, 24, w["dt"], w["masses"], steps_per_launch=spl)
+        e.set_state(w["q"], np.random.default_rng(0).normal(size=w["q"].shape) * 20, w["beta"])
+        for n in runs:
+            e.run(n)
+        outs.append(e.get_state()[0])
+        e.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_orthogonal_mode_equals_reference_mode_on_symmetric_beads():
+    nb = 8
+    w = workloads.h3_nve(nbeads=nb)
+    rev = (-np.arange(nb)) % nb
+    q = 0.5 * (w["q"] + w["q"][rev])
+    p = 0.5 * (w["p"] + w["p"][rev])
+    res = []
+    for mode in (cam.CAM_NM_REFERENCE, cam.CAM_NM_ORTHOGONAL):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_H3, nb, 1, w["dt"], w["masses"], nm_mode=mode)
+        e.set_state(q[None], p[None], w["beta"])
+        e.run(50)
+        res.append(e.get_state()[0][0])
+        e.close()
+    assert np.abs(res[0] - res[1]).max() < 1e-10
+
+
+def test_orthogonal_mode_conserves_energy_for_general_beads():
+    nb = 16
+    e, w = _h3_engine(nb, nm_mode=cam.CAM_NM_ORTHOGONAL)
+    e.set_state(w["q"][None], w["p"][None], w["beta"])
+    def h():
+        o = e.observables()
+        return o["epot"][0] + o["ekin"][0] + o["espring"][0]
+    h0 = h()
+    dev = []
+    for _ in range(20):
+        e.run(20)
+        dev.append(h() - h0)
+    assert np.max(np.abs(dev)) < 1e-4
+    e.close()
+
+
+def test_andersen_draws_match_oracle():
+    w = workloads.ch4h_children(ntraj=6, nbeads=4)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 4, 6, w["dt"], w["masses"], seed=1234,
+                       ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"])
+    e.set_state(w["q"], w["p"], w["beta"])
+    e.draw_momenta(draw=77, antithetic_pairs=True)
+    _, p = e.get_state()
+    s = ol.make_sys(2, 4, w["dt"], w["masses"])
+    for t in range(6):
+        ref = np.zeros((4, 6, 3))
+        ol.rpmd().oracle_andersen(ol.ctypes.byref(s), ref.ctypes.data_as(ol._dp), float(w["beta"][t]), 1234,
+                                  t & ~1, 77)
+        sign = -1.0 if t & 1 else 1.0
+        np.testing.assert_allclose(p[t], sign * ref, rtol=1e-13, atol=1e-12)
+    e.close()
+
+
+def test_constraint_matches_oracle():
+    w = workloads.ch4h_children(ntraj=8, nbeads=8)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 8, 8, w["dt"], w["masses"], ensemble=cam.CAM_ENS_CHILD,
+                       reaction=w["reaction"])
+    p0 = np.random.default_rng(2).normal(size=w["q"].shape) * 20
+    e.set_state(w["q"], p0, w["beta"])
+    st = e.constrain(project_p=True)
+    assert not st.any()
+    q, p = e.get_state()
+    s = ol.make_sys(2, 8, w["dt"], w["masses"])
+    rx = ol.make_rx(w["reaction"])
+    for t in range(8):
+        qt, pt = w["q"][t].copy(), p0[t].copy()
+        c = qt.mean(axis=0)
+        _, dxi = ol.calc_xi(s, rx, c, 2)
+        assert ol.rpmd().oracle_constrain_q(ol.ctypes.byref(s), ol.ctypes.byref(rx), c.ctypes.data_as(ol._dp),
+                                            dxi.ctypes.data_as(ol._dp), qt.ctypes.data_as(ol._dp),
+                                            pt.ctypes.data_as(ol._dp)) == 0
+        ol.rpmd().oracle_constrain_p(ol.ctypes.byref(s), dxi.ctypes.data_as(ol._dp), pt.ctypes.data_as(ol._dp))
+        np.testing.assert_allclose(q[t], qt, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(p[t], pt, rtol=1e-10, atol=1e-10)
+        assert abs(ol.calc_xi(s, rx, q[t].mean(axis=0), 2)[0]) < 1e-8
+    e.close()
+
+
+def test_children_flags_and_kappa_match_oracle():
+    ntraj, nb, nsteps = 16, 8, 60
+    w = workloads.ch4h_children(ntraj=ntraj, nbeads=nb)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nb, ntraj, w["dt"], w["masses"], ensemble=cam.CAM_ENS_CHILD,
+                       reaction=w["reaction"], seed=99, steps_per_launch=25)
+    e.set_state(w["q"], w["p"], w["beta"])
+    assert not e.constrain(project_p=False).any()
+    e.draw_momenta(draw=1, antithetic_pairs=True)
+    q0, p0 = e.get_state()
+    e.begin_children()
+    e.run(nsteps)
+    num, den = e.kappa(nsteps)
+    q1, _ = e.get_state()
+    s = ol.make_sys(2, nb, w["dt"], w["masses"])
+    rx = ol.make_rx(w["reaction"])
+    num_ref = np.zeros(nsteps)
+    den_ref = 0.0
+    for t in range(ntraj):
+        wt = ol.rpmd().oracle_child_weight(ol.ctypes.byref(s), ol.ctypes.byref(rx),
+                                           np.ascontiguousarray(q0[t]).ctypes.data_as(ol._dp),
+                                           np.ascontiguousarray(p0[t]).ctypes.data_as(ol._dp), float(w["beta"][t]))
+        if wt > 0:
+            den_ref += wt
+        tr = ol.OracleTraj(s, q0[t], p0[t], float(w["beta"][t]), rx=rx, ensemble=2)
+        for l in range(nsteps):
+            xi = tr.step()
+            if xi > 0:
+                num_ref[l] += wt
+        np.testing.assert_allclose(q1[t], tr.q, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(den, den_ref, rtol=1e-12)
+    np.testing.assert_allclose(num, num_ref, rtol=1e-12, atol=1e-12)
+    e.close()
+
+
+def test_mixed_precision_engine_tracks_fp64():
+    w = workloads.ch4h_children(ntraj=64, nbeads=16)
+    p0 = np.random.default_rng(4).normal(size=w["q"].shape) * 15
+    res = []
+    for prec in (cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 16, 64, w["dt"], w["masses"], precision=prec)
+        e.set_state(w["q"], p0, w["beta"])
+        e.run(100)
+        res.append(e.get_state()[0])
+        e.close()
+    err = np.abs(res[0] - res[1]).max()
+    print(f"mixed vs fp64 after 100 steps: max |dq| = {err:.2e} bohr")
+    assert err < 1e-5
