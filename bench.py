@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Throughput of the RPMD hot path on one MI355X.
+
+Workload (the batched recrossing-children configuration): CH4 + H on the CH5
+surface, 4096 trajectories x 32 beads, centroid constrained to the dividing
+surface xi = 1, Maxwell-Boltzmann momenta at 500 K in +-p pairs, FP64, dt =
+0.1 fs, child steps with the reaction coordinate of the centroid and the
+recrossing-factor numerator every step, all steps of a run in one HIP graph.
+A step is one velocity-Verlet RPMD step of every trajectory.
+
+Printed: one JSON line.  value = bead-force-evals/s of the batched FP64 run
+(inputs resident in HBM).  Also reported: the same batch in mixed precision
+(FP32 force arithmetic, FP64 state and accumulation) with its force error,
+the plain FP64 one-trajectory-per-launch form, the roofline of the step
+kernel and a CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic work (oracle/flopcount.cpp, the reference's own arithmetic):
+# +,-,*,/ per bead-force-evaluation of the CH5 surface; elementary functions
+# are counted separately and not included in the flop figure.
+CH4H_FLOPS_PER_BEAD = 12707
+CH4H_TRANSCENDENTALS_PER_BEAD = 321
+FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
+
+
+def step_flops(natoms, nbeads):
+    """Algorithmic flops of one RPMD step of one trajectory: two half kicks
+    (2 flops per coordinate each), the normal-mode transform pair on p and q
+    (24 N n^2) plus the per-mode propagator (6 per coordinate), and the surface."""
+    nc = 3 * natoms * nbeads
+    return 4 * nc + 24 * natoms * nbeads ** 2 + 6 * nc + nbeads * CH4H_FLOPS_PER_BEAD
+
+
+def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0):
+    w = workloads.ch4h_children(ntraj=ntraj, nbeads=nbeads, seed=seed)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nbeads, ntraj, w["dt"], w["masses"], precision=precision,
+                       ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"], seed=seed + 17,
+                       steps_per_launch=steps_per_launch)
+    e.set_state(w["q"], w["p"], w["beta"])
+    st = e.constrain(project_p=False)
+    if st.any():
+        raise RuntimeError(f"{int(st.sum())} trajectories failed the centroid constraint")
+    e.draw_momenta(draw=1, antithetic_pairs=True)
+    e.begin_children()
+    return e, w
+
+
+def timed_run(e, steps, warmup):
+    e.prepare(warmup)
+    e.prepare(steps)
+    e.run(warmup)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.run(steps, sync=False)
+    e.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_launch, spl, nl = e.last_timing()
+    return wall, ms_launch, spl, nl
+
+
+def force_error_mixed(cam, workloads, ntraj, nbeads):
+    """max over beads of |F_mixed - F_fp64|_inf / |F_fp64|_inf on the workload's start state."""
+    g = {}
+    for prec in (cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED):
+        e, _ = prepare_children(cam, workloads, ntraj, nbeads, prec)
+        g[prec] = e.get_forces()[1]
+        e.close()
+    a, b = g[cam.CAM_PREC_FP64], g[cam.CAM_PREC_MIXED]
+    num = np.abs(a - b).reshape(a.shape[0] * a.shape[1], -1).max(axis=1)
+    den = np.abs(a).reshape(a.shape[0] * a.shape[1], -1).max(axis=1)
+    return float(np.max(num / den))
+
+
+def energy_drift_mixed(cam, workloads, nsteps=2000):
+    """NVE drift of the ring-polymer energy (hartree per ps per trajectory) in
+    mixed precision, orthogonal normal modes, 64 CH4H trajectories."""
+    w = workloads.ch4h_children(ntraj=64, nbeads=32, seed=5)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 32, 64, w["dt"], w["masses"], precision=cam.CAM_PREC_MIXED,
+                       nm_mode=cam.CAM_NM_ORTHOGONAL)
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=w["q"].shape) * np.sqrt(w["masses"][None, None, :, None] * 32 / w["beta"][:, None, None, None])
+    e.set_state(w["q"], p, w["beta"])
+    ts, hs = [], []
+    chunk = 100
+    for i in range(nsteps // chunk + 1):
+        o = e.observables()
+        ts.append(i * chunk * w["dt"])
+        hs.append(o["epot"] + o["ekin"] + o["espring"])
+        if i < nsteps // chunk:
+            e.run(chunk)
+    hs = np.array(hs)
+    t_ps = np.array(ts) * 2.4188843265857e-5
+    slopes = np.polyfit(t_ps, hs - hs[0], 1)[0]
+    e.close()
+    return float(np.max(np.abs(slopes)))
+
+
+def plain_fp64(cam, workloads, steps):
+    """The plain form: one trajectory per launch, one step per launch, FP64."""
+    e, _ = prepare_children(cam, workloads, 2, 32, cam.CAM_PREC_FP64, steps_per_launch=1)
+    e.prepare(1)
+    e.run(5)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.run(1, sync=False)
+    e.sync()
+    wall = time.perf_counter() - t0
+    e.close()
+    return 2 * 32 * steps / wall
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle's RPMD child step (a C port of the reference's verlet and
+    egrad_ch4h) on the host cores, one trajectory per process like the
+    reference's MPI task farm; a bounded sample of the same workload."""
+    import multiprocessing as mp
+    cores = min(16, os.cpu_count() or 1)
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(i, seconds) for i in range(cores)])
+    steps = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    return dict(value=steps * 32 / wall, unit="bead-force-evals/s", cores=cores, kind="port",
+                sample=f"{cores} processes x {int(steps / cores)} child steps of one 32-bead CH4H trajectory "
+                       f"(oracle C port of verlet + egrad_ch4h, gcc -O2), {wall:.1f} s")
+
+
+def _cpu_worker(args):
+    idx, seconds = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from caracal_amd import workloads
+    w = workloads.ch4h_children(ntraj=cpu_count_for_sample(), nbeads=32, seed=idx)
+    s = ol.make_sys(2, 32, w["dt"], w["masses"])
+    rx = ol.make_rx(w["reaction"])
+    rng = np.random.default_rng(idx)
+    p = rng.normal(size=w["q"][0].shape) * 20
+    t = ol.OracleTraj(s, w["q"][0], p, float(w["beta"][0]), rx=rx, ensemble=2)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            t.step()
+        n += 10
+    return n, time.perf_counter() - t0
+
+
+def cpu_count_for_sample():
+    return 1
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--ntraj", type=int, default=4096)
+    ap.add_argument("--nbeads", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "
+              "without communication, see the sharding section of the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import caracal_amd as cam
+    from caracal_amd import workloads
+    torch.cuda.set_device(0)
+
+    ntraj, nb = args.ntraj, args.nbeads
+    e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
+    wall, ms_launch, spl, nl = timed_run(e, args.steps, args.warmup)
+    num, den = e.kappa(args.steps)
+    e.close()
+    bead_evals = ntraj * nb * args.steps
+    value = bead_evals / wall
+    steps_per_s = ntraj * args.steps / wall
+    flops_launch = step_flops(6, nb) * ntraj * spl
+    achieved = flops_launch / (ms_launch * 1e-3) / 1e12
+
+    out = {
+        "metric": "bead-force-evals/s (batched RPMD children, CH4H, 32 beads x 4096 trajectories)",
+        "value": value,
+        "unit": "bead-force-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded saddle perturbations, Maxwell-Boltzmann momenta)",
+        "config": {"workload": "ch4h_children", "surface": "CH5 (egrad_ch4h)", "ntraj": ntraj, "nbeads": nb,
+                   "natoms": 6, "dt_fs": 0.1, "kelvin": 500, "precision": "fp64",
+                   "nm_mode": "reference cosine transform", "steps_per_launch": spl},
+        "rpmd_steps_per_s": steps_per_s,
+        "kappa_final": float(num[-1] / den) if den > 0 else None,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "step_kernel<SurfCH4H,32,double,CHILD>", "ms_per_launch": ms_launch,
+                     "flops_per_launch": flops_launch,
+                     "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
+    }
+    if not args.skip_extras:
+        e, _ = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_MIXED)
+        wall_m, ms_m, _, _ = timed_run(e, args.steps, args.warmup)
+        e.close()
+        mixed_value = bead_evals / wall_m
+        plain = plain_fp64(cam, workloads, 200)
+        out["mixed"] = {"value": mixed_value, "dtype": "fp32 forces / fp64 state",
+                        "max_rel_force_err": force_error_mixed(cam, workloads, 256, nb),
+                        "nve_energy_drift_eh_per_ps": energy_drift_mixed(cam, workloads)}
+        out["plain_fp64_one_traj_per_launch"] = {"value": plain}
+        out["speedup_mixed_vs_plain"] = mixed_value / plain
+        out["speedup_fp64_batched_vs_plain"] = value / plain
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -62,6 +62,7 @@ PROTOTYPES = {
     "cam_engine_constrain": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "cam_engine_begin_children": (ctypes.c_int, [_vp]),
     "cam_engine_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_engine_prepare": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_engine_sync": (ctypes.c_int, [_vp]),
     "cam_engine_kappa": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int]),
     "cam_engine_observables": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _ip]),

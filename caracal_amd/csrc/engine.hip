@@ -161,8 +161,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr int BPB = kBlock / L;          // beads per block
     constexpr int TPB = BPB / NB;            // trajectories per block
     static_assert(TPB >= 1, "a trajectory must fit in one block");
-    __shared__ double X[TPB * NC * NB];
-    __shared__ double cent[TPB * NC];
+    __shared__ __align__(16) double X[TPB * NC * NB];
+    __shared__ __align__(16) double cent[TPB * NC];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -1141,6 +1141,43 @@ int cam_engine_begin_children(cam_engine *e)
     return CAM_OK;
 }
 
+static int get_graph(cam_engine *e, int nsteps, hipGraphExec_t *out)
+{
+    if (e->cfg.ensemble == CAM_ENS_CHILD) {
+        int rc = ensure_num(e, nsteps);
+        if (rc != CAM_OK) return rc;
+    }
+    auto it = e->graphs.find(nsteps);
+    if (it == e->graphs.end()) {
+        CAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_run(e, nsteps);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        if (rc != CAM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        CAM_HIP(ce);
+        hipGraphExec_t exec;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CAM_HIP(ie);
+        it = e->graphs.emplace(nsteps, exec).first;
+    }
+    *out = it->second;
+    return CAM_OK;
+}
+
+int cam_engine_prepare(cam_engine *e, int nsteps)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(nsteps >= 1, "nsteps must be >= 1");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    clear_error();
+    hipGraphExec_t g;
+    return get_graph(e, nsteps, &g);
+}
+
 int cam_engine_run(cam_engine *e, int nsteps)
 {
     CAM_CHECK_ARG(e, "null engine");
@@ -1148,31 +1185,11 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_CHECK_ARG(e->have_state, "no state uploaded");
     clear_error();
     if (nsteps == 0) return CAM_OK;
-    if (e->cfg.ensemble == CAM_ENS_CHILD) {
-        int rc = ensure_num(e, nsteps);
-        if (rc != CAM_OK) return rc;
-    }
-    auto it = e->graphs.find(nsteps);
-    if (it == e->graphs.end()) {
-        hipGraph_t graph;
-        CAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_run(e, nsteps);
-        hipGraph_t captured = nullptr;
-        hipError_t ce = hipStreamEndCapture(e->stream, &captured);
-        if (rc != CAM_OK) {
-            if (captured) (void)hipGraphDestroy(captured);
-            return rc;
-        }
-        CAM_HIP(ce);
-        graph = captured;
-        hipGraphExec_t exec;
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        CAM_HIP(ie);
-        it = e->graphs.emplace(nsteps, exec).first;
-    }
+    hipGraphExec_t g;
+    int rc = get_graph(e, nsteps, &g);
+    if (rc != CAM_OK) return rc;
     CAM_HIP(hipEventRecord(e->ev0, e->stream));
-    CAM_HIP(hipGraphLaunch(it->second, e->stream));
+    CAM_HIP(hipGraphLaunch(g, e->stream));
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
     e->last_launches = (nsteps + e->spl - 1) / e->spl;
     e->last_ms = -1.0;

@@ -128,6 +128,10 @@ class RPMDEngine:
         if sync:
             self.sync()
 
+    def prepare(self, nsteps):
+        """Capture the HIP graph of an nsteps run without running it."""
+        check("cam_engine_prepare", lib().cam_engine_prepare(self._h, nsteps))
+
     def sync(self):
         check("cam_engine_sync", lib().cam_engine_sync(self._h))
 

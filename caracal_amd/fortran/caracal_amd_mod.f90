@@ -1,0 +1,191 @@
+!
+!   caracal_amd_mod: iso_c_binding interface of the MI355X RPMD engine
+!   (include/caracal_amd.h) for a Fortran host such as Caracal's dynamic.x
+!   and calc_rate.x.  Every function returns the integer status of the C
+!   entry point (0 = success); cam_error_message returns the text of the
+!   last failure.
+!
+!   Arrays keep the reference layout: q(3,natoms,nbeads) per trajectory, a
+!   batch is q(3,natoms,nbeads,ntraj).  Units: bohr, hartree, electron
+!   masses, atomic time units.
+!
+module caracal_amd_mod
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+
+   integer(c_int), parameter, public :: CAM_OK = 0
+   integer(c_int32_t), parameter, public :: CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2
+   integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
+   integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
+   integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2
+   integer, parameter, public :: CAM_MAX_ATOMS = 16, CAM_MAX_BONDS = 4, CAM_MAX_REAC = 4
+
+   ! cam_reaction: the BIMOLEC reaction coordinate (MECHA section keys)
+   type, bind(C), public :: cam_reaction
+      integer(c_int32_t) :: n_form = 0, n_break = 0
+      integer(c_int32_t) :: bond_form(2, CAM_MAX_BONDS) = 0
+      integer(c_int32_t) :: bond_break(2, CAM_MAX_BONDS) = 0
+      real(c_double) :: form_ref(CAM_MAX_BONDS) = 0.d0
+      real(c_double) :: break_ref(CAM_MAX_BONDS) = 0.d0
+      integer(c_int32_t) :: n_reac = 0
+      integer(c_int32_t) :: reac_natoms(CAM_MAX_REAC) = 0
+      integer(c_int32_t) :: reac_atoms(CAM_MAX_ATOMS, CAM_MAX_REAC) = 0
+      real(c_double) :: r_inf = 0.d0, xi_ideal = 0.d0
+   end type cam_reaction
+
+   type, bind(C), public :: cam_config
+      integer(c_int32_t) :: surface = CAM_SURFACE_H3, natoms = 3, nbeads = 1, ntraj = 1
+      integer(c_int32_t) :: precision = CAM_PREC_FP64, nm_mode = CAM_NM_REFERENCE
+      integer(c_int32_t) :: ensemble = CAM_ENS_NVE, remove_com = 0, andersen_step = 0
+      integer(c_int32_t) :: steps_per_launch = 0
+      real(c_double) :: dt = 0.d0
+      real(c_double) :: mass(CAM_MAX_ATOMS) = 0.d0
+      integer(c_int64_t) :: seed = 0
+      type(cam_reaction) :: reaction
+   end type cam_config
+
+   interface
+      function cam_last_error() bind(C, name="cam_last_error") result(msg)
+         import :: c_ptr
+         type(c_ptr) :: msg
+      end function
+      ! drop-in for egrad_h3(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_h3.f:29)
+      function cam_egrad_h3(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_h3") result(rc)
+         import :: c_int, c_double
+         real(c_double), intent(in) :: q(*)
+         integer(c_int), value :: natoms, nbeads
+         real(c_double), intent(out) :: v(*), dvdq(*)
+         integer(c_int) :: rc
+      end function
+      ! drop-in for egrad_ch4h(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_ch4h.f:74)
+      function cam_egrad_ch4h(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_ch4h") result(rc)
+         import :: c_int, c_double
+         real(c_double), intent(in) :: q(*)
+         integer(c_int), value :: natoms, nbeads
+         real(c_double), intent(out) :: v(*), dvdq(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_create(cfg, eng) bind(C, name="cam_engine_create") result(rc)
+         import :: c_int, c_ptr, cam_config
+         type(cam_config), intent(in) :: cfg
+         type(c_ptr), intent(out) :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_engine_destroy(eng) bind(C, name="cam_engine_destroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_engine_set_state(eng, q, p, beta) bind(C, name="cam_engine_set_state") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(in) :: q(*), p(*), beta(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_get_state(eng, q, p) bind(C, name="cam_engine_get_state") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: q(*), p(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_get_forces(eng, dvdq, vbead) bind(C, name="cam_engine_get_forces") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: dvdq(*), vbead(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_draw_momenta(eng, draw, pair) bind(C, name="cam_engine_draw_momenta") result(rc)
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: eng
+         integer(c_int64_t), value :: draw
+         integer(c_int), value :: pair
+         integer(c_int) :: rc
+      end function
+      function cam_engine_constrain(eng, project_p, status) bind(C, name="cam_engine_constrain") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int), value :: project_p
+         integer(c_int), intent(out) :: status(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_begin_children(eng) bind(C, name="cam_engine_begin_children") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_engine_run(eng, nsteps) bind(C, name="cam_engine_run") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int), value :: nsteps
+         integer(c_int) :: rc
+      end function
+      function cam_engine_sync(eng) bind(C, name="cam_engine_sync") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_engine_kappa(eng, num, den, nsteps) bind(C, name="cam_engine_kappa") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: num(*), den
+         integer(c_int), value :: nsteps
+         integer(c_int) :: rc
+      end function
+      function cam_engine_observables(eng, epot, ekin, espring, xi, flags) &
+            bind(C, name="cam_engine_observables") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: epot(*), ekin(*), espring(*), xi(*)
+         integer(c_int), intent(out) :: flags(*)
+         integer(c_int) :: rc
+      end function
+   end interface
+
+   public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
+   public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
+   public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
+   public :: cam_engine_begin_children, cam_engine_run, cam_engine_sync, cam_engine_kappa
+   public :: cam_engine_observables
+   public :: egrad_h3_gpu, egrad_ch4h_gpu, cam_error_message
+
+contains
+
+   ! Same argument list as the reference's egrad_h3 / egrad_ch4h, so a host
+   ! can swap the call in gradient.f90 (src/gradient.f90:181) unchanged.
+   subroutine egrad_h3_gpu(q, natoms, nbeads, v, dvdq, info)
+      integer, intent(in) :: natoms, nbeads
+      real(c_double), intent(in) :: q(3, natoms, nbeads)
+      real(c_double), intent(out) :: v(nbeads), dvdq(3, natoms, nbeads)
+      integer, intent(out) :: info
+      info = cam_egrad_h3(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
+   end subroutine egrad_h3_gpu
+
+   subroutine egrad_ch4h_gpu(q, natoms, nbeads, v, dvdq, info)
+      integer, intent(in) :: natoms, nbeads
+      real(c_double), intent(in) :: q(3, natoms, nbeads)
+      real(c_double), intent(out) :: v(nbeads), dvdq(3, natoms, nbeads)
+      integer, intent(out) :: info
+      info = cam_egrad_ch4h(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
+   end subroutine egrad_ch4h_gpu
+
+   function cam_error_message() result(msg)
+      character(len=:), allocatable :: msg
+      character(kind=c_char), pointer :: chars(:)
+      type(c_ptr) :: p
+      integer :: n
+      interface
+         function c_strlen(s) bind(C, name="strlen") result(l)
+            import :: c_ptr, c_size_t
+            type(c_ptr), value :: s
+            integer(c_size_t) :: l
+         end function
+      end interface
+      p = cam_last_error()
+      n = int(c_strlen(p))
+      call c_f_pointer(p, chars, [n])
+      allocate(character(len=n) :: msg)
+      msg = transfer(chars(1:n), msg)
+   end function cam_error_message
+
+end module caracal_amd_mod

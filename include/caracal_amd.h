@@ -148,15 +148,19 @@ int cam_engine_constrain(cam_engine *eng, int project_p, int *status);
 
 /* Start a recrossing child batch: evaluates xi and its gradient at the
  * centroid and the flux factor vs/fs of every trajectory
- * (src/recross_serial.f90:189-233), adds vs/fs to the denominator where
- * vs > 0 and clears the numerator.  The engine's nsteps-long numerator
- * buffer is sized by the next cam_engine_run call. */
+ * (src/recross_serial.f90:189-233) and adds vs/fs to the denominator where
+ * vs > 0.  The following cam_engine_run accumulates the numerator num(t)
+ * step by step; like the reference's num_total/denom_total, both keep
+ * accumulating over successive child batches of the same engine. */
 int cam_engine_begin_children(cam_engine *eng);
 
 /* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
  * engine's stream; all steps are captured in one HIP graph on first use and
  * replayed afterwards.  Asynchronous: call cam_engine_sync to wait. */
 int cam_engine_run(cam_engine *eng, int nsteps);
+/* Capture and instantiate the graph of an nsteps run without running it, so
+ * a later cam_engine_run(nsteps) only replays it. */
+int cam_engine_prepare(cam_engine *eng, int nsteps);
 int cam_engine_sync(cam_engine *eng);
 
 /* Recrossing factor kappa(t) = num(t)/den accumulated since
