@@ -36,6 +36,20 @@ typedef struct {
     double r_inf, xi_ideal;
 } oracle_rx;
 
+/* QMDFF-form periodic force field (oracle/ff_box.c) */
+#define ORACLE_FF_NPAR 10
+typedef struct {
+    int natoms, nbonds, nangles;
+    const int *type;          /* 0 = O, 1 = H */
+    const int *mol;           /* molecule index of each atom */
+    const int *bonds;         /* 2 per bond, 0-based */
+    const int *angles;        /* 3 per angle (i, vertex, k), 0-based */
+    double box, rcut, zahn_a, zahn_par;
+    double sr42[2][2], r0bj[2][2], zrep[2][2], arep[2][2];
+} oracle_ff;
+double oracle_ff_eval(const oracle_ff *f, const double *theta, const double *x, double *g, double *dth,
+                      const double *v, double *dvth);
+
 /* analytic surfaces (src/egrad_h3.f, src/egrad_ch4h.f) */
 void oracle_h3_pote(const double r[3], double *pe, double dpe[3]);
 void oracle_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq);

@@ -201,3 +201,47 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     rpmd().oracle_philox4x32_10(c, k, o)
     return list(o)
+
+
+# ---- QMDFF-form box (oracle/ff_box.c) ----
+class OracleFF(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int), ("nbonds", ctypes.c_int), ("nangles", ctypes.c_int),
+                ("type", _ip), ("mol", _ip), ("bonds", _ip), ("angles", _ip),
+                ("box", ctypes.c_double), ("rcut", ctypes.c_double), ("zahn_a", ctypes.c_double),
+                ("zahn_par", ctypes.c_double),
+                ("sr42", (ctypes.c_double * 2) * 2), ("r0bj", (ctypes.c_double * 2) * 2),
+                ("zrep", (ctypes.c_double * 2) * 2), ("arep", (ctypes.c_double * 2) * 2)]
+
+
+class FFOracle:
+    def __init__(self, system, rc_angstrom=10.0):
+        from caracal_amd import ffbox
+        self.keep = [np.ascontiguousarray(system[k], dtype=np.int32) for k in ("type", "mol", "bonds", "angles")]
+        f = OracleFF()
+        f.natoms = len(system["type"])
+        f.nbonds = len(system["bonds"])
+        f.nangles = len(system["angles"])
+        f.type, f.mol, f.bonds, f.angles = (a.ctypes.data_as(_ip) for a in self.keep)
+        f.box = system["box"]
+        za, rc, zp = ffbox.zahn_params(rc_angstrom)
+        f.rcut, f.zahn_a, f.zahn_par = rc, za, zp
+        sr42, r0, zrep, arep = ffbox.pair_tables()
+        for a in range(2):
+            for b in range(2):
+                f.sr42[a][b], f.r0bj[a][b], f.zrep[a][b], f.arep[a][b] = sr42[a, b], r0[a, b], zrep[a, b], arep[a, b]
+        self.f = f
+        o = oracle()
+        o.oracle_ff_eval.restype = ctypes.c_double
+        o.oracle_ff_eval.argtypes = [ctypes.POINTER(OracleFF), _dp, _dp, _dp, _dp, _dp, _dp]
+
+    def eval(self, theta, x, v=None, want_grad=True):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x) if want_grad else None
+        dth = np.zeros(10)
+        dvth = np.zeros(10) if v is not None else None
+        vv = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+        e = oracle().oracle_ff_eval(ctypes.byref(self.f), _ptr(theta), _ptr(x),
+                                    _ptr(g) if g is not None else None, _ptr(dth),
+                                    _ptr(vv) if vv is not None else None, _ptr(dvth) if dvth is not None else None)
+        return e, g, dth, dvth
