@@ -41,6 +41,18 @@ class Config(ctypes.Structure):
     ]
 
 
+class FFDesc(ctypes.Structure):
+    _fields_ = [
+        ("natoms", ctypes.c_int32), ("nbonds", ctypes.c_int32), ("nangles", ctypes.c_int32),
+        ("type", ctypes.POINTER(ctypes.c_int32)), ("mol", ctypes.POINTER(ctypes.c_int32)),
+        ("bonds", ctypes.POINTER(ctypes.c_int32)), ("angles", ctypes.POINTER(ctypes.c_int32)),
+        ("box", ctypes.c_double), ("rcut", ctypes.c_double), ("zahn_a", ctypes.c_double),
+        ("zahn_par", ctypes.c_double),
+        ("sr42", (ctypes.c_double * 2) * 2), ("r0bj", (ctypes.c_double * 2) * 2),
+        ("zrep", (ctypes.c_double * 2) * 2), ("arep", (ctypes.c_double * 2) * 2),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
@@ -69,6 +81,22 @@ PROTOTYPES = {
     "cam_engine_device_state": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cam_engine_last_timing": (ctypes.c_int, [_vp, _dp, _ip, _ip]),
+    "cam_ff_create": (ctypes.c_int, [ctypes.POINTER(FFDesc), ctypes.POINTER(_vp)]),
+    "cam_ff_destroy": (ctypes.c_int, [_vp]),
+    "cam_ff_energy": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "cam_ff_forces": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "cam_box_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "cam_box_destroy": (ctypes.c_int, [_vp]),
+    "cam_box_set_params": (ctypes.c_int, [_vp, _dp]),
+    "cam_box_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_box_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_box_prepare": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_box_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_box_sync": (ctypes.c_int, [_vp, _dp]),
+    "cam_box_energies": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "cam_box_device_state": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
 }
 
 _lib = None

@@ -761,7 +761,10 @@ bool pick_nb(int nb, int ens, int wpe, step_fn &st, force_fn &fo)
     }
 }
 
-void build_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double> &inv)
+}  // namespace
+
+namespace cam {
+void nm_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double> &inv)
 {
     // element (j,k) of the stored arrays is T[k][j]
     fwd.assign((size_t)nb * nb, 0.0);
@@ -799,6 +802,10 @@ void build_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<doub
             }
     }
 }
+
+}  // namespace cam
+
+namespace {
 
 TrajArgs traj_args(cam_engine *e)
 {
@@ -982,7 +989,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_HIP(hipMalloc(&e->tfwd, (size_t)nb * nb * sizeof(double)));
     CAM_HIP(hipMalloc(&e->tinv, (size_t)nb * nb * sizeof(double)));
     std::vector<double> fwd, inv;
-    build_matrices(nb, cfg->nm_mode, fwd, inv);
+    nm_matrices(nb, cfg->nm_mode, fwd, inv);
     CAM_HIP(hipMemcpy(e->tfwd, fwd.data(), fwd.size() * sizeof(double), hipMemcpyHostToDevice));
     CAM_HIP(hipMemcpy(e->tinv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
     CAM_HIP(hipMemset(e->step_base, 0, sizeof(uint64_t)));

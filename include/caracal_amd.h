@@ -186,6 +186,62 @@ int cam_engine_device_state(cam_engine *eng, double **q, double **p, double **dv
  * number of steps one launch advances. */
 int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_per_launch, int *launches);
 
+/* ---- (3) QMDFF-form periodic box: energy kernel, hand-written adjoint and
+ * a ring-polymer engine ----
+ * Terms follow the reference's QMDFF force field: bond stretch and bend of
+ * ff_eg (src/ff_eg.f90:87-299), BJ-damped dispersion, Pauli repulsion and the
+ * Zahn-damped Coulomb of ff_nonb (src/ff_nonb.f90:198-394) between atoms of
+ * different molecules, minimum image as box_image (src/box_image.f90:36-64).
+ * Forms: Morse De(1-exp(-a(r-r0)))^2 bonds, harmonic k(theta-theta0)^2
+ * angles.  Two atom types (0, 1).  Parameter vector theta (10 doubles):
+ * De, a, r0, k, theta0, q_0, q_1, C6_00, C6_01, C6_11. */
+typedef struct {
+    int32_t natoms, nbonds, nangles;
+    const int32_t *type;      /* natoms, 0 or 1 */
+    const int32_t *mol;       /* natoms, molecule index (nonbonded terms skip equal indices) */
+    const int32_t *bonds;     /* 2*nbonds, 0-based */
+    const int32_t *angles;    /* 3*nangles (i, vertex, k), 0-based */
+    double box;               /* cubic box edge, bohr */
+    double rcut;              /* nonbonded cutoff, bohr (<= box/2) */
+    double zahn_a, zahn_par;  /* Zahn damping (src/set_periodic.f90:93-97) */
+    double sr42[2][2], r0bj[2][2];   /* BJ dispersion: C8/C6 factor and damping radius */
+    double zrep[2][2], arep[2][2];   /* repulsion prefactor and exponent */
+} cam_ff_desc;
+
+typedef struct cam_ff cam_ff;
+int cam_ff_create(const cam_ff_desc *desc, cam_ff **out);
+int cam_ff_destroy(cam_ff *ff);
+/* Energy of nbeads replicas: x device [nbeads][natoms][3] bohr, e device
+ * [nbeads] hartree.  theta is a host array of 10.  Asynchronous on stream. */
+int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, int precision,
+                  void *stream);
+/* Adjoint of the energy kernel: dE/dx (device, same layout as x); optionally
+ * e [nbeads], dE/dtheta [nbeads][10] and, for a direction v (device, layout
+ * of x), d(v . dE/dx)/dtheta [nbeads][10] -- the parameter gradient of a
+ * force-matching loss.  No counterpart in the reference, which fits QMDFF
+ * parameters without derivatives (src/lm_qmdff.f90:36-84). */
+int cam_ff_forces(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, double *dedx,
+                  double *dedtheta, const double *v, double *dvdtheta, int precision, void *stream);
+
+/* Ring-polymer dynamics of one box (replaces verlet's per-bead force loop
+ * over gradient -> ff_eg/ff_nonb, src/verlet.f90:772-777, for a QMDFF box).
+ * andersen_step 0 = NVE.  State arrays are host (3,natoms,nbeads). */
+typedef struct cam_box cam_box;
+int cam_box_create(cam_ff *ff, int nbeads, double dt, const double *mass, double beta, int nm_mode, int precision,
+                   int andersen_step, uint64_t seed, cam_box **out);
+int cam_box_destroy(cam_box *box);
+int cam_box_set_params(cam_box *box, const double *theta);
+int cam_box_set_state(cam_box *box, const double *q, const double *p);
+int cam_box_get_state(cam_box *box, double *q, double *p);
+int cam_box_prepare(cam_box *box, int nsteps);
+int cam_box_run(cam_box *box, int nsteps);
+/* Waits for the last run; ms (may be NULL) receives its duration measured
+ * with HIP events on the box's stream. */
+int cam_box_sync(cam_box *box, double *ms);
+/* Per-bead potential energies (nbeads), total kinetic and spring energy. */
+int cam_box_energies(cam_box *box, double *epot_beads, double *ekin, double *espring);
+int cam_box_device_state(cam_box *box, double **q, double **p, double **dvdq, void **stream);
+
 #ifdef __cplusplus
 }
 #endif

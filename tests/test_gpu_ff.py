@@ -1,0 +1,137 @@
+"""GPU parity of the QMDFF-form box: energy kernel, its hand-written adjoint
+(forces, parameter gradients, force-matching contraction) against the oracle
+and against central differences of the GPU energy kernel (h = 1e-5), the
+torch.autograd wrapper, and the box ring-polymer engine."""
+import numpy as np
+import pytest
+
+import caracal_amd as cam
+import oracle_lib as ol
+from caracal_amd import ff as cff
+from caracal_amd import ffbox
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(nmol, box_a, seed, rc_a, jitter=0.05):
+    s = ffbox.box_system(nmol=nmol, box_angstrom=box_a, seed=seed)
+    s["x"] = s["x"] + np.random.default_rng(seed).normal(0, jitter, s["x"].shape)
+    return s, cff.ForceField(s, rc_angstrom=rc_a), ol.FFOracle(s, rc_angstrom=rc_a)
+
+
+def _beads(s, nb, seed, sigma=0.05):
+    rng = np.random.default_rng(seed)
+    return s["x"][None] + rng.normal(0, sigma, (nb,) + s["x"].shape)
+
+
+def test_energy_and_forces_match_oracle_full_size_box():
+    import torch
+    s, ff, fo = _box(1000, 31.0, 2, 10.0)
+    th = ffbox.theta_ref()
+    xb = _beads(s, 2, 7)
+    x = torch.tensor(xb, device="cuda")
+    e, g, _, _ = ff.forces(th, x)
+    e = e.cpu().numpy()
+    g = g.cpu().numpy()
+    for b in range(2):
+        er, gr, _, _ = fo.eval(th, xb[b])
+        assert abs(e[b] - er) < 1e-10 * abs(er)
+        np.testing.assert_allclose(g[b], gr, rtol=0, atol=1e-10 * np.abs(gr).max())
+    em = ff.energy(th, x, precision=cam.CAM_PREC_MIXED).cpu().numpy()
+    _, gm, _, _ = ff.forces(th, x, precision=cam.CAM_PREC_MIXED)
+    gm = gm.cpu().numpy()
+    # mixed: FP32 pair arithmetic, compensated accumulation
+    fe = np.max(np.abs(gm - g).reshape(-1, 3).max(1)) / np.abs(g).max()
+    print(f"box mixed-precision max force error relative to max |F|: {fe:.2e}")
+    assert fe < 1e-5
+    assert np.max(np.abs(em - e) / np.abs(e)) < 1e-5
+
+
+def test_adjoint_matches_central_differences_of_energy_kernel():
+    import torch
+    s, ff, _ = _box(64, 12.0, 3, 5.9)
+    th = ffbox.theta_ref()
+    x = torch.tensor(_beads(s, 1, 1), device="cuda")
+    _, g, dth, _ = ff.forces(th, x, want_params=True)
+    g, dth = g.cpu().numpy()[0], dth.cpu().numpy()[0]
+    h = 1e-5
+    rng = np.random.default_rng(2)
+    for idx in rng.choice(x.numel(), 30, replace=False):
+        xp, xm = x.clone().view(-1), x.clone().view(-1)
+        xp[idx] += h
+        xm[idx] -= h
+        fd = (ff.energy(th, xp.view_as(x)).item() - ff.energy(th, xm.view_as(x)).item()) / (2 * h)
+        assert abs(fd - g.ravel()[idx]) < 2e-8 + 2e-6 * abs(fd)
+    for p in range(10):
+        hp = h * max(1.0, abs(th[p]))
+        tp, tm = th.copy(), th.copy()
+        tp[p] += hp
+        tm[p] -= hp
+        fd = (ff.energy(tp, x).item() - ff.energy(tm, x).item()) / (2 * hp)
+        assert abs(fd - dth[p]) < 1e-6 * max(1.0, abs(fd)), (p, fd, dth[p])
+
+
+def test_parameter_gradients_match_oracle():
+    import torch
+    s, ff, fo = _box(64, 12.0, 5, 5.9)
+    th = ffbox.theta_ref() * (1 + 0.05 * np.random.default_rng(0).normal(size=10))
+    xb = _beads(s, 3, 4)
+    v = np.random.default_rng(9).normal(size=xb.shape)
+    x, vt = torch.tensor(xb, device="cuda"), torch.tensor(v, device="cuda")
+    e, g, dth, dvth = ff.forces(th, x, v=vt, want_params=True)
+    for b in range(3):
+        er, gr, dr, dvr = fo.eval(th, xb[b], v=v[b])
+        np.testing.assert_allclose(dth.cpu().numpy()[b], dr, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(dvth.cpu().numpy()[b], dvr, rtol=1e-10, atol=1e-12)
+
+
+def test_autograd_force_matching_gradient():
+    import torch
+    s, ff, _ = _box(64, 12.0, 6, 5.9)
+    fn = cff.make_autograd(ff)
+    x = torch.tensor(_beads(s, 2, 3), device="cuda")
+    th_ref = torch.tensor(ffbox.theta_ref(), device="cuda")
+    with torch.no_grad():
+        _, f_ref = fn(x, th_ref)
+    th = (th_ref * 1.05).clone().requires_grad_(True)
+    e, f = fn(x, th)
+    loss = ((f - f_ref) ** 2).sum() + 0.1 * e.sum()
+    loss.backward()
+    grad = th.grad.cpu().numpy()
+    for p in range(10):
+        h = 1e-6 * max(1.0, abs(th[p].item()))
+        tp, tm = th.detach().clone(), th.detach().clone()
+        tp[p] += h
+        tm[p] -= h
+        with torch.no_grad():
+            ep, fp = fn(x, tp)
+            em, fm = fn(x, tm)
+            lp = ((fp - f_ref) ** 2).sum() + 0.1 * ep.sum()
+            lm = ((fm - f_ref) ** 2).sum() + 0.1 * em.sum()
+        fd = ((lp - lm) / (2 * h)).item()
+        assert abs(fd - grad[p]) < 1e-5 * max(1.0, abs(fd)), (p, fd, grad[p])
+
+
+def test_box_engine_nve_conserves_energy():
+    s = ffbox.box_system(nmol=125, box_angstrom=15.5, seed=8)
+    ff = cff.ForceField(s, rc_angstrom=7.5)
+    nb = 8
+    m = ffbox.masses(s["type"])
+    beta = float(cam.beta_of(300.0))
+    rng = np.random.default_rng(1)
+    q = s["x"][None] + rng.normal(0, 0.02, (nb,) + s["x"].shape)
+    p = rng.normal(size=q.shape) * np.sqrt(m[None, :, None] * nb / beta)
+    eng = cff.BoxEngine(ff, nb, 0.1 * cam.FS_TO_AU, m, beta, ffbox.theta_ref(), precision=cam.CAM_PREC_FP64,
+                        nm_mode=cam.CAM_NM_ORTHOGONAL)
+    eng.set_state(q, p)
+    def h():
+        ep, ek, es = eng.energies()
+        return ep.sum() + ek + es
+    h0 = h()
+    hs = []
+    for _ in range(10):
+        eng.run(20)
+        hs.append(h() - h0)
+    print("box NVE energy deviations", np.array(hs))
+    assert np.max(np.abs(hs)) < 2e-3 * nb
+    eng.close()
