@@ -11,6 +11,7 @@ CAM_SURFACE_H3, CAM_SURFACE_CH4H = 1, 2
 CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
 CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD = 0, 1, 2
+CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA = 0, 1, 2
 CAM_MAX_ATOMS, CAM_MAX_BONDS, CAM_MAX_REAC = 16, 4, 4
 
 
@@ -33,7 +34,7 @@ class Config(ctypes.Structure):
         ("surface", ctypes.c_int32), ("natoms", ctypes.c_int32), ("nbeads", ctypes.c_int32),
         ("ntraj", ctypes.c_int32), ("precision", ctypes.c_int32), ("nm_mode", ctypes.c_int32),
         ("ensemble", ctypes.c_int32), ("remove_com", ctypes.c_int32), ("andersen_step", ctypes.c_int32),
-        ("steps_per_launch", ctypes.c_int32),
+        ("steps_per_launch", ctypes.c_int32), ("nm_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dt", ctypes.c_double),
         ("mass", ctypes.c_double * CAM_MAX_ATOMS),
         ("seed", ctypes.c_uint64),

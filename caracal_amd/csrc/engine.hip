@@ -150,19 +150,102 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 }
 
 // ---------------------------------------------------------------------------
+// Normal-mode step on the matrix cores (FP64 MFMA 16x16x4), nbeads 16, 32 or
+// 64.  The block's p and q columns are staged in LDS as [column][bead] with
+// column = 2*(tl*NC + coord) + (0: p, 1: q) and stride NB + 1.  Each wave owns
+// one 16-row band of the transform (its A operand stays in registers) and
+// sweeps the 16-column tiles of its band: pass 0 computes the modes Y = T X
+// into ys and applies the exact 2x2 free ring-polymer map to the accumulators
+// (p and q of one coordinate sit in neighbouring lanes, swapped with DPP);
+// pass 1 transforms ys back into xs.
+// Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and receives D[4*(l/16)+r][l%16], r = 0..3.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NB, int TPB, int NC>
+__device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *prop, const double *tfwd,
+                                        const double *tinv, const double *mass, double dt)
+{
+    constexpr int NBP = NB + 1;
+    constexpr int NCOL = TPB * 2 * NC;
+    constexpr int CT = (NCOL + 15) / 16;
+    constexpr int RT = NB / 16;
+    constexpr int NWAVE = kBlock / 64;
+    static_assert(NB % 16 == 0 && NWAVE % RT == 0, "row bands must map onto whole waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row0 = (wave % RT) * 16;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        const double *T = pass == 0 ? tfwd : tinv;
+        const double *src = pass == 0 ? xs : ys;
+        double *dst = pass == 0 ? ys : xs;
+        double av[NB / 4];
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) av[kk] = T[(4 * kk + lk) * NB + row0 + li];   // T[row][k]
+#pragma unroll 1
+        for (int ct = wave / RT; ct < CT; ct += NWAVE / RT) {
+            const int col = ct * 16 + li;
+            const bool ok = col < NCOL;
+            const double *bcol = src + (ok ? col : 0) * NBP;
+            dbl4 acc = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+            for (int kk = 0; kk < NB / 4; ++kk) {
+                const double bv = ok ? bcol[4 * kk + lk] : 0.0;                         // X[k][col]
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc, 0, 0, 0);
+            }
+            if (pass == 0) {
+                // free ring-polymer map of each mode on (p, q) (src/verlet.f90:434-447)
+                const bool isq = (col & 1) != 0;
+                const int pair = (ok ? col : 0) >> 1;
+                const int tl = pair / NC, c = pair % NC;
+                const double m = mass[c / 3];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(acc[r]);
+                    const int mode = row0 + 4 * lk + r;
+                    const double *pr = prop + (tl * NB + mode) * 3;
+                    const double cw = pr[0], sw = pr[1], wk = pr[2];
+                    double p2, p3;
+                    if (mode == 0) {
+                        p2 = 0.0;
+                        p3 = dt / m;
+                    } else {
+                        const double wm = wk * m;
+                        p2 = -wm * sw;
+                        p3 = sw / wm;
+                    }
+                    acc[r] = isq ? other * p3 + acc[r] * cw : acc[r] * cw + other * p2;
+                }
+            }
+            if (ok) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) dst[col * NBP + row0 + 4 * lk + r] = acc[r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN or CAM_ENS_CHILD.
 // One step: p -= dt/2 g; free ring-polymer propagation in normal-mode space;
 // [CHILD: xi of the centroid]; forces; p -= dt/2 g; [ANDERSEN: resample].
 // ---------------------------------------------------------------------------
-template <class S, int NB, typename T, int ENS, int WPE>
+template <class S, int NB, typename T, int ENS, int WPE, bool MF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void step_kernel(StepArgs a)
 {
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS;
     constexpr int BPB = kBlock / L;          // beads per block
     constexpr int TPB = BPB / NB;            // trajectories per block
     static_assert(TPB >= 1, "a trajectory must fit in one block");
-    __shared__ __align__(16) double X[TPB * NC * NB];
+    constexpr int XS = MF ? TPB * 2 * NC * (NB + 1) : TPB * NC * NB;
+    __shared__ __align__(16) double X[XS];
     __shared__ __align__(16) double cent[TPB * NC];
+    __shared__ __align__(16) double Y[MF ? XS : 1];
+    __shared__ double prop[MF ? TPB * NB * 3 : 1];
+    __shared__ double smass[MF ? S::NA : 1];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -209,6 +292,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     const double hdt = 0.5 * a.dt;
     double vb = 0.0;
+    if constexpr (MF) {
+        if (threadIdx.x < S::NA) smass[threadIdx.x] = a.mass[threadIdx.x];
+        // per-trajectory mode table: cos(w dt), sin(w dt), w  (src/verlet.f90:413-432)
+        for (int i = threadIdx.x; i < TPB * NB; i += kBlock) {
+            const int t = i / NB, kk0 = i % NB;
+            const int tr = min(blockIdx.x * TPB + t, a.ntraj - 1);
+            const int kk = kk0 <= NB / 2 ? kk0 : NB - kk0;
+            const double bt = a.beta[tr];
+            const double wk = (2.0 / (bt / NB)) * sin(kk * (3.141592653589793 / NB));
+            prop[i * 3] = kk0 == 0 ? 1.0 : cos(wk * a.dt);
+            prop[i * 3 + 1] = kk0 == 0 ? 0.0 : sin(wk * a.dt);
+            prop[i * 3 + 2] = wk;
+        }
+    }
 
     for (int it = 0; it < a.nsteps; ++it) {
         // half kick (src/verlet.f90:216-218)
@@ -218,6 +315,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         if constexpr (NB == 1) {
 #pragma unroll
             for (int s = 0; s < SL; ++s) q[s] = p[s] * p3[s] + q[s];
+        } else if constexpr (MF) {
+            double *xm = X + tl * 2 * NC * (NB + 1);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < SL; ++s)
+                if (S::valid(s, r)) {
+                    xm[(2 * cc[s]) * (NB + 1) + k] = p[s];
+                    xm[(2 * cc[s] + 1) * (NB + 1) + k] = q[s];
+                }
+            __syncthreads();
+            nm_mfma<NB, TPB, NC>(X, Y, prop, a.tfwd, a.tinv, smass, a.dt);
+#pragma unroll
+            for (int s = 0; s < SL; ++s) {
+                p[s] = xm[(2 * cc[s]) * (NB + 1) + k];
+                q[s] = xm[(2 * cc[s] + 1) * (NB + 1) + k];
+            }
         } else {
             // bead -> normal modes, propagate, back (src/verlet.f90:377-463)
             double yp[SL], yq[SL];
@@ -719,6 +832,7 @@ struct cam_engine {
     bool have_state = false;
     int spl = 1;                      // steps per launch
     int wpe = 1;
+    bool mfma = false;
     step_fn step = nullptr;
     force_fn force = nullptr;
     std::map<int, hipGraphExec_t> graphs;
@@ -731,21 +845,27 @@ struct cam_engine {
 namespace {
 
 template <class S, int NB, typename T>
-step_fn pick_step(int ens, int wpe)
+step_fn pick_step(int ens, int wpe, bool mfma)
 {
     (void)wpe;
-    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1>;
-    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1>;
-    return step_kernel<S, NB, T, CAM_ENS_CHILD, 1>;
+    constexpr bool MF = NB >= 16 && NB % 16 == 0;
+    if (mfma && MF) {
+        if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1, MF>;
+        if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1, MF>;
+        return step_kernel<S, NB, T, CAM_ENS_CHILD, 1, MF>;
+    }
+    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1, false>;
+    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1, false>;
+    return step_kernel<S, NB, T, CAM_ENS_CHILD, 1, false>;
 }
 
 template <class S, typename T>
-bool pick_nb(int nb, int ens, int wpe, step_fn &st, force_fn &fo)
+bool pick_nb(int nb, int ens, int wpe, bool mfma, step_fn &st, force_fn &fo)
 {
     switch (nb) {
 #define CAM_NB_CASE(N)                               \
     case N:                                          \
-        st = pick_step<S, N, T>(ens, wpe);           \
+        st = pick_step<S, N, T>(ens, wpe, mfma);     \
         fo = force_kernel<S, N, T>;                  \
         return true;
         CAM_NB_CASE(1)
@@ -911,6 +1031,10 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
     CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_CHILD, "bad ensemble");
+    CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_MFMA, "bad nm_kernel");
+    CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
+                  "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
+    CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
@@ -963,13 +1087,14 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
     if (cfg->remove_com) e->spl = 1;
     e->wpe = 1;
+    e->mfma = cfg->nm_kernel == CAM_NMK_MFMA || (cfg->nm_kernel == CAM_NMK_AUTO && nb >= 16);
     bool ok;
     if (surf == CAM_SURFACE_H3)
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, e->step, e->force)
-                                             : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, e->step, e->force);
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force)
+                                             : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force);
     else
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, e->step, e->force)
-                                             : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, e->step, e->force);
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force)
+                                             : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
         return CAM_ERR_UNSUPPORTED;

@@ -52,7 +52,7 @@ class RPMDEngine:
 
     def __init__(self, surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64,
                  nm_mode=_abi.CAM_NM_REFERENCE, ensemble=_abi.CAM_ENS_NVE, remove_com=False,
-                 andersen_step=0, steps_per_launch=0, seed=0, reaction=None):
+                 andersen_step=0, steps_per_launch=0, seed=0, reaction=None, nm_kernel=_abi.CAM_NMK_AUTO):
         natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
         masses = np.asarray(masses, dtype=np.float64)
         if masses.shape != (natoms,):
@@ -61,6 +61,7 @@ class RPMDEngine:
         c.surface, c.natoms, c.nbeads, c.ntraj = surface, natoms, nbeads, ntraj
         c.precision, c.nm_mode, c.ensemble = precision, nm_mode, ensemble
         c.remove_com, c.andersen_step, c.steps_per_launch = int(remove_com), andersen_step, steps_per_launch
+        c.nm_kernel = nm_kernel
         c.dt = dt
         for i, m in enumerate(masses):
             c.mass[i] = m

@@ -19,6 +19,7 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2
+   integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2
    integer, parameter, public :: CAM_MAX_ATOMS = 16, CAM_MAX_BONDS = 4, CAM_MAX_REAC = 4
 
    ! cam_reaction: the BIMOLEC reaction coordinate (MECHA section keys)
@@ -38,7 +39,7 @@ module caracal_amd_mod
       integer(c_int32_t) :: surface = CAM_SURFACE_H3, natoms = 3, nbeads = 1, ntraj = 1
       integer(c_int32_t) :: precision = CAM_PREC_FP64, nm_mode = CAM_NM_REFERENCE
       integer(c_int32_t) :: ensemble = CAM_ENS_NVE, remove_com = 0, andersen_step = 0
-      integer(c_int32_t) :: steps_per_launch = 0
+      integer(c_int32_t) :: steps_per_launch = 0, nm_kernel = 0, reserved = 0
       real(c_double) :: dt = 0.d0
       real(c_double) :: mass(CAM_MAX_ATOMS) = 0.d0
       integer(c_int64_t) :: seed = 0

@@ -77,6 +77,11 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
  *              reaction coordinate of the centroid is evaluated each step and
  *              the recrossing-factor numerator is accumulated. */
 enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
+/* Which unit computes the normal-mode transform inside the step kernel:
+ * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
+ * nbeads a multiple of 16).  AUTO takes MFMA where it applies.  Both give the
+ * same transform to rounding. */
+enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2 };
 
 #define CAM_MAX_ATOMS 16
 #define CAM_MAX_BONDS 4
@@ -110,6 +115,8 @@ typedef struct {
                                  step as the reference's transrot does (src/transrot.f90) */
     int32_t andersen_step;    /* ANDERSEN: resample every andersen_step steps */
     int32_t steps_per_launch; /* steps fused into one kernel launch; 0 = pick automatically */
+    int32_t nm_kernel;        /* CAM_NMK_* */
+    int32_t reserved;         /* must be 0 */
     double dt;                /* time step, atomic time units */
     double mass[CAM_MAX_ATOMS];   /* electron masses */
     uint64_t seed;            /* counter-based RNG key for Andersen draws */

@@ -17,11 +17,14 @@ def _h3_engine(nb, ntraj=1, **kw):
     return e, w
 
 
-@pytest.mark.parametrize("nb", [16, 1, 4, 64])
-def test_h3_nve_trajectory_matches_reference_100_steps(nb):
+@pytest.mark.parametrize("nb,nmk", [(16, cam.CAM_NMK_VALU), (16, cam.CAM_NMK_MFMA), (1, cam.CAM_NMK_AUTO),
+                                     (4, cam.CAM_NMK_AUTO), (32, cam.CAM_NMK_MFMA), (64, cam.CAM_NMK_VALU),
+                                     (64, cam.CAM_NMK_MFMA)])
+def test_h3_nve_trajectory_matches_reference_100_steps(nb, nmk):
     """dynamic.x NVE with transrot, reference normal-mode transform: positions
-    within 1e-10 relative of the oracle after 100 steps."""
-    e, w = _h3_engine(nb, remove_com=True)
+    within 1e-10 relative of the oracle after 100 steps, with the transform on
+    the vector ALUs and on the matrix cores."""
+    e, w = _h3_engine(nb, remove_com=True, nm_kernel=nmk)
     e.set_state(w["q"][None], w["p"][None], w["beta"])
     s = ol.make_sys(1, nb, w["dt"], w["masses"])
     t = ol.OracleTraj(s, w["q"], w["p"], w["beta"], ensemble=0, remove_com=True)
@@ -32,7 +35,7 @@ def test_h3_nve_trajectory_matches_reference_100_steps(nb):
         q, p = e.get_state()
         scale = np.abs(t.q).max()
         err = np.abs(q[0] - t.q).max() / scale
-        print(f"nb={nb} step={25 * (block + 1)} max rel position error {err:.2e}")
+        print(f"nb={nb} nm_kernel={nmk} step={25 * (block + 1)} max rel position error {err:.2e}")
         assert err <= 1e-10
         assert np.abs(p[0] - t.p).max() <= 1e-9 * np.abs(t.p).max()
     e.close()
@@ -172,3 +175,33 @@ def test_mixed_precision_engine_tracks_fp64():
     err = np.abs(res[0] - res[1]).max()
     print(f"mixed vs fp64 after 100 steps: max |dq| = {err:.2e} bohr")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+def test_mfma_and_valu_transforms_agree_on_a_ch4h_batch(prec):
+    """CH4H children, 32 beads, 40 trajectories (a partial last block): the
+    matrix-core normal-mode step and the vector-ALU one stay within rounding
+    of each other over 200 steps, and the recrossing flags agree."""
+    w = workloads.ch4h_children(ntraj=40, nbeads=32, seed=4)
+    res = []
+    for nmk in (cam.CAM_NMK_VALU, cam.CAM_NMK_MFMA):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 32, 40, w["dt"], w["masses"], precision=prec,
+                           ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"], seed=3, nm_kernel=nmk)
+        e.set_state(w["q"], w["p"], w["beta"])
+        e.begin_children()
+        e.run(200)
+        q, p = e.get_state()
+        num, den = e.kappa(200)
+        res.append((q, p, num, den))
+        e.close()
+    (q0, p0, n0, d0), (q1, p1, n1, d1) = res
+    err = np.abs(q0 - q1).max() / np.abs(q0).max()
+    print(f"prec={prec} MFMA vs VALU max rel position difference {err:.2e}")
+    assert d0 == d1
+    if prec == cam.CAM_PREC_FP64:
+        assert err <= 1e-11
+        assert np.abs(p0 - p1).max() <= 1e-9 * np.abs(p0).max()
+        np.testing.assert_allclose(n0, n1, rtol=0, atol=1e-12 * max(1.0, np.abs(n0).max()))
+    else:
+        # FP32 forces turn rounding-level position differences into FP32-level ones
+        assert err <= 1e-6
