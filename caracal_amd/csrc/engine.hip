@@ -159,7 +159,7 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 // (p and q of one coordinate sit in neighbouring lanes, swapped with DPP);
 // pass 1 transforms ys back into xs.
 // Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] and receives D[4*(l/16)+r][l%16], r = 0..3.
+// B[l/16][l%16] and receives D[l/16 + 4r][l%16] in accumulator element r.
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -204,7 +204,7 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(acc[r]);
-                    const int mode = row0 + 4 * lk + r;
+                    const int mode = row0 + lk + 4 * r;
                     const double *pr = prop + (tl * NB + mode) * 3;
                     const double cw = pr[0], sw = pr[1], wk = pr[2];
                     double p2, p3;
@@ -221,7 +221,7 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
             }
             if (ok) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) dst[col * NBP + row0 + 4 * lk + r] = acc[r];
+                for (int r = 0; r < 4; ++r) dst[col * NBP + row0 + lk + 4 * r] = acc[r];
             }
         }
         __syncthreads();
@@ -1087,7 +1087,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
     if (cfg->remove_com) e->spl = 1;
     e->wpe = 1;
-    e->mfma = cfg->nm_kernel == CAM_NMK_MFMA || (cfg->nm_kernel == CAM_NMK_AUTO && nb >= 16);
+    e->mfma = cfg->nm_kernel == CAM_NMK_MFMA;
     bool ok;
     if (surf == CAM_SURFACE_H3)
         ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force)

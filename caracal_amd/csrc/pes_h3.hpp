@@ -31,24 +31,27 @@ __device__ __forceinline__ void singlet(T r, T& e, T& de)
             a16 = T(-1.406619156782167e-10);
     const T r0 = T(3.5284882), dd = T(0.160979391), c6 = T(6.499027), c8 = T(124.3991),
             c10 = T(3285.828);
-    // Horner form of sum_{i=1..16} a_i r^(i-1) and of its derivative.
-    T poly = a16;
-    poly = poly * r + a15; poly = poly * r + a14; poly = poly * r + a13;
-    poly = poly * r + a12; poly = poly * r + a11; poly = poly * r + a10;
-    poly = poly * r + a9;  poly = poly * r + a8;  poly = poly * r + a7;
-    poly = poly * r + a6;  poly = poly * r + a5;  poly = poly * r + a4;
-    poly = poly * r + a3;  poly = poly * r + a2;  poly = poly * r + a1;
-    T dpoly = T(15) * a16;
-    dpoly = dpoly * r + T(14) * a15; dpoly = dpoly * r + T(13) * a14;
-    dpoly = dpoly * r + T(12) * a13; dpoly = dpoly * r + T(11) * a12;
-    dpoly = dpoly * r + T(10) * a11; dpoly = dpoly * r + T(9) * a10;
-    dpoly = dpoly * r + T(8) * a9;   dpoly = dpoly * r + T(7) * a8;
-    dpoly = dpoly * r + T(6) * a7;   dpoly = dpoly * r + T(5) * a6;
-    dpoly = dpoly * r + T(4) * a5;   dpoly = dpoly * r + T(3) * a4;
-    dpoly = dpoly * r + T(2) * a3;   dpoly = dpoly * r + a2;
+    // Horner form of sum_{i=1..16} a_i r^(i-1) and of its derivative, always
+    // in FP64: the terms reach |a_i r^(i-1)| ~ 1e3 around r = 4.5 bohr while
+    // their sum stays O(1), so a binary32 evaluation would lose ~1e-4 of alpha.
+    const double rd = double(r);
+    double poly = double(a16);
+    poly = poly * rd + double(a15); poly = poly * rd + double(a14); poly = poly * rd + double(a13);
+    poly = poly * rd + double(a12); poly = poly * rd + double(a11); poly = poly * rd + double(a10);
+    poly = poly * rd + double(a9);  poly = poly * rd + double(a8);  poly = poly * rd + double(a7);
+    poly = poly * rd + double(a6);  poly = poly * rd + double(a5);  poly = poly * rd + double(a4);
+    poly = poly * rd + double(a3);  poly = poly * rd + double(a2);  poly = poly * rd + double(a1);
+    double dpoly = 15.0 * double(a16);
+    dpoly = dpoly * rd + 14.0 * double(a15); dpoly = dpoly * rd + 13.0 * double(a14);
+    dpoly = dpoly * rd + 12.0 * double(a13); dpoly = dpoly * rd + 11.0 * double(a12);
+    dpoly = dpoly * rd + 10.0 * double(a11); dpoly = dpoly * rd + 9.0 * double(a10);
+    dpoly = dpoly * rd + 8.0 * double(a9);   dpoly = dpoly * rd + 7.0 * double(a8);
+    dpoly = dpoly * rd + 6.0 * double(a7);   dpoly = dpoly * rd + 5.0 * double(a6);
+    dpoly = dpoly * rd + 4.0 * double(a5);   dpoly = dpoly * rd + 3.0 * double(a4);
+    dpoly = dpoly * rd + 2.0 * double(a3);   dpoly = dpoly * rd + double(a2);
     const T rinv = T(1) / r;
-    const T alpha = a0 * rinv + poly;
-    const T dalpha = -a0 * rinv * rinv + dpoly;
+    const T alpha = T(double(a0) / rd + poly);
+    const T dalpha = T(-double(a0) / (rd * rd) + dpoly);
     const T ex = rt::exp(alpha);
     const T exm1 = ex - T(1);
     const T r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
@@ -148,6 +151,125 @@ static __constant__ const double kCbend[2][30] = {
           CAM_SP(-.5145757859E+02), CAM_SP(0.2046097265E+01), CAM_SP(0.2540775558E+01),
           CAM_SP(-.4889246569E+00), CAM_SP(-.1127439280E+04), CAM_SP(-.2269932295E+01) } };
 
+
+// Bending terms (VBCB95).  Evaluated in FP64 on both paths: the A- and
+// F-sums of the VB families are O(100) and their exponentially damped products
+// cancel to O(0.1) near the saddle, so binary32 exponentials there would cost
+// ~1e-6 Eh/bohr in the gradient.  v and dv accumulate in place.
+template <typename T>
+__device__ __forceinline__ void bend(const T r[3], int icompc, const T t[3], const T dt[3], T& v, T dv[3])
+{
+    const T rs = r[0] + r[1] + r[2];
+    const T rsq = rs * rs;
+    const T r1 = r[0], r2 = r[1], r3 = r[2];
+    const T t1 = r1 * r1 - r2 * r2 - r3 * r3;
+    const T t2 = r2 * r2 - r3 * r3 - r1 * r1;
+    const T t3 = r3 * r3 - r1 * r1 - r2 * r2;
+    const T cs1 = t1 / (-T(2) * r2 * r3), cs2 = t2 / (-T(2) * r3 * r1), cs3 = t3 / (-T(2) * r1 * r2);
+    const T csum = cs1 + cs2 + cs3;
+    const T cos3 = (T(4) * cs1 * cs1 * cs1 - T(3) * cs1) + (T(4) * cs2 * cs2 * cs2 - T(3) * cs2)
+                   + (T(4) * cs3 * cs3 * cs3 - T(3) * cs3);
+    const T b1fam[2] = { T(1) - csum, T(1) - (T(0.625) * cos3 + T(0.375) * csum) };
+    // dC_j/dR_i
+    const T dc1[3] = { -r1 / (r2 * r3), (t1 / (r2 * r2) + T(2)) / (T(2) * r3), (t1 / (r3 * r3) + T(2)) / (T(2) * r2) };
+    const T dc2[3] = { (t2 / (r1 * r1) + T(2)) / (T(2) * r3), -r2 / (r1 * r3), (t2 / (r3 * r3) + T(2)) / (T(2) * r1) };
+    const T dc3[3] = { (t3 / (r1 * r1) + T(2)) / (T(2) * r2), (t3 / (r2 * r2) + T(2)) / (T(2) * r1), -r3 / (r1 * r2) };
+    const T q1 = T(12) * cs1 * cs1 - T(3), q2 = T(12) * cs2 * cs2 - T(3), q3 = T(12) * cs3 * cs3 - T(3);
+    T db1[2][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        db1[0][i] = -(dc1[i] + dc2[i] + dc3[i]);
+        db1[1][i] = -T(0.625) * (q1 * dc1[i] + q2 * dc2[i] + q3 * dc3[i]) - T(0.375) * (dc1[i] + dc2[i] + dc3[i]);
+    }
+    const T b2 = T(1) / r1 + T(1) / r2 + T(1) / r3;
+    const T db2[3] = { -T(1) / (r1 * r1), -T(1) / (r2 * r2), -T(1) / (r3 * r3) };
+    const T b3 = (r2 - r1) * (r2 - r1) + (r3 - r2) * (r3 - r2) + (r1 - r3) * (r1 - r3);
+    const T b3b = rt::sqrt(b3 + T(1.0e-12));
+    const T db3[3] = { T(4) * r1 - T(2) * r2 - T(2) * r3, T(4) * r2 - T(2) * r3 - T(2) * r1,
+                       T(4) * r3 - T(2) * r1 - T(2) * r2 };
+    const T ib3b = T(0.5) / b3b;
+    const T beta1 = T(0.52), beta2 = T(0.052), beta3 = T(0.79);
+    const T exp1 = rt::exp(-beta1 * rs), exp2 = rt::exp(-beta2 * rsq), exp7 = rt::exp(-beta3 * rs);
+    const T dexp1 = -beta1 * exp1, dexp2 = -T(2) * beta2 * rs * exp2, dexp7 = -beta3 * exp7;
+#pragma unroll
+    for (int fam = 0; fam < 2; ++fam) {
+        const double* c = kVbend[fam];
+        const T b1 = b1fam[fam];
+        const T b12 = b1 * b1, b13 = b12 * b1, b14 = b13 * b1, b15 = b14 * b1;
+        const T asum = T(c[0]) + T(c[1]) * rs + T(c[2]) * rsq;
+        const T bsum = T(c[3]) * b12 + T(c[4]) * b13 + T(c[5]) * b14 + T(c[6]) * b15;
+        const T csm = T(c[7]) * b1 * exp1 + T(c[8]) * b12 * exp2;
+        const T ds1 = T(c[9]) * exp1 + T(c[10]) * exp2;
+        const T ds2 = T(c[11]) * exp1 + T(c[12]) * exp2;
+        const T fsum = T(c[13]) + T(c[14]) * rs + T(c[15]) * rsq;
+        v += b1 * asum * exp1 + bsum * exp2 + b2 * csm + (b1 * b3 * ds1 + b1 * b3b * ds2) + b1 * fsum * exp7;
+        const T dasum = T(c[1]) + T(2) * T(c[2]) * rs;
+        const T dbsum = T(2) * T(c[3]) * b1 + T(3) * T(c[4]) * b12 + T(4) * T(c[5]) * b13 + T(5) * T(c[6]) * b14;
+        const T dds1 = T(c[9]) * dexp1 + T(c[10]) * dexp2;
+        const T dds2 = T(c[11]) * dexp1 + T(c[12]) * dexp2;
+        const T dfsum = T(c[14]) + T(2) * T(c[15]) * rs;
+        // parts of the derivative that do not depend on i
+        const T k0 = b1 * dasum * exp1 + b1 * asum * dexp1 + bsum * dexp2
+                     + b2 * (T(c[7]) * b1 * dexp1 + T(c[8]) * b12 * dexp2)
+                     + b1 * b3 * dds1 + b1 * b3b * dds2 + b1 * dfsum * exp7 + b1 * fsum * dexp7;
+        const T kb1 = asum * exp1 + dbsum * exp2 + b2 * (T(c[7]) * exp1 + T(2) * T(c[8]) * b1 * exp2)
+                      + b3 * ds1 + b3b * ds2 + fsum * exp7;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            dv[i] += k0 + db1[fam][i] * kb1 + db2[i] * csm + b1 * db3[i] * ds1 + b1 * db3[i] * ib3b * ds2;
+    }
+    if (icompc == 0) return;
+    const T sumt = t[0] + t[1] + t[2];
+    const T rcu = rsq * rs;
+    const T p = r1 * r2 * r3;
+    const T ip = T(1) / p;
+    const T ip2 = ip * ip, ip3 = ip2 * ip;
+    const T dp[3] = { r2 * r3, r3 * r1, r1 * r2 };
+    const T e7 = rt::exp(-beta2 * rcu);
+    const T de7 = -T(3) * beta2 * rsq * e7;
+    const T irs = T(1) / rs, irs2 = irs * irs, irs3 = irs2 * irs;
+#pragma unroll
+    for (int fam = 0; fam < 2; ++fam) {
+        const double* c = kCbend[fam];
+        const T b1 = b1fam[fam];
+        const T b12 = b1 * b1, b13 = b12 * b1, b14 = b13 * b1, b15 = b14 * b1;
+        const T x1 = T(c[15] + c[28]);          // C51 + C83
+        const T asum = T(c[0]) + T(c[1]) * rs + T(c[2]) * rsq + T(c[3]) * irs + T(c[4]) * irs2;
+        const T bsum = T(c[5]) * b12 + T(c[6]) * b13 + T(c[7]) * b14 + T(c[8]) * b15;
+        const T csm = T(c[9]) * b1 * exp1 + T(c[10]) * b12 * exp2;
+        const T ds1 = T(c[11]) * exp1 + T(c[12]) * exp2;
+        const T ds2 = T(c[13]) * exp1 + T(c[14]) * exp2;
+        const T fsum = x1 + T(c[16]) * rs + T(c[17]) * rsq;
+        const T gsum = T(c[18]) + T(c[19]) * irs + T(c[20]) * irs2;
+        const T aasum = T(c[21]) + T(c[22]) * p + T(c[23]) * p * p + T(c[24]) * ip + T(c[25]) * ip2;
+        const T ffsum = T(c[26]) * p + T(c[27]) * p * p + T(c[29]) * ip2;
+        const T dasum = T(c[1]) + T(2) * T(c[2]) * rs - T(c[3]) * irs2 - T(2) * T(c[4]) * irs3;
+        const T dbsum = T(2) * T(c[5]) * b1 + T(3) * T(c[6]) * b12 + T(4) * T(c[7]) * b13 + T(5) * T(c[8]) * b14;
+        const T dds1 = T(c[11]) * dexp1 + T(c[12]) * dexp2;
+        const T dds2 = T(c[13]) * dexp1 + T(c[14]) * dexp2;
+        const T dfsum = T(c[16]) + T(2) * T(c[17]) * rs;
+        const T dgsum = -T(c[19]) * irs2 - T(2) * T(c[20]) * irs3;
+        const T daasum = T(c[22]) + T(2) * T(c[23]) * p - T(c[24]) * ip2 - T(2) * T(c[25]) * ip3;
+        const T dffsum = T(c[26]) + T(2) * T(c[27]) * p - T(2) * T(c[29]) * ip3;
+        const T cbnds = b1 * asum * exp1 * ip + bsum * exp2 + b2 * csm + (b1 * b3 * ds1 + b1 * b3b * ds2)
+                        + b1 * fsum * e7 * ip + b1 * gsum * e7 + b1 * aasum * exp2 + b1 * ffsum * e7;
+        v += sumt * cbnds;
+        const T k0 = b1 * dasum * exp1 * ip + b1 * asum * dexp1 * ip + bsum * dexp2
+                     + b2 * (T(c[9]) * b1 * dexp1 + T(c[10]) * b12 * dexp2)
+                     + b1 * b3 * dds1 + b1 * b3b * dds2
+                     + b1 * dfsum * e7 * ip + b1 * fsum * de7 * ip
+                     + b1 * dgsum * e7 + b1 * gsum * de7 + b1 * aasum * dexp2 + b1 * ffsum * de7;
+        const T kb1 = asum * exp1 * ip + dbsum * exp2 + b2 * (T(c[9]) * exp1 + T(2) * T(c[10]) * b1 * exp2)
+                      + b3 * ds1 + b3b * ds2 + fsum * e7 * ip + gsum * e7 + aasum * exp2 + ffsum * e7;
+        const T kdp = -b1 * asum * exp1 * ip2 - b1 * fsum * e7 * ip2 + b1 * daasum * exp2 + b1 * dffsum * e7;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const T dsum = k0 + db1[fam][i] * kb1 + db2[i] * csm + b1 * db3[i] * ds1
+                           + b1 * db3[i] * ib3b * ds2 + dp[i] * kdp;
+            dv[i] += dt[i] * cbnds + sumt * dsum;
+        }
+    }
+}
 
 // V(R1,R2,R3) and dV/dR_i of the BKMP2 surface (POTE).  R(1)=r_AB, R(2)=r_AC, R(3)=r_BC.
 template <typename T>
@@ -265,114 +387,16 @@ __device__ __forceinline__ void pote(const T r[3], T& v, T dv[3])
                      + determ * sumt * a2 * term1 * series;
         }
     }
-    // ---- bending terms (VBCB95) ----
-    const T r1 = r[0], r2 = r[1], r3 = r[2];
-    const T t1 = r1 * r1 - r2 * r2 - r3 * r3;
-    const T t2 = r2 * r2 - r3 * r3 - r1 * r1;
-    const T t3 = r3 * r3 - r1 * r1 - r2 * r2;
-    const T cs1 = t1 / (-T(2) * r2 * r3), cs2 = t2 / (-T(2) * r3 * r1), cs3 = t3 / (-T(2) * r1 * r2);
-    const T csum = cs1 + cs2 + cs3;
-    const T cos3 = (T(4) * cs1 * cs1 * cs1 - T(3) * cs1) + (T(4) * cs2 * cs2 * cs2 - T(3) * cs2)
-                   + (T(4) * cs3 * cs3 * cs3 - T(3) * cs3);
-    const T b1fam[2] = { T(1) - csum, T(1) - (T(0.625) * cos3 + T(0.375) * csum) };
-    // dC_j/dR_i
-    const T dc1[3] = { -r1 / (r2 * r3), (t1 / (r2 * r2) + T(2)) / (T(2) * r3), (t1 / (r3 * r3) + T(2)) / (T(2) * r2) };
-    const T dc2[3] = { (t2 / (r1 * r1) + T(2)) / (T(2) * r3), -r2 / (r1 * r3), (t2 / (r3 * r3) + T(2)) / (T(2) * r1) };
-    const T dc3[3] = { (t3 / (r1 * r1) + T(2)) / (T(2) * r2), (t3 / (r2 * r2) + T(2)) / (T(2) * r1), -r3 / (r1 * r2) };
-    const T q1 = T(12) * cs1 * cs1 - T(3), q2 = T(12) * cs2 * cs2 - T(3), q3 = T(12) * cs3 * cs3 - T(3);
-    T db1[2][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        db1[0][i] = -(dc1[i] + dc2[i] + dc3[i]);
-        db1[1][i] = -T(0.625) * (q1 * dc1[i] + q2 * dc2[i] + q3 * dc3[i]) - T(0.375) * (dc1[i] + dc2[i] + dc3[i]);
-    }
-    const T b2 = T(1) / r1 + T(1) / r2 + T(1) / r3;
-    const T db2[3] = { -T(1) / (r1 * r1), -T(1) / (r2 * r2), -T(1) / (r3 * r3) };
-    const T b3 = (r2 - r1) * (r2 - r1) + (r3 - r2) * (r3 - r2) + (r1 - r3) * (r1 - r3);
-    const T b3b = rt::sqrt(b3 + T(1.0e-12));
-    const T db3[3] = { T(4) * r1 - T(2) * r2 - T(2) * r3, T(4) * r2 - T(2) * r3 - T(2) * r1,
-                       T(4) * r3 - T(2) * r1 - T(2) * r2 };
-    const T ib3b = T(0.5) / b3b;
-    const T beta1 = T(0.52), beta2 = T(0.052), beta3 = T(0.79);
-    const T exp1 = rt::exp(-beta1 * rs), exp2 = rt::exp(-beta2 * rsq), exp7 = rt::exp(-beta3 * rs);
-    const T dexp1 = -beta1 * exp1, dexp2 = -T(2) * beta2 * rs * exp2, dexp7 = -beta3 * exp7;
-#pragma unroll
-    for (int fam = 0; fam < 2; ++fam) {
-        const double* c = kVbend[fam];
-        const T b1 = b1fam[fam];
-        const T b12 = b1 * b1, b13 = b12 * b1, b14 = b13 * b1, b15 = b14 * b1;
-        const T asum = T(c[0]) + T(c[1]) * rs + T(c[2]) * rsq;
-        const T bsum = T(c[3]) * b12 + T(c[4]) * b13 + T(c[5]) * b14 + T(c[6]) * b15;
-        const T csm = T(c[7]) * b1 * exp1 + T(c[8]) * b12 * exp2;
-        const T ds1 = T(c[9]) * exp1 + T(c[10]) * exp2;
-        const T ds2 = T(c[11]) * exp1 + T(c[12]) * exp2;
-        const T fsum = T(c[13]) + T(c[14]) * rs + T(c[15]) * rsq;
-        v += b1 * asum * exp1 + bsum * exp2 + b2 * csm + (b1 * b3 * ds1 + b1 * b3b * ds2) + b1 * fsum * exp7;
-        const T dasum = T(c[1]) + T(2) * T(c[2]) * rs;
-        const T dbsum = T(2) * T(c[3]) * b1 + T(3) * T(c[4]) * b12 + T(4) * T(c[5]) * b13 + T(5) * T(c[6]) * b14;
-        const T dds1 = T(c[9]) * dexp1 + T(c[10]) * dexp2;
-        const T dds2 = T(c[11]) * dexp1 + T(c[12]) * dexp2;
-        const T dfsum = T(c[14]) + T(2) * T(c[15]) * rs;
-        // parts of the derivative that do not depend on i
-        const T k0 = b1 * dasum * exp1 + b1 * asum * dexp1 + bsum * dexp2
-                     + b2 * (T(c[7]) * b1 * dexp1 + T(c[8]) * b12 * dexp2)
-                     + b1 * b3 * dds1 + b1 * b3b * dds2 + b1 * dfsum * exp7 + b1 * fsum * dexp7;
-        const T kb1 = asum * exp1 + dbsum * exp2 + b2 * (T(c[7]) * exp1 + T(2) * T(c[8]) * b1 * exp2)
-                      + b3 * ds1 + b3b * ds2 + fsum * exp7;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            dv[i] += k0 + db1[fam][i] * kb1 + db2[i] * csm + b1 * db3[i] * ds1 + b1 * db3[i] * ib3b * ds2;
-    }
-    if (icompc == 0) return;
-    const T sumt = t[0] + t[1] + t[2];
-    const T rcu = rsq * rs;
-    const T p = r1 * r2 * r3;
-    const T ip = T(1) / p;
-    const T ip2 = ip * ip, ip3 = ip2 * ip;
-    const T dp[3] = { r2 * r3, r3 * r1, r1 * r2 };
-    const T e7 = rt::exp(-beta2 * rcu);
-    const T de7 = -T(3) * beta2 * rsq * e7;
-    const T irs = T(1) / rs, irs2 = irs * irs, irs3 = irs2 * irs;
-#pragma unroll
-    for (int fam = 0; fam < 2; ++fam) {
-        const double* c = kCbend[fam];
-        const T b1 = b1fam[fam];
-        const T b12 = b1 * b1, b13 = b12 * b1, b14 = b13 * b1, b15 = b14 * b1;
-        const T x1 = T(c[15] + c[28]);          // C51 + C83
-        const T asum = T(c[0]) + T(c[1]) * rs + T(c[2]) * rsq + T(c[3]) * irs + T(c[4]) * irs2;
-        const T bsum = T(c[5]) * b12 + T(c[6]) * b13 + T(c[7]) * b14 + T(c[8]) * b15;
-        const T csm = T(c[9]) * b1 * exp1 + T(c[10]) * b12 * exp2;
-        const T ds1 = T(c[11]) * exp1 + T(c[12]) * exp2;
-        const T ds2 = T(c[13]) * exp1 + T(c[14]) * exp2;
-        const T fsum = x1 + T(c[16]) * rs + T(c[17]) * rsq;
-        const T gsum = T(c[18]) + T(c[19]) * irs + T(c[20]) * irs2;
-        const T aasum = T(c[21]) + T(c[22]) * p + T(c[23]) * p * p + T(c[24]) * ip + T(c[25]) * ip2;
-        const T ffsum = T(c[26]) * p + T(c[27]) * p * p + T(c[29]) * ip2;
-        const T dasum = T(c[1]) + T(2) * T(c[2]) * rs - T(c[3]) * irs2 - T(2) * T(c[4]) * irs3;
-        const T dbsum = T(2) * T(c[5]) * b1 + T(3) * T(c[6]) * b12 + T(4) * T(c[7]) * b13 + T(5) * T(c[8]) * b14;
-        const T dds1 = T(c[11]) * dexp1 + T(c[12]) * dexp2;
-        const T dds2 = T(c[13]) * dexp1 + T(c[14]) * dexp2;
-        const T dfsum = T(c[16]) + T(2) * T(c[17]) * rs;
-        const T dgsum = -T(c[19]) * irs2 - T(2) * T(c[20]) * irs3;
-        const T daasum = T(c[22]) + T(2) * T(c[23]) * p - T(c[24]) * ip2 - T(2) * T(c[25]) * ip3;
-        const T dffsum = T(c[26]) + T(2) * T(c[27]) * p - T(2) * T(c[29]) * ip3;
-        const T cbnds = b1 * asum * exp1 * ip + bsum * exp2 + b2 * csm + (b1 * b3 * ds1 + b1 * b3b * ds2)
-                        + b1 * fsum * e7 * ip + b1 * gsum * e7 + b1 * aasum * exp2 + b1 * ffsum * e7;
-        v += sumt * cbnds;
-        const T k0 = b1 * dasum * exp1 * ip + b1 * asum * dexp1 * ip + bsum * dexp2
-                     + b2 * (T(c[9]) * b1 * dexp1 + T(c[10]) * b12 * dexp2)
-                     + b1 * b3 * dds1 + b1 * b3b * dds2
-                     + b1 * dfsum * e7 * ip + b1 * fsum * de7 * ip
-                     + b1 * dgsum * e7 + b1 * gsum * de7 + b1 * aasum * dexp2 + b1 * ffsum * de7;
-        const T kb1 = asum * exp1 * ip + dbsum * exp2 + b2 * (T(c[9]) * exp1 + T(2) * T(c[10]) * b1 * exp2)
-                      + b3 * ds1 + b3b * ds2 + fsum * e7 * ip + gsum * e7 + aasum * exp2 + ffsum * e7;
-        const T kdp = -b1 * asum * exp1 * ip2 - b1 * fsum * e7 * ip2 + b1 * daasum * exp2 + b1 * dffsum * e7;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const T dsum = k0 + db1[fam][i] * kb1 + db2[i] * csm + b1 * db3[i] * ds1
-                           + b1 * db3[i] * ib3b * ds2 + dp[i] * kdp;
-            dv[i] += dt[i] * cbnds + sumt * dsum;
-        }
+    {
+        const double rb[3] = { double(r[0]), double(r[1]), double(r[2]) };
+        const double tb[3] = { double(t[0]), double(t[1]), double(t[2]) };
+        const double dtb[3] = { double(dt[0]), double(dt[1]), double(dt[2]) };
+        double vb = double(v), dvb[3] = { double(dv[0]), double(dv[1]), double(dv[2]) };
+        bend<double>(rb, icompc, tb, dtb, vb, dvb);
+        v = T(vb);
+        dv[0] = T(dvb[0]);
+        dv[1] = T(dvb[1]);
+        dv[2] = T(dvb[2]);
     }
 }
 

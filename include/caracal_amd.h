@@ -79,7 +79,7 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
 enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
- * nbeads a multiple of 16).  AUTO takes MFMA where it applies.  Both give the
+ * nbeads a multiple of 16).  AUTO takes the vector-ALU form.  Both give the
  * same transform to rounding. */
 enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2 };
 

@@ -1,0 +1,112 @@
+"""Ring-polymer dynamics of a periodic box on the CPU -- test infrastructure
+(the oracle), used only to check the GPU box engine.
+
+One velocity-Verlet RPMD step as the reference's verlet does it
+(src/verlet.f90:216-218 half kick, :377-463 free ring-polymer propagation in
+normal-mode space, :772-777 forces, :1031-1034 half kick), with the forces of
+every bead from the oracle force field (oracle/ff_box.c).  The normal-mode
+transform is the reference's rfft/irfft pair: both directions apply the
+symmetric matrix cos(2 pi k n / N)/sqrt(N) (src/rfft.f90:36-62,
+src/irfft.f90:36-63).  The numpy propagation is pinned against the C
+oracle's oracle_free_rp on a small system (tests/test_oracle_box.py).
+"""
+import numpy as np
+
+
+def cosine_matrix(nb):
+    k = np.arange(nb)
+    return np.cos(2.0 * np.pi * np.outer(k, k) / nb) / np.sqrt(nb)
+
+
+def orthogonal_matrix(nb):
+    """Orthogonal real-DFT normal-mode basis: row k = 0 constant, rows
+    1..nb/2 cosines, rows above nb/2 sines, so that modes k and nb - k share
+    the frequency 2 sin(k pi / nb) / beta_n."""
+    n = np.arange(nb)
+    u = np.empty((nb, nb))
+    u[0] = 1.0 / np.sqrt(nb)
+    for k in range(1, nb):
+        if k < nb / 2:
+            u[k] = np.sqrt(2.0 / nb) * np.cos(2.0 * np.pi * k * n / nb)
+        elif k == nb / 2:
+            u[k] = (-1.0) ** n / np.sqrt(nb)
+        else:
+            u[k] = np.sqrt(2.0 / nb) * np.sin(2.0 * np.pi * k * n / nb)
+    return u
+
+
+def propagator(nb, dt, masses, beta):
+    """poly[c, k, atom] of src/verlet.f90:405-432: (p, q) -> (p c0 + q c1, p c2 + q c3)."""
+    na = len(masses)
+    poly = np.zeros((4, nb, na))
+    poly[0, 0] = 1.0
+    poly[2, 0] = dt / masses
+    poly[3, 0] = 1.0
+    if nb > 1:
+        beta_n = beta / nb
+        for k in range(1, nb // 2 + 1):
+            wk = (2.0 / beta_n) * np.sin(k * np.pi / nb)
+            wt = wk * dt
+            wm = wk * masses
+            poly[0, k] = np.cos(wt)
+            poly[1, k] = -wm * np.sin(wt)
+            poly[2, k] = np.sin(wt) / wm
+            poly[3, k] = np.cos(wt)
+        for k in range(1, (nb - 1) // 2 + 1):
+            poly[:, nb - k] = poly[:, k]
+    return poly
+
+
+def free_rp(q, p, dt, masses, beta, orthogonal=False):
+    """q, p: (nbeads, natoms, 3); returns the propagated pair.  orthogonal
+    selects the orthogonal real-DFT basis (inverse = transpose) instead of the
+    reference's cosine pair."""
+    nb = q.shape[0]
+    if nb == 1:
+        return q + p * (dt / masses)[None, :, None], p.copy()
+    fwd = orthogonal_matrix(nb) if orthogonal else cosine_matrix(nb)
+    inv = fwd.T
+    poly = propagator(nb, dt, masses, beta)
+    pm = np.einsum("kn,naj->kaj", fwd, p)
+    qm = np.einsum("kn,naj->kaj", fwd, q)
+    pn = pm * poly[0][:, :, None] + qm * poly[1][:, :, None]
+    qn = pm * poly[2][:, :, None] + qm * poly[3][:, :, None]
+    return np.einsum("kn,naj->kaj", inv, qn), np.einsum("kn,naj->kaj", inv, pn)
+
+
+class BoxRPMD:
+    """The oracle's ring polymer of one box; arrays (nbeads, natoms, 3), bohr
+    and atomic units."""
+
+    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False):
+        self.orthogonal = orthogonal
+        self.ffo, self.theta = ffo, np.asarray(theta, dtype=np.float64)
+        self.q, self.p = np.array(q, dtype=np.float64), np.array(p, dtype=np.float64)
+        self.dt, self.m, self.beta = dt, np.asarray(masses, dtype=np.float64), beta
+        self.v, self.g = self.forces(self.q)
+
+    def forces(self, q):
+        v = np.empty(q.shape[0])
+        g = np.empty_like(q)
+        for b in range(q.shape[0]):
+            e, gb, _, _ = self.ffo.eval(self.theta, q[b])
+            v[b], g[b] = e, gb
+        return v, g
+
+    def step(self):
+        hdt = 0.5 * self.dt
+        self.p = self.p - hdt * self.g
+        self.q, self.p = free_rp(self.q, self.p, self.dt, self.m, self.beta, self.orthogonal)
+        self.v, self.g = self.forces(self.q)
+        self.p = self.p - hdt * self.g
+
+    def energies(self):
+        """Per-bead potential, kinetic energy and ring-polymer spring energy
+        0.5 m omega_n^2 sum_k |q_k - q_k+1|^2 with omega_n = nbeads / beta, the
+        spring whose normal modes have the frequencies of src/verlet.f90:413-418."""
+        nb = self.q.shape[0]
+        ek = 0.5 * np.sum(self.p ** 2 / self.m[None, :, None])
+        wn = nb / self.beta
+        d = self.q - np.roll(self.q, -1, axis=0)
+        es = 0.5 * wn * wn * np.sum(self.m[None, :, None] * d * d) if nb > 1 else 0.0
+        return self.v.copy(), ek, es

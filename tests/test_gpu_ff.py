@@ -112,26 +112,72 @@ def test_autograd_force_matching_gradient():
         assert abs(fd - grad[p]) < 1e-5 * max(1.0, abs(fd)), (p, fd, grad[p])
 
 
-def test_box_engine_nve_conserves_energy():
-    s = ffbox.box_system(nmol=125, box_angstrom=15.5, seed=8)
-    ff = cff.ForceField(s, rc_angstrom=7.5)
-    nb = 8
+def _box_start(nmol, box_a, seed, nb, kelvin=300.0):
+    s = ffbox.box_system(nmol=nmol, box_angstrom=box_a, seed=seed)
     m = ffbox.masses(s["type"])
-    beta = float(cam.beta_of(300.0))
+    beta = float(cam.beta_of(kelvin))
     rng = np.random.default_rng(1)
     q = s["x"][None] + rng.normal(0, 0.02, (nb,) + s["x"].shape)
     p = rng.normal(size=q.shape) * np.sqrt(m[None, :, None] * nb / beta)
-    eng = cff.BoxEngine(ff, nb, 0.1 * cam.FS_TO_AU, m, beta, ffbox.theta_ref(), precision=cam.CAM_PREC_FP64,
-                        nm_mode=cam.CAM_NM_ORTHOGONAL)
+    return s, m, beta, q, p
+
+
+@pytest.mark.parametrize("nm_mode", [cam.CAM_NM_REFERENCE, cam.CAM_NM_ORTHOGONAL])
+def test_box_engine_matches_oracle_trajectory(nm_mode):
+    """125 molecules, 8 beads, 7.5 angstrom cutoff, FP64: 40 steps of the GPU
+    box engine against the oracle's ring polymer (oracle force field + the
+    free ring-polymer step); positions and the energy terms agree."""
+    import box_oracle as bo
+    nb = 8
+    s, m, beta, q, p = _box_start(125, 15.5, 8, nb)
+    ff = cff.ForceField(s, rc_angstrom=7.5)
+    dt = 0.1 * cam.FS_TO_AU
+    eng = cff.BoxEngine(ff, nb, dt, m, beta, ffbox.theta_ref(), precision=cam.CAM_PREC_FP64, nm_mode=nm_mode)
     eng.set_state(q, p)
-    def h():
-        ep, ek, es = eng.energies()
-        return ep.sum() + ek + es
-    h0 = h()
-    hs = []
-    for _ in range(10):
-        eng.run(20)
-        hs.append(h() - h0)
-    print("box NVE energy deviations", np.array(hs))
-    assert np.max(np.abs(hs)) < 2e-3 * nb
+    t = bo.BoxRPMD(ol.FFOracle(s, rc_angstrom=7.5), ffbox.theta_ref(), q, p, dt, m, beta,
+                   orthogonal=nm_mode == cam.CAM_NM_ORTHOGONAL)
+    for block in range(4):
+        eng.run(10)
+        for _ in range(10):
+            t.step()
+        qg, pg = eng.get_state()
+        err = np.abs(qg - t.q).max() / np.abs(t.q).max()
+        print(f"nm_mode={nm_mode} step={10 * (block + 1)} max rel position error {err:.2e}")
+        assert err < 1e-10
+        assert np.abs(pg - t.p).max() < 1e-8 * np.abs(t.p).max()
+    ep, ek, es = eng.energies()
+    vo, eko, eso = t.energies()
+    np.testing.assert_allclose(ep, vo, rtol=1e-9)
+    np.testing.assert_allclose([ek, es], [eko, eso], rtol=1e-9)
     eng.close()
+
+
+def test_box_engine_nve_conserves_energy():
+    """Orthogonal normal modes, cutoff beyond the largest minimum-image
+    distance (no pair crosses it, so the energy is smooth): the ring-polymer
+    energy stays bounded over 200 steps and its error is second order in dt.
+    (With a cutoff inside the box the reference's energy jumps by
+    q_i q_j erfc(a rc)/rc whenever a pair crosses it.)"""
+    nb = 8
+    s, m, beta, q, p = _box_start(125, 15.5, 8, nb)
+    ff = cff.ForceField(s, rc_angstrom=13.5)
+    dev = {}
+    for dtfs, nsteps in ((0.1, 200), (0.05, 400)):
+        eng = cff.BoxEngine(ff, nb, dtfs * cam.FS_TO_AU, m, beta, ffbox.theta_ref(), precision=cam.CAM_PREC_FP64,
+                            nm_mode=cam.CAM_NM_ORTHOGONAL)
+        eng.set_state(q, p)
+
+        def h():
+            ep, ek, es = eng.energies()
+            return ep.sum() + ek + es
+        h0 = h()
+        hs = []
+        for _ in range(20):
+            eng.run(nsteps // 20)
+            hs.append(h() - h0)
+        eng.close()
+        dev[dtfs] = np.array(hs)
+        print(f"dt={dtfs} fs energy deviations {dev[dtfs]}")
+    assert np.abs(dev[0.1]).max() < 0.02
+    ratio = np.abs(dev[0.1]).mean() / np.abs(dev[0.05]).mean()
+    assert 2.5 < ratio < 6.0
