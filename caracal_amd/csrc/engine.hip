@@ -702,7 +702,10 @@ __global__ void child_begin_kernel(TrajArgs a, double *weight, double *den_part)
 }
 
 // num(t) += sum_traj weight * flag(t, traj), one block per step, fixed order.
-__global__ void kappa_reduce_kernel(const uint8_t *flag, const double *weight, int ntraj, double *num)
+// num[t] for t counted from the last begin_children: the run's first step is
+// child step ctr[1] (the device counter, so that a replayed graph continues).
+__global__ void kappa_reduce_kernel(const uint8_t *flag, const double *weight, int ntraj, double *num,
+                                    const uint64_t *ctr)
 {
     __shared__ double part[256];
     const int s = blockIdx.x;
@@ -714,7 +717,7 @@ __global__ void kappa_reduce_kernel(const uint8_t *flag, const double *weight, i
         if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) num[s] += part[0];
+    if (threadIdx.x == 0) num[ctr[1] + (uint64_t)s] += part[0];
 }
 
 __global__ void sum_kernel(const double *v, int n, double *out)
@@ -731,7 +734,14 @@ __global__ void sum_kernel(const double *v, int n, double *out)
     if (threadIdx.x == 0) *out += part[0];
 }
 
-__global__ void advance_kernel(uint64_t *ctr, int n) { *ctr += (uint64_t)n; }
+// ctr[0]: steps since creation (Andersen draws); ctr[1]: steps since begin_children.
+__global__ void advance_kernel(uint64_t *ctr, int n)
+{
+    ctr[0] += (uint64_t)n;
+    ctr[1] += (uint64_t)n;
+}
+
+__global__ void reset_child_kernel(uint64_t *ctr) { ctr[1] = 0; }
 
 // Redraw momenta (andersen, src/andersen.f90:37-74).
 __global__ void draw_kernel(double *p, const double *beta, int ntraj, int natoms, int nbeads, uint64_t seed,
@@ -826,9 +836,11 @@ struct cam_engine {
     double *tfwd = nullptr, *tinv = nullptr;
     double *weight = nullptr, *den_part = nullptr, *den = nullptr, *num = nullptr;
     uint8_t *xi_flag = nullptr;
-    uint64_t *step_base = nullptr;
+    uint64_t *step_base = nullptr;    // [0] steps since creation, [1] steps since begin_children
     double *scratch_io = nullptr;     // staging for layout conversion
-    int num_cap = 0;
+    int num_cap = 0;                  // doubles in num (child steps)
+    int flag_cap = 0;                 // steps of one run that xi_flag holds
+    long child_steps = 0;             // host mirror of step_base[1]
     bool have_state = false;
     int spl = 1;                      // steps per launch
     int wpe = 1;
@@ -952,20 +964,39 @@ int run_forces(cam_engine *e)
     return CAM_OK;
 }
 
-int ensure_num(cam_engine *e, int nsteps)
+void drop_graphs(cam_engine *e)
 {
-    if (nsteps <= e->num_cap) return CAM_OK;
-    if (e->num) CAM_HIP(hipFree(e->num));
-    if (e->xi_flag) CAM_HIP(hipFree(e->xi_flag));
-    e->num = nullptr;
-    e->xi_flag = nullptr;
-    CAM_HIP(hipMalloc(&e->num, (size_t)nsteps * sizeof(double)));
-    CAM_HIP(hipMalloc(&e->xi_flag, (size_t)nsteps * e->cfg.ntraj));
-    CAM_HIP(hipMemsetAsync(e->num, 0, (size_t)nsteps * sizeof(double), e->stream));
-    e->num_cap = nsteps;
-    // captured graphs refer to the old buffers
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     e->graphs.clear();
+}
+
+// Room for the flags of an nsteps run and for the numerator of child steps
+// up to 'total'; the numerator keeps its contents when it grows.  Captured
+// graphs refer to the buffers, so growth drops them.
+int ensure_child_buffers(cam_engine *e, int nsteps, long total)
+{
+    if (nsteps > e->flag_cap) {
+        if (e->xi_flag) CAM_HIP(hipFree(e->xi_flag));
+        e->xi_flag = nullptr;
+        CAM_HIP(hipMalloc(&e->xi_flag, (size_t)nsteps * e->cfg.ntraj));
+        e->flag_cap = nsteps;
+        drop_graphs(e);
+    }
+    if (total > e->num_cap) {
+        const long cap = std::max<long>(total, std::max<long>(2L * e->num_cap, 8192L));
+        CAM_CHECK_ARG(cap <= (1L << 30), "too many child steps (%ld)", total);
+        double *nn = nullptr;
+        CAM_HIP(hipMalloc(&nn, (size_t)cap * sizeof(double)));
+        CAM_HIP(hipMemsetAsync(nn, 0, (size_t)cap * sizeof(double), e->stream));
+        if (e->num) {
+            CAM_HIP(hipMemcpyAsync(nn, e->num, (size_t)e->num_cap * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+            CAM_HIP(hipStreamSynchronize(e->stream));
+            CAM_HIP(hipFree(e->num));
+        }
+        e->num = nn;
+        e->num_cap = (int)cap;
+        drop_graphs(e);
+    }
     return CAM_OK;
 }
 
@@ -1005,7 +1036,7 @@ int enqueue_run(cam_engine *e, int nsteps)
     }
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
         hipLaunchKernelGGL(kappa_reduce_kernel, dim3(nsteps), dim3(256), 0, e->stream, e->xi_flag, e->weight,
-                           e->cfg.ntraj, e->num);
+                           e->cfg.ntraj, e->num, e->step_base);
         CAM_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base, nsteps);
@@ -1110,14 +1141,14 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_HIP(hipMalloc(&e->weight, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den_part, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den, sizeof(double)));
-    CAM_HIP(hipMalloc(&e->step_base, sizeof(uint64_t)));
+    CAM_HIP(hipMalloc(&e->step_base, 2 * sizeof(uint64_t)));
     CAM_HIP(hipMalloc(&e->tfwd, (size_t)nb * nb * sizeof(double)));
     CAM_HIP(hipMalloc(&e->tinv, (size_t)nb * nb * sizeof(double)));
     std::vector<double> fwd, inv;
     nm_matrices(nb, cfg->nm_mode, fwd, inv);
     CAM_HIP(hipMemcpy(e->tfwd, fwd.data(), fwd.size() * sizeof(double), hipMemcpyHostToDevice));
     CAM_HIP(hipMemcpy(e->tinv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
-    CAM_HIP(hipMemset(e->step_base, 0, sizeof(uint64_t)));
+    CAM_HIP(hipMemset(e->step_base, 0, 2 * sizeof(uint64_t)));
     CAM_HIP(hipMemset(e->den, 0, sizeof(double)));
     CAM_HIP(hipMemset(e->weight, 0, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipEventCreate(&e->ev0));
@@ -1269,14 +1300,17 @@ int cam_engine_begin_children(cam_engine *e)
     CAM_LAUNCH_CHECK();
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, e->stream, e->den_part, e->cfg.ntraj, e->den);
     CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);
+    CAM_LAUNCH_CHECK();
     CAM_HIP(hipStreamSynchronize(e->stream));
+    e->child_steps = 0;
     return CAM_OK;
 }
 
 static int get_graph(cam_engine *e, int nsteps, hipGraphExec_t *out)
 {
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
-        int rc = ensure_num(e, nsteps);
+        int rc = ensure_child_buffers(e, nsteps, e->child_steps + nsteps);
         if (rc != CAM_OK) return rc;
     }
     auto it = e->graphs.find(nsteps);
@@ -1323,6 +1357,7 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_HIP(hipEventRecord(e->ev0, e->stream));
     CAM_HIP(hipGraphLaunch(g, e->stream));
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
+    e->child_steps += nsteps;
     e->last_launches = (nsteps + e->spl - 1) / e->spl;
     e->last_ms = -1.0;
     return CAM_OK;
@@ -1345,7 +1380,7 @@ int cam_engine_kappa(cam_engine *e, double *num, double *den, int nsteps)
 {
     CAM_CHECK_ARG(e && num && den, "null argument");
     CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_CHILD, "kappa needs ensemble CHILD");
-    CAM_CHECK_ARG(nsteps >= 0 && nsteps <= e->num_cap, "nsteps exceeds the last run (%d)", e->num_cap);
+    CAM_CHECK_ARG(nsteps >= 0 && nsteps <= e->num_cap, "nsteps exceeds the child steps recorded (%d)", e->num_cap);
     clear_error();
     if (nsteps) CAM_HIP(hipMemcpyAsync(num, e->num, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     CAM_HIP(hipMemcpyAsync(den, e->den, sizeof(double), hipMemcpyDeviceToHost, e->stream));

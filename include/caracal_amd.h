@@ -170,10 +170,12 @@ int cam_engine_run(cam_engine *eng, int nsteps);
 int cam_engine_prepare(cam_engine *eng, int nsteps);
 int cam_engine_sync(cam_engine *eng);
 
-/* Recrossing factor kappa(t) = num(t)/den accumulated since
- * cam_engine_begin_children, for t = 1..nsteps of the last run (host array of
- * nsteps doubles); den is returned separately.  Sums run over the batch in a
- * fixed order, so the result does not depend on scheduling. */
+/* Recrossing factor kappa(t) = num(t)/den.  num(t) counts child step t since
+ * the last cam_engine_begin_children (successive cam_engine_run calls
+ * continue the same children) and, like den, accumulates over successive
+ * child batches.  Host array of nsteps doubles for t = 1..nsteps; den is
+ * returned separately.  Sums run over the batch in a fixed order, so the
+ * result does not depend on scheduling. */
 int cam_engine_kappa(cam_engine *eng, double *num, double *den, int nsteps);
 
 /* Per-trajectory observables of the current state (host arrays of ntraj):

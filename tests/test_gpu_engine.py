@@ -174,6 +174,25 @@ def test_children_flags_and_kappa_match_oracle():
     e.close()
 
 
+def test_children_continue_across_runs():
+    """Two runs of 30 steps continue the same children: the kappa numerator
+    and the state equal those of one 60-step run, bit for bit."""
+    w = workloads.ch4h_children(ntraj=48, nbeads=8, seed=6)
+    out = []
+    for runs in ([60], [30, 30]):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 8, 48, w["dt"], w["masses"], ensemble=cam.CAM_ENS_CHILD,
+                           reaction=w["reaction"], seed=2, steps_per_launch=10)
+        e.set_state(w["q"], w["p"], w["beta"])
+        e.begin_children()
+        for n in runs:
+            e.run(n)
+        num, den = e.kappa(60)
+        out.append((num, den, e.get_state()[0]))
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_mixed_precision_engine_tracks_fp64():
     w = workloads.ch4h_children(ntraj=64, nbeads=16)
     p0 = np.random.default_rng(4).normal(size=w["q"].shape) * 15
