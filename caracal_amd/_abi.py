@@ -42,6 +42,17 @@ class Config(ctypes.Structure):
     ]
 
 
+CAM_MAX_TRIALS = 16
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("chosen", ctypes.c_int32),
+        ("nm_kernel", ctypes.c_int32 * CAM_MAX_TRIALS), ("steps_per_launch", ctypes.c_int32 * CAM_MAX_TRIALS),
+        ("us_per_step", ctypes.c_double * CAM_MAX_TRIALS),
+    ]
+
+
 class FFDesc(ctypes.Structure):
     _fields_ = [
         ("natoms", ctypes.c_int32), ("nbonds", ctypes.c_int32), ("nangles", ctypes.c_int32),
@@ -82,6 +93,7 @@ PROTOTYPES = {
     "cam_engine_device_state": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cam_engine_last_timing": (ctypes.c_int, [_vp, _dp, _ip, _ip]),
+    "cam_engine_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Tuning)]),
     "cam_ff_create": (ctypes.c_int, [ctypes.POINTER(FFDesc), ctypes.POINTER(_vp)]),
     "cam_ff_destroy": (ctypes.c_int, [_vp]),
     "cam_ff_energy": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),

@@ -163,9 +163,16 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-template <int NB, int TPB, int NC>
+template <int NB>
+constexpr double kSqrt()
+{
+    return NB == 1 ? 1.0 : NB == 2 ? 1.4142135623730951 : NB == 4 ? 2.0 : NB == 8 ? 2.8284271247461903
+         : NB == 16 ? 4.0 : NB == 32 ? 5.656854249492381 : 8.0;
+}
+
+template <int NB, int TPB, int NC, bool CENT>
 __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *prop, const double *tfwd,
-                                        const double *tinv, const double *mass, double dt)
+                                        const double *tinv, const double *mass, double dt, double *cent)
 {
     constexpr int NBP = NB + 1;
     constexpr int NCOL = TPB * 2 * NC;
@@ -218,6 +225,9 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
                     }
                     acc[r] = isq ? other * p3 + acc[r] * cw : acc[r] * cw + other * p2;
                 }
+                // The centroid of the propagated beads is mode 0 of q over sqrt(NB):
+                // every column of either transform sums to sqrt(NB) times delta_k0.
+                if (CENT && isq && ok && row0 == 0 && lk == 0) cent[pair] = acc[0] * (1.0 / kSqrt<NB>());
             }
             if (ok) {
 #pragma unroll
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     xm[(2 * cc[s] + 1) * (NB + 1) + k] = q[s];
                 }
             __syncthreads();
-            nm_mfma<NB, TPB, NC>(X, Y, prop, a.tfwd, a.tinv, smass, a.dt);
+            nm_mfma<NB, TPB, NC, ENS == CAM_ENS_CHILD>(X, Y, prop, a.tfwd, a.tinv, smass, a.dt, cent);
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
                 p[s] = xm[(2 * cc[s]) * (NB + 1) + k];
@@ -366,6 +376,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 yq[s] = yp[s] * p3[s] + yq[s] * p1;
                 yp[s] = pn;
             }
+            if constexpr (ENS == CAM_ENS_CHILD) {
+                // centroid of the propagated beads = mode 0 of q / sqrt(NB)
+                if (k == 0) {
+#pragma unroll
+                    for (int s = 0; s < SL; ++s)
+                        if (S::valid(s, r)) cent[tl * NC + cc[s]] = yq[s] * (1.0 / kSqrt<NB>());
+                }
+            }
 #pragma unroll
             for (int pass = 0; pass < 2; ++pass) {
                 __syncthreads();
@@ -396,23 +414,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 
         if constexpr (ENS == CAM_ENS_CHILD) {
             // centroid (src/get_centroid.f90:66-85) and xi of the child step
-            // (umbrella mode 1 -> calc_xi mode 2, src/verlet.f90:1049-1050)
-            __syncthreads();
-#pragma unroll
-            for (int s = 0; s < SL; ++s)
-                if (S::valid(s, r)) xs[cc[s] * NB + k] = q[s];
-            __syncthreads();
-            if (k == 0) {
+            // (umbrella mode 1 -> calc_xi mode 2, src/verlet.f90:1049-1050).
+            // With beads, the normal-mode step above left the centroid in cent
+            // behind a barrier; one bead is its own centroid.
+            if constexpr (NB == 1) {
+                __syncthreads();
 #pragma unroll
                 for (int s = 0; s < SL; ++s)
-                    if (S::valid(s, r)) {
-                        double acc = 0.0;
-#pragma unroll 1
-                        for (int j = 0; j < NB; ++j) acc = acc + xs[cc[s] * NB + j];
-                        cent[tl * NC + cc[s]] = acc / NB;
-                    }
+                    if (S::valid(s, r)) cent[tl * NC + cc[s]] = q[s];
+                __syncthreads();
             }
-            __syncthreads();
             if (k == 0 && r == 0 && act) {
                 const double xi = calc_xi<S::NA>(a.rx, a.mass, cent + tl * NC, nullptr);
                 a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
@@ -939,6 +950,17 @@ void nm_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double>
 
 namespace {
 
+bool select_kernels(cam_engine *e, bool mfma, step_fn &st, force_fn &fo)
+{
+    const cam_config *cfg = &e->cfg;
+    const int nb = cfg->nbeads;
+    if (cfg->surface == CAM_SURFACE_H3)
+        return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, mfma, st, fo)
+                                               : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, mfma, st, fo);
+    return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, mfma, st, fo)
+                                           : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, mfma, st, fo);
+}
+
 TrajArgs traj_args(cam_engine *e)
 {
     TrajArgs a{};
@@ -1001,7 +1023,7 @@ int ensure_child_buffers(cam_engine *e, int nsteps, long total)
 }
 
 // Enqueue one run of nsteps (used both for capture and eagerly).
-int enqueue_run(cam_engine *e, int nsteps)
+StepArgs step_args(cam_engine *e)
 {
     StepArgs a{};
     a.q = e->q;
@@ -1019,6 +1041,12 @@ int enqueue_run(cam_engine *e, int nsteps)
     a.dt = e->cfg.dt;
     std::memcpy(a.mass, e->cfg.mass, sizeof a.mass);
     a.rx = e->rx;
+    return a;
+}
+
+int enqueue_run(cam_engine *e, int nsteps)
+{
+    StepArgs a = step_args(e);
     const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
     const TrajArgs ta = traj_args(e);
     const int tgrid = (e->cfg.ntraj + 127) / 128;
@@ -1119,13 +1147,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     if (cfg->remove_com) e->spl = 1;
     e->wpe = 1;
     e->mfma = cfg->nm_kernel == CAM_NMK_MFMA;
-    bool ok;
-    if (surf == CAM_SURFACE_H3)
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force)
-                                             : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force);
-    else
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force)
-                                             : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, e->mfma, e->step, e->force);
+    const bool ok = select_kernels(e.get(), e->mfma, e->step, e->force);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
         return CAM_ERR_UNSUPPORTED;
@@ -1423,6 +1445,120 @@ int cam_engine_device_state(cam_engine *e, double **q, double **p, double **dvdq
     if (p) *p = e->p;
     if (dvdq) *dvdq = e->g;
     if (stream) *stream = (void *)e->stream;
+    return CAM_OK;
+}
+
+int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(trial_steps >= 1 && trial_steps <= 100000, "trial_steps must be in 1..100000");
+    clear_error();
+    const cam_config &c = e->cfg;
+    const int nb = c.nbeads;
+    std::vector<int> nmks, spls;
+    if (c.nm_kernel != CAM_NMK_AUTO) nmks.push_back(c.nm_kernel);
+    else {
+        nmks.push_back(CAM_NMK_VALU);
+        if (nb >= 16 && nb % 16 == 0) nmks.push_back(CAM_NMK_MFMA);
+    }
+    if (c.remove_com) spls.push_back(1);
+    else if (c.steps_per_launch > 0) spls.push_back(c.steps_per_launch);
+    else
+        for (int v : { 10, 25, 50, 100 }) spls.push_back(v);
+    // Trials run on a copy of the state with their own flag rows and step
+    // counters, so the engine's trajectories, kappa sums and draws are untouched.
+    const size_t nstate = (size_t)c.ntraj * 3 * c.natoms * nb;
+    double *buf = nullptr;
+    uint8_t *flags = nullptr;
+    uint64_t *ctr = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = CAM_OK;
+    struct Trial { int nmk, spl; double us; };
+    std::vector<Trial> trials;
+    auto fail = [&](hipError_t err) {
+        set_error("HIP error in autotune: %s", hipGetErrorString(err));
+        rc = CAM_ERR_DEVICE;
+    };
+    hipError_t err = hipMalloc(&buf, (3 * nstate + (size_t)c.ntraj * nb) * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&flags, (size_t)trial_steps * c.ntraj);
+    if (err == hipSuccess) err = hipMalloc(&ctr, 2 * sizeof(uint64_t));
+    if (err == hipSuccess) err = hipEventCreate(&t0);
+    if (err == hipSuccess) err = hipEventCreate(&t1);
+    if (err != hipSuccess) fail(err);
+    for (int nmk : nmks) {
+        if (rc != CAM_OK) break;
+        step_fn st;
+        force_fn fo;
+        if (!select_kernels(e, nmk == CAM_NMK_MFMA, st, fo)) continue;
+        for (int spl : spls) {
+            StepArgs a = step_args(e);
+            a.q = buf;
+            a.p = buf + nstate;
+            a.g = buf + 2 * nstate;
+            a.vbead = buf + 3 * nstate;
+            a.xi_flag = flags;
+            a.step_base = ctr;
+            const int grid = (c.ntraj + e->tpb - 1) / e->tpb;
+            err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
+            if (err == hipSuccess) err = hipMemcpyAsync(a.q, e->q, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+            if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+            if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+            if (err != hipSuccess) {
+                fail(err);
+                break;
+            }
+            // one untimed launch, then the timed trial
+            a.nsteps = std::min(spl, trial_steps);
+            a.step_off = 0;
+            hipLaunchKernelGGL(st, dim3(grid), dim3(kBlock), 0, e->stream, a);
+            err = hipGetLastError();
+            if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
+            for (int done = 0; err == hipSuccess && done < trial_steps; done += a.nsteps) {
+                a.nsteps = std::min(spl, trial_steps - done);
+                a.step_off = done;
+                hipLaunchKernelGGL(st, dim3(grid), dim3(kBlock), 0, e->stream, a);
+                err = hipGetLastError();
+            }
+            if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(t1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+            if (err != hipSuccess) {
+                fail(err);
+                break;
+            }
+            trials.push_back({ nmk, spl, 1e3 * ms / trial_steps });
+        }
+    }
+    if (buf) (void)hipFree(buf);
+    if (flags) (void)hipFree(flags);
+    if (ctr) (void)hipFree(ctr);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (rc != CAM_OK) return rc;
+    CAM_CHECK_ARG(!trials.empty(), "no candidate schedule");
+    size_t best = 0;
+    for (size_t i = 1; i < trials.size(); ++i)
+        if (trials[i].us < trials[best].us) best = i;
+    step_fn st;
+    force_fn fo;
+    select_kernels(e, trials[best].nmk == CAM_NMK_MFMA, st, fo);
+    e->mfma = trials[best].nmk == CAM_NMK_MFMA;
+    e->step = st;
+    e->force = fo;
+    e->spl = trials[best].spl;
+    drop_graphs(e);
+    if (out) {
+        std::memset(out, 0, sizeof *out);
+        out->n = (int)std::min<size_t>(trials.size(), CAM_MAX_TRIALS);
+        out->chosen = (int)best;
+        for (int i = 0; i < out->n; ++i) {
+            out->nm_kernel[i] = trials[i].nmk;
+            out->steps_per_launch[i] = trials[i].spl;
+            out->us_per_step[i] = trials[i].us;
+        }
+    }
     return CAM_OK;
 }
 

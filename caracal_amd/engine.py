@@ -150,6 +150,15 @@ class RPMDEngine:
               lib().cam_engine_observables(self._h, _ptr(ep), _ptr(ek), _ptr(es), _ptr(xi), fl.ctypes.data_as(_ip)))
         return dict(epot=ep, ekin=ek, espring=es, xi=xi, flags=fl)
 
+    def autotune(self, trial_steps=100):
+        """Time the candidate schedules on a copy of the state and keep the
+        fastest; returns the trials as a list of dicts and the chosen one."""
+        t = _abi.Tuning()
+        check("cam_engine_autotune", lib().cam_engine_autotune(self._h, trial_steps, ctypes.byref(t)))
+        trials = [dict(nm_kernel=t.nm_kernel[i], steps_per_launch=t.steps_per_launch[i], us_per_step=t.us_per_step[i])
+                  for i in range(t.n)]
+        return trials, trials[t.chosen]
+
     def last_timing(self):
         ms = ctypes.c_double()
         spl = ctypes.c_int()

@@ -46,6 +46,13 @@ module caracal_amd_mod
       type(cam_reaction) :: reaction
    end type cam_config
 
+   integer, parameter, public :: CAM_MAX_TRIALS = 16
+   type, bind(C), public :: cam_tuning
+      integer(c_int32_t) :: n = 0, chosen = 0
+      integer(c_int32_t) :: nm_kernel(CAM_MAX_TRIALS) = 0, steps_per_launch(CAM_MAX_TRIALS) = 0
+      real(c_double) :: us_per_step(CAM_MAX_TRIALS) = 0.d0
+   end type cam_tuning
+
    interface
       function cam_last_error() bind(C, name="cam_last_error") result(msg)
          import :: c_ptr
@@ -141,13 +148,26 @@ module caracal_amd_mod
          integer(c_int), intent(out) :: flags(*)
          integer(c_int) :: rc
       end function
+      function cam_engine_prepare(eng, nsteps) bind(C, name="cam_engine_prepare") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int), value :: nsteps
+         integer(c_int) :: rc
+      end function
+      function cam_engine_autotune(eng, trial_steps, tuning) bind(C, name="cam_engine_autotune") result(rc)
+         import :: c_int, c_ptr, cam_tuning
+         type(c_ptr), value :: eng
+         integer(c_int), value :: trial_steps
+         type(cam_tuning), intent(out) :: tuning
+         integer(c_int) :: rc
+      end function
    end interface
 
    public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_run, cam_engine_sync, cam_engine_kappa
-   public :: cam_engine_observables
+   public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune
    public :: egrad_h3_gpu, egrad_ch4h_gpu, cam_error_message
 
 contains

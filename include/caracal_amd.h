@@ -190,6 +190,23 @@ int cam_engine_observables(cam_engine *eng, double *epot, double *ekin, double *
  * layout [traj][3*natoms][nbeads] and the hipStream_t it runs on. */
 int cam_engine_device_state(cam_engine *eng, double **q, double **p, double **dvdq, void **stream);
 
+/* Schedule autotuner.  Times candidate schedules of the step kernel on a
+ * copy of the current state (the engine's trajectories, kappa sums and RNG
+ * counters are not touched) for trial_steps steps each, and keeps the fastest:
+ * the normal-mode unit (VALU or MFMA; only when nm_kernel is AUTO) and the
+ * steps fused per launch (10, 25, 50, 100; only when steps_per_launch is 0
+ * and remove_com is off).  Captured graphs are dropped.  No counterpart in
+ * the reference, which has one schedule. */
+#define CAM_MAX_TRIALS 16
+typedef struct {
+    int32_t n;                              /* candidates timed */
+    int32_t chosen;                         /* index of the schedule kept */
+    int32_t nm_kernel[CAM_MAX_TRIALS];      /* CAM_NMK_VALU or CAM_NMK_MFMA */
+    int32_t steps_per_launch[CAM_MAX_TRIALS];
+    double us_per_step[CAM_MAX_TRIALS];     /* microseconds per step of the whole batch */
+} cam_tuning;
+int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);
+
 /* Mean duration in milliseconds of the step kernel launches of the last
  * cam_engine_run, measured with HIP events on the engine's stream, and the
  * number of steps one launch advances. */

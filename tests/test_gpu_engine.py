@@ -193,6 +193,33 @@ def test_children_continue_across_runs():
     assert np.array_equal(out[0][2], out[1][2])
 
 
+def test_autotune_leaves_the_state_and_keeps_parity():
+    """The autotuner times schedules on a copy: state, kappa sums and step
+    counters are untouched, and the schedule it keeps gives the same
+    trajectory as the default one to rounding."""
+    w = workloads.ch4h_children(ntraj=64, nbeads=16, seed=7)
+    res = []
+    for tune in (False, True):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 16, 64, w["dt"], w["masses"], ensemble=cam.CAM_ENS_CHILD,
+                           reaction=w["reaction"], seed=2)
+        e.set_state(w["q"], w["p"], w["beta"])
+        e.begin_children()
+        q0, p0 = e.get_state()
+        if tune:
+            trials, best = e.autotune(20)
+            print("autotune trials", trials, "chosen", best)
+            assert len(trials) == 8 and best in trials
+            q1, p1 = e.get_state()
+            assert np.array_equal(q0, q1) and np.array_equal(p0, p1)
+        e.run(60)
+        res.append((e.get_state()[0], e.kappa(60)))
+        e.close()
+    (qa, (na, da)), (qb, (nb_, db)) = res
+    assert np.abs(qa - qb).max() <= 1e-11 * np.abs(qa).max()
+    assert da == db
+    np.testing.assert_allclose(na, nb_, rtol=0, atol=1e-12 * max(1.0, np.abs(na).max()))
+
+
 def test_mixed_precision_engine_tracks_fp64():
     w = workloads.ch4h_children(ntraj=64, nbeads=16)
     p0 = np.random.default_rng(4).normal(size=w["q"].shape) * 15
