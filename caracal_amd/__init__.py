@@ -65,4 +65,4 @@ def egrad_dev(surface, q_ptr, nbeads, v_ptr, g_ptr, precision=CAM_PREC_FP64, str
     check(name, getattr(lib(), name)(q_ptr, na, nbeads, v_ptr, g_ptr, precision, stream))
 
 
-from .engine import RPMDEngine, Reaction  # noqa: E402,F401
+from .engine import RaggedBatch, RPMDEngine, Reaction, make_config  # noqa: E402,F401

@@ -94,6 +94,12 @@ PROTOTYPES = {
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cam_engine_last_timing": (ctypes.c_int, [_vp, _dp, _ip, _ip]),
     "cam_engine_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "cam_batch_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cam_batch_destroy": (ctypes.c_int, [_vp]),
+    "cam_batch_engine": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cam_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_batch_sync": (ctypes.c_int, [_vp, _dp]),
     "cam_ff_create": (ctypes.c_int, [ctypes.POINTER(FFDesc), ctypes.POINTER(_vp)]),
     "cam_ff_destroy": (ctypes.c_int, [_vp]),
     "cam_ff_energy": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),

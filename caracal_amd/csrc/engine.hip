@@ -1562,6 +1562,126 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     return CAM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Ragged batches: one engine per (surface, nbeads) bucket, each on its own
+// stream, launched longest-first and joined on a batch stream.
+// ---------------------------------------------------------------------------
+struct cam_batch {
+    std::vector<cam_engine *> eng;
+    std::vector<int> order;            // launch order, decreasing work estimate
+    std::vector<hipEvent_t> done;
+    hipStream_t stream = nullptr;      // join stream
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool pending = false;
+    double last_ms = 0.0;
+};
+
+int cam_batch_destroy(cam_batch *b)
+{
+    if (!b) return CAM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (cam_engine *e : b->eng) cam_engine_destroy(e);
+    for (hipEvent_t ev : b->done)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->start) (void)hipEventDestroy(b->start);
+    if (b->stop) (void)hipEventDestroy(b->stop);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CAM_OK;
+}
+
+int cam_batch_create(const cam_config *cfgs, int nbuckets, cam_batch **out)
+{
+    CAM_CHECK_ARG(cfgs && out, "null argument");
+    CAM_CHECK_ARG(nbuckets >= 1 && nbuckets <= 1024, "nbuckets must be in 1..1024 (got %d)", nbuckets);
+    *out = nullptr;
+    cam_batch *b = new cam_batch();
+    std::vector<double> work(nbuckets);
+    for (int i = 0; i < nbuckets; ++i) {
+        cam_engine *e = nullptr;
+        const int rc = cam_engine_create(&cfgs[i], &e);
+        if (rc != CAM_OK) {
+            cam_batch_destroy(b);
+            return rc;
+        }
+        b->eng.push_back(e);
+        // flops of one step with elementary functions weighted 20 flops each
+        const double bead = cfgs[i].surface == CAM_SURFACE_H3 ? 1438.0 + 20.0 * 22 : 12707.0 + 20.0 * 321;
+        work[i] = (double)cfgs[i].ntraj * cfgs[i].nbeads * bead;
+    }
+    b->order.resize(nbuckets);
+    for (int i = 0; i < nbuckets; ++i) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    b->done.assign(nbuckets, nullptr);
+    hipError_t err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreate(&b->start);
+    if (err == hipSuccess) err = hipEventCreate(&b->stop);
+    for (int i = 0; err == hipSuccess && i < nbuckets; ++i) err = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming);
+    if (err != hipSuccess) {
+        set_error("HIP error: %s", hipGetErrorString(err));
+        cam_batch_destroy(b);
+        return CAM_ERR_DEVICE;
+    }
+    *out = b;
+    return CAM_OK;
+}
+
+int cam_batch_engine(cam_batch *b, int i, cam_engine **eng)
+{
+    CAM_CHECK_ARG(b && eng, "null argument");
+    CAM_CHECK_ARG(i >= 0 && i < (int)b->eng.size(), "bucket %d out of range", i);
+    *eng = b->eng[i];
+    return CAM_OK;
+}
+
+int cam_batch_autotune(cam_batch *b, int trial_steps)
+{
+    CAM_CHECK_ARG(b, "null batch");
+    for (cam_engine *e : b->eng) {
+        const int rc = cam_engine_autotune(e, trial_steps, nullptr);
+        if (rc != CAM_OK) return rc;
+    }
+    return CAM_OK;
+}
+
+int cam_batch_run(cam_batch *b, int nsteps)
+{
+    CAM_CHECK_ARG(b, "null batch");
+    CAM_CHECK_ARG(nsteps >= 0, "nsteps must be >= 0");
+    clear_error();
+    CAM_HIP(hipEventRecord(b->start, b->stream));
+    for (int i : b->order) {
+        cam_engine *e = b->eng[i];
+        CAM_HIP(hipStreamWaitEvent(e->stream, b->start, 0));
+        const int rc = cam_engine_run(e, nsteps);
+        if (rc != CAM_OK) return rc;
+        CAM_HIP(hipEventRecord(b->done[i], e->stream));
+        CAM_HIP(hipStreamWaitEvent(b->stream, b->done[i], 0));
+    }
+    CAM_HIP(hipEventRecord(b->stop, b->stream));
+    b->pending = true;
+    return CAM_OK;
+}
+
+int cam_batch_sync(cam_batch *b, double *ms)
+{
+    CAM_CHECK_ARG(b, "null batch");
+    clear_error();
+    CAM_HIP(hipStreamSynchronize(b->stream));
+    if (b->pending) {
+        float t = 0.f;
+        CAM_HIP(hipEventElapsedTime(&t, b->start, b->stop));
+        b->last_ms = t;
+        b->pending = false;
+        for (cam_engine *e : b->eng) {
+            const int rc = cam_engine_sync(e);
+            if (rc != CAM_OK) return rc;
+        }
+    }
+    if (ms) *ms = b->last_ms;
+    return CAM_OK;
+}
+
 int cam_engine_last_timing(cam_engine *e, double *ms_per_launch, int *steps_per_launch, int *launches)
 {
     CAM_CHECK_ARG(e, "null engine");

@@ -46,6 +46,28 @@ class Reaction:
         return r
 
 
+def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64, nm_mode=_abi.CAM_NM_REFERENCE,
+                ensemble=_abi.CAM_ENS_NVE, remove_com=False, andersen_step=0, steps_per_launch=0, seed=0,
+                reaction=None, nm_kernel=_abi.CAM_NMK_AUTO):
+    """cam_config of one engine (or of one bucket of a batch)."""
+    natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
+    masses = np.asarray(masses, dtype=np.float64)
+    if masses.shape != (natoms,):
+        raise ValueError(f"need {natoms} masses")
+    c = _abi.Config()
+    c.surface, c.natoms, c.nbeads, c.ntraj = surface, natoms, nbeads, ntraj
+    c.precision, c.nm_mode, c.ensemble = precision, nm_mode, ensemble
+    c.remove_com, c.andersen_step, c.steps_per_launch = int(remove_com), andersen_step, steps_per_launch
+    c.nm_kernel = nm_kernel
+    c.dt = dt
+    for i, m in enumerate(masses):
+        c.mass[i] = m
+    c.seed = seed
+    if reaction is not None:
+        c.reaction = reaction.to_c()
+    return c
+
+
 class RPMDEngine:
     """A batch of ntraj ring polymers of nbeads beads on one analytic surface,
     resident on the GPU.  State arrays have shape (ntraj, nbeads, natoms, 3)."""
@@ -53,31 +75,30 @@ class RPMDEngine:
     def __init__(self, surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64,
                  nm_mode=_abi.CAM_NM_REFERENCE, ensemble=_abi.CAM_ENS_NVE, remove_com=False,
                  andersen_step=0, steps_per_launch=0, seed=0, reaction=None, nm_kernel=_abi.CAM_NMK_AUTO):
-        natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
-        masses = np.asarray(masses, dtype=np.float64)
-        if masses.shape != (natoms,):
-            raise ValueError(f"need {natoms} masses")
-        c = _abi.Config()
-        c.surface, c.natoms, c.nbeads, c.ntraj = surface, natoms, nbeads, ntraj
-        c.precision, c.nm_mode, c.ensemble = precision, nm_mode, ensemble
-        c.remove_com, c.andersen_step, c.steps_per_launch = int(remove_com), andersen_step, steps_per_launch
-        c.nm_kernel = nm_kernel
-        c.dt = dt
-        for i, m in enumerate(masses):
-            c.mass[i] = m
-        c.seed = seed
-        if reaction is not None:
-            c.reaction = reaction.to_c()
-        self.cfg = c
-        self.natoms, self.nbeads, self.ntraj = natoms, nbeads, ntraj
+        c = make_config(surface, nbeads, ntraj, dt, masses, precision=precision, nm_mode=nm_mode,
+                        ensemble=ensemble, remove_com=remove_com, andersen_step=andersen_step,
+                        steps_per_launch=steps_per_launch, seed=seed, reaction=reaction, nm_kernel=nm_kernel)
         h = ctypes.c_void_p()
         check("cam_engine_create", lib().cam_engine_create(ctypes.byref(c), ctypes.byref(h)))
-        self._h = h
+        self._attach(c, h, owned=True)
+
+    def _attach(self, cfg, handle, owned):
+        self.cfg = cfg
+        self.natoms, self.nbeads, self.ntraj = cfg.natoms, cfg.nbeads, cfg.ntraj
+        self._h = handle
+        self._owned = owned
+
+    @classmethod
+    def _borrowed(cls, cfg, handle):
+        """An engine owned by a batch (closing it is a no-op)."""
+        e = cls.__new__(cls)
+        e._attach(cfg, handle, owned=False)
+        return e
 
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
+        if getattr(self, "_h", None) is not None and self._h.value and getattr(self, "_owned", False):
             lib().cam_engine_destroy(self._h)
-            self._h = None
+        self._h = None
 
     __del__ = close
 
@@ -166,3 +187,50 @@ class RPMDEngine:
         check("cam_engine_last_timing",
               lib().cam_engine_last_timing(self._h, ctypes.byref(ms), ctypes.byref(spl), ctypes.byref(nl)))
         return ms.value, spl.value, nl.value
+
+
+class RaggedBatch:
+    """Trajectories of different surfaces and bead counts as one batch: one
+    engine per bucket, run concurrently by the native batch layer.
+    buckets: list of (surface, nbeads, dict(q, p, beta, masses, dt)) as
+    workloads.ragged returns; every bucket gets the same ensemble options."""
+
+    def __init__(self, buckets, **options):
+        self.buckets = buckets
+        cfgs = (_abi.Config * len(buckets))()
+        for i, (surf, nb, d) in enumerate(buckets):
+            cfgs[i] = make_config(surf, nb, len(d["beta"]), d["dt"], d["masses"], **options)
+        h = ctypes.c_void_p()
+        check("cam_batch_create", lib().cam_batch_create(cfgs, len(buckets), ctypes.byref(h)))
+        self._h = h
+        self.engines = []
+        for i in range(len(buckets)):
+            eh = ctypes.c_void_p()
+            check("cam_batch_engine", lib().cam_batch_engine(h, i, ctypes.byref(eh)))
+            self.engines.append(RPMDEngine._borrowed(cfgs[i], eh))
+
+    def set_states(self):
+        for e, (_, _, d) in zip(self.engines, self.buckets):
+            e.set_state(d["q"], d["p"], d["beta"])
+
+    def autotune(self, trial_steps=50):
+        check("cam_batch_autotune", lib().cam_batch_autotune(self._h, trial_steps))
+
+    def run(self, nsteps, sync=True):
+        check("cam_batch_run", lib().cam_batch_run(self._h, nsteps))
+        if sync:
+            return self.sync()
+
+    def sync(self):
+        ms = ctypes.c_double()
+        check("cam_batch_sync", lib().cam_batch_sync(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            for e in self.engines:
+                e._h = None
+            lib().cam_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close

@@ -212,6 +212,24 @@ int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);
  * number of steps one launch advances. */
 int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_per_launch, int *launches);
 
+/* Ragged batches (trajectories of different surfaces and bead counts): one
+ * engine per bucket of equal (surface, nbeads), each on its own stream.
+ * cam_batch_run launches every bucket's graph, longest bucket first by a
+ * work estimate (trajectories x beads x surface flops), and joins them; the
+ * buckets run concurrently on the GPU.  Engines are reached with
+ * cam_batch_engine to set state and read results; the batch owns them.  No
+ * counterpart in the reference, which runs one trajectory per MPI rank
+ * (src/recross.f90:334-415). */
+typedef struct cam_batch cam_batch;
+int cam_batch_create(const cam_config *cfgs, int nbuckets, cam_batch **out);
+int cam_batch_destroy(cam_batch *batch);
+int cam_batch_engine(cam_batch *batch, int bucket, cam_engine **eng);
+int cam_batch_autotune(cam_batch *batch, int trial_steps);
+int cam_batch_run(cam_batch *batch, int nsteps);
+/* Waits for the last run; ms = its GPU time from the first launch to the
+ * last bucket's end (HIP events). */
+int cam_batch_sync(cam_batch *batch, double *ms);
+
 /* ---- (3) QMDFF-form periodic box: energy kernel, hand-written adjoint and
  * a ring-polymer engine ----
  * Terms follow the reference's QMDFF force field: bond stretch and bend of
