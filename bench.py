@@ -9,12 +9,15 @@ recrossing-factor numerator every step, all steps of a run in one HIP graph.
 A step is one velocity-Verlet RPMD step of every trajectory.
 
 Printed: one JSON line.  value = bead-force-evals/s of the batched FP64 run
-(inputs resident in HBM).  Also reported: the same batch in mixed precision
-(FP32 force arithmetic, FP64 state and accumulation) with its force error,
-the plain FP64 one-trajectory-per-launch form, the roofline of the step
-kernel and a CPU baseline.
+(inputs resident in HBM, schedule chosen by the engine's autotuner).  Also
+reported: the same batch in mixed precision (FP32 force arithmetic, FP64
+state and accumulation) with its force error and energy drift, the plain
+FP64 one-trajectory-per-launch form, the roofline of the step kernel, a CPU
+baseline, and the other workloads: a ragged batch of 8192 trajectories (H3
+and CH4H, 4..64 beads), a 3000-atom box with 16 beads (Andersen NVT) and 50
+Adam steps of force matching on that box with 64 beads.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--skip-extras]
 """
 import argparse
 import json
@@ -43,11 +46,13 @@ def step_flops(natoms, nbeads):
     return 4 * nc + 24 * natoms * nbeads ** 2 + 6 * nc + nbeads * CH4H_FLOPS_PER_BEAD
 
 
-def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0):
+def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0,
+                     nm_kernel=None):
     w = workloads.ch4h_children(ntraj=ntraj, nbeads=nbeads, seed=seed)
     e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nbeads, ntraj, w["dt"], w["masses"], precision=precision,
                        ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"], seed=seed + 17,
-                       steps_per_launch=steps_per_launch)
+                       steps_per_launch=steps_per_launch,
+                       nm_kernel=cam.CAM_NMK_AUTO if nm_kernel is None else nm_kernel)
     e.set_state(w["q"], w["p"], w["beta"])
     st = e.constrain(project_p=False)
     if st.any():
@@ -86,28 +91,119 @@ def force_error_mixed(cam, workloads, ntraj, nbeads):
     return float(num.max() / den.max()), float(np.max(num / den))
 
 
-def energy_drift_mixed(cam, workloads, nsteps=2000):
-    """NVE drift of the ring-polymer energy (hartree per ps per trajectory) in
-    mixed precision, orthogonal normal modes, 64 CH4H trajectories."""
-    w = workloads.ch4h_children(ntraj=64, nbeads=32, seed=5)
-    e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 32, 64, w["dt"], w["masses"], precision=cam.CAM_PREC_MIXED,
-                       nm_mode=cam.CAM_NM_ORTHOGONAL)
-    rng = np.random.default_rng(5)
-    p = rng.normal(size=w["q"].shape) * np.sqrt(w["masses"][None, None, :, None] * 32 / w["beta"][:, None, None, None])
-    e.set_state(w["q"], p, w["beta"])
-    ts, hs = [], []
-    chunk = 100
-    for i in range(nsteps // chunk + 1):
-        o = e.observables()
-        ts.append(i * chunk * w["dt"])
-        hs.append(o["epot"] + o["ekin"] + o["espring"])
-        if i < nsteps // chunk:
-            e.run(chunk)
-    hs = np.array(hs)
-    t_ps = np.array(ts) * 2.4188843265857e-5
-    slopes = np.polyfit(t_ps, hs - hs[0], 1)[0]
-    e.close()
-    return float(np.max(np.abs(slopes)))
+def energy_drift(cam, workloads, nsteps=20000):
+    """NVE drift of the ring-polymer energy (hartree per ps per trajectory):
+    64 CH4H trajectories of 32 beads, orthogonal normal modes, 2 ps at 0.1 fs,
+    FP64 and mixed.  Per trajectory, the slope of a least-squares line through
+    H(t) sampled every 100 steps; reported as the mean of |slope| and as the
+    slope of the ensemble-mean energy."""
+    out = {}
+    for name, prec in (("fp64", cam.CAM_PREC_FP64), ("mixed", cam.CAM_PREC_MIXED)):
+        w = workloads.ch4h_children(ntraj=64, nbeads=32, seed=5)
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, 32, 64, w["dt"], w["masses"], precision=prec,
+                           nm_mode=cam.CAM_NM_ORTHOGONAL)
+        rng = np.random.default_rng(5)
+        p = rng.normal(size=w["q"].shape) * np.sqrt(w["masses"][None, None, :, None] * 32 / w["beta"][:, None, None, None])
+        e.set_state(w["q"], p, w["beta"])
+        chunk = 100
+        ts, hs = [], []
+        for i in range(nsteps // chunk + 1):
+            o = e.observables()
+            ts.append(i * chunk * w["dt"])
+            hs.append(o["epot"] + o["ekin"] + o["espring"])
+            if i < nsteps // chunk:
+                e.run(chunk)
+        e.close()
+        hs = np.array(hs)
+        t_ps = np.array(ts) * 2.4188843265857e-5
+        slopes = np.polyfit(t_ps, hs - hs[0], 1)[0]
+        out[name] = {"mean_abs_drift_eh_per_ps": float(np.abs(slopes).mean()),
+                     "ensemble_drift_eh_per_ps": float(np.polyfit(t_ps, (hs - hs[0]).mean(axis=1), 1)[0]),
+                     "rms_fluctuation_eh": float((hs - hs[0]).std(axis=0).mean()), "t_ps": float(t_ps[-1])}
+    return out
+
+
+def ragged_batch(cam, workloads, steps=200):
+    """8192 trajectories, half H3 and half CH4H, nbeads uniform on {4..64},
+    T uniform on [200, 1000] K, NVE, as one concurrent batch (one engine per
+    bucket).  Throughput in FP64 and mixed; mixed-vs-FP64 force error per
+    (surface, nbeads) bucket on the start state."""
+    import torch
+    buckets = workloads.ragged(ntraj=8192, seed=1)
+    beads = sum(len(d["beta"]) * nb for _, nb, d in buckets)
+    out, forces = {}, {}
+    for name, prec in (("fp64", cam.CAM_PREC_FP64), ("mixed", cam.CAM_PREC_MIXED)):
+        rb = cam.RaggedBatch(buckets, precision=prec)
+        rb.set_states()
+        forces[name] = [e.get_forces()[1] for e in rb.engines]
+        rb.autotune(20)
+        for e in rb.engines:
+            e.prepare(steps)
+        rb.run(steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms = rb.run(steps)
+        wall = time.perf_counter() - t0
+        rb.close()
+        out[name] = {"bead_force_evals_per_s": beads * steps / wall, "ms_gpu": ms, "wall_s": wall}
+    err = {}
+    for i, (surf, nb, _) in enumerate(buckets):
+        a, b = forces["fp64"][i], forces["mixed"][i]
+        num = np.abs(a - b).reshape(a.shape[0] * a.shape[1], -1).max(axis=1)
+        den = np.abs(a).reshape(a.shape[0] * a.shape[1], -1).max(axis=1)
+        err[f"{'H3' if surf == cam.CAM_SURFACE_H3 else 'CH4H'}/{nb}"] = float(num.max() / den.max())
+    out["rel_force_err_per_bucket"] = err
+    out["trajectories"], out["beads"], out["steps"] = 8192, beads, steps
+    return out
+
+
+def box_nvt(cam, steps=500):
+    """3000-atom box (1000 bent triatomics, 31 angstrom, 10 angstrom cutoff),
+    16 beads, Andersen NVT at 300 K (all beads redrawn every 100 steps),
+    0.1 fs: bead-force-evals/s where one evaluation is one bead of the box."""
+    import torch
+    from caracal_amd import ff as cff, ffbox
+    s = ffbox.box_system(nmol=1000, box_angstrom=31.0, seed=2)
+    ff = cff.ForceField(s, rc_angstrom=10.0)
+    m = ffbox.masses(s["type"])
+    beta = float(cam.beta_of(300.0))
+    nb = 16
+    rng = np.random.default_rng(1)
+    q = s["x"][None] + rng.normal(0, 0.02, (nb,) + s["x"].shape)
+    p = rng.normal(size=q.shape) * np.sqrt(m[None, :, None] * nb / beta)
+    out = {}
+    for name, prec in (("fp64", cam.CAM_PREC_FP64), ("mixed", cam.CAM_PREC_MIXED)):
+        eng = cff.BoxEngine(ff, nb, 0.1 * cam.FS_TO_AU, m, beta, ffbox.theta_ref(), precision=prec,
+                            nm_mode=cam.CAM_NM_REFERENCE, andersen_step=100, seed=5)
+        eng.set_state(q, p)
+        eng.prepare(steps)
+        eng.run(10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms = eng.run(steps)
+        wall = time.perf_counter() - t0
+        eng.close()
+        out[name] = {"bead_force_evals_per_s": nb * steps / wall, "ms_per_step": wall / steps * 1e3, "ms_gpu": ms}
+    out["atoms"], out["nbeads"], out["steps"] = len(s["type"]), nb, steps
+    ff.close()
+    return out
+
+
+def force_matching(cam):
+    """50 Adam steps of force matching on the 3000-atom box with 64 beads,
+    from the reference parameters perturbed by 10% (seed 3), gradients from
+    the adjoint kernel."""
+    import torch
+    from caracal_amd import ff as cff, ffbox, fit
+    s, xb = fit.default_problem(nbeads=64)
+    ff = cff.ForceField(s, rc_angstrom=10.0)
+    x = torch.tensor(xb, device="cuda")
+    th_ref = ffbox.theta_ref()
+    res = fit.force_match(ff, x, th_ref, fit.perturbed_theta(th_ref, 0.10, seed=3), steps=50, lr=0.02)
+    ff.close()
+    return {"loss_first": res["loss"][0], "loss_last": res["loss"][-1], "ms_per_adam_step": res["s_per_step"] * 1e3,
+            "max_rel_param_err_start": res["rel_param_err0"], "max_rel_param_err_end": res["rel_param_err"],
+            "nbeads": 64, "atoms": len(s["type"])}
 
 
 def plain_fp64(cam, workloads, steps):
@@ -187,6 +283,7 @@ def main():
 
     ntraj, nb = args.ntraj, args.nbeads
     e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
+    trials, best = e.autotune(100)
     wall, ms_launch, spl, nl = timed_run(e, args.steps, args.warmup)
     num, den = e.kappa(args.steps)
     e.close()
@@ -195,6 +292,8 @@ def main():
     steps_per_s = ntraj * args.steps / wall
     flops_launch = step_flops(6, nb) * ntraj * spl
     achieved = flops_launch / (ms_launch * 1e-3) / 1e12
+    mf = best["nm_kernel"] == cam.CAM_NMK_MFMA
+    kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes>"
 
     out = {
         "metric": "bead-force-evals/s (batched RPMD children, CH4H, 32 beads x 4096 trajectories)",
@@ -211,28 +310,34 @@ def main():
         "data": "synthetic (seeded saddle perturbations, Maxwell-Boltzmann momenta)",
         "config": {"workload": "ch4h_children", "surface": "CH5 (egrad_ch4h)", "ntraj": ntraj, "nbeads": nb,
                    "natoms": 6, "dt_fs": 0.1, "kelvin": 500, "precision": "fp64",
-                   "nm_mode": "reference cosine transform", "steps_per_launch": spl},
+                   "nm_mode": "reference cosine transform", "steps_per_launch": spl,
+                   "nm_kernel": "mfma" if mf else "valu"},
         "rpmd_steps_per_s": steps_per_s,
         "kappa_final": float(num[-1] / den) if den > 0 else None,
+        "autotune": trials,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "step_kernel<SurfCH4H,32,double,CHILD>", "ms_per_launch": ms_launch,
-                     "flops_per_launch": flops_launch,
+                     "kernel": kernel, "ms_per_launch": ms_launch, "flops_per_launch": flops_launch,
                      "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
     }
     if not args.skip_extras:
         e, _ = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_MIXED)
-        wall_m, ms_m, _, _ = timed_run(e, args.steps, args.warmup)
+        mtrials, mbest = e.autotune(100)
+        wall_m, ms_m, spl_m, _ = timed_run(e, args.steps, args.warmup)
         e.close()
         mixed_value = bead_evals / wall_m
         plain = plain_fp64(cam, workloads, 200)
         ferr, ferr_bead = force_error_mixed(cam, workloads, 256, nb)
-        out["mixed"] = {"value": mixed_value, "dtype": "fp32 forces / fp64 state",
-                        "rel_force_err": ferr, "rel_force_err_worst_bead": ferr_bead,
-                        "nve_energy_drift_eh_per_ps": energy_drift_mixed(cam, workloads)}
+        out["mixed"] = {"value": mixed_value, "rpmd_steps_per_s": ntraj * args.steps / wall_m,
+                        "dtype": "fp32 forces / fp64 state", "rel_force_err": ferr,
+                        "rel_force_err_worst_bead": ferr_bead, "schedule": mbest}
+        out["energy_drift"] = energy_drift(cam, workloads)
         out["plain_fp64_one_traj_per_launch"] = {"value": plain}
         out["speedup_mixed_vs_plain"] = mixed_value / plain
         out["speedup_fp64_batched_vs_plain"] = value / plain
+        out["ragged"] = ragged_batch(cam, workloads)
+        out["box"] = box_nvt(cam)
+        out["force_matching"] = force_matching(cam)
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 
