@@ -94,6 +94,8 @@ PROTOTYPES = {
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cam_engine_last_timing": (ctypes.c_int, [_vp, _dp, _ip, _ip]),
     "cam_engine_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(Tuning)]),
+    "cam_water_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(_vp)]),
     "cam_batch_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(_vp)]),
     "cam_batch_destroy": (ctypes.c_int, [_vp]),
     "cam_batch_engine": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),

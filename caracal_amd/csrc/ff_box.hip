@@ -16,6 +16,7 @@
 // q(3,natoms,nbeads)).  The nonbonded kernel tiles j-atoms through LDS
 // (64 per tile) and splits the j range over several blocks; every partial
 // sum is reduced in a fixed order, so results are bitwise reproducible.
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -393,12 +394,193 @@ __global__ void ff_reduce_kernel(const double *epart, int nblk_nb, int nblk_b, i
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The reference's flexible water model (src/egrad_water.f90:36-333), atoms in
+// the order O, H, H per molecule.  Intermolecular pairs: Coulomb, plain or
+// the Zahn form, with the reference's gradient e0/r^2 * d (the derivative of
+// q_i q_j / r); LJ whenever the lower-index atom of the pair is an oxygen
+// (the reference tests name(i) twice, :293).  With periodic boxes: minimum
+// image and the cutoff r < rc on both terms.
+// ---------------------------------------------------------------------------
+struct WaterDev {
+    int natoms, periodic, zahn;
+    double box, rc, za, zp;
+    double r0, r0hh, de, a, khh, krt, krr, qh, qo, sig, eps;
+};
+
+template <typename T, bool GRAD>
+__global__ __launch_bounds__(kTile) void water_nb_kernel(WaterDev w, const double *__restrict__ x, int nsplit,
+                                                         double *__restrict__ gpart, double *__restrict__ epart)
+{
+    __shared__ T sx[kTile][3];
+    __shared__ double red[kTile];
+    const int n = w.natoms;
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * kTile + tid;
+    const int split = blockIdx.y, b = blockIdx.z;
+    const double *xb = x + (size_t)b * n * 3;
+    const bool act = i < n;
+    const int ii = act ? i : 0;
+    const T xi0 = (T)xb[3 * ii], xi1 = (T)xb[3 * ii + 1], xi2 = (T)xb[3 * ii + 2];
+    const int mi = ii / 3;
+    const bool io = ii % 3 == 0;
+    const T box = (T)w.box, rc = (T)w.rc, za = (T)w.za, zp = (T)w.zp;
+    const T qi = (T)(io ? w.qo : w.qh), qo = (T)w.qo, qh = (T)w.qh;
+    const T sig2 = (T)(w.sig * w.sig), eps = (T)w.eps;
+    Acc<T> g0, g1, g2;
+    double e = 0.0;
+    const int chunk = ((n + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
+    const int j0 = split * chunk, j1 = min(n, j0 + chunk);
+    for (int jt = j0; jt < j1; jt += kTile) {
+        __syncthreads();
+        const int j = jt + tid;
+        if (j < j1) {
+            sx[tid][0] = (T)xb[3 * j];
+            sx[tid][1] = (T)xb[3 * j + 1];
+            sx[tid][2] = (T)xb[3 * j + 2];
+        }
+        __syncthreads();
+        if (!act) continue;
+        const int nt = min(kTile, j1 - jt);
+        for (int jj = 0; jj < nt; ++jj) {
+            const int jg = jt + jj;
+            if (jg / 3 == mi) continue;
+            T d0 = xi0 - sx[jj][0], d1 = xi1 - sx[jj][1], d2 = xi2 - sx[jj][2];
+            if (w.periodic) {
+                d0 = min_image(d0, box);
+                d1 = min_image(d1, box);
+                d2 = min_image(d2, box);
+            }
+            const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            const T r = sqrt_t(r2);
+            if (w.periodic && !(r < rc)) continue;
+            const T rinv = (T)1 / r;
+            const T qj = jg % 3 == 0 ? qo : qh;
+            const T e0 = w.zahn ? qi * qj * (erfc_t(za * r) * rinv - zp * (r - rc)) : qi * qj * rinv;
+            T fp = e0 * rinv * rinv;                  // reference gradient of the Coulomb term
+            T ep = e0;
+            // LJ when the lower-index atom of the pair is an oxygen
+            if ((i < jg) ? io : (jg % 3 == 0)) {
+                const T s2 = sig2 * rinv * rinv;
+                const T s6 = s2 * s2 * s2;
+                ep += (T)4 * eps * (s6 * s6 - s6);
+                fp += (T)24 * eps * rinv * rinv * s6 * ((T)2 * s6 - (T)1);
+            }
+            e += 0.5 * (double)ep;
+            if (GRAD) {
+                g0.add(-fp * d0);
+                g1.add(-fp * d1);
+                g2.add(-fp * d2);
+            }
+        }
+    }
+    if (GRAD && act) {
+        double *gp = gpart + (((size_t)split * gridDim.z + b) * n + i) * 3;
+        gp[0] = g0.get();
+        gp[1] = g1.get();
+        gp[2] = g2.get();
+    }
+    const int nblk = gridDim.x * gridDim.y;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    double *ep = epart + ((size_t)b * (nblk + 1) + blk) * kNE;
+    __syncthreads();
+    red[tid] = act ? e : 0.0;
+    __syncthreads();
+    for (int s = kTile / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ep[0] = red[0];
+        for (int k = 1; k < kNE; ++k) ep[k] = 0.0;
+    }
+}
+
+// Intramolecular terms, one thread per (atom, bead): the thread evaluates its
+// molecule (three distances) and adds its atom's share to the summed
+// nonbonded partials; the oxygen's thread books the molecule's energy.
+template <bool GRAD>
+__global__ __launch_bounds__(kTile) void water_intra_kernel(WaterDev w, const double *__restrict__ x, int nsplit,
+                                                            const double *__restrict__ gpart, double *__restrict__ g,
+                                                            double *__restrict__ epart, int nblk_nb)
+{
+    __shared__ double red[kTile];
+    const int n = w.natoms;
+    const int tid = threadIdx.x;
+    const int a = blockIdx.x * kTile + tid;
+    const int b = blockIdx.y;
+    const bool act = a < n;
+    const double *xb = x + (size_t)b * n * 3;
+    double e = 0.0;
+    if (act) {
+        double gg[3] = { 0.0, 0.0, 0.0 };
+        if (GRAD)
+            for (int s = 0; s < nsplit; ++s) {
+                const double *gp = gpart + (((size_t)s * gridDim.y + b) * n + a) * 3;
+                gg[0] += gp[0];
+                gg[1] += gp[1];
+                gg[2] += gp[2];
+            }
+        const int o = a / 3 * 3, role = a % 3;
+        double v1[3], v2[3], vh[3], r1 = 0.0, r2 = 0.0, rh = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            v1[c] = xb[3 * o + c] - xb[3 * (o + 1) + c];
+            v2[c] = xb[3 * o + c] - xb[3 * (o + 2) + c];
+            vh[c] = xb[3 * (o + 1) + c] - xb[3 * (o + 2) + c];
+            if (w.periodic) {
+                v1[c] -= w.box * rint(v1[c] / w.box);
+                v2[c] -= w.box * rint(v2[c] / w.box);
+                vh[c] -= w.box * rint(vh[c] / w.box);
+            }
+            r1 += v1[c] * v1[c];
+            r2 += v2[c] * v2[c];
+            rh += vh[c] * vh[c];
+        }
+        r1 = sqrt(r1);
+        r2 = sqrt(r2);
+        rh = sqrt(rh);
+        const double d1 = r1 - w.r0, d2 = r2 - w.r0, dh = rh - w.r0hh;
+        const double e1 = exp(w.a * d1), e2 = exp(w.a * d2);
+        if (role == 0)
+            e = w.de * (1.0 - e1) * (1.0 - e1) + w.de * (1.0 - e2) * (1.0 - e2) + 0.5 * w.khh * dh * dh
+                + w.krt * dh * (d1 + d2) + w.krr * d1 * d2;
+        if (GRAD) {
+            const double gr1 = -2.0 * w.a * w.de * e1 * (1.0 - e1) + w.krt * dh + w.krr * d2;   // dE/dr_OH1
+            const double gr2 = -2.0 * w.a * w.de * e2 * (1.0 - e2) + w.krt * dh + w.krr * d1;
+            const double grh = w.khh * dh + w.krt * (d1 + d2);
+            for (int c = 0; c < 3; ++c) {
+                const double a1 = gr1 * v1[c] / r1, a2 = gr2 * v2[c] / r2, ah = grh * vh[c] / rh;
+                gg[c] += role == 0 ? a1 + a2 : role == 1 ? ah - a1 : -a2 - ah;
+            }
+            double *go = g + ((size_t)b * n + a) * 3;
+            go[0] = gg[0];
+            go[1] = gg[1];
+            go[2] = gg[2];
+        }
+    }
+    __syncthreads();
+    red[tid] = act ? e : 0.0;
+    __syncthreads();
+    for (int s = kTile / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    double *bp = epart + ((size_t)gridDim.y * (nblk_nb + 1)) * kNE + ((size_t)b * gridDim.x + blockIdx.x) * kNE;
+    if (tid == 0) {
+        bp[0] = red[0];
+        for (int k = 1; k < kNE; ++k) bp[k] = 0.0;
+    }
+}
+
 }  // namespace
 }  // namespace cam
 
 using namespace cam;
 
 struct cam_ff {
+    int kind = 0;                     // 0: QMDFF form, 1: the reference's water model
+    WaterDev w{};
     FFDev d{};
     int natoms = 0;
     int *type = nullptr, *mol = nullptr, *bonds = nullptr, *angles = nullptr, *csr_ptr = nullptr, *csr_term = nullptr;
@@ -435,6 +617,26 @@ int ff_launch(cam_ff *ff, const Theta &th, const double *x, int nbeads, double *
     const bool grad = g != nullptr;
     const bool par = dth != nullptr || dvth != nullptr;
     dim3 gnb(nbx, ff->nsplit, nbeads);
+    if (ff->kind == 1) {
+        if (grad)
+            hipLaunchKernelGGL((water_nb_kernel<T, true>), gnb, dim3(kTile), 0, s, ff->w, x, ff->nsplit, ff->gpart,
+                               ff->epart);
+        else
+            hipLaunchKernelGGL((water_nb_kernel<T, false>), gnb, dim3(kTile), 0, s, ff->w, x, ff->nsplit, ff->gpart,
+                               ff->epart);
+        CAM_LAUNCH_CHECK();
+        if (grad)
+            hipLaunchKernelGGL((water_intra_kernel<true>), dim3(nbx, nbeads), dim3(kTile), 0, s, ff->w, x, ff->nsplit,
+                               ff->gpart, g, ff->epart, nblk_nb);
+        else
+            hipLaunchKernelGGL((water_intra_kernel<false>), dim3(nbx, nbeads), dim3(kTile), 0, s, ff->w, x,
+                               ff->nsplit, ff->gpart, g, ff->epart, nblk_nb);
+        CAM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(ff_reduce_kernel, dim3(nbeads), dim3(32), 0, s, ff->epart, nblk_nb, nbx, nbeads, e, nullptr,
+                           nullptr);
+        CAM_LAUNCH_CHECK();
+        return CAM_OK;
+    }
     if (grad && par)
         hipLaunchKernelGGL((nonbonded_kernel<T, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v, ff->nsplit,
                            ff->gpart, ff->epart);
@@ -538,6 +740,46 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out)
     return CAM_OK;
 }
 
+int cam_water_create(int nwater, int periodic, double box, int zahn, double rcut, cam_ff **out)
+{
+    CAM_CHECK_ARG(out, "null argument");
+    *out = nullptr;
+    CAM_CHECK_ARG(nwater >= 1 && nwater <= (1 << 24), "nwater out of range (%d)", nwater);
+    CAM_CHECK_ARG(!periodic || (box > 0.0 && rcut > 0.0 && rcut <= 0.5 * box),
+                  "periodic boxes need 0 < rcut <= box/2 (minimum image)");
+    CAM_CHECK_ARG(!zahn || periodic, "the Zahn form needs a periodic box");
+    clear_error();
+    std::unique_ptr<cam_ff> ff(new cam_ff());
+    ff->kind = 1;
+    ff->natoms = 3 * nwater;
+    WaterDev &w = ff->w;
+    w.natoms = 3 * nwater;
+    w.periodic = periodic ? 1 : 0;
+    w.zahn = zahn ? 1 : 0;
+    w.box = periodic ? box : 0.0;
+    w.rc = periodic ? rcut : 0.0;
+    // Zahn parameters as set_periodic sets them (src/set_periodic.f90:93-97)
+    const double bohr = 0.52917721092, hartree = 627.5094743;
+    w.za = 0.2 * bohr;
+    const double acut = w.za * w.rc;
+    w.zp = w.zahn ? std::erfc(acut) / (w.rc * w.rc) + 2.0 * w.za / 1.7724538509055160 * std::exp(-acut * acut) / w.rc
+                  : 0.0;
+    // water_init (src/water_init.f90:83-106); k_rr and eps_OO are default REAL there
+    w.r0 = 1.0 / bohr;
+    w.r0hh = 1.633 / bohr;
+    w.de = 101.9188 / hartree;
+    w.a = 2.567 * bohr;
+    w.khh = 328.645606 / hartree * bohr * bohr;
+    w.krt = -211.4672 / hartree * bohr * bohr;
+    w.krr = (double)111.70765f / hartree * bohr * bohr;
+    w.qh = 0.41;
+    w.qo = -0.82;
+    w.sig = 3.166 / bohr;
+    w.eps = (double)0.1554f / hartree;
+    *out = ff.release();
+    return CAM_OK;
+}
+
 int cam_ff_destroy(cam_ff *ff)
 {
     if (!ff) return CAM_OK;
@@ -553,12 +795,13 @@ int cam_ff_destroy(cam_ff *ff)
 int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, int precision,
                   void *stream)
 {
-    CAM_CHECK_ARG(ff && theta && x && e, "null argument");
+    CAM_CHECK_ARG(ff && x && e, "null argument");
+    CAM_CHECK_ARG(theta || ff->kind == 1, "theta is required for this force field");
     CAM_CHECK_ARG(nbeads >= 1, "nbeads must be >= 1");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision");
     clear_error();
-    Theta th;
-    std::memcpy(th.p, theta, sizeof th.p);
+    Theta th{};
+    if (theta) std::memcpy(th.p, theta, sizeof th.p);
     hipStream_t s = (hipStream_t)stream;
     return precision == CAM_PREC_MIXED ? ff_launch<float>(ff, th, x, nbeads, e, nullptr, nullptr, nullptr, nullptr, s)
                                        : ff_launch<double>(ff, th, x, nbeads, e, nullptr, nullptr, nullptr, nullptr, s);
@@ -567,13 +810,15 @@ int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, 
 int cam_ff_forces(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, double *dedx,
                   double *dedtheta, const double *v, double *dvdtheta, int precision, void *stream)
 {
-    CAM_CHECK_ARG(ff && theta && x && dedx, "null argument");
+    CAM_CHECK_ARG(ff && x && dedx, "null argument");
+    CAM_CHECK_ARG(theta || ff->kind == 1, "theta is required for this force field");
+    CAM_CHECK_ARG(ff->kind == 0 || (!dedtheta && !dvdtheta), "the water model has no parameter derivatives");
     CAM_CHECK_ARG(nbeads >= 1, "nbeads must be >= 1");
     CAM_CHECK_ARG(!dvdtheta || v, "dvdtheta needs the direction v");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision");
     clear_error();
-    Theta th;
-    std::memcpy(th.p, theta, sizeof th.p);
+    Theta th{};
+    if (theta) std::memcpy(th.p, theta, sizeof th.p);
     hipStream_t s = (hipStream_t)stream;
     return precision == CAM_PREC_MIXED
                ? ff_launch<float>(ff, th, x, nbeads, e, dedx, dedtheta, dvdtheta ? v : nullptr, dvdtheta, s)

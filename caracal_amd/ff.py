@@ -38,6 +38,16 @@ class ForceField:
         self._h = h
         self.natoms = d.natoms
 
+    @classmethod
+    def water(cls, nwater, periodic=False, box=0.0, zahn=False, rcut=0.0):
+        """The reference's flexible water model (atoms O, H, H per molecule);
+        box and rcut in bohr.  theta arguments of the methods are ignored."""
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        check("cam_water_create", lib().cam_water_create(nwater, int(periodic), box, int(zahn), rcut, ctypes.byref(h)))
+        self._h, self.natoms, self._keep = h, 3 * nwater, []
+        return self
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().cam_ff_destroy(self._h)
@@ -48,18 +58,18 @@ class ForceField:
     def energy(self, theta, x, precision=_abi.CAM_PREC_FP64):
         """x: cuda float64 tensor (nbeads, natoms, 3) -> energies (nbeads,)."""
         import torch
-        th = np.ascontiguousarray(theta, dtype=np.float64)
+        thp = None if theta is None else _p(np.ascontiguousarray(theta, dtype=np.float64))
         x = x.contiguous()
         nb = x.shape[0]
         e = torch.empty(nb, dtype=torch.float64, device=x.device)
-        check("cam_ff_energy", lib().cam_ff_energy(self._h, _p(th), x.data_ptr(), nb, e.data_ptr(), precision,
+        check("cam_ff_energy", lib().cam_ff_energy(self._h, thp, x.data_ptr(), nb, e.data_ptr(), precision,
                                                   torch.cuda.current_stream().cuda_stream))
         return e
 
     def forces(self, theta, x, v=None, want_params=False, precision=_abi.CAM_PREC_FP64):
         """-> (e (nb,), dE/dx (nb,natoms,3), dE/dtheta (nb,10) or None, d(v.dE/dx)/dtheta (nb,10) or None)."""
         import torch
-        th = np.ascontiguousarray(theta, dtype=np.float64)
+        thp = None if theta is None else _p(np.ascontiguousarray(theta, dtype=np.float64))
         x = x.contiguous()
         nb = x.shape[0]
         e = torch.empty(nb, dtype=torch.float64, device=x.device)
@@ -67,7 +77,7 @@ class ForceField:
         dth = torch.empty((nb, 10), dtype=torch.float64, device=x.device) if want_params else None
         dvth = torch.empty((nb, 10), dtype=torch.float64, device=x.device) if v is not None else None
         vptr = v.contiguous().data_ptr() if v is not None else None
-        check("cam_ff_forces", lib().cam_ff_forces(self._h, _p(th), x.data_ptr(), nb, e.data_ptr(), g.data_ptr(),
+        check("cam_ff_forces", lib().cam_ff_forces(self._h, thp, x.data_ptr(), nb, e.data_ptr(), g.data_ptr(),
                                                   dth.data_ptr() if dth is not None else None, vptr,
                                                   dvth.data_ptr() if dvth is not None else None, precision,
                                                   torch.cuda.current_stream().cuda_stream))
@@ -118,7 +128,8 @@ class BoxEngine:
         check("cam_box_create", lib().cam_box_create(ff._h, nbeads, dt, _p(m), beta, nm_mode, precision,
                                                      andersen_step, seed, ctypes.byref(h)))
         self._h, self.ff, self.nbeads, self.natoms = h, ff, nbeads, ff.natoms
-        self.set_params(theta)
+        if theta is not None:
+            self.set_params(theta)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -254,6 +254,17 @@ typedef struct {
 
 typedef struct cam_ff cam_ff;
 int cam_ff_create(const cam_ff_desc *desc, cam_ff **out);
+/* The reference's flexible water model (src/egrad_water.f90:36-333) as a
+ * force field for the calls below and for cam_box_create: nwater molecules,
+ * atoms O, H, H per molecule, parameters of water_init
+ * (src/water_init.f90:83-106).  periodic: cubic box of side box (bohr),
+ * minimum image, cutoff rcut (bohr, <= box/2) on the intermolecular terms;
+ * zahn: Zahn-damped Coulomb with the constants of set_periodic
+ * (src/set_periodic.f90:93-97).  The intermolecular gradient is the
+ * reference's (e0/r^2 per pair) and LJ acts when the lower-index atom of a
+ * pair is an oxygen, as in the reference.  theta is ignored (may be NULL);
+ * parameter derivatives are not available. */
+int cam_water_create(int nwater, int periodic, double box, int zahn, double rcut, cam_ff **out);
 int cam_ff_destroy(cam_ff *ff);
 /* Energy of nbeads replicas: x device [nbeads][natoms][3] bohr, e device
  * [nbeads] hartree.  theta is a host array of 10.  Asynchronous on stream. */

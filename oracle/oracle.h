@@ -47,6 +47,14 @@ typedef struct {
     double box, rcut, zahn_a, zahn_par;
     double sr42[2][2], r0bj[2][2], zrep[2][2], arep[2][2];
 } oracle_ff;
+/* The reference's flexible water model (src/egrad_water.f90:36-333),
+ * nwater molecules O,H,H; x, g: (3, 3*nwater).  Parameters in atomic units
+ * (src/water_init.f90:83-106): r0_OH, r0_HH, De, a, k_HH, k_rtheta, k_rr,
+ * q_H, q_O, sigma_OO, eps_OO. */
+void oracle_water_params(double par[11]);
+double oracle_water_eval(int nwater, const double *x, double *g, int periodic, double box, int zahn,
+                         double zahn_a, double zahn_par, double coul_cut);
+
 double oracle_ff_eval(const oracle_ff *f, const double *theta, const double *x, double *g, double *dth,
                       const double *v, double *dvth);
 

@@ -45,3 +45,24 @@ def ch4h_base(d_hb=2.0, stretch=1.0):
 def ch4h_cloud(n, sigma, seed, d_hb=2.0):
     rng = np.random.default_rng(seed)
     return ch4h_base(d_hb)[None] + rng.normal(0.0, sigma, (n, 6, 3))
+
+
+def water_box(nwater, box, seed, jitter=0.05):
+    """nwater flexible waters (O, H, H) near r_OH = 1 angstrom and the H-H
+    distance 1.633 angstrom, on a cubic lattice of side box (bohr) with random
+    orientations, plus Gaussian jitter (bohr)."""
+    rng = np.random.default_rng(seed)
+    bohr = 0.52917721092
+    r0, rhh = 1.0 / bohr, 1.633 / bohr
+    half = np.arcsin(0.5 * rhh / r0)
+    m = int(np.ceil(nwater ** (1.0 / 3.0)))
+    a = box / m
+    sites = np.array([(i, j, k) for i in range(m) for j in range(m) for k in range(m)], dtype=np.float64)
+    sites = (sites[rng.permutation(len(sites))[:nwater]] + 0.5) * a
+    out = []
+    for s in sites:
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        h1 = r0 * np.array([np.sin(half), np.cos(half), 0.0])
+        h2 = r0 * np.array([-np.sin(half), np.cos(half), 0.0])
+        out += [s, s + q @ h1, s + q @ h2]
+    return np.array(out) + rng.normal(0.0, jitter, (3 * nwater, 3))

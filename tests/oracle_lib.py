@@ -245,3 +245,41 @@ class FFOracle:
                                     _ptr(g) if g is not None else None, _ptr(dth),
                                     _ptr(vv) if vv is not None else None, _ptr(dvth) if dvth is not None else None)
         return e, g, dth, dvth
+
+
+# ---- water surface (src/egrad_water.f90) ----
+def water_params():
+    par = np.zeros(11)
+    oracle().oracle_water_params(_ptr(par))
+    return par
+
+
+def _water_args(x, periodic, box, zahn, zahn_a, zahn_par, cut):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return x, np.zeros_like(x), (int(periodic), float(box), int(zahn), float(zahn_a), float(zahn_par), float(cut))
+
+
+def water_eval(x, periodic=False, box=0.0, zahn=False, zahn_a=0.0, zahn_par=0.0, cut=0.0):
+    """x: (natoms, 3) bohr, O,H,H order -> (E, g)."""
+    x, g, (ip, bx, iz, za, zp, ct) = _water_args(x, periodic, box, zahn, zahn_a, zahn_par, cut)
+    f = oracle().oracle_water_eval
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_double]
+    e = f(len(x) // 3, _ptr(x), _ptr(g), ip, bx, iz, za, zp, ct)
+    return e, g
+
+
+def ref_water_eval(x, periodic=False, box=0.0, zahn=False, zahn_a=0.0, zahn_par=0.0, cut=0.0):
+    """The compiled reference's egrad_water through the harness, or None."""
+    r = ref()
+    if r is None:
+        return None
+    x, g, (ip, bx, iz, za, zp, ct) = _water_args(x, periodic, box, zahn, zahn_a, zahn_par, cut)
+    e = ctypes.c_double()
+    f = r.ref_water_eval
+    f.restype = None
+    f.argtypes = [ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_double,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    f(len(x), _ptr(x), _ptr(g), ctypes.byref(e), ip, bx, iz, za, zp, ct)
+    return e.value, g
