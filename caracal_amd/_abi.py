@@ -34,7 +34,7 @@ class Config(ctypes.Structure):
         ("surface", ctypes.c_int32), ("natoms", ctypes.c_int32), ("nbeads", ctypes.c_int32),
         ("ntraj", ctypes.c_int32), ("precision", ctypes.c_int32), ("nm_mode", ctypes.c_int32),
         ("ensemble", ctypes.c_int32), ("remove_com", ctypes.c_int32), ("andersen_step", ctypes.c_int32),
-        ("steps_per_launch", ctypes.c_int32), ("nm_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("steps_per_launch", ctypes.c_int32), ("nm_kernel", ctypes.c_int32), ("waves_per_eu", ctypes.c_int32),
         ("dt", ctypes.c_double),
         ("mass", ctypes.c_double * CAM_MAX_ATOMS),
         ("seed", ctypes.c_uint64),
@@ -49,7 +49,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int32), ("chosen", ctypes.c_int32),
         ("nm_kernel", ctypes.c_int32 * CAM_MAX_TRIALS), ("steps_per_launch", ctypes.c_int32 * CAM_MAX_TRIALS),
-        ("us_per_step", ctypes.c_double * CAM_MAX_TRIALS),
+        ("waves_per_eu", ctypes.c_int32 * CAM_MAX_TRIALS), ("us_per_step", ctypes.c_double * CAM_MAX_TRIALS),
     ]
 
 

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -867,19 +868,29 @@ struct cam_engine {
 
 namespace {
 
+template <class S, int NB, typename T, int WPE>
+step_fn pick_step_w(int ens, bool mfma)
+{
+    constexpr bool MF = NB >= 16 && NB % 16 == 0;
+    if (mfma && MF) {
+        if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, MF>;
+        if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, MF>;
+        return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, MF>;
+    }
+    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, false>;
+    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, false>;
+    return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, false>;
+}
+
+// wpe: minimum waves per SIMD the compiler is asked to fit (1 or 2).  Only
+// the FP64 CH5 step offers 2: it needs more than 256 registers at 1, and at 2
+// it spills; which is faster is measured (cam_engine_autotune).
 template <class S, int NB, typename T>
 step_fn pick_step(int ens, int wpe, bool mfma)
 {
-    (void)wpe;
-    constexpr bool MF = NB >= 16 && NB % 16 == 0;
-    if (mfma && MF) {
-        if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1, MF>;
-        if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1, MF>;
-        return step_kernel<S, NB, T, CAM_ENS_CHILD, 1, MF>;
-    }
-    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, 1, false>;
-    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, 1, false>;
-    return step_kernel<S, NB, T, CAM_ENS_CHILD, 1, false>;
+    if constexpr (S::L == 4 && std::is_same<T, double>::value)
+        if (wpe == 2) return pick_step_w<S, NB, T, 2>(ens, mfma);
+    return pick_step_w<S, NB, T, 1>(ens, mfma);
 }
 
 template <class S, typename T>
@@ -1093,7 +1104,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_MFMA, "bad nm_kernel");
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
                   "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
-    CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
+    CAM_CHECK_ARG(cfg->waves_per_eu >= 0 && cfg->waves_per_eu <= 2, "waves_per_eu must be 0, 1 or 2");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
@@ -1145,7 +1156,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     // steps fused per launch: state stays in registers for this many steps
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
     if (cfg->remove_com) e->spl = 1;
-    e->wpe = 1;
+    e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 ? 2 : 1;
     e->mfma = cfg->nm_kernel == CAM_NMK_MFMA;
     const bool ok = select_kernels(e.get(), e->mfma, e->step, e->force);
     if (!ok) {
@@ -1456,7 +1467,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     clear_error();
     const cam_config &c = e->cfg;
     const int nb = c.nbeads;
-    std::vector<int> nmks, spls;
+    std::vector<int> nmks, spls, wpes;
+    if (c.waves_per_eu != 0 || c.surface != CAM_SURFACE_CH4H || c.precision != CAM_PREC_FP64) wpes.push_back(e->wpe);
+    else wpes = { 1, 2 };
     if (c.nm_kernel != CAM_NMK_AUTO) nmks.push_back(c.nm_kernel);
     else {
         nmks.push_back(CAM_NMK_VALU);
@@ -1474,7 +1487,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     uint64_t *ctr = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = CAM_OK;
-    struct Trial { int nmk, spl; double us; };
+    struct Trial { int nmk, wpe, spl; double us; };
     std::vector<Trial> trials;
     auto fail = [&](hipError_t err) {
         set_error("HIP error in autotune: %s", hipGetErrorString(err));
@@ -1486,11 +1499,16 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     if (err == hipSuccess) err = hipEventCreate(&t0);
     if (err == hipSuccess) err = hipEventCreate(&t1);
     if (err != hipSuccess) fail(err);
-    for (int nmk : nmks) {
+    const int wpe0 = e->wpe;
+    for (int nmk : nmks)
+      for (int wpe : wpes) {
         if (rc != CAM_OK) break;
         step_fn st;
         force_fn fo;
-        if (!select_kernels(e, nmk == CAM_NMK_MFMA, st, fo)) continue;
+        e->wpe = wpe;
+        const bool okk = select_kernels(e, nmk == CAM_NMK_MFMA, st, fo);
+        e->wpe = wpe0;
+        if (!okk) continue;
         for (int spl : spls) {
             StepArgs a = step_args(e);
             a.q = buf;
@@ -1528,7 +1546,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
                 fail(err);
                 break;
             }
-            trials.push_back({ nmk, spl, 1e3 * ms / trial_steps });
+            trials.push_back({ nmk, wpe, spl, 1e3 * ms / trial_steps });
         }
     }
     if (buf) (void)hipFree(buf);
@@ -1543,6 +1561,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         if (trials[i].us < trials[best].us) best = i;
     step_fn st;
     force_fn fo;
+    e->wpe = trials[best].wpe;
     select_kernels(e, trials[best].nmk == CAM_NMK_MFMA, st, fo);
     e->mfma = trials[best].nmk == CAM_NMK_MFMA;
     e->step = st;
@@ -1556,6 +1575,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         for (int i = 0; i < out->n; ++i) {
             out->nm_kernel[i] = trials[i].nmk;
             out->steps_per_launch[i] = trials[i].spl;
+            out->waves_per_eu[i] = trials[i].wpe;
             out->us_per_step[i] = trials[i].us;
         }
     }

@@ -39,7 +39,7 @@ module caracal_amd_mod
       integer(c_int32_t) :: surface = CAM_SURFACE_H3, natoms = 3, nbeads = 1, ntraj = 1
       integer(c_int32_t) :: precision = CAM_PREC_FP64, nm_mode = CAM_NM_REFERENCE
       integer(c_int32_t) :: ensemble = CAM_ENS_NVE, remove_com = 0, andersen_step = 0
-      integer(c_int32_t) :: steps_per_launch = 0, nm_kernel = 0, reserved = 0
+      integer(c_int32_t) :: steps_per_launch = 0, nm_kernel = 0, waves_per_eu = 0
       real(c_double) :: dt = 0.d0
       real(c_double) :: mass(CAM_MAX_ATOMS) = 0.d0
       integer(c_int64_t) :: seed = 0
@@ -50,6 +50,7 @@ module caracal_amd_mod
    type, bind(C), public :: cam_tuning
       integer(c_int32_t) :: n = 0, chosen = 0
       integer(c_int32_t) :: nm_kernel(CAM_MAX_TRIALS) = 0, steps_per_launch(CAM_MAX_TRIALS) = 0
+      integer(c_int32_t) :: waves_per_eu(CAM_MAX_TRIALS) = 0
       real(c_double) :: us_per_step(CAM_MAX_TRIALS) = 0.d0
    end type cam_tuning
 

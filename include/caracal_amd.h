@@ -116,7 +116,8 @@ typedef struct {
     int32_t andersen_step;    /* ANDERSEN: resample every andersen_step steps */
     int32_t steps_per_launch; /* steps fused into one kernel launch; 0 = pick automatically */
     int32_t nm_kernel;        /* CAM_NMK_* */
-    int32_t reserved;         /* must be 0 */
+    int32_t waves_per_eu;     /* register budget of the step kernel: 1 = up to 512 registers per lane,
+                                 2 = 256 (two waves per SIMD; FP64 CH4H only); 0 = 1 unless tuned */
     double dt;                /* time step, atomic time units */
     double mass[CAM_MAX_ATOMS];   /* electron masses */
     uint64_t seed;            /* counter-based RNG key for Andersen draws */
@@ -193,9 +194,10 @@ int cam_engine_device_state(cam_engine *eng, double **q, double **p, double **dv
 /* Schedule autotuner.  Times candidate schedules of the step kernel on a
  * copy of the current state (the engine's trajectories, kappa sums and RNG
  * counters are not touched) for trial_steps steps each, and keeps the fastest:
- * the normal-mode unit (VALU or MFMA; only when nm_kernel is AUTO) and the
- * steps fused per launch (10, 25, 50, 100; only when steps_per_launch is 0
- * and remove_com is off).  Captured graphs are dropped.  No counterpart in
+ * the normal-mode unit (VALU or MFMA; only when nm_kernel is AUTO), the
+ * register budget (waves_per_eu 1 or 2; FP64 CH4H with waves_per_eu 0) and
+ * the steps fused per launch (10, 25, 50, 100; only when steps_per_launch is
+ * 0 and remove_com is off).  Captured graphs are dropped.  No counterpart in
  * the reference, which has one schedule. */
 #define CAM_MAX_TRIALS 16
 typedef struct {
@@ -203,6 +205,7 @@ typedef struct {
     int32_t chosen;                         /* index of the schedule kept */
     int32_t nm_kernel[CAM_MAX_TRIALS];      /* CAM_NMK_VALU or CAM_NMK_MFMA */
     int32_t steps_per_launch[CAM_MAX_TRIALS];
+    int32_t waves_per_eu[CAM_MAX_TRIALS];
     double us_per_step[CAM_MAX_TRIALS];     /* microseconds per step of the whole batch */
 } cam_tuning;
 int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);
