@@ -9,6 +9,8 @@
 // to binary32 first, exactly as the compiled reference sees them.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
+
 #include "rt_math.hpp"
 
 namespace cam {
@@ -271,11 +273,10 @@ __device__ __forceinline__ void bend(const T r[3], int icompc, const T t[3], con
     }
 }
 
-// V(R1,R2,R3) and dV/dR_i of the BKMP2 surface (POTE).  R(1)=r_AB, R(2)=r_AC, R(3)=r_BC.
+// London term (H3LOND95): singlet and triplet H2 curves combined.
 template <typename T>
-__device__ __forceinline__ void pote(const T r[3], T& v, T dv[3])
+__device__ __forceinline__ void london(const T r[3], T& v, T dv[3])
 {
-    // ---- London term (H3LOND95) ----
     T e1[3], de1[3], e3[3], de3[3], jj[3];
     T sumq = T(0);
 #pragma unroll
@@ -285,15 +286,29 @@ __device__ __forceinline__ void pote(const T r[3], T& v, T dv[3])
         sumq += T(0.5) * (e1[i] + e3[i]);
         jj[i] = T(0.5) * (e1[i] - e3[i]);
     }
-    {
-        const T d21 = jj[1] - jj[0], d32 = jj[2] - jj[1], d31 = jj[2] - jj[0];
-        const T root = rt::sqrt(T(0.5) * (d21 * d21 + d32 * d32 + d31 * d31) + T(1.0e-12));
-        v = sumq - root;
-        const T q = T(0.25) / root;
-        dv[0] = T(0.5) * (de1[0] + de3[0]) - (T(2) * jj[0] - jj[1] - jj[2]) * (de1[0] - de3[0]) * q;
-        dv[1] = T(0.5) * (de1[1] + de3[1]) - (T(2) * jj[1] - jj[2] - jj[0]) * (de1[1] - de3[1]) * q;
-        dv[2] = T(0.5) * (de1[2] + de3[2]) - (T(2) * jj[2] - jj[0] - jj[1]) * (de1[2] - de3[2]) * q;
-    }
+    const T d21 = jj[1] - jj[0], d32 = jj[2] - jj[1], d31 = jj[2] - jj[0];
+    const T root = rt::sqrt(T(0.5) * (d21 * d21 + d32 * d32 + d31 * d31) + T(1.0e-12));
+    v = sumq - root;
+    const T q = T(0.25) / root;
+    dv[0] = T(0.5) * (de1[0] + de3[0]) - (T(2) * jj[0] - jj[1] - jj[2]) * (de1[0] - de3[0]) * q;
+    dv[1] = T(0.5) * (de1[1] + de3[1]) - (T(2) * jj[1] - jj[2] - jj[0]) * (de1[1] - de3[1]) * q;
+    dv[2] = T(0.5) * (de1[2] + de3[2]) - (T(2) * jj[2] - jj[0] - jj[1]) * (de1[2] - de3[2]) * q;
+}
+
+// V(R1,R2,R3) and dV/dR_i of the BKMP2 surface (POTE).  R(1)=r_AB, R(2)=r_AC, R(3)=r_BC.
+// The London and bending terms are evaluated in FP64 for either T: their
+// O(1) parts cancel to the O(1e-2) forces of the saddle region, and a
+// binary32 evaluation (inputs included) there costs ~1e-5 of max|F|.  T is
+// the type of the asymmetric and compact corrections.
+template <typename T>
+__device__ __forceinline__ void pote(const double rd[3], double& vd, double dvd[3])
+{
+    london<double>(rd, vd, dvd);
+    const T r[3] = { T(rd[0]), T(rd[1]), T(rd[2]) };
+    // FP64 keeps the reference's order of accumulation; binary32 corrections
+    // are summed on their own and added once.
+    constexpr bool F64 = std::is_same<T, double>::value;
+    T v = F64 ? T(vd) : T(0), dv[3] = { F64 ? T(dvd[0]) : T(0), F64 ? T(dvd[1]) : T(0), F64 ? T(dvd[2]) : T(0) };
     const T rs = r[0] + r[1] + r[2];
     const T rsq = rs * rs;
     T da[3];
@@ -388,15 +403,20 @@ __device__ __forceinline__ void pote(const T r[3], T& v, T dv[3])
         }
     }
     {
-        const double rb[3] = { double(r[0]), double(r[1]), double(r[2]) };
         const double tb[3] = { double(t[0]), double(t[1]), double(t[2]) };
         const double dtb[3] = { double(dt[0]), double(dt[1]), double(dt[2]) };
-        double vb = double(v), dvb[3] = { double(dv[0]), double(dv[1]), double(dv[2]) };
-        bend<double>(rb, icompc, tb, dtb, vb, dvb);
-        v = T(vb);
-        dv[0] = T(dvb[0]);
-        dv[1] = T(dvb[1]);
-        dv[2] = T(dvb[2]);
+        if (F64) {
+            vd = double(v);
+            dvd[0] = double(dv[0]);
+            dvd[1] = double(dv[1]);
+            dvd[2] = double(dv[2]);
+        } else {
+            vd += double(v);
+            dvd[0] += double(dv[0]);
+            dvd[1] += double(dv[1]);
+            dvd[2] += double(dv[2]);
+        }
+        bend<double>(rd, icompc, tb, dtb, vd, dvd);
     }
 }
 
@@ -418,10 +438,10 @@ __device__ __forceinline__ void egrad_bead(const double x[9], double& v, double 
     const double rab = ::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
     const double rac = ::sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
     const double rbc = ::sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
-    T r[3] = { T(rab), T(rac), T(rbc) }, dr[3], vt;
-    pote(r, vt, dr);
-    v = double(vt);
-    const double f0 = double(dr[0]) / rab, f1 = double(dr[1]) / rac, f2 = double(dr[2]) / rbc;
+    const double r[3] = { rab, rac, rbc };
+    double dr[3];
+    pote<T>(r, v, dr);
+    const double f0 = dr[0] / rab, f1 = dr[1] / rac, f2 = dr[2] / rbc;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         g[d] = f1 * ac[d] - f0 * ab[d];
