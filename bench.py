@@ -270,6 +270,11 @@ def main():
     ap.add_argument("--nbeads", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed step-kernel schedule 'valu|mfma,steps_per_launch' instead of the autotuner "
+                         "(for profiling: every launch then has the same shape)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "
@@ -280,10 +285,26 @@ def main():
     import caracal_amd as cam
     from caracal_amd import workloads
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 8 * n, "copies": 5}))
+        return
 
     ntraj, nb = args.ntraj, args.nbeads
-    e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
-    trials, best = e.autotune(100)
+    if args.schedule:
+        unit, spl_fixed = args.schedule.split(",")
+        nmk = cam.CAM_NMK_MFMA if unit == "mfma" else cam.CAM_NMK_VALU
+        e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64, steps_per_launch=int(spl_fixed),
+                                nm_kernel=nmk)
+        trials, best = [], {"nm_kernel": nmk, "steps_per_launch": int(spl_fixed)}
+    else:
+        e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
+        trials, best = e.autotune(100)
     wall, ms_launch, spl, nl = timed_run(e, args.steps, args.warmup)
     num, den = e.kappa(args.steps)
     e.close()

@@ -54,6 +54,27 @@ def test_steps_per_launch_and_graph_replay_are_bitwise_invariant():
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("surface,nb", [(cam.CAM_SURFACE_H3, 4), (cam.CAM_SURFACE_H3, 8), (cam.CAM_SURFACE_H3, 32),
+                                        (cam.CAM_SURFACE_CH4H, 4), (cam.CAM_SURFACE_CH4H, 32)])
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+def test_fused_steps_equal_single_steps(surface, nb, prec):
+    """Steps fused in one launch (state in registers) give bit for bit the
+    state of one step per launch, for every surface, bead count and precision,
+    on the ragged workload's hot initial conditions (up to 1000 K)."""
+    bucket = [b for b in workloads.ragged(ntraj=1200, seed=1) if b[0] == surface and b[1] == nb][0]
+    d = bucket[2]
+    out = []
+    for spl in (1, 10):
+        e = cam.RPMDEngine(surface, nb, len(d["beta"]), d["dt"], d["masses"], precision=prec, steps_per_launch=spl)
+        e.set_state(d["q"], d["p"], d["beta"])
+        e.run(20)
+        out.append(e.get_state() + (e.get_forces()[1],))
+        e.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.isfinite(b).all()
+        assert np.array_equal(a, b)
+
+
 def test_orthogonal_mode_equals_reference_mode_on_symmetric_beads():
     nb = 8
     w = workloads.h3_nve(nbeads=nb)
