@@ -36,6 +36,11 @@ sys.path.insert(0, ROOT)
 CH4H_FLOPS_PER_BEAD = 12707
 CH4H_TRANSCENDENTALS_PER_BEAD = 321
 FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
+# HBM bytes of one step-kernel launch at 4096 trajectories x 32 beads, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a plain 1 GiB copy
+# (profiles/step_kernel_ch4h_children_fp64_mfma.md); the state is read and
+# written once per launch, so the figure does not depend on steps per launch.
+MEASURED_TRAFFIC = {(4096, 32): 117.0e6}
 
 
 def step_flops(natoms, nbeads):
@@ -337,7 +342,7 @@ def main():
         "kappa_final": float(num[-1] / den) if den > 0 else None,
         "autotune": trials,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get((ntraj, nb)),
                      "kernel": kernel, "ms_per_launch": ms_launch, "flops_per_launch": flops_launch,
                      "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
     }
