@@ -204,10 +204,11 @@ def force_matching(cam):
     ff = cff.ForceField(s, rc_angstrom=10.0)
     x = torch.tensor(xb, device="cuda")
     th_ref = ffbox.theta_ref()
-    res = fit.force_match(ff, x, th_ref, fit.perturbed_theta(th_ref, 0.10, seed=3), steps=50, lr=0.02)
+    res = fit.force_match(ff, x, th_ref, fit.perturbed_theta(th_ref, 0.10, seed=3), steps=50, lr=0.01)
     ff.close()
     return {"loss_first": res["loss"][0], "loss_last": res["loss"][-1], "ms_per_adam_step": res["s_per_step"] * 1e3,
             "max_rel_param_err_start": res["rel_param_err0"], "max_rel_param_err_end": res["rel_param_err"],
+            "rel_param_err_end": [float(v) for v in np.round(res["theta"] / th_ref - 1.0, 4)],
             "nbeads": 64, "atoms": len(s["type"])}
 
 

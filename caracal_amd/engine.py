@@ -103,7 +103,12 @@ class RPMDEngine:
             lib().cam_engine_destroy(self._h)
         self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the library handle may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -236,4 +241,9 @@ class RaggedBatch:
             lib().cam_batch_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the library handle may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -53,7 +53,12 @@ class ForceField:
             lib().cam_ff_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the library handle may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def energy(self, theta, x, precision=_abi.CAM_PREC_FP64):
         """x: cuda float64 tensor (nbeads, natoms, 3) -> energies (nbeads,)."""
@@ -136,7 +141,12 @@ class BoxEngine:
             lib().cam_box_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the library handle may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def set_params(self, theta):
         th = np.ascontiguousarray(theta, dtype=np.float64)
