@@ -137,6 +137,18 @@ struct Own {
     T s1, ds1, s2, ds2, s3, ds3, sphi, dsphi, sth, dsth;
 };
 
+// 1 - tanh(x) and sech^2(x) from one exponential: with E = exp(2x),
+// 1 - tanh(x) = 2/(1 + E) and sech^2(x) = (1 - tanh)(1 + tanh).  The reference
+// evaluates tanh and cosh separately; this form needs one elementary function
+// instead of two and has no cancellation where tanh is close to 1.
+template <typename T>
+__device__ __forceinline__ void one_minus_tanh(T x, T& s, T& sech2)
+{
+    const T e = rt::exp(T(2) * x);
+    s = T(2) / (T(1) + e);
+    sech2 = s * (T(2) - s);
+}
+
 template <typename T>
 __device__ __forceinline__ void own_switches(Own<T>& o)
 {
@@ -146,26 +158,29 @@ __device__ __forceinline__ void own_switches(Own<T>& o)
         const T y = r - T(Par::b1s), y2 = y * y, y4 = y2 * y2, y8 = y4 * y4, y7 = y4 * y2 * y;
         const T arg = T(Par::a1s) * dr * y8;
         const bool on = arg < T(19);
-        const T c = rt::cosh(arg);
-        o.s1 = on ? T(1) - rt::tanh(arg) : T(0);
-        o.ds1 = on ? -(T(Par::a1s) * (y8 + T(8) * dr * y7)) / (c * c) : T(0);
+        T sw, sh;
+        one_minus_tanh(arg, sw, sh);
+        o.s1 = on ? sw : T(0);
+        o.ds1 = on ? -(T(Par::a1s) * (y8 + T(8) * dr * y7)) * sh : T(0);
     }
     {
         const T y = r - T(Par::b2s), y2 = y * y, y4 = y2 * y2, y6 = y4 * y2, y5 = y4 * y;
         const T arg = T(Par::a2s) * dr * y6;
         const bool on = arg < T(19);
-        const T c = rt::cosh(arg);
-        o.s2 = on ? T(1) - rt::tanh(arg) : T(0);
-        o.ds2 = on ? -(T(Par::a2s) * (y6 + T(6) * dr * y5)) / (c * c) : T(0);
+        T sw, sh;
+        one_minus_tanh(arg, sw, sh);
+        o.s2 = on ? sw : T(0);
+        o.ds2 = on ? -(T(Par::a2s) * (y6 + T(6) * dr * y5)) * sh : T(0);
     }
     {
         const T y = r - T(Par::b3s);
         const T arg = T(Par::a3s) * dr * (y * y);
         const bool on = arg < T(19);
-        const T c = rt::cosh(arg);
-        o.s3 = on ? T(1) - rt::tanh(arg) : T(0);
+        T sw, sh;
+        one_minus_tanh(arg, sw, sh);
+        o.s3 = on ? sw : T(0);
         o.ds3 = on ? -(T(Par::a3s) * (T(3) * r * r - T(2) * r * (r0ch + T(2) * T(Par::b3s))
-                                      + T(Par::b3s) * (T(Par::b3s) + T(2) * r0ch))) / (c * c)
+                                      + T(Par::b3s) * (T(Par::b3s) + T(2) * r0ch))) * sh
                    : T(0);
     }
     const bool near = r < T(3.8);
@@ -173,17 +188,19 @@ __device__ __forceinline__ void own_switches(Own<T>& o)
         const T y = r - T(Par::cphi);
         const T e = rt::exp(T(Par::bphi) * y * y * y);
         const T arg = T(Par::aphi) * dr * e;
-        const T c = rt::cosh(arg);
-        o.sphi = near ? T(1) - rt::tanh(arg) : T(0);
-        o.dsphi = near ? -(T(Par::aphi) * (T(1) + T(3) * T(Par::bphi) * dr * y * y) * e) / (c * c) : T(0);
+        T sw, sh;
+        one_minus_tanh(arg, sw, sh);
+        o.sphi = near ? sw : T(0);
+        o.dsphi = near ? -(T(Par::aphi) * (T(1) + T(3) * T(Par::bphi) * dr * y * y) * e) * sh : T(0);
     }
     {
         const T y = r - T(Par::ctheta);
         const T e = rt::exp(T(Par::btheta) * y * y * y);
         const T arg = T(Par::atheta) * dr * e;
-        const T c = rt::cosh(arg);
-        o.sth = near ? T(1) - rt::tanh(arg) : T(0);
-        o.dsth = near ? -(T(Par::atheta) * (T(1) + T(3) * T(Par::btheta) * dr * y * y) * e) / (c * c) : T(0);
+        T sw, sh;
+        one_minus_tanh(arg, sw, sh);
+        o.sth = near ? sw : T(0);
+        o.dsth = near ? -(T(Par::atheta) * (T(1) + T(3) * T(Par::btheta) * dr * y * y) * e) * sh : T(0);
     }
 }
 
@@ -286,9 +303,10 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
         const T arga = T(Par::c1ch) * (rav - r0ch);
         T ach;
         if (arga < T(19)) {
-            const T c = rt::cosh(arga);
-            ach = T(Par::a1ch) + T(Par::b1ch) * (rt::tanh(arga) + T(1)) * T(0.5);
-            dumach = T(Par::b1ch) * T(Par::c1ch) / (T(2) * (c * c));
+            T sw, sh;
+            one_minus_tanh(arga, sw, sh);
+            ach = T(Par::a1ch) + T(Par::b1ch) * (T(2) - sw) * T(0.5);
+            dumach = T(Par::b1ch) * T(Par::c1ch) * sh * T(0.5);
         } else {
             ach = T(Par::a1ch + Par::b1ch);
             dumach = T(0);
