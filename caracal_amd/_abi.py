@@ -12,6 +12,7 @@ CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
 CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD = 0, 1, 2
 CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA = 0, 1, 2
+CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT = 0, 1, 2
 CAM_MAX_ATOMS, CAM_MAX_BONDS, CAM_MAX_REAC = 16, 4, 4
 
 
@@ -35,6 +36,7 @@ class Config(ctypes.Structure):
         ("ntraj", ctypes.c_int32), ("precision", ctypes.c_int32), ("nm_mode", ctypes.c_int32),
         ("ensemble", ctypes.c_int32), ("remove_com", ctypes.c_int32), ("andersen_step", ctypes.c_int32),
         ("steps_per_launch", ctypes.c_int32), ("nm_kernel", ctypes.c_int32), ("waves_per_eu", ctypes.c_int32),
+        ("schedule", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dt", ctypes.c_double),
         ("mass", ctypes.c_double * CAM_MAX_ATOMS),
         ("seed", ctypes.c_uint64),
@@ -42,14 +44,15 @@ class Config(ctypes.Structure):
     ]
 
 
-CAM_MAX_TRIALS = 16
+CAM_MAX_TRIALS = 32
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int32), ("chosen", ctypes.c_int32),
         ("nm_kernel", ctypes.c_int32 * CAM_MAX_TRIALS), ("steps_per_launch", ctypes.c_int32 * CAM_MAX_TRIALS),
-        ("waves_per_eu", ctypes.c_int32 * CAM_MAX_TRIALS), ("us_per_step", ctypes.c_double * CAM_MAX_TRIALS),
+        ("waves_per_eu", ctypes.c_int32 * CAM_MAX_TRIALS), ("split", ctypes.c_int32 * CAM_MAX_TRIALS),
+        ("us_per_step", ctypes.c_double * CAM_MAX_TRIALS),
     ]
 
 

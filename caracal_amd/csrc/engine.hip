@@ -243,8 +243,14 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
 // The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN or CAM_ENS_CHILD.
 // One step: p -= dt/2 g; free ring-polymer propagation in normal-mode space;
 // [CHILD: xi of the centroid]; forces; p -= dt/2 g; [ANDERSEN: resample].
+//
+// SPLIT: the same step without its force phase, one step per launch, for the
+// split schedule (forces in force_kernel at its own register budget).  The
+// launch first finishes the previous step of the run (p -= dt/2 g with that
+// step's forces, Andersen), then kicks, propagates and evaluates xi; the run's
+// last half kick is final_kick_kernel.
 // ---------------------------------------------------------------------------
-template <class S, int NB, typename T, int ENS, int WPE, bool MF>
+template <class S, int NB, typename T, int ENS, int WPE, bool MF, bool SPLIT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void step_kernel(StepArgs a)
 {
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS;
@@ -255,8 +261,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     __shared__ __align__(16) double X[XS];
     __shared__ __align__(16) double cent[TPB * NC];
     __shared__ __align__(16) double Y[MF ? XS : 1];
-    __shared__ double prop[MF ? TPB * NB * 3 : 1];
-    __shared__ double smass[MF ? S::NA : 1];
+    // With two waves per SIMD (WPE 2) the step keeps nothing but the state in
+    // registers: the propagator table and the masses come from LDS, and q, p
+    // wait in LDS while the forces are evaluated, so that the force phase has
+    // the register budget of the force kernel alone.
+    constexpr bool STASH = WPE >= 2;
+    constexpr bool TAB = MF || STASH;
+    __shared__ double prop[TAB ? TPB * NB * 3 : 1];
+    __shared__ double smass[TAB ? S::NA : 1];
+    __shared__ double stash[STASH ? 2 * SL * kBlock : 1];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -282,7 +295,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // free ring-polymer propagator of mode k (src/verlet.f90:405-432)
     const double beta = a.beta[traj];
     double p1 = 1.0, p2[SL], p3[SL];
-    if (NB > 1 && k != 0) {
+    if constexpr (TAB) {
+        // coefficients come from the LDS table inside the step loop
+    } else if (NB > 1 && k != 0) {
         const int kk = k <= NB / 2 ? k : NB - k;
         const double wk = (2.0 / (beta / NB)) * sin(kk * (3.141592653589793 / NB));
         const double wt = wk * a.dt;
@@ -303,7 +318,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     const double hdt = 0.5 * a.dt;
     double vb = 0.0;
-    if constexpr (MF) {
+    if constexpr (TAB) {
         if (threadIdx.x < S::NA) smass[threadIdx.x] = a.mass[threadIdx.x];
         // per-trajectory mode table: cos(w dt), sin(w dt), w  (src/verlet.f90:413-432)
         for (int i = threadIdx.x; i < TPB * NB; i += kBlock) {
@@ -318,14 +333,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
     }
 
+    if constexpr (SPLIT) {
+        if (a.step_off > 0) {
+            // second half kick of the previous step (src/verlet.f90:1060-1062) and its Andersen draw
+#pragma unroll
+            for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
+            if constexpr (ENS == CAM_ENS_ANDERSEN) {
+                const uint64_t istep = *a.step_base + (uint64_t)a.step_off;
+                if (istep % (uint64_t)a.andersen_step == 0) {
+                    __syncthreads();      // the mass table is written before the first step
+#pragma unroll
+                    for (int s = 0; s < SL; ++s) {
+                        const double sd = sqrt((TAB ? smass[cc[s] / 3] : m[s]) / (beta / NB));
+                        p[s] = andersen_normal(a.seed, (uint32_t)traj, istep, cc[s], k) * sd;
+                    }
+                }
+            }
+        }
+    }
     for (int it = 0; it < a.nsteps; ++it) {
         // half kick (src/verlet.f90:216-218)
 #pragma unroll
         for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
 
         if constexpr (NB == 1) {
+            if constexpr (TAB) {
+                __syncthreads();          // the table is written before the first step
 #pragma unroll
-            for (int s = 0; s < SL; ++s) q[s] = p[s] * p3[s] + q[s];
+                for (int s = 0; s < SL; ++s) q[s] = p[s] * (a.dt / smass[cc[s] / 3]) + q[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < SL; ++s) q[s] = p[s] * p3[s] + q[s];
+            }
         } else if constexpr (MF) {
             double *xm = X + tl * 2 * NC * (NB + 1);
             __syncthreads();
@@ -371,11 +410,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     else yq[s] = y[s];
                 }
             }
+            if constexpr (TAB) {
+                // the same coefficients, from the table (mode k = this lane's bead index)
+                const double *pr = prop + (tl * NB + k) * 3;
+                const double cw = pr[0], sw = pr[1], wk = pr[2];
 #pragma unroll
-            for (int s = 0; s < SL; ++s) {
-                const double pn = yp[s] * p1 + yq[s] * p2[s];
-                yq[s] = yp[s] * p3[s] + yq[s] * p1;
-                yp[s] = pn;
+                for (int s = 0; s < SL; ++s) {
+                    const double mm = smass[cc[s] / 3];
+                    double c2, c3;
+                    if (k == 0) {
+                        c2 = 0.0;
+                        c3 = a.dt / mm;
+                    } else {
+                        const double wm = wk * mm;
+                        c2 = -wm * sw;
+                        c3 = sw / wm;
+                    }
+                    const double pn = yp[s] * cw + yq[s] * c2;
+                    yq[s] = yp[s] * c3 + yq[s] * cw;
+                    yp[s] = pn;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    const double pn = yp[s] * p1 + yq[s] * p2[s];
+                    yq[s] = yp[s] * p3[s] + yq[s] * p1;
+                    yp[s] = pn;
+                }
             }
             if constexpr (ENS == CAM_ENS_CHILD) {
                 // centroid of the propagated beads = mode 0 of q / sqrt(NB)
@@ -431,12 +492,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
         }
 
+        if constexpr (SPLIT) continue;     // one step per launch; forces come next
         // forces of every bead (src/verlet.f90:772-777)
         {
             double x[SL];
 #pragma unroll
             for (int s = 0; s < SL; ++s) x[s] = q[s];
+            if constexpr (STASH) {
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    stash[s * kBlock + tid] = q[s];
+                    stash[(SL + s) * kBlock + tid] = p[s];
+                }
+            }
             S::template egrad<T>(x, vb, g);
+            if constexpr (STASH) {
+#pragma unroll
+                for (int s = 0; s < SL; ++s) {
+                    q[s] = stash[s * kBlock + tid];
+                    p[s] = stash[(SL + s) * kBlock + tid];
+                }
+            }
         }
         // second half kick (src/verlet.f90:1060-1062)
 #pragma unroll
@@ -449,7 +525,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (istep % (uint64_t)a.andersen_step == 0) {
 #pragma unroll
                 for (int s = 0; s < SL; ++s) {
-                    const double sd = sqrt(m[s] / (beta / NB));
+                    const double sd = sqrt((TAB ? smass[cc[s] / 3] : m[s]) / (beta / NB));
                     p[s] = andersen_normal(a.seed, (uint32_t)traj, istep, cc[s], k) * sd;
                 }
             }
@@ -462,16 +538,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const size_t o = base + (size_t)cc[s] * NB + k;
                 a.q[o] = q[s];
                 a.p[o] = p[s];
-                a.g[o] = g[s];
+                if (!SPLIT) a.g[o] = g[s];
             }
-        if (r == 0) a.vbead[(size_t)traj * NB + k] = vb;
+        if (r == 0 && !SPLIT) a.vbead[(size_t)traj * NB + k] = vb;
     }
 }
 
-// Forces and bead energies of the current positions (mdinit, src/mdinit.f90:79-83).
-template <class S, int NB, typename T>
-__global__ __launch_bounds__(kBlock) void force_kernel(const double *__restrict__ qa, double *__restrict__ ga,
-                                                       double *__restrict__ vbead, int ntraj)
+// Last half kick of a split-schedule run (and its Andersen draw), over the
+// engine layout [traj][coord][bead].
+__global__ void final_kick_kernel(StepArgs a, int natoms, int nb)
+{
+    const size_t nc = (size_t)3 * natoms;
+    const size_t n = (size_t)a.ntraj * nc * nb;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = (int)(i % nb);
+    const int c = (int)((i / nb) % nc);
+    const int traj = (int)(i / (nc * nb));
+    double p = a.p[i] - 0.5 * a.dt * a.g[i];
+    if (a.andersen_step > 0) {
+        const uint64_t istep = *a.step_base + (uint64_t)a.nsteps;
+        if (istep % (uint64_t)a.andersen_step == 0) {
+            const double sd = sqrt(a.mass[c / 3] / (a.beta[traj] / nb));
+            p = andersen_normal(a.seed, (uint32_t)traj, istep, c, k) * sd;
+        }
+    }
+    a.p[i] = p;
+}
+
+// Forces and bead energies of the current positions (mdinit, src/mdinit.f90:79-83;
+// also the force phase of the split schedule).  WPE: waves per SIMD the
+// register budget is sized for.
+template <class S, int NB, typename T, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void force_kernel(
+    const double *__restrict__ qa, double *__restrict__ ga, double *__restrict__ vbead, int ntraj)
 {
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS, BPB = kBlock / L, TPB = BPB / NB;
     const int tid = threadIdx.x, r = tid % L, bib = tid / L, tl = bib / NB, k = bib % NB;
@@ -840,6 +940,13 @@ using namespace cam;
 typedef void (*step_fn)(StepArgs);
 typedef void (*force_fn)(const double *, double *, double *, int);
 
+// The kernels of one schedule: the fused step, the initial-force kernel, and
+// the normal-mode step and force kernel of the split schedule.
+struct Kernels {
+    step_fn step = nullptr, nm = nullptr;
+    force_fn force = nullptr, fsplit = nullptr;
+};
+
 struct cam_engine {
     cam_config cfg{};
     Rx rx{};
@@ -857,8 +964,8 @@ struct cam_engine {
     int spl = 1;                      // steps per launch
     int wpe = 1;
     bool mfma = false;
-    step_fn step = nullptr;
-    force_fn force = nullptr;
+    Kernels kern;                     // kernels of the current schedule
+    bool split = false;               // split schedule: force phase in its own kernel
     std::map<int, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -868,18 +975,18 @@ struct cam_engine {
 
 namespace {
 
-template <class S, int NB, typename T, int WPE>
+template <class S, int NB, typename T, int WPE, bool SPLIT = false>
 step_fn pick_step_w(int ens, bool mfma)
 {
     constexpr bool MF = NB >= 16 && NB % 16 == 0;
     if (mfma && MF) {
-        if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, MF>;
-        if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, MF>;
-        return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, MF>;
+        if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, MF, SPLIT>;
+        if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, MF, SPLIT>;
+        return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, MF, SPLIT>;
     }
-    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, false>;
-    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, false>;
-    return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, false>;
+    if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, false, SPLIT>;
+    if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, false, SPLIT>;
+    return step_kernel<S, NB, T, CAM_ENS_CHILD, WPE, false, SPLIT>;
 }
 
 // wpe: minimum waves per SIMD the compiler is asked to fit (1 or 2).  Only
@@ -893,14 +1000,25 @@ step_fn pick_step(int ens, int wpe, bool mfma)
     return pick_step_w<S, NB, T, 1>(ens, mfma);
 }
 
+// The force kernel of the split schedule gets two waves per SIMD for the FP64
+// CH5 surface (it then spills a few bytes), the default budget otherwise.
+template <class S, int NB, typename T>
+force_fn pick_force_split()
+{
+    if constexpr (S::L == 4 && std::is_same<T, double>::value) return force_kernel<S, NB, T, 2>;
+    else return force_kernel<S, NB, T, 1>;
+}
+
 template <class S, typename T>
-bool pick_nb(int nb, int ens, int wpe, bool mfma, step_fn &st, force_fn &fo)
+bool pick_nb(int nb, int ens, int wpe, bool mfma, step_fn &st, force_fn &fo, step_fn &nm, force_fn &fs)
 {
     switch (nb) {
-#define CAM_NB_CASE(N)                               \
-    case N:                                          \
-        st = pick_step<S, N, T>(ens, wpe, mfma);     \
-        fo = force_kernel<S, N, T>;                  \
+#define CAM_NB_CASE(N)                                                   \
+    case N:                                                              \
+        st = pick_step<S, N, T>(ens, wpe, mfma);                         \
+        fo = force_kernel<S, N, T, 1>;                                   \
+        nm = pick_step_w<S, N, double, 1, true>(ens, mfma);              \
+        fs = pick_force_split<S, N, T>();                                \
         return true;
         CAM_NB_CASE(1)
         CAM_NB_CASE(2)
@@ -961,15 +1079,15 @@ void nm_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double>
 
 namespace {
 
-bool select_kernels(cam_engine *e, bool mfma, step_fn &st, force_fn &fo)
+bool select_kernels(cam_engine *e, int wpe, bool mfma, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
-    const int nb = cfg->nbeads;
+    const int nb = cfg->nbeads, ens = cfg->ensemble;
     if (cfg->surface == CAM_SURFACE_H3)
-        return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, cfg->ensemble, e->wpe, mfma, st, fo)
-                                               : pick_nb<SurfH3, float>(nb, cfg->ensemble, e->wpe, mfma, st, fo);
-    return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, cfg->ensemble, e->wpe, mfma, st, fo)
-                                           : pick_nb<SurfCH4H, float>(nb, cfg->ensemble, e->wpe, mfma, st, fo);
+        return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                                               : pick_nb<SurfH3, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+    return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                                           : pick_nb<SurfCH4H, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
 }
 
 TrajArgs traj_args(cam_engine *e)
@@ -992,7 +1110,7 @@ TrajArgs traj_args(cam_engine *e)
 int run_forces(cam_engine *e)
 {
     const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
-    hipLaunchKernelGGL(e->force, dim3(grid), dim3(kBlock), 0, e->stream, e->q, e->g, e->vbead, e->cfg.ntraj);
+    hipLaunchKernelGGL(e->kern.force, dim3(grid), dim3(kBlock), 0, e->stream, e->q, e->g, e->vbead, e->cfg.ntraj);
     CAM_LAUNCH_CHECK();
     return CAM_OK;
 }
@@ -1055,24 +1173,50 @@ StepArgs step_args(cam_engine *e)
     return a;
 }
 
+// The kernel launches of nsteps steps on the state a points to: fused (spl
+// steps per launch, transrot after each launch when requested) or split (per
+// step the normal-mode kernel, then the force kernel; one last half kick).
+hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, StepArgs a, int nsteps,
+                        const TrajArgs *transrot)
+{
+    const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
+    hipError_t err = hipSuccess;
+    if (split) {
+        for (int j = 0; j < nsteps && err == hipSuccess; ++j) {
+            a.nsteps = 1;
+            a.step_off = j;
+            hipLaunchKernelGGL(kn.nm, dim3(grid), dim3(kBlock), 0, e->stream, a);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            hipLaunchKernelGGL(kn.fsplit, dim3(grid), dim3(kBlock), 0, e->stream, a.q, a.g, a.vbead, a.ntraj);
+            err = hipGetLastError();
+        }
+        if (err != hipSuccess) return err;
+        a.nsteps = nsteps;
+        a.andersen_step = e->cfg.ensemble == CAM_ENS_ANDERSEN ? e->cfg.andersen_step : 0;
+        const size_t n = (size_t)e->cfg.ntraj * 3 * e->cfg.natoms * e->cfg.nbeads;
+        hipLaunchKernelGGL(final_kick_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, a,
+                           e->cfg.natoms, e->cfg.nbeads);
+        return hipGetLastError();
+    }
+    const int tgrid = (e->cfg.ntraj + 127) / 128;
+    for (int done = 0; done < nsteps && err == hipSuccess; done += a.nsteps) {
+        a.nsteps = std::min(spl, nsteps - done);
+        a.step_off = done;
+        hipLaunchKernelGGL(kn.step, dim3(grid), dim3(kBlock), 0, e->stream, a);
+        err = hipGetLastError();
+        if (err == hipSuccess && transrot) {
+            hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, e->stream, *transrot);
+            err = hipGetLastError();
+        }
+    }
+    return err;
+}
+
 int enqueue_run(cam_engine *e, int nsteps)
 {
-    StepArgs a = step_args(e);
-    const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
     const TrajArgs ta = traj_args(e);
-    const int tgrid = (e->cfg.ntraj + 127) / 128;
-    int done = 0;
-    while (done < nsteps) {
-        a.nsteps = std::min(e->spl, nsteps - done);
-        a.step_off = done;
-        hipLaunchKernelGGL(e->step, dim3(grid), dim3(kBlock), 0, e->stream, a);
-        CAM_LAUNCH_CHECK();
-        if (e->cfg.remove_com) {
-            hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, e->stream, ta);
-            CAM_LAUNCH_CHECK();
-        }
-        done += a.nsteps;
-    }
+    CAM_HIP(launch_steps(e, e->kern, e->split, e->spl, step_args(e), nsteps, e->cfg.remove_com ? &ta : nullptr));
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
         hipLaunchKernelGGL(kappa_reduce_kernel, dim3(nsteps), dim3(256), 0, e->stream, e->xi_flag, e->weight,
                            e->cfg.ntraj, e->num, e->step_base);
@@ -1105,6 +1249,10 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
                   "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
     CAM_CHECK_ARG(cfg->waves_per_eu >= 0 && cfg->waves_per_eu <= 2, "waves_per_eu must be 0, 1 or 2");
+    CAM_CHECK_ARG(cfg->schedule >= CAM_SCHED_AUTO && cfg->schedule <= CAM_SCHED_SPLIT, "bad schedule");
+    CAM_CHECK_ARG(cfg->schedule != CAM_SCHED_SPLIT || !cfg->remove_com,
+                  "remove_com needs the fused schedule (transrot after every step)");
+    CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
@@ -1158,7 +1306,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     if (cfg->remove_com) e->spl = 1;
     e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 ? 2 : 1;
     e->mfma = cfg->nm_kernel == CAM_NMK_MFMA;
-    const bool ok = select_kernels(e.get(), e->mfma, e->step, e->force);
+    e->split = cfg->schedule == CAM_SCHED_SPLIT;
+    const bool ok = select_kernels(e.get(), e->wpe, e->mfma, e->kern);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
         return CAM_ERR_UNSUPPORTED;
@@ -1391,7 +1540,7 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_HIP(hipGraphLaunch(g, e->stream));
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
     e->child_steps += nsteps;
-    e->last_launches = (nsteps + e->spl - 1) / e->spl;
+    e->last_launches = e->split ? nsteps : (nsteps + e->spl - 1) / e->spl;
     e->last_ms = -1.0;
     return CAM_OK;
 }
@@ -1487,8 +1636,15 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     uint64_t *ctr = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = CAM_OK;
-    struct Trial { int nmk, wpe, spl; double us; };
-    std::vector<Trial> trials;
+    struct Trial { int nmk, wpe, spl, split; double us; };
+    std::vector<Trial> cand, trials;
+    const bool try_fused = c.schedule != CAM_SCHED_SPLIT, try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
+    for (int nmk : nmks) {
+        if (try_fused)
+            for (int wpe : wpes)
+                for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
+        if (try_split) cand.push_back({ nmk, 1, 1, 1, 0.0 });
+    }
     auto fail = [&](hipError_t err) {
         set_error("HIP error in autotune: %s", hipGetErrorString(err));
         rc = CAM_ERR_DEVICE;
@@ -1499,55 +1655,36 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     if (err == hipSuccess) err = hipEventCreate(&t0);
     if (err == hipSuccess) err = hipEventCreate(&t1);
     if (err != hipSuccess) fail(err);
-    const int wpe0 = e->wpe;
-    for (int nmk : nmks)
-      for (int wpe : wpes) {
+    for (const Trial &tr : cand) {
         if (rc != CAM_OK) break;
-        step_fn st;
-        force_fn fo;
-        e->wpe = wpe;
-        const bool okk = select_kernels(e, nmk == CAM_NMK_MFMA, st, fo);
-        e->wpe = wpe0;
-        if (!okk) continue;
-        for (int spl : spls) {
-            StepArgs a = step_args(e);
-            a.q = buf;
-            a.p = buf + nstate;
-            a.g = buf + 2 * nstate;
-            a.vbead = buf + 3 * nstate;
-            a.xi_flag = flags;
-            a.step_base = ctr;
-            const int grid = (c.ntraj + e->tpb - 1) / e->tpb;
-            err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
-            if (err == hipSuccess) err = hipMemcpyAsync(a.q, e->q, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
-            if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
-            if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
-            if (err != hipSuccess) {
-                fail(err);
-                break;
-            }
-            // one untimed launch, then the timed trial
-            a.nsteps = std::min(spl, trial_steps);
-            a.step_off = 0;
-            hipLaunchKernelGGL(st, dim3(grid), dim3(kBlock), 0, e->stream, a);
-            err = hipGetLastError();
-            if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
-            for (int done = 0; err == hipSuccess && done < trial_steps; done += a.nsteps) {
-                a.nsteps = std::min(spl, trial_steps - done);
-                a.step_off = done;
-                hipLaunchKernelGGL(st, dim3(grid), dim3(kBlock), 0, e->stream, a);
-                err = hipGetLastError();
-            }
-            if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
-            if (err == hipSuccess) err = hipEventSynchronize(t1);
-            float ms = 0.f;
-            if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
-            if (err != hipSuccess) {
-                fail(err);
-                break;
-            }
-            trials.push_back({ nmk, wpe, spl, 1e3 * ms / trial_steps });
+        Kernels kn;
+        if (!select_kernels(e, tr.wpe, tr.nmk == CAM_NMK_MFMA, kn)) continue;
+        StepArgs a = step_args(e);
+        a.q = buf;
+        a.p = buf + nstate;
+        a.g = buf + 2 * nstate;
+        a.vbead = buf + 3 * nstate;
+        a.xi_flag = flags;
+        a.step_base = ctr;
+        err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(a.q, e->q, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+        if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+        // one untimed launch sequence, then the timed trial
+        if (err == hipSuccess) err = launch_steps(e, kn, tr.split != 0, tr.spl, a, std::min(tr.spl, trial_steps), nullptr);
+        if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
+        if (err == hipSuccess) err = launch_steps(e, kn, tr.split != 0, tr.spl, a, trial_steps, nullptr);
+        if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+        if (err != hipSuccess) {
+            fail(err);
+            break;
         }
+        Trial done = tr;
+        done.us = 1e3 * ms / trial_steps;
+        trials.push_back(done);
     }
     if (buf) (void)hipFree(buf);
     if (flags) (void)hipFree(flags);
@@ -1559,14 +1696,13 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     size_t best = 0;
     for (size_t i = 1; i < trials.size(); ++i)
         if (trials[i].us < trials[best].us) best = i;
-    step_fn st;
-    force_fn fo;
+    Kernels kn;
     e->wpe = trials[best].wpe;
-    select_kernels(e, trials[best].nmk == CAM_NMK_MFMA, st, fo);
+    select_kernels(e, e->wpe, trials[best].nmk == CAM_NMK_MFMA, kn);
     e->mfma = trials[best].nmk == CAM_NMK_MFMA;
-    e->step = st;
-    e->force = fo;
+    e->kern = kn;
     e->spl = trials[best].spl;
+    e->split = trials[best].split != 0;
     drop_graphs(e);
     if (out) {
         std::memset(out, 0, sizeof *out);
@@ -1576,6 +1712,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
             out->nm_kernel[i] = trials[i].nmk;
             out->steps_per_launch[i] = trials[i].spl;
             out->waves_per_eu[i] = trials[i].wpe;
+            out->split[i] = trials[i].split;
             out->us_per_step[i] = trials[i].us;
         }
     }

@@ -48,7 +48,7 @@ class Reaction:
 
 def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64, nm_mode=_abi.CAM_NM_REFERENCE,
                 ensemble=_abi.CAM_ENS_NVE, remove_com=False, andersen_step=0, steps_per_launch=0, seed=0,
-                reaction=None, nm_kernel=_abi.CAM_NMK_AUTO, waves_per_eu=0):
+                reaction=None, nm_kernel=_abi.CAM_NMK_AUTO, waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO):
     """cam_config of one engine (or of one bucket of a batch)."""
     natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
     masses = np.asarray(masses, dtype=np.float64)
@@ -60,6 +60,7 @@ def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_F
     c.remove_com, c.andersen_step, c.steps_per_launch = int(remove_com), andersen_step, steps_per_launch
     c.nm_kernel = nm_kernel
     c.waves_per_eu = waves_per_eu
+    c.schedule = schedule
     c.dt = dt
     for i, m in enumerate(masses):
         c.mass[i] = m
@@ -76,11 +77,11 @@ class RPMDEngine:
     def __init__(self, surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64,
                  nm_mode=_abi.CAM_NM_REFERENCE, ensemble=_abi.CAM_ENS_NVE, remove_com=False,
                  andersen_step=0, steps_per_launch=0, seed=0, reaction=None, nm_kernel=_abi.CAM_NMK_AUTO,
-                 waves_per_eu=0):
+                 waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO):
         c = make_config(surface, nbeads, ntraj, dt, masses, precision=precision, nm_mode=nm_mode,
                         ensemble=ensemble, remove_com=remove_com, andersen_step=andersen_step,
                         steps_per_launch=steps_per_launch, seed=seed, reaction=reaction, nm_kernel=nm_kernel,
-                        waves_per_eu=waves_per_eu)
+                        waves_per_eu=waves_per_eu, schedule=schedule)
         h = ctypes.c_void_p()
         check("cam_engine_create", lib().cam_engine_create(ctypes.byref(c), ctypes.byref(h)))
         self._attach(c, h, owned=True)
@@ -184,8 +185,8 @@ class RPMDEngine:
         fastest; returns the trials as a list of dicts and the chosen one."""
         t = _abi.Tuning()
         check("cam_engine_autotune", lib().cam_engine_autotune(self._h, trial_steps, ctypes.byref(t)))
-        trials = [dict(nm_kernel=t.nm_kernel[i], waves_per_eu=t.waves_per_eu[i], steps_per_launch=t.steps_per_launch[i],
-                       us_per_step=t.us_per_step[i]) for i in range(t.n)]
+        trials = [dict(nm_kernel=t.nm_kernel[i], split=t.split[i], waves_per_eu=t.waves_per_eu[i],
+                       steps_per_launch=t.steps_per_launch[i], us_per_step=t.us_per_step[i]) for i in range(t.n)]
         return trials, trials[t.chosen]
 
     def last_timing(self):

@@ -20,6 +20,7 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2
    integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2
+   integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
    integer, parameter, public :: CAM_MAX_ATOMS = 16, CAM_MAX_BONDS = 4, CAM_MAX_REAC = 4
 
    ! cam_reaction: the BIMOLEC reaction coordinate (MECHA section keys)
@@ -39,18 +40,18 @@ module caracal_amd_mod
       integer(c_int32_t) :: surface = CAM_SURFACE_H3, natoms = 3, nbeads = 1, ntraj = 1
       integer(c_int32_t) :: precision = CAM_PREC_FP64, nm_mode = CAM_NM_REFERENCE
       integer(c_int32_t) :: ensemble = CAM_ENS_NVE, remove_com = 0, andersen_step = 0
-      integer(c_int32_t) :: steps_per_launch = 0, nm_kernel = 0, waves_per_eu = 0
+      integer(c_int32_t) :: steps_per_launch = 0, nm_kernel = 0, waves_per_eu = 0, schedule = 0, reserved = 0
       real(c_double) :: dt = 0.d0
       real(c_double) :: mass(CAM_MAX_ATOMS) = 0.d0
       integer(c_int64_t) :: seed = 0
       type(cam_reaction) :: reaction
    end type cam_config
 
-   integer, parameter, public :: CAM_MAX_TRIALS = 16
+   integer, parameter, public :: CAM_MAX_TRIALS = 32
    type, bind(C), public :: cam_tuning
       integer(c_int32_t) :: n = 0, chosen = 0
       integer(c_int32_t) :: nm_kernel(CAM_MAX_TRIALS) = 0, steps_per_launch(CAM_MAX_TRIALS) = 0
-      integer(c_int32_t) :: waves_per_eu(CAM_MAX_TRIALS) = 0
+      integer(c_int32_t) :: waves_per_eu(CAM_MAX_TRIALS) = 0, split(CAM_MAX_TRIALS) = 0
       real(c_double) :: us_per_step(CAM_MAX_TRIALS) = 0.d0
    end type cam_tuning
 

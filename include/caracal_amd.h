@@ -82,6 +82,13 @@ enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
  * nbeads a multiple of 16).  AUTO takes the vector-ALU form.  Both give the
  * same transform to rounding. */
 enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2 };
+/* How a step is launched.  FUSED: the whole step, forces included, in one
+ * kernel that advances steps_per_launch steps with the state in registers.
+ * SPLIT: per step a normal-mode kernel and a force kernel (each at its own
+ * register budget), the state in HBM in between.  AUTO: FUSED unless the
+ * autotuner measures SPLIT faster.  Both give the same trajectory to
+ * rounding. */
+enum { CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2 };
 
 #define CAM_MAX_ATOMS 16
 #define CAM_MAX_BONDS 4
@@ -118,6 +125,8 @@ typedef struct {
     int32_t nm_kernel;        /* CAM_NMK_* */
     int32_t waves_per_eu;     /* register budget of the step kernel: 1 = up to 512 registers per lane,
                                  2 = 256 (two waves per SIMD; FP64 CH4H only); 0 = 1 unless tuned */
+    int32_t schedule;         /* CAM_SCHED_* */
+    int32_t reserved;         /* must be 0 */
     double dt;                /* time step, atomic time units */
     double mass[CAM_MAX_ATOMS];   /* electron masses */
     uint64_t seed;            /* counter-based RNG key for Andersen draws */
@@ -194,18 +203,20 @@ int cam_engine_device_state(cam_engine *eng, double **q, double **p, double **dv
 /* Schedule autotuner.  Times candidate schedules of the step kernel on a
  * copy of the current state (the engine's trajectories, kappa sums and RNG
  * counters are not touched) for trial_steps steps each, and keeps the fastest:
- * the normal-mode unit (VALU or MFMA; only when nm_kernel is AUTO), the
- * register budget (waves_per_eu 1 or 2; FP64 CH4H with waves_per_eu 0) and
- * the steps fused per launch (10, 25, 50, 100; only when steps_per_launch is
- * 0 and remove_com is off).  Captured graphs are dropped.  No counterpart in
+ * the normal-mode unit (VALU or MFMA; only when nm_kernel is AUTO), fused or
+ * split launches (schedule AUTO, remove_com off), and for the fused schedule
+ * the register budget (waves_per_eu 1 or 2; FP64 CH4H with waves_per_eu 0)
+ * and the steps fused per launch (10, 25, 50, 100; only when
+ * steps_per_launch is 0 and remove_com is off).  Captured graphs are dropped.  No counterpart in
  * the reference, which has one schedule. */
-#define CAM_MAX_TRIALS 16
+#define CAM_MAX_TRIALS 32
 typedef struct {
     int32_t n;                              /* candidates timed */
     int32_t chosen;                         /* index of the schedule kept */
     int32_t nm_kernel[CAM_MAX_TRIALS];      /* CAM_NMK_VALU or CAM_NMK_MFMA */
     int32_t steps_per_launch[CAM_MAX_TRIALS];
     int32_t waves_per_eu[CAM_MAX_TRIALS];
+    int32_t split[CAM_MAX_TRIALS];          /* 1: the split schedule */
     double us_per_step[CAM_MAX_TRIALS];     /* microseconds per step of the whole batch */
 } cam_tuning;
 int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);

@@ -229,7 +229,7 @@ def test_autotune_leaves_the_state_and_keeps_parity():
         if tune:
             trials, best = e.autotune(20)
             print("autotune trials", trials, "chosen", best)
-            assert len(trials) == 16 and best in trials
+            assert len(trials) == 18 and best in trials
             q1, p1 = e.get_state()
             assert np.array_equal(q0, q1) and np.array_equal(p0, p1)
         e.run(60)
@@ -239,6 +239,43 @@ def test_autotune_leaves_the_state_and_keeps_parity():
     assert np.abs(qa - qb).max() <= 1e-11 * np.abs(qa).max()
     assert da == db
     np.testing.assert_allclose(na, nb_, rtol=0, atol=1e-12 * max(1.0, np.abs(na).max()))
+
+
+@pytest.mark.parametrize("ens", [cam.CAM_ENS_NVE, cam.CAM_ENS_ANDERSEN, cam.CAM_ENS_CHILD])
+@pytest.mark.parametrize("surface,nb", [(cam.CAM_SURFACE_CH4H, 32), (cam.CAM_SURFACE_CH4H, 8), (cam.CAM_SURFACE_H3, 16)])
+def test_split_schedule_matches_fused(ens, surface, nb):
+    """The split schedule (normal-mode kernel + force kernel per step, last
+    half kick at the end) follows the fused trajectory to rounding over 60
+    steps, including Andersen redraws and the recrossing flags."""
+    if surface == cam.CAM_SURFACE_CH4H:
+        w = workloads.ch4h_children(ntraj=40, nbeads=nb, seed=8)
+        q, p, beta, rx = w["q"], w["p"], w["beta"], w["reaction"]
+        rng = np.random.default_rng(8)
+        p = rng.normal(size=q.shape) * np.sqrt(w["masses"][None, None, :, None] * nb / beta[:, None, None, None])
+    else:
+        w = workloads.h3_nve(nbeads=nb)
+        q, p, beta, rx = np.stack([w["q"]] * 8), np.stack([w["p"]] * 8), np.full(8, w["beta"]), workloads.h3_reaction()
+    out = []
+    for sched in (cam.CAM_SCHED_FUSED, cam.CAM_SCHED_SPLIT):
+        e = cam.RPMDEngine(surface, nb, q.shape[0], w["dt"], w["masses"], ensemble=ens, andersen_step=7, seed=5,
+                           reaction=rx, schedule=sched, steps_per_launch=25)
+        e.set_state(q, p, beta)
+        if ens == cam.CAM_ENS_CHILD:
+            e.begin_children()
+        e.run(30)
+        e.run(30)
+        qq, pp = e.get_state()
+        kap = e.kappa(60) if ens == cam.CAM_ENS_CHILD else None
+        out.append((qq, pp, kap))
+        e.close()
+    (q0, p0, k0), (q1, p1, k1) = out
+    err = np.abs(q0 - q1).max() / np.abs(q0).max()
+    print(f"ens={ens} surface={surface} nb={nb}: split vs fused max rel position difference {err:.2e}")
+    assert err <= 1e-11
+    assert np.abs(p0 - p1).max() <= 1e-9 * np.abs(p0).max()
+    if k0 is not None:
+        assert k0[1] == k1[1]
+        np.testing.assert_allclose(k0[0], k1[0], rtol=0, atol=1e-12 * max(1.0, np.abs(k0[0]).max()))
 
 
 def test_mixed_precision_engine_tracks_fp64():
