@@ -52,11 +52,11 @@ def step_flops(natoms, nbeads):
 
 
 def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0,
-                     nm_kernel=None):
+                     nm_kernel=None, schedule=0):
     w = workloads.ch4h_children(ntraj=ntraj, nbeads=nbeads, seed=seed)
     e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nbeads, ntraj, w["dt"], w["masses"], precision=precision,
                        ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"], seed=seed + 17,
-                       steps_per_launch=steps_per_launch,
+                       steps_per_launch=steps_per_launch, schedule=schedule,
                        nm_kernel=cam.CAM_NMK_AUTO if nm_kernel is None else nm_kernel)
     e.set_state(w["q"], w["p"], w["beta"])
     st = e.constrain(project_p=False)
@@ -277,8 +277,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
     ap.add_argument("--schedule", default=None,
-                    help="fixed step-kernel schedule 'valu|mfma,steps_per_launch' instead of the autotuner "
-                         "(for profiling: every launch then has the same shape)")
+                    help="fixed schedule 'valu|mfma,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
+                         "the autotuner (for profiling: every launch then has the same shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -305,9 +305,10 @@ def main():
     if args.schedule:
         unit, spl_fixed = args.schedule.split(",")
         nmk = cam.CAM_NMK_MFMA if unit == "mfma" else cam.CAM_NMK_VALU
-        e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64, steps_per_launch=int(spl_fixed),
-                                nm_kernel=nmk)
-        trials, best = [], {"nm_kernel": nmk, "steps_per_launch": int(spl_fixed)}
+        split = spl_fixed == "split"
+        e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64, steps_per_launch=0 if split else int(spl_fixed),
+                                nm_kernel=nmk, schedule=cam.CAM_SCHED_SPLIT if split else cam.CAM_SCHED_FUSED)
+        trials, best = [], {"nm_kernel": nmk, "split": int(split), "steps_per_launch": 1 if split else int(spl_fixed)}
     else:
         e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
         trials, best = e.autotune(100)
@@ -320,7 +321,11 @@ def main():
     flops_launch = step_flops(6, nb) * ntraj * spl
     achieved = flops_launch / (ms_launch * 1e-3) / 1e12
     mf = best["nm_kernel"] == cam.CAM_NMK_MFMA
-    kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes>"
+    if best.get("split"):
+        kernel = (f"split step: step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes, "
+                  f"no forces> + force_kernel<SurfCH4H, {nb}, double>")
+    else:
+        kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes>"
 
     out = {
         "metric": "bead-force-evals/s (batched RPMD children, CH4H, 32 beads x 4096 trajectories)",
@@ -338,7 +343,7 @@ def main():
         "config": {"workload": "ch4h_children", "surface": "CH5 (egrad_ch4h)", "ntraj": ntraj, "nbeads": nb,
                    "natoms": 6, "dt_fs": 0.1, "kelvin": 500, "precision": "fp64",
                    "nm_mode": "reference cosine transform", "steps_per_launch": spl,
-                   "nm_kernel": "mfma" if mf else "valu"},
+                   "nm_kernel": "mfma" if mf else "valu", "schedule": "split" if best.get("split") else "fused"},
         "rpmd_steps_per_s": steps_per_s,
         "kappa_final": float(num[-1] / den) if den > 0 else None,
         "autotune": trials,

@@ -63,6 +63,7 @@ struct StepArgs {
     const double *beta;          // [ntraj]
     const double *tfwd, *tinv;   // [nbeads][nbeads], element (j,k) = T[k][j]
     uint8_t *xi_flag;            // CHILD: [nsteps_run][ntraj]
+    double *cbuf;                // CHILD, fused: centroids of this launch's steps [step][traj][3*natoms]
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
     int andersen_step;
@@ -71,6 +72,55 @@ struct StepArgs {
     double mass[CAM_MAX_ATOMS];
     Rx rx;
 };
+
+// xi of the centroid without its gradient, for the per-step child flags
+// (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213).  One lane evaluates it
+// while its block waits, so everything it reads is in LDS or registers:
+// wcom[k][at] = mass[at] / M_k for the atoms of fragment k (0 otherwise), and
+// the bond loops are unrolled to their maximum under run-time guards.  The
+// fragment centres of mass are sums of weighted positions, which differs from
+// calc_xi's sum of mass * position / M_k only in rounding.
+template <int NA>
+__device__ double calc_xi_value(const Rx &rx, const double (*wcom)[NA], const double *c)
+{
+    double s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < CAM_MAX_BONDS; ++i)
+        if (i < rx.n_break) {
+            const int a = rx.brk[i][0], b = rx.brk[i][1];
+            const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
+            s1 = s1 + (sqrt(dx * dx + dy * dy + dz * dz) - rx.break_ref[i]) / (double)rx.n_break;
+        }
+#pragma unroll
+    for (int i = 0; i < CAM_MAX_BONDS; ++i)
+        if (i < rx.n_form) {
+            const int a = rx.form[i][0], b = rx.form[i][1];
+            const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
+            s1 = s1 - (sqrt(dx * dx + dy * dy + dz * dz) - rx.form_ref[i]) / (double)rx.n_form;
+        }
+    double com[CAM_MAX_REAC][3];
+#pragma unroll
+    for (int k = 0; k < CAM_MAX_REAC; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int at = 0; at < NA; ++at) acc = fma(wcom[k][at], c[3 * at + j], acc);
+            com[k][j] = acc;
+        }
+    double s0 = 0.0;
+    const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
+#pragma unroll
+    for (int i = 0; i < CAM_MAX_REAC; ++i)
+#pragma unroll
+        for (int j = i + 1; j < CAM_MAX_REAC; ++j)
+            if (j < rx.n_reac) {
+                const double dx = com[j][0] - com[i][0], dy = com[j][1] - com[i][1], dz = com[j][2] - com[i][2];
+                s0 = s0 + (rx.r_inf - sqrt(dx * dx + dy * dy + dz * dz));
+            }
+    s0 = s0 / (double)terms;
+    return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
+}
 
 // xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213),
 // c: centroid (3*natoms, atom-major).  Optionally the gradient dxi (3*natoms).
@@ -270,6 +320,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     __shared__ double prop[TAB ? TPB * NB * 3 : 1];
     __shared__ double smass[TAB ? S::NA : 1];
     __shared__ double stash[STASH ? 2 * SL * kBlock : 1];
+    __shared__ double wcom[ENS == CAM_ENS_CHILD ? CAM_MAX_REAC : 1][S::NA];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -318,6 +369,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     const double hdt = 0.5 * a.dt;
     double vb = 0.0;
+    if constexpr (ENS == CAM_ENS_CHILD) {
+        // fragment weights of the centre-of-mass sums (calc_com, src/calc_com.f90:36-59)
+        if (threadIdx.x < CAM_MAX_REAC * S::NA) {
+            const int kf = threadIdx.x / S::NA, at = threadIdx.x % S::NA;
+            double w = 0.0;
+            if (kf < a.rx.n_reac)
+                for (int i = 0; i < a.rx.reac_n[kf]; ++i)
+                    if (a.rx.reac_at[kf][i] == at) w = a.mass[at] / a.rx.reac_mass[kf];
+            wcom[kf][at] = w;
+        }
+    }
     if constexpr (TAB) {
         if (threadIdx.x < S::NA) smass[threadIdx.x] = a.mass[threadIdx.x];
         // per-trajectory mode table: cos(w dt), sin(w dt), w  (src/verlet.f90:413-432)
@@ -486,9 +548,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     if (S::valid(s, r)) cent[tl * NC + cc[s]] = q[s];
                 __syncthreads();
             }
-            if (k == 0 && r == 0 && act) {
-                const double xi = calc_xi<S::NA>(a.rx, a.mass, cent + tl * NC, nullptr);
-                a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
+            if constexpr (SPLIT) {
+                if (k == 0 && r == 0 && act) {
+                    const double xi = calc_xi_value<S::NA>(a.rx, wcom, cent + tl * NC);
+                    a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
+                }
+            } else {
+                // xi of these centroids is evaluated by xi_flag_kernel after the
+                // launch, at full occupancy, instead of by one lane while its SIMD waits
+                for (int i = tid; i < TPB * NC; i += kBlock) {
+                    const int tr = blockIdx.x * TPB + i / NC;
+                    if (tr < a.ntraj) a.cbuf[((size_t)it * a.ntraj + tr) * NC + i % NC] = cent[i];
+                }
             }
         }
 
@@ -564,6 +635,31 @@ __global__ void final_kick_kernel(StepArgs a, int natoms, int nb)
         }
     }
     a.p[i] = p;
+}
+
+// xi flags of the child steps of one fused launch: one thread per (step,
+// trajectory) on the centroids the step kernel wrote.
+template <int NA>
+__global__ __launch_bounds__(256) void xi_flag_kernel(StepArgs a)
+{
+    __shared__ double wcom[CAM_MAX_REAC][NA];
+    if (threadIdx.x < CAM_MAX_REAC * NA) {
+        const int kf = threadIdx.x / NA, at = threadIdx.x % NA;
+        double w = 0.0;
+        if (kf < a.rx.n_reac)
+            for (int i = 0; i < a.rx.reac_n[kf]; ++i)
+                if (a.rx.reac_at[kf][i] == at) w = a.mass[at] / a.rx.reac_mass[kf];
+        wcom[kf][at] = w;
+    }
+    __syncthreads();
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)a.nsteps * a.ntraj) return;
+    const int it = (int)(i / a.ntraj), traj = (int)(i % a.ntraj);
+    double c[3 * NA];
+#pragma unroll
+    for (int j = 0; j < 3 * NA; ++j) c[j] = a.cbuf[i * 3 * NA + j];
+    const double xi = calc_xi_value<NA>(a.rx, wcom, c);
+    a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
 }
 
 // Forces and bead energies of the current positions (mdinit, src/mdinit.f90:79-83;
@@ -955,6 +1051,7 @@ struct cam_engine {
     double *tfwd = nullptr, *tinv = nullptr;
     double *weight = nullptr, *den_part = nullptr, *den = nullptr, *num = nullptr;
     uint8_t *xi_flag = nullptr;
+    double *cbuf = nullptr;           // CHILD: centroids of one fused launch
     uint64_t *step_base = nullptr;    // [0] steps since creation, [1] steps since begin_children
     double *scratch_io = nullptr;     // staging for layout conversion
     int num_cap = 0;                  // doubles in num (child steps)
@@ -1163,6 +1260,7 @@ StepArgs step_args(cam_engine *e)
     a.tfwd = e->tfwd;
     a.tinv = e->tinv;
     a.xi_flag = e->xi_flag;
+    a.cbuf = e->cbuf;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
     a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
@@ -1205,6 +1303,12 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
         a.step_off = done;
         hipLaunchKernelGGL(kn.step, dim3(grid), dim3(kBlock), 0, e->stream, a);
         err = hipGetLastError();
+        if (err == hipSuccess && e->cfg.ensemble == CAM_ENS_CHILD) {
+            const unsigned xgrid = (unsigned)(((size_t)a.nsteps * a.ntraj + 255) / 256);
+            if (e->cfg.natoms == 3) hipLaunchKernelGGL(xi_flag_kernel<3>, dim3(xgrid), dim3(256), 0, e->stream, a);
+            else hipLaunchKernelGGL(xi_flag_kernel<6>, dim3(xgrid), dim3(256), 0, e->stream, a);
+            err = hipGetLastError();
+        }
         if (err == hipSuccess && transrot) {
             hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, e->stream, *transrot);
             err = hipGetLastError();
@@ -1320,6 +1424,11 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_HIP(hipMalloc(&e->scratch_io, nstate * sizeof(double)));
     CAM_HIP(hipMalloc(&e->vbead, (size_t)cfg->ntraj * nb * sizeof(double)));
     CAM_HIP(hipMalloc(&e->beta, (size_t)cfg->ntraj * sizeof(double)));
+    if (cfg->ensemble == CAM_ENS_CHILD) {
+        // centroids of one fused launch: the longest schedule the autotuner tries, or the configured one
+        const size_t steps = (size_t)std::max(e->spl, 100);
+        CAM_HIP(hipMalloc(&e->cbuf, steps * cfg->ntraj * 3 * na * sizeof(double)));
+    }
     CAM_HIP(hipMalloc(&e->weight, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den_part, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den, sizeof(double)));
@@ -1345,7 +1454,7 @@ int cam_engine_destroy(cam_engine *e)
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->weight, e->den_part, e->den,
-                       e->num, e->scratch_io };
+                       e->num, e->scratch_io, e->cbuf };
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (e->xi_flag) (void)hipFree(e->xi_flag);
