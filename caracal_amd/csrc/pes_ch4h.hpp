@@ -53,67 +53,75 @@ __device__ __forceinline__ double akk() { return 0.1353000 * Par::fact2; }
 __device__ __forceinline__ constexpr int comp0(int i, int j) { return (i != 0 && j != 0) ? 0 : (i != 1 && j != 1) ? 1 : 2; }
 __device__ __forceinline__ constexpr int comp1(int i, int j) { return (i != 3 && j != 3) ? 3 : (i != 2 && j != 2) ? 2 : 1; }
 
+// Value of lane r + n of the quad (frame index n); n is a compile-time
+// constant wherever it is used, so this folds to one DPP move.
 template <typename T>
-struct Switch {
-    T s1[4], ds1[4], s2[4], ds2[4], s3[4], ds3[4], sphi[4], dsphi[4], sth[4], dsth[4];
-};
+__device__ __forceinline__ T fr(T v, int n)
+{
+    switch (n & 3) {
+    case 0: return v;
+    case 1: return quad::rot<1>(v);
+    case 2: return quad::rot<2>(v);
+    default: return quad::rot<3>(v);
+    }
+}
 
 // theta0(i,j) and d theta0(i,j) / d rch(k) (refangles_ch4h, src/egrad_ch4h.f:367-501)
-template <typename T>
-__device__ __forceinline__ T theta0(const Switch<T>& s, int i, int j)
+template <typename T, class V>
+__device__ __forceinline__ T theta0(const V& s, int i, int j)
 {
     const T tau = T(1.9106332362490186);           // acos(-1/3)
     const T a = tau - T(1.5707963267948966), b = tau - T(2.0943951023931953);
     const int k0 = comp0(i, j), l0 = comp1(i, j);
-    return tau + a * (s.sphi[i] * s.sphi[j] - T(1)) + b * (s.sth[k0] * s.sth[l0] - T(1));
+    return tau + a * (s.sphi(i) * s.sphi(j) - T(1)) + b * (s.sth(k0) * s.sth(l0) - T(1));
 }
-template <typename T>
-__device__ __forceinline__ T dtheta0(const Switch<T>& s, int i, int j, int k)
+template <typename T, class V>
+__device__ __forceinline__ T dtheta0(const V& s, int i, int j, int k)
 {
     const T tau = T(1.9106332362490186);
     const T a = tau - T(1.5707963267948966), b = tau - T(2.0943951023931953);
     const int k0 = comp0(i, j), l0 = comp1(i, j);
-    if (k == i) return a * s.dsphi[i] * s.sphi[j];
-    if (k == j) return a * s.sphi[i] * s.dsphi[j];
-    if (k == k0) return b * s.dsth[k0] * s.sth[l0];
-    return b * s.sth[k0] * s.dsth[l0];
+    if (k == i) return a * s.dsphi(i) * s.sphi(j);
+    if (k == j) return a * s.sphi(i) * s.dsphi(j);
+    if (k == k0) return b * s.dsth(k0) * s.sth(l0);
+    return b * s.sth(k0) * s.dsth(l0);
 }
 
 // Out-of-plane switch sw(i) = (1 - s3_i) prod_{m != i} s3_m and d sw(i)/d rch(j).
-template <typename T>
-__device__ __forceinline__ T opswitch(const Switch<T>& s, int i)
+template <typename T, class V>
+__device__ __forceinline__ T opswitch(const V& s, int i)
 {
     const int a = (i + 1) & 3, b = (i + 2) & 3, c = (i + 3) & 3;
-    return (T(1) - s.s3[i]) * s.s3[a] * s.s3[b] * s.s3[c];
+    return (T(1) - s.s3(i)) * s.s3(a) * s.s3(b) * s.s3(c);
 }
-template <typename T>
-__device__ __forceinline__ T dopswitch(const Switch<T>& s, int i, int j)
+template <typename T, class V>
+__device__ __forceinline__ T dopswitch(const V& s, int i, int j)
 {
     const int a = (i + 1) & 3, b = (i + 2) & 3, c = (i + 3) & 3;
-    if (j == i) return -s.ds3[i] * s.s3[a] * s.s3[b] * s.s3[c];
-    const T o = T(1) - s.s3[i];
-    if (j == a) return o * s.ds3[a] * s.s3[b] * s.s3[c];
-    if (j == b) return o * s.s3[a] * s.ds3[b] * s.s3[c];
-    return o * s.s3[a] * s.s3[b] * s.ds3[c];
+    if (j == i) return -s.ds3(i) * s.s3(a) * s.s3(b) * s.s3(c);
+    const T o = T(1) - s.s3(i);
+    if (j == a) return o * s.ds3(a) * s.s3(b) * s.s3(c);
+    if (j == b) return o * s.s3(a) * s.ds3(b) * s.s3(c);
+    return o * s.s3(a) * s.s3(b) * s.ds3(c);
 }
 
 // In-plane force constant fk0(i,j) and its derivative wrt rch(k) (ipforce_ch4h).
-template <typename T>
-__device__ __forceinline__ T fk0(const Switch<T>& s, int i, int j)
+template <typename T, class V>
+__device__ __forceinline__ T fk0(const V& s, int i, int j)
 {
     const T f0 = T(fkinf() + akk()), f2 = T(fkinf());
     const int k0 = comp0(i, j), l0 = comp1(i, j);
-    return f0 + f0 * (s.s1[i] * s.s1[j] - T(1)) + (f0 - f2) * (s.s2[k0] * s.s2[l0] - T(1));
+    return f0 + f0 * (s.s1(i) * s.s1(j) - T(1)) + (f0 - f2) * (s.s2(k0) * s.s2(l0) - T(1));
 }
-template <typename T>
-__device__ __forceinline__ T dfk0(const Switch<T>& s, int i, int j, int k)
+template <typename T, class V>
+__device__ __forceinline__ T dfk0(const V& s, int i, int j, int k)
 {
     const T f0 = T(fkinf() + akk()), f2 = T(fkinf());
     const int k0 = comp0(i, j), l0 = comp1(i, j);
-    if (k == i) return f0 * s.ds1[i] * s.s1[j];
-    if (k == j) return f0 * s.s1[i] * s.ds1[j];
-    if (k == k0) return (f0 - f2) * s.ds2[k0] * s.s2[l0];
-    return (f0 - f2) * s.s2[k0] * s.ds2[l0];
+    if (k == i) return f0 * s.ds1(i) * s.s1(j);
+    if (k == j) return f0 * s.s1(i) * s.ds1(j);
+    if (k == k0) return (f0 - f2) * s.ds2(k0) * s.s2(l0);
+    return (f0 - f2) * s.s2(k0) * s.ds2(l0);
 }
 
 // ---------------------------------------------------------------------------
@@ -124,7 +132,7 @@ __device__ __forceinline__ T dfk0(const Switch<T>& s, int i, int j, int k)
 //   - the switching functions of its own H,
 //   - the LEPS stretch of bond C-H_r / Hb-H_r,
 //   - the out-of-plane term centred on H_r (opbend_ch4h rotation i = r),
-//   - the in-plane pair (r, r+1), and for r < 2 the pair (r, r+2).
+//   - the in-plane pair (r, r+1), and half of the pair (r, r+2).
 // Every factor that depends on a bond length only is accumulated as a scalar
 // dV/dr; the angular terms are differentiated in vector form.  Results are
 // reduced over the quad with DPP.  This is the reference's analytic gradient
@@ -204,25 +212,25 @@ __device__ __forceinline__ void own_switches(Own<T>& o)
     }
 }
 
-// Rotated-frame arrays: entry n comes from lane r + n of the quad.
+// The rotated frame: entry n is the quantity of methane H r + n, read from
+// lane r + n of the quad with DPP where it is used rather than held in
+// registers for the whole evaluation.
 template <typename T>
 struct Frame {
-    T t[3][4], rinv[4];
-    Switch<T> s;
+    const Own<T>& o;
+    __device__ T t(int d, int n) const { return fr(o.t[d], n); }
+    __device__ T rinv(int n) const { return fr(o.rinv, n); }
+    __device__ T s1(int n) const { return fr(o.s1, n); }
+    __device__ T ds1(int n) const { return fr(o.ds1, n); }
+    __device__ T s2(int n) const { return fr(o.s2, n); }
+    __device__ T ds2(int n) const { return fr(o.ds2, n); }
+    __device__ T s3(int n) const { return fr(o.s3, n); }
+    __device__ T ds3(int n) const { return fr(o.ds3, n); }
+    __device__ T sphi(int n) const { return fr(o.sphi, n); }
+    __device__ T dsphi(int n) const { return fr(o.dsphi, n); }
+    __device__ T sth(int n) const { return fr(o.sth, n); }
+    __device__ T dsth(int n) const { return fr(o.dsth, n); }
 };
-
-template <int N, typename T>
-__device__ __forceinline__ void fill_frame(const Own<T>& o, Frame<T>& f)
-{
-#pragma unroll
-    for (int d = 0; d < 3; ++d) f.t[d][N] = quad::rot<N>(o.t[d]);
-    f.rinv[N] = quad::rot<N>(o.rinv);
-    f.s.s1[N] = quad::rot<N>(o.s1);     f.s.ds1[N] = quad::rot<N>(o.ds1);
-    f.s.s2[N] = quad::rot<N>(o.s2);     f.s.ds2[N] = quad::rot<N>(o.ds2);
-    f.s.s3[N] = quad::rot<N>(o.s3);     f.s.ds3[N] = quad::rot<N>(o.ds3);
-    f.s.sphi[N] = quad::rot<N>(o.sphi); f.s.dsphi[N] = quad::rot<N>(o.dsphi);
-    f.s.sth[N] = quad::rot<N>(o.sth);   f.s.dsth[N] = quad::rot<N>(o.dsth);
-}
 
 // Sum over the quad of rotated-frame contributions: lane m receives
 // sum_o (value at frame index o on lane m - o), i.e. the total for its own H.
@@ -232,31 +240,32 @@ __device__ __forceinline__ A unrotate_sum(const A (&loc)[4])
     return loc[0] + quad::rot<3>(loc[1]) + quad::rot<2>(loc[2]) + quad::rot<1>(loc[3]);
 }
 
-// In-plane pair (0, J) of the rotated frame (ipbend_ch4h / ipforce_ch4h).
+// In-plane pair (0, J) of the rotated frame (ipbend_ch4h / ipforce_ch4h),
+// its energy and derivatives scaled by wt.
 template <int J, typename T>
 __device__ __forceinline__ void inplane_pair(const Frame<T>& R, const T (&f1)[4], const T (&df1dc)[4],
-                                             const T (&df1dh)[4], double& vtot, T (&lg)[3][4], T (&ldc)[4],
+                                             const T (&df1dh)[4], T wt, double& vtot, T (&lg)[3][4], T (&ldc)[4],
                                              T (&ldh)[4])
 {
-    const T costh = (R.t[0][0] * R.t[0][J] + R.t[1][0] * R.t[1][J] + R.t[2][0] * R.t[2][J]) * R.rinv[0] * R.rinv[J];
-    const T dth = rt::acos(costh) - theta0(R.s, 0, J);
-    const T k0 = fk0(R.s, 0, J);
+    const T costh = (R.t(0, 0) * R.t(0, J) + R.t(1, 0) * R.t(1, J) + R.t(2, 0) * R.t(2, J)) * R.rinv(0) * R.rinv(J);
+    const T dth = rt::acos(costh) - theta0<T>(R, 0, J);
+    const T k0 = wt * fk0<T>(R, 0, J);
     const T kk = k0 * f1[0] * f1[J];
     vtot += double(T(0.5) * kk * dth * dth);
     const T pc = -kk * dth / rt::sqrt(T(1) - costh * costh);     // dV/d cos(theta)
     const T h2 = T(0.5) * dth * dth;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const T ui = -R.t[d][0] * R.rinv[0], uj = -R.t[d][J] * R.rinv[J];
-        lg[d][0] += pc * (uj - costh * ui) * R.rinv[0];
-        lg[d][J] += pc * (ui - costh * uj) * R.rinv[J];
+        const T ui = -R.t(d, 0) * R.rinv(0), uj = -R.t(d, J) * R.rinv(J);
+        lg[d][0] += pc * (uj - costh * ui) * R.rinv(0);
+        lg[d][J] += pc * (ui - costh * uj) * R.rinv(J);
     }
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-        T dk = dfk0(R.s, 0, J, n) * f1[0] * f1[J];
+        T dk = wt * dfk0<T>(R, 0, J, n) * f1[0] * f1[J];
         if (n == 0) dk += k0 * df1dc[0] * f1[J];
         if (n == J) dk += k0 * f1[0] * df1dc[J];
-        ldc[n] += -kk * dth * dtheta0(R.s, 0, J, n) + h2 * dk;
+        ldc[n] += -kk * dth * dtheta0<T>(R, 0, J, n) + h2 * dk;
     }
     ldh[0] += h2 * k0 * df1dh[0] * f1[J];
     ldh[J] += h2 * k0 * f1[0] * df1dh[J];
@@ -340,11 +349,7 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
         drcb = dqcb + djcb * gbb;                     // summed over the quad below
     }
 
-    Frame<T> R;
-    fill_frame<0>(o, R);
-    fill_frame<1>(o, R);
-    fill_frame<2>(o, R);
-    fill_frame<3>(o, R);
+    const Frame<T> R{ o };
 
     // ---- out-of-plane bend centred on the own H (opbend_ch4h / calcdelta_ch4h) ----
     {
@@ -352,8 +357,8 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
         T a[3], b[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            a[d] = R.t[d][1] - R.t[d][2];         // q_Hk - q_Hj
-            b[d] = R.t[d][1] - R.t[d][3];         // q_Hl - q_Hj
+            a[d] = R.t(d, 1) - R.t(d, 2);         // q_Hk - q_Hj
+            b[d] = R.t(d, 1) - R.t(d, 3);         // q_Hl - q_Hj
         }
         const T nx = a[1] * b[2] - a[2] * b[1], ny = a[2] * b[0] - a[0] * b[2], nz = a[0] * b[1] - a[1] * b[0];
         const T inrm = T(1) / rt::sqrt(nx * nx + ny * ny + nz * nz);
@@ -362,13 +367,13 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
 #pragma unroll
         for (int ii = 0; ii < 3; ++ii) {
             const int m = ii + 1;
-            argd[ii] = -(nx * R.t[0][m] + ny * R.t[1][m] + nz * R.t[2][m]) * R.rinv[m] * inrm;
+            argd[ii] = -(nx * R.t(0, m) + ny * R.t(1, m) + nz * R.t(2, m)) * R.rinv(m) * inrm;
             npos += argd[ii] > T(0) ? 1 : 0;
         }
         // the reference swaps k and l once per positive projection
         // (src/egrad_ch4h.f:821-833): the normal flips when the count is odd
         const T sg = (npos & 1) ? T(-1) : T(1);
-        const T sw = opswitch(R.s, 0);
+        const T sw = opswitch<T>(R, 0);
         const T fi = Fch3 * sw, hi = Hch3 * sw;
         const T nh[3] = { sg * nx * inrm, sg * ny * inrm, sg * nz * inrm };
         T s2 = T(0), s4 = T(0);
@@ -377,25 +382,25 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
         for (int ii = 0; ii < 3; ++ii) {
             const int m = ii + 1;
             const T ad = sg * argd[ii];
-            const T dm = rt::acos(ad) - theta0(R.s, 0, m);
+            const T dm = rt::acos(ad) - theta0<T>(R, 0, m);
             const T d2 = dm * dm;
             s2 += d2;
             s4 += d2 * d2;
             const T wgt = T(2) * fi * dm + T(4) * hi * dm * d2;         // dV/d delta
 #pragma unroll
-            for (int n = 0; n < 4; ++n) ldc[n] -= wgt * dtheta0(R.s, 0, m, n);
+            for (int n = 0; n < 4; ++n) ldc[n] -= wgt * dtheta0<T>(R, 0, m, n);
             const T pm = -wgt / rt::sqrt(T(1) - ad * ad);            // dV/d argd
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const T um = -R.t[d][m] * R.rinv[m];
-                lg[d][m] += pm * (nh[d] - ad * um) * R.rinv[m];
+                const T um = -R.t(d, m) * R.rinv(m);
+                lg[d][m] += pm * (nh[d] - ad * um) * R.rinv(m);
                 vv[d] += pm * sg * inrm * (um - ad * nh[d]);
             }
         }
         vtot += double(fi * s2 + hi * s4);
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
-            const T dsw = dopswitch(R.s, 0, n);
+            const T dsw = dopswitch<T>(R, 0, n);
             ldc[n] += (dsw * Fch3) * s2 + (dsw * Hch3) * s4;
         }
         const T bxv[3] = { b[1] * vv[2] - b[2] * vv[1], b[2] * vv[0] - b[0] * vv[2], b[0] * vv[1] - b[1] * vv[0] };
@@ -428,8 +433,10 @@ __device__ __forceinline__ void egrad_quad(const double own[6], double& v_out, d
         f1[1] = quad::rot<1>(f1o); df1dc[1] = quad::rot<1>(dco); df1dh[1] = quad::rot<1>(dho);
         f1[2] = quad::rot<2>(f1o); df1dc[2] = quad::rot<2>(dco); df1dh[2] = quad::rot<2>(dho);
         f1[3] = quad::rot<3>(f1o); df1dc[3] = quad::rot<3>(dco); df1dh[3] = quad::rot<3>(dho);
-        inplane_pair<1>(R, f1, df1dc, df1dh, vtot, lg, ldc, ldh);
-        if (r < 2) inplane_pair<2>(R, f1, df1dc, df1dh, vtot, lg, ldc, ldh);
+        inplane_pair<1>(R, f1, df1dc, df1dh, T(1), vtot, lg, ldc, ldh);
+        // the pair (r, r+2) is the pair (r+2, r) of lane r+2: every lane takes
+        // half of it, so that the whole quad stays active for the DPP reads
+        inplane_pair<2>(R, f1, df1dc, df1dh, T(0.5), vtot, lg, ldc, ldh);
     }
 
     // ---- quad reductions and the chain rule to Cartesians ----
