@@ -201,14 +201,29 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 }
 
 // ---------------------------------------------------------------------------
+// Propagator table of a block, per (trajectory, mode): cos(w_k dt) and, per
+// atom a, the two off-diagonal elements of the exact free ring-polymer map
+// (src/verlet.f90:413-432):
+//   p' = cw p + p2_a q,   q' = p3_a p + cw q,
+//   p2_a = -w_k m_a sin(w_k dt),  p3_a = sin(w_k dt) / (w_k m_a)  (mode 0: 0, dt/m_a).
+// ---------------------------------------------------------------------------
+template <int NA>
+struct PropTab {
+    static constexpr int W = 1 + 2 * NA;      // doubles per (trajectory, mode)
+};
+
+// ---------------------------------------------------------------------------
 // Normal-mode step on the matrix cores (FP64 MFMA 16x16x4), nbeads 16, 32 or
 // 64.  The block's p and q columns are staged in LDS as [column][bead] with
-// column = 2*(tl*NC + coord) + (0: p, 1: q) and stride NB + 1.  Each wave owns
-// one 16-row band of the transform (its A operand stays in registers) and
-// sweeps the 16-column tiles of its band: pass 0 computes the modes Y = T X
-// into ys and applies the exact 2x2 free ring-polymer map to the accumulators
-// (p and q of one coordinate sit in neighbouring lanes, swapped with DPP);
-// pass 1 transforms ys back into xs.
+// column = 2*(tl*NC + coord) + (0: p, 1: q) and an odd stride NB + 1, so
+// that the 16 columns of a tile fall in distinct banks.  Each
+// wave owns one 16-row band of the transform (its A operand stays in
+// registers) and the column tiles c0, c0 + CPW, ... of that band, whose
+// accumulation chains it interleaves so that consecutive MFMAs are
+// independent.  Pass 0 computes the modes Y = T X into ys and applies the
+// exact 2x2 free ring-polymer map to the accumulators (p and q of one
+// coordinate sit in neighbouring lanes, swapped with DPP); pass 1 transforms
+// ys back into xs.
 // Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16] and receives D[l/16 + 4r][l%16] in accumulator element r.
 // ---------------------------------------------------------------------------
@@ -221,19 +236,27 @@ constexpr double kSqrt()
          : NB == 16 ? 4.0 : NB == 32 ? 5.656854249492381 : 8.0;
 }
 
+template <int NB>
+constexpr int nm_stride() { return NB + 1; }
+
 template <int NB, int TPB, int NC, bool CENT>
 __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *prop, const double *tfwd,
-                                        const double *tinv, const double *mass, double dt, double *cent)
+                                        const double *tinv, double *cent)
 {
-    constexpr int NBP = NB + 1;
+    constexpr int NA = NC / 3;
+    constexpr int NBP = nm_stride<NB>();
     constexpr int NCOL = TPB * 2 * NC;
     constexpr int CT = (NCOL + 15) / 16;
     constexpr int RT = NB / 16;
     constexpr int NWAVE = kBlock / 64;
     static_assert(NB % 16 == 0 && NWAVE % RT == 0, "row bands must map onto whole waves");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int CPW = NWAVE / RT;                 // waves sharing a row band
+    constexpr int MAXT = (CT + CPW - 1) / CPW;      // column tiles per wave
+    constexpr int PW = PropTab<NA>::W;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    const int row0 = (wave % RT) * 16;
+    const int row0 = (wave % RT) * 16, c0 = wave / RT;
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         const double *T = pass == 0 ? tfwd : tinv;
@@ -242,47 +265,43 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
         double av[NB / 4];
 #pragma unroll
         for (int kk = 0; kk < NB / 4; ++kk) av[kk] = T[(4 * kk + lk) * NB + row0 + li];   // T[row][k]
-#pragma unroll 1
-        for (int ct = wave / RT; ct < CT; ct += NWAVE / RT) {
-            const int col = ct * 16 + li;
-            const bool ok = col < NCOL;
-            const double *bcol = src + (ok ? col : 0) * NBP;
-            dbl4 acc = { 0.0, 0.0, 0.0, 0.0 };
+        dbl4 acc[MAXT];
 #pragma unroll
-            for (int kk = 0; kk < NB / 4; ++kk) {
-                const double bv = ok ? bcol[4 * kk + lk] : 0.0;                         // X[k][col]
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc, 0, 0, 0);
+        for (int j = 0; j < MAXT; ++j) acc[j] = dbl4{ 0.0, 0.0, 0.0, 0.0 };
+        // Every wave runs MAXT tiles, so that no MFMA sits under a branch; a
+        // tile past the last one reads zeros and stores nothing.
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk)
+#pragma unroll
+            for (int j = 0; j < MAXT; ++j) {
+                const int col = (c0 + j * CPW) * 16 + li;
+                const double bv = col < NCOL ? src[col * NBP + 4 * kk + lk] : 0.0;   // X[k][col]
+                acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc[j], 0, 0, 0);
             }
+#pragma unroll
+        for (int j = 0; j < MAXT; ++j) {
+            const int col = (c0 + j * CPW) * 16 + li;
+            const bool ok = col < NCOL;
             if (pass == 0) {
                 // free ring-polymer map of each mode on (p, q) (src/verlet.f90:434-447)
                 const bool isq = (col & 1) != 0;
                 const int pair = (ok ? col : 0) >> 1;
-                const int tl = pair / NC, c = pair % NC;
-                const double m = mass[c / 3];
+                const int tl = pair / NC, at = (pair % NC) / 3;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(acc[r]);
+                    const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(acc[j][r]);
                     const int mode = row0 + lk + 4 * r;
-                    const double *pr = prop + (tl * NB + mode) * 3;
-                    const double cw = pr[0], sw = pr[1], wk = pr[2];
-                    double p2, p3;
-                    if (mode == 0) {
-                        p2 = 0.0;
-                        p3 = dt / m;
-                    } else {
-                        const double wm = wk * m;
-                        p2 = -wm * sw;
-                        p3 = sw / wm;
-                    }
-                    acc[r] = isq ? other * p3 + acc[r] * cw : acc[r] * cw + other * p2;
+                    const double *pr = prop + (tl * NB + mode) * PW;
+                    const double cw = pr[0], p2 = pr[1 + 2 * at], p3 = pr[2 + 2 * at];
+                    acc[j][r] = isq ? other * p3 + acc[j][r] * cw : acc[j][r] * cw + other * p2;
                 }
                 // The centroid of the propagated beads is mode 0 of q over sqrt(NB):
                 // every column of either transform sums to sqrt(NB) times delta_k0.
-                if (CENT && isq && ok && row0 == 0 && lk == 0) cent[pair] = acc[0] * (1.0 / kSqrt<NB>());
+                if (CENT && isq && ok && row0 == 0 && lk == 0) cent[pair] = acc[j][0] * (1.0 / kSqrt<NB>());
             }
             if (ok) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) dst[col * NBP + row0 + lk + 4 * r] = acc[r];
+                for (int r = 0; r < 4; ++r) dst[col * NBP + row0 + lk + 4 * r] = acc[j][r];
             }
         }
         __syncthreads();
@@ -307,7 +326,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr int BPB = kBlock / L;          // beads per block
     constexpr int TPB = BPB / NB;            // trajectories per block
     static_assert(TPB >= 1, "a trajectory must fit in one block");
-    constexpr int XS = MF ? TPB * 2 * NC * (NB + 1) : TPB * NC * NB;
+    constexpr int XS = MF ? TPB * 2 * NC * nm_stride<NB>() : TPB * NC * NB;
+    constexpr int PW = PropTab<S::NA>::W;
     __shared__ __align__(16) double X[XS];
     __shared__ __align__(16) double cent[TPB * NC];
     __shared__ __align__(16) double Y[MF ? XS : 1];
@@ -317,7 +337,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // the register budget of the force kernel alone.
     constexpr bool STASH = WPE >= 2;
     constexpr bool TAB = MF || STASH;
-    __shared__ double prop[TAB ? TPB * NB * 3 : 1];
+    __shared__ double prop[TAB ? TPB * NB * PW : 1];
     __shared__ double smass[TAB ? S::NA : 1];
     __shared__ double stash[STASH ? 2 * SL * kBlock : 1];
     __shared__ double wcom[ENS == CAM_ENS_CHILD ? CAM_MAX_REAC : 1][S::NA];
@@ -382,16 +402,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     if constexpr (TAB) {
         if (threadIdx.x < S::NA) smass[threadIdx.x] = a.mass[threadIdx.x];
-        // per-trajectory mode table: cos(w dt), sin(w dt), w  (src/verlet.f90:413-432)
+        // per-trajectory mode table (src/verlet.f90:413-432), as PropTab lays it out
         for (int i = threadIdx.x; i < TPB * NB; i += kBlock) {
             const int t = i / NB, kk0 = i % NB;
             const int tr = min(blockIdx.x * TPB + t, a.ntraj - 1);
             const int kk = kk0 <= NB / 2 ? kk0 : NB - kk0;
             const double bt = a.beta[tr];
             const double wk = (2.0 / (bt / NB)) * sin(kk * (3.141592653589793 / NB));
-            prop[i * 3] = kk0 == 0 ? 1.0 : cos(wk * a.dt);
-            prop[i * 3 + 1] = kk0 == 0 ? 0.0 : sin(wk * a.dt);
-            prop[i * 3 + 2] = wk;
+            const double wt = wk * a.dt;
+            const double cw = cos(wt), sw = sin(wt);
+            double *pr = prop + i * PW;
+            pr[0] = kk0 == 0 ? 1.0 : cw;
+#pragma unroll
+            for (int at = 0; at < S::NA; ++at) {
+                const double m = a.mass[at];
+                if (kk0 == 0) {
+                    pr[1 + 2 * at] = 0.0;
+                    pr[2 + 2 * at] = a.dt / m;
+                } else {
+                    const double wm = wk * m;
+                    pr[1 + 2 * at] = -wm * sw;
+                    pr[2 + 2 * at] = sw / wm;
+                }
+            }
         }
     }
 
@@ -428,20 +461,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 for (int s = 0; s < SL; ++s) q[s] = p[s] * p3[s] + q[s];
             }
         } else if constexpr (MF) {
-            double *xm = X + tl * 2 * NC * (NB + 1);
+            double *xm = X + tl * 2 * NC * nm_stride<NB>();
             __syncthreads();
 #pragma unroll
             for (int s = 0; s < SL; ++s)
                 if (S::valid(s, r)) {
-                    xm[(2 * cc[s]) * (NB + 1) + k] = p[s];
-                    xm[(2 * cc[s] + 1) * (NB + 1) + k] = q[s];
+                    xm[(2 * cc[s]) * nm_stride<NB>() + k] = p[s];
+                    xm[(2 * cc[s] + 1) * nm_stride<NB>() + k] = q[s];
                 }
             __syncthreads();
-            nm_mfma<NB, TPB, NC, ENS == CAM_ENS_CHILD>(X, Y, prop, a.tfwd, a.tinv, smass, a.dt, cent);
+            nm_mfma<NB, TPB, NC, ENS == CAM_ENS_CHILD>(X, Y, prop, a.tfwd, a.tinv, cent);
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
-                p[s] = xm[(2 * cc[s]) * (NB + 1) + k];
-                q[s] = xm[(2 * cc[s] + 1) * (NB + 1) + k];
+                p[s] = xm[(2 * cc[s]) * nm_stride<NB>() + k];
+                q[s] = xm[(2 * cc[s] + 1) * nm_stride<NB>() + k];
             }
         } else {
             // bead -> normal modes, propagate, back (src/verlet.f90:377-463)
@@ -474,20 +507,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
             if constexpr (TAB) {
                 // the same coefficients, from the table (mode k = this lane's bead index)
-                const double *pr = prop + (tl * NB + k) * 3;
-                const double cw = pr[0], sw = pr[1], wk = pr[2];
+                const double *pr = prop + (tl * NB + k) * PW;
+                const double cw = pr[0];
 #pragma unroll
                 for (int s = 0; s < SL; ++s) {
-                    const double mm = smass[cc[s] / 3];
-                    double c2, c3;
-                    if (k == 0) {
-                        c2 = 0.0;
-                        c3 = a.dt / mm;
-                    } else {
-                        const double wm = wk * mm;
-                        c2 = -wm * sw;
-                        c3 = sw / wm;
-                    }
+                    const int at = cc[s] / 3;
+                    const double c2 = pr[1 + 2 * at], c3 = pr[2 + 2 * at];
                     const double pn = yp[s] * cw + yq[s] * c2;
                     yq[s] = yp[s] * c3 + yq[s] * cw;
                     yp[s] = pn;
