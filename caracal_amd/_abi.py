@@ -11,7 +11,7 @@ CAM_SURFACE_H3, CAM_SURFACE_CH4H = 1, 2
 CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
 CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD = 0, 1, 2
-CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA = 0, 1, 2
+CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA, CAM_NMK_FOLDED = 0, 1, 2, 3
 CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT = 0, 1, 2
 CAM_MAX_ATOMS, CAM_MAX_BONDS, CAM_MAX_REAC = 16, 4, 4
 

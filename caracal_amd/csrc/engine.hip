@@ -16,62 +16,11 @@
 #include <type_traits>
 #include <vector>
 
-#include "common.hpp"
-#include "pes_ch4h.hpp"
-#include "pes_h3.hpp"
 #include "philox.hpp"
+#include "step_common.hpp"
 
 namespace cam {
 namespace {
-
-constexpr int kBlock = 256;
-
-// ---------------------------------------------------------------------------
-// Surface policies: how the coordinates of one bead are spread over its L lanes.
-// ---------------------------------------------------------------------------
-struct SurfH3 {
-    static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
-    __device__ static int coord(int slot, int) { return slot; }
-    __device__ static bool valid(int, int) { return true; }
-    template <typename T>
-    __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<T>(own, v, g); }
-};
-
-struct SurfCH4H {
-    static constexpr int NA = 6, NC = 18, L = 4, SLOTS = 6;
-    // slot 0..2: methane H number r; slot 3..5: C on lane 0, Hb on lane 1
-    __device__ static int coord(int slot, int r)
-    {
-        return slot < 3 ? 3 * ch4h::hidx(r) + slot : 3 * (r == 0 ? ch4h::kC : ch4h::kB) + slot - 3;
-    }
-    __device__ static bool valid(int slot, int r) { return slot < 3 || r < 2; }
-    template <typename T>
-    __device__ static void egrad(const double *own, double &v, double *g) { ch4h::egrad_quad<T>(own, v, g); }
-};
-
-struct Rx {                                  // reaction coordinate, 0-based indices
-    int n_form, n_break, n_reac;
-    int form[CAM_MAX_BONDS][2], brk[CAM_MAX_BONDS][2];
-    double form_ref[CAM_MAX_BONDS], break_ref[CAM_MAX_BONDS];
-    int reac_n[CAM_MAX_REAC], reac_at[CAM_MAX_REAC][CAM_MAX_ATOMS];
-    double reac_mass[CAM_MAX_REAC];
-    double r_inf, xi_ideal;
-};
-
-struct StepArgs {
-    double *q, *p, *g, *vbead;
-    const double *beta;          // [ntraj]
-    const double *tfwd, *tinv;   // [nbeads][nbeads], element (j,k) = T[k][j]
-    uint8_t *xi_flag;            // CHILD: [nsteps_run][ntraj]
-    double *cbuf;                // CHILD, fused: centroids of this launch's steps [step][traj][3*natoms]
-    const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
-    int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
-    int andersen_step;
-    uint64_t seed;
-    double dt;
-    double mass[CAM_MAX_ATOMS];
-    Rx rx;
-};
 
 // xi of the centroid without its gradient, for the per-step child flags
 // (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213).  One lane evaluates it
@@ -201,18 +150,6 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 }
 
 // ---------------------------------------------------------------------------
-// Propagator table of a block, per (trajectory, mode): cos(w_k dt) and, per
-// atom a, the two off-diagonal elements of the exact free ring-polymer map
-// (src/verlet.f90:413-432):
-//   p' = cw p + p2_a q,   q' = p3_a p + cw q,
-//   p2_a = -w_k m_a sin(w_k dt),  p3_a = sin(w_k dt) / (w_k m_a)  (mode 0: 0, dt/m_a).
-// ---------------------------------------------------------------------------
-template <int NA>
-struct PropTab {
-    static constexpr int W = 1 + 2 * NA;      // doubles per (trajectory, mode)
-};
-
-// ---------------------------------------------------------------------------
 // Normal-mode step on the matrix cores (FP64 MFMA 16x16x4), nbeads 16, 32 or
 // 64.  The block's p and q columns are staged in LDS as [column][bead] with
 // column = 2*(tl*NC + coord) + (0: p, 1: q) and an odd stride NB + 1, so
@@ -227,14 +164,6 @@ struct PropTab {
 // Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16] and receives D[l/16 + 4r][l%16] in accumulator element r.
 // ---------------------------------------------------------------------------
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-template <int NB>
-constexpr double kSqrt()
-{
-    return NB == 1 ? 1.0 : NB == 2 ? 1.4142135623730951 : NB == 4 ? 2.0 : NB == 8 ? 2.8284271247461903
-         : NB == 16 ? 4.0 : NB == 32 ? 5.656854249492381 : 8.0;
-}
 
 template <int NB>
 constexpr int nm_stride() { return NB + 1; }
@@ -1058,7 +987,6 @@ __global__ void from_device_layout(const double *in, double *out, int ntraj, int
 // ---------------------------------------------------------------------------
 using namespace cam;
 
-typedef void (*step_fn)(StepArgs);
 typedef void (*force_fn)(const double *, double *, double *, int);
 
 // The kernels of one schedule: the fused step, the initial-force kernel, and
@@ -1066,6 +994,7 @@ typedef void (*force_fn)(const double *, double *, double *, int);
 struct Kernels {
     step_fn step = nullptr, nm = nullptr;
     force_fn force = nullptr, fsplit = nullptr;
+    int step_tpb = 1;                 // trajectories per block of step
 };
 
 struct cam_engine {
@@ -1074,6 +1003,7 @@ struct cam_engine {
     hipStream_t stream = nullptr;
     double *q = nullptr, *p = nullptr, *g = nullptr, *vbead = nullptr, *beta = nullptr;
     double *tfwd = nullptr, *tinv = nullptr;
+    double *tfold = nullptr;          // folded reference transform (nm_mode REFERENCE, nbeads >= 4)
     double *weight = nullptr, *den_part = nullptr, *den = nullptr, *num = nullptr;
     uint8_t *xi_flag = nullptr;
     double *cbuf = nullptr;           // CHILD: centroids of one fused launch
@@ -1085,7 +1015,7 @@ struct cam_engine {
     bool have_state = false;
     int spl = 1;                      // steps per launch
     int wpe = 1;
-    bool mfma = false;
+    int nmk = CAM_NMK_VALU;           // normal-mode kernel of the current schedule
     Kernels kern;                     // kernels of the current schedule
     bool split = false;               // split schedule: force phase in its own kernel
     std::map<int, hipGraphExec_t> graphs;
@@ -1201,15 +1131,36 @@ void nm_matrices(int nb, int mode, std::vector<double> &fwd, std::vector<double>
 
 namespace {
 
-bool select_kernels(cam_engine *e, int wpe, bool mfma, Kernels &k)
+// The folded step kernel applies to the reference transform on NVE and child
+// trajectories of the fused schedule without transrot (fold.hip).
+bool fold_eligible(const cam_config *cfg)
+{
+    return cfg->nm_mode == CAM_NM_REFERENCE && cfg->nbeads >= 4 &&
+           (cfg->ensemble == CAM_ENS_NVE || cfg->ensemble == CAM_ENS_CHILD) && !cfg->remove_com &&
+           cfg->schedule != CAM_SCHED_SPLIT;
+}
+
+bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
     const int nb = cfg->nbeads, ens = cfg->ensemble;
+    const bool mfma = nmk == CAM_NMK_MFMA;
+    bool ok;
     if (cfg->surface == CAM_SURFACE_H3)
-        return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
-                                               : pick_nb<SurfH3, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
-    return cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
-                                           : pick_nb<SurfCH4H, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                                             : pick_nb<SurfH3, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+    else
+        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                                             : pick_nb<SurfCH4H, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+    k.step_tpb = e->tpb;
+    if (ok && nmk == CAM_NMK_FOLDED) {
+        int tpb = 0;
+        const step_fn f = fold_eligible(cfg) ? fold_step_kernel_for(cfg->surface, cfg->precision, nb, ens, &tpb) : nullptr;
+        if (!f) return false;
+        k.step = f;
+        k.step_tpb = tpb;
+    }
+    return ok;
 }
 
 TrajArgs traj_args(cam_engine *e)
@@ -1284,6 +1235,7 @@ StepArgs step_args(cam_engine *e)
     a.beta = e->beta;
     a.tfwd = e->tfwd;
     a.tinv = e->tinv;
+    a.tfold = e->tfold;
     a.xi_flag = e->xi_flag;
     a.cbuf = e->cbuf;
     a.step_base = e->step_base;
@@ -1323,10 +1275,11 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
         return hipGetLastError();
     }
     const int tgrid = (e->cfg.ntraj + 127) / 128;
+    const int sgrid = (e->cfg.ntraj + kn.step_tpb - 1) / kn.step_tpb;
     for (int done = 0; done < nsteps && err == hipSuccess; done += a.nsteps) {
         a.nsteps = std::min(spl, nsteps - done);
         a.step_off = done;
-        hipLaunchKernelGGL(kn.step, dim3(grid), dim3(kBlock), 0, e->stream, a);
+        hipLaunchKernelGGL(kn.step, dim3(sgrid), dim3(kBlock), 0, e->stream, a);
         err = hipGetLastError();
         if (err == hipSuccess && e->cfg.ensemble == CAM_ENS_CHILD) {
             const unsigned xgrid = (unsigned)(((size_t)a.nsteps * a.ntraj + 255) / 256);
@@ -1374,9 +1327,12 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
     CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_CHILD, "bad ensemble");
-    CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_MFMA, "bad nm_kernel");
+    CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_FOLDED, "bad nm_kernel");
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
                   "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
+    CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_FOLDED || fold_eligible(cfg),
+                  "nm_kernel FOLDED needs nm_mode REFERENCE, nbeads >= 4, ensemble NVE or CHILD, no remove_com "
+                  "and the fused schedule");
     CAM_CHECK_ARG(cfg->waves_per_eu >= 0 && cfg->waves_per_eu <= 2, "waves_per_eu must be 0, 1 or 2");
     CAM_CHECK_ARG(cfg->schedule >= CAM_SCHED_AUTO && cfg->schedule <= CAM_SCHED_SPLIT, "bad schedule");
     CAM_CHECK_ARG(cfg->schedule != CAM_SCHED_SPLIT || !cfg->remove_com,
@@ -1433,10 +1389,11 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     // steps fused per launch: state stays in registers for this many steps
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
     if (cfg->remove_com) e->spl = 1;
-    e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 ? 2 : 1;
-    e->mfma = cfg->nm_kernel == CAM_NMK_MFMA;
+    e->nmk = cfg->nm_kernel == CAM_NMK_AUTO ? CAM_NMK_VALU : cfg->nm_kernel;
+    e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
+                     e->nmk != CAM_NMK_FOLDED ? 2 : 1;
     e->split = cfg->schedule == CAM_SCHED_SPLIT;
-    const bool ok = select_kernels(e.get(), e->wpe, e->mfma, e->kern);
+    const bool ok = select_kernels(e.get(), e->wpe, e->nmk, e->kern);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
         return CAM_ERR_UNSUPPORTED;
@@ -1464,6 +1421,12 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     nm_matrices(nb, cfg->nm_mode, fwd, inv);
     CAM_HIP(hipMemcpy(e->tfwd, fwd.data(), fwd.size() * sizeof(double), hipMemcpyHostToDevice));
     CAM_HIP(hipMemcpy(e->tinv, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (cfg->nm_mode == CAM_NM_REFERENCE && nb >= 4) {
+        std::vector<double> fm((size_t)(nb / 2) * (nb / 2));
+        fold_matrix(nb, fm.data());
+        CAM_HIP(hipMalloc(&e->tfold, fm.size() * sizeof(double)));
+        CAM_HIP(hipMemcpy(e->tfold, fm.data(), fm.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     CAM_HIP(hipMemset(e->step_base, 0, 2 * sizeof(uint64_t)));
     CAM_HIP(hipMemset(e->den, 0, sizeof(double)));
     CAM_HIP(hipMemset(e->weight, 0, (size_t)cfg->ntraj * sizeof(double)));
@@ -1478,8 +1441,8 @@ int cam_engine_destroy(cam_engine *e)
     if (!e) return CAM_OK;
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
-    double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->weight, e->den_part, e->den,
-                       e->num, e->scratch_io, e->cbuf };
+    double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->tfold, e->weight, e->den_part,
+                       e->den, e->num, e->scratch_io, e->cbuf };
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (e->xi_flag) (void)hipFree(e->xi_flag);
@@ -1757,6 +1720,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     else {
         nmks.push_back(CAM_NMK_VALU);
         if (nb >= 16 && nb % 16 == 0) nmks.push_back(CAM_NMK_MFMA);
+        if (fold_eligible(&c)) nmks.push_back(CAM_NMK_FOLDED);
     }
     if (c.remove_com) spls.push_back(1);
     else if (c.steps_per_launch > 0) spls.push_back(c.steps_per_launch);
@@ -1774,6 +1738,10 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     std::vector<Trial> cand, trials;
     const bool try_fused = c.schedule != CAM_SCHED_SPLIT, try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
     for (int nmk : nmks) {
+        if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
+            for (int spl : spls) cand.push_back({ nmk, 1, spl, 0, 0.0 });
+            continue;
+        }
         if (try_fused)
             for (int wpe : wpes)
                 for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
@@ -1792,7 +1760,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     for (const Trial &tr : cand) {
         if (rc != CAM_OK) break;
         Kernels kn;
-        if (!select_kernels(e, tr.wpe, tr.nmk == CAM_NMK_MFMA, kn)) continue;
+        if (!select_kernels(e, tr.wpe, tr.nmk, kn)) continue;
         StepArgs a = step_args(e);
         a.q = buf;
         a.p = buf + nstate;
@@ -1832,8 +1800,8 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         if (trials[i].us < trials[best].us) best = i;
     Kernels kn;
     e->wpe = trials[best].wpe;
-    select_kernels(e, e->wpe, trials[best].nmk == CAM_NMK_MFMA, kn);
-    e->mfma = trials[best].nmk == CAM_NMK_MFMA;
+    select_kernels(e, e->wpe, trials[best].nmk, kn);
+    e->nmk = trials[best].nmk;
     e->kern = kn;
     e->spl = trials[best].spl;
     e->split = trials[best].split != 0;

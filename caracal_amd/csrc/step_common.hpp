@@ -1,0 +1,96 @@
+// Definitions shared by the step kernels of the trajectory-batch engine
+// (engine.hip) and the folded step kernel (fold.hip): the launch block size,
+// how a surface spreads the coordinates of one bead over its lanes, the
+// kernel arguments and the layout of the per-block propagator table.
+#pragma once
+#include <cstdint>
+
+#include "common.hpp"
+#include "pes_ch4h.hpp"
+#include "pes_h3.hpp"
+
+namespace cam {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Surface policies: how the coordinates of one bead are spread over its L lanes.
+// ---------------------------------------------------------------------------
+struct SurfH3 {
+    static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
+    __device__ static int coord(int slot, int) { return slot; }
+    __device__ static bool valid(int, int) { return true; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<T>(own, v, g); }
+};
+
+struct SurfCH4H {
+    static constexpr int NA = 6, NC = 18, L = 4, SLOTS = 6;
+    // slot 0..2: methane H number r; slot 3..5: C on lane 0, Hb on lane 1
+    __device__ static int coord(int slot, int r)
+    {
+        return slot < 3 ? 3 * ch4h::hidx(r) + slot : 3 * (r == 0 ? ch4h::kC : ch4h::kB) + slot - 3;
+    }
+    __device__ static bool valid(int slot, int r) { return slot < 3 || r < 2; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { ch4h::egrad_quad<T>(own, v, g); }
+};
+
+struct Rx {                                  // reaction coordinate, 0-based indices
+    int n_form, n_break, n_reac;
+    int form[CAM_MAX_BONDS][2], brk[CAM_MAX_BONDS][2];
+    double form_ref[CAM_MAX_BONDS], break_ref[CAM_MAX_BONDS];
+    int reac_n[CAM_MAX_REAC], reac_at[CAM_MAX_REAC][CAM_MAX_ATOMS];
+    double reac_mass[CAM_MAX_REAC];
+    double r_inf, xi_ideal;
+};
+
+struct StepArgs {
+    double *q, *p, *g, *vbead;
+    const double *beta;          // [ntraj]
+    const double *tfwd, *tinv;   // [nbeads][nbeads], element (j,k) = T[k][j]
+    const double *tfold;         // folded reference transform [nbeads/2][nbeads/2] (fold.hip)
+    uint8_t *xi_flag;            // CHILD: [nsteps_run][ntraj]
+    double *cbuf;                // CHILD, fused: centroids of this launch's steps [step][traj][3*natoms]
+    const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
+    int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
+    int andersen_step;
+    uint64_t seed;
+    double dt;
+    double mass[CAM_MAX_ATOMS];
+    Rx rx;
+};
+
+// ---------------------------------------------------------------------------
+// Propagator table of a block, per (trajectory, mode): cos(w_k dt) and, per
+// atom a, the two off-diagonal elements of the exact free ring-polymer map
+// (src/verlet.f90:413-432):
+//   p' = cw p + p2_a q,   q' = p3_a p + cw q,
+//   p2_a = -w_k m_a sin(w_k dt),  p3_a = sin(w_k dt) / (w_k m_a)  (mode 0: 0, dt/m_a).
+// ---------------------------------------------------------------------------
+template <int NA>
+struct PropTab {
+    static constexpr int W = 1 + 2 * NA;      // doubles per (trajectory, mode)
+};
+
+// FP64 MFMA accumulator of v_mfma_f64_16x16x4f64
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+constexpr double kSqrt()
+{
+    return NB == 1 ? 1.0 : NB == 2 ? 1.4142135623730951 : NB == 4 ? 2.0 : NB == 8 ? 2.8284271247461903
+         : NB == 16 ? 4.0 : NB == 32 ? 5.656854249492381 : 8.0;
+}
+
+typedef void (*step_fn)(StepArgs);
+
+// The folded step kernel (fold.hip) of a configuration, or nullptr if it has
+// none; *tpb receives its trajectories per block.
+step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int *tpb);
+
+// Folded reference transform M[r][c] = w_c cos(2 pi r c / nbeads) / sqrt(nbeads),
+// r, c < nbeads/2, w_0 = 1 and w_c = 2 otherwise, row-major.
+void fold_matrix(int nbeads, double *m);
+
+}  // namespace cam

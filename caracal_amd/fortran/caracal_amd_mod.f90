@@ -19,7 +19,8 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2
-   integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2
+   integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, &
+                                            CAM_NMK_FOLDED = 3
    integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
    integer, parameter, public :: CAM_MAX_ATOMS = 16, CAM_MAX_BONDS = 4, CAM_MAX_REAC = 4
 

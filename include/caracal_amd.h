@@ -79,9 +79,15 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
 enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
- * nbeads a multiple of 16).  AUTO takes the vector-ALU form.  Both give the
- * same transform to rounding. */
-enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2 };
+ * nbeads a multiple of 16).  FOLDED: the reference transform (rfft/irfft, the
+ * real part of the DFT both ways) maps every ring polymer onto one that is
+ * symmetric under the bead reflection j -> nbeads - j, so a step only has
+ * nbeads/2 + 1 distinct beads; this kernel carries those, evaluates their
+ * forces only, and writes each bead and its mirror (nm_mode REFERENCE,
+ * nbeads >= 4, NVE or CHILD, fused schedule, no remove_com).  AUTO takes the
+ * vector-ALU form unless the autotuner measures another faster.  All give the
+ * same trajectory to rounding. */
+enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, CAM_NMK_FOLDED = 3 };
 /* How a step is launched.  FUSED: the whole step, forces included, in one
  * kernel that advances steps_per_launch steps with the state in registers.
  * SPLIT: per step a normal-mode kernel and a force kernel (each at its own
