@@ -40,15 +40,25 @@ FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a plain 1 GiB copy
 # (profiles/step_kernel_ch4h_children_fp64_mfma.md); the state is read and
 # written once per launch, so the figure does not depend on steps per launch.
-MEASURED_TRAFFIC = {(4096, 32): 117.0e6}
+MEASURED_TRAFFIC = {("mfma", 4096, 32): 117.0e6}
 
 
-def step_flops(natoms, nbeads):
+NMK_NAMES = {1: "valu", 2: "mfma", 3: "folded"}
+
+
+def step_flops(natoms, nbeads, folded=False):
     """Algorithmic flops of one RPMD step of one trajectory: two half kicks
     (2 flops per coordinate each), the normal-mode transform pair on p and q
-    (24 N n^2) plus the per-mode propagator (6 per coordinate), and the surface."""
-    nc = 3 * natoms * nbeads
-    return 4 * nc + 24 * natoms * nbeads ** 2 + 6 * nc + nbeads * CH4H_FLOPS_PER_BEAD
+    (24 N n^2) plus the per-mode propagator (6 per coordinate), and the surface.
+    Folded (the reference transform keeps the beads j and n - j equal): the
+    same on the n/2 + 1 distinct beads, with the transform pair on the
+    (n/2) x (n/2) block plus its rank-1 row and column (2 h^2 + 4 h per column)."""
+    if not folded:
+        nc = 3 * natoms * nbeads
+        return 4 * nc + 24 * natoms * nbeads ** 2 + 6 * nc + nbeads * CH4H_FLOPS_PER_BEAD
+    h = nbeads // 2
+    nc = 3 * natoms * (h + 1)
+    return 4 * nc + 12 * natoms * (2 * h * h + 4 * h) + 6 * nc + (h + 1) * CH4H_FLOPS_PER_BEAD
 
 
 def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0,
@@ -213,8 +223,10 @@ def force_matching(cam):
 
 
 def plain_fp64(cam, workloads, steps):
-    """The plain form: one trajectory per launch, one step per launch, FP64."""
-    e, _ = prepare_children(cam, workloads, 2, 32, cam.CAM_PREC_FP64, steps_per_launch=1)
+    """The plain form: one step per launch of a pair of trajectories, FP64,
+    with the unfolded vector-ALU step (every bead transformed and evaluated)."""
+    e, _ = prepare_children(cam, workloads, 2, 32, cam.CAM_PREC_FP64, steps_per_launch=1,
+                            nm_kernel=cam.CAM_NMK_VALU)
     e.prepare(1)
     e.run(5)
     import torch
@@ -277,7 +289,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
     ap.add_argument("--schedule", default=None,
-                    help="fixed schedule 'valu|mfma,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
+                    help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
                          "the autotuner (for profiling: every launch then has the same shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
@@ -304,7 +316,7 @@ def main():
     ntraj, nb = args.ntraj, args.nbeads
     if args.schedule:
         unit, spl_fixed = args.schedule.split(",")
-        nmk = cam.CAM_NMK_MFMA if unit == "mfma" else cam.CAM_NMK_VALU
+        nmk = {"mfma": cam.CAM_NMK_MFMA, "folded": cam.CAM_NMK_FOLDED}.get(unit, cam.CAM_NMK_VALU)
         split = spl_fixed == "split"
         e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64, steps_per_launch=0 if split else int(spl_fixed),
                                 nm_kernel=nmk, schedule=cam.CAM_SCHED_SPLIT if split else cam.CAM_SCHED_FUSED)
@@ -318,14 +330,18 @@ def main():
     bead_evals = ntraj * nb * args.steps
     value = bead_evals / wall
     steps_per_s = ntraj * args.steps / wall
-    flops_launch = step_flops(6, nb) * ntraj * spl
+    folded = best["nm_kernel"] == cam.CAM_NMK_FOLDED
+    flops_launch = step_flops(6, nb, folded) * ntraj * spl
     achieved = flops_launch / (ms_launch * 1e-3) / 1e12
-    mf = best["nm_kernel"] == cam.CAM_NMK_MFMA
+    nmk_name = NMK_NAMES[best["nm_kernel"]]
     if best.get("split"):
-        kernel = (f"split step: step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes, "
+        kernel = (f"split step: step_kernel<SurfCH4H, {nb}, double, CHILD, {nmk_name.upper()} normal modes, "
                   f"no forces> + force_kernel<SurfCH4H, {nb}, double>")
+    elif folded:
+        kernel = f"fold_step_kernel<SurfCH4H, {nb}, double, CHILD>"
     else:
-        kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {'MFMA' if mf else 'VALU'} normal modes>"
+        kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {nmk_name.upper()} normal modes>"
+    distinct = nb // 2 + 1 if folded else nb
 
     out = {
         "metric": "bead-force-evals/s (batched RPMD children, CH4H, 32 beads x 4096 trajectories)",
@@ -343,12 +359,16 @@ def main():
         "config": {"workload": "ch4h_children", "surface": "CH5 (egrad_ch4h)", "ntraj": ntraj, "nbeads": nb,
                    "natoms": 6, "dt_fs": 0.1, "kelvin": 500, "precision": "fp64",
                    "nm_mode": "reference cosine transform", "steps_per_launch": spl,
-                   "nm_kernel": "mfma" if mf else "valu", "schedule": "split" if best.get("split") else "fused"},
+                   "nm_kernel": nmk_name, "schedule": "split" if best.get("split") else "fused"},
         "rpmd_steps_per_s": steps_per_s,
+        "distinct_bead_force_evals_per_s": ntraj * distinct * args.steps / wall,
+        "value_note": ("value counts the forces of all nbeads beads of every trajectory step, as the reference "
+                       "evaluates them; the reference transform keeps beads j and nbeads - j equal, so the folded "
+                       f"kernel evaluates {distinct} distinct beads per step and mirrors the rest"),
         "kappa_final": float(num[-1] / den) if den > 0 else None,
         "autotune": trials,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get((ntraj, nb)),
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get((nmk_name, ntraj, nb)),
                      "kernel": kernel, "ms_per_launch": ms_launch, "flops_per_launch": flops_launch,
                      "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
     }

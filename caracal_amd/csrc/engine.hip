@@ -1140,6 +1140,14 @@ bool fold_eligible(const cam_config *cfg)
            cfg->schedule != CAM_SCHED_SPLIT;
 }
 
+// ... and has a kernel for the surface, bead count and precision.
+bool fold_available(const cam_config *cfg)
+{
+    int tpb = 0;
+    return fold_eligible(cfg) &&
+           fold_step_kernel_for(cfg->surface, cfg->precision, cfg->nbeads, cfg->ensemble, &tpb) != nullptr;
+}
+
 bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
@@ -1333,6 +1341,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_FOLDED || fold_eligible(cfg),
                   "nm_kernel FOLDED needs nm_mode REFERENCE, nbeads >= 4, ensemble NVE or CHILD, no remove_com "
                   "and the fused schedule");
+    CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_FOLDED || fold_available(cfg),
+                  "nm_kernel FOLDED has no kernel for this surface, bead count and precision");
     CAM_CHECK_ARG(cfg->waves_per_eu >= 0 && cfg->waves_per_eu <= 2, "waves_per_eu must be 0, 1 or 2");
     CAM_CHECK_ARG(cfg->schedule >= CAM_SCHED_AUTO && cfg->schedule <= CAM_SCHED_SPLIT, "bad schedule");
     CAM_CHECK_ARG(cfg->schedule != CAM_SCHED_SPLIT || !cfg->remove_com,
@@ -1389,7 +1399,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     // steps fused per launch: state stays in registers for this many steps
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
     if (cfg->remove_com) e->spl = 1;
-    e->nmk = cfg->nm_kernel == CAM_NMK_AUTO ? CAM_NMK_VALU : cfg->nm_kernel;
+    if (cfg->nm_kernel != CAM_NMK_AUTO) e->nmk = cfg->nm_kernel;
+    else e->nmk = fold_available(cfg) ? CAM_NMK_FOLDED : CAM_NMK_VALU;
     e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
                      e->nmk != CAM_NMK_FOLDED ? 2 : 1;
     e->split = cfg->schedule == CAM_SCHED_SPLIT;
@@ -1720,7 +1731,7 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     else {
         nmks.push_back(CAM_NMK_VALU);
         if (nb >= 16 && nb % 16 == 0) nmks.push_back(CAM_NMK_MFMA);
-        if (fold_eligible(&c)) nmks.push_back(CAM_NMK_FOLDED);
+        if (fold_available(&c)) nmks.push_back(CAM_NMK_FOLDED);
     }
     if (c.remove_com) spls.push_back(1);
     else if (c.steps_per_launch > 0) spls.push_back(c.steps_per_launch);

@@ -62,6 +62,10 @@ int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, doubl
 
 /* ---- (2) trajectory-batch RPMD engine ---- */
 enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2 };
+/* Precision of the engine.  MIXED: CH4H forces in FP32 arithmetic with FP64
+ * positions, momenta, accumulation and normal modes.  The H3 surface is
+ * evaluated in FP64 by the engine under either setting (its FP32 correction
+ * terms gave wrong forces in multi-step kernels); cam_egrad_h3_dev keeps them. */
 enum { CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1 };
 /* Normal-mode transform.  REFERENCE reproduces the reference's rfft/irfft pair
  * (src/rfft.f90:36-62, src/irfft.f90:36-63): both directions are the
@@ -84,9 +88,10 @@ enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
  * symmetric under the bead reflection j -> nbeads - j, so a step only has
  * nbeads/2 + 1 distinct beads; this kernel carries those, evaluates their
  * forces only, and writes each bead and its mirror (nm_mode REFERENCE,
- * nbeads >= 4, NVE or CHILD, fused schedule, no remove_com).  AUTO takes the
- * vector-ALU form unless the autotuner measures another faster.  All give the
- * same trajectory to rounding. */
+ * nbeads >= 4, NVE or CHILD, fused schedule, no remove_com).  AUTO takes
+ * FOLDED where it applies and the vector-ALU form otherwise, unless the
+ * autotuner measures another faster.  All give the same trajectory to
+ * rounding. */
 enum { CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, CAM_NMK_FOLDED = 3 };
 /* How a step is launched.  FUSED: the whole step, forces included, in one
  * kernel that advances steps_per_launch steps with the state in registers.

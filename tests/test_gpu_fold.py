@@ -137,6 +137,56 @@ def test_folded_fused_steps_equal_single_steps(surface, nb, ntraj, prec):
     assert _mirror_symmetric(out[0][0]) and _mirror_symmetric(out[0][1])
 
 
+def _batch(surface, nb, ntraj, seed):
+    if surface == cam.CAM_SURFACE_H3:
+        w = workloads.h3_nve(nbeads=nb)
+        rng = np.random.default_rng(seed)
+        q = np.stack([w["q"]] * ntraj) + rng.normal(0, 0.02, (ntraj,) + w["q"].shape)
+        beta = np.full(ntraj, w["beta"]) * rng.uniform(0.5, 2.0, ntraj)
+        rx = workloads.h3_reaction()
+    else:
+        w = workloads.ch4h_children(ntraj=ntraj, nbeads=nb, seed=seed)
+        q, beta, rx = w["q"], w["beta"], w["reaction"]
+        rng = np.random.default_rng(seed)
+    p = rng.normal(size=q.shape) * np.sqrt(w["masses"][None, None, :, None] * nb / beta[:, None, None, None])
+    return w, q, p, beta, rx
+
+
+@pytest.mark.parametrize("surface", [cam.CAM_SURFACE_H3, cam.CAM_SURFACE_CH4H])
+@pytest.mark.parametrize("nb", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+@pytest.mark.parametrize("ens", [cam.CAM_ENS_NVE, cam.CAM_ENS_CHILD])
+def test_every_folded_kernel_is_deterministic_and_tracks_the_unfolded_one(surface, nb, prec, ens):
+    """Every folded instantiation, on three blocks of trajectories at
+    spread temperatures and with asymmetric starting momenta: two runs agree
+    bit for bit, and 5 steps stay within rounding of the unfolded vector-ALU
+    kernel (FP32-force level in mixed precision)."""
+    L = 1 if surface == cam.CAM_SURFACE_H3 else 4
+    tpb = (256 // L) // (nb // 2 + 1)
+    ntraj = 2 * tpb + 1
+    w, q, p, beta, rx = _batch(surface, nb, ntraj, seed=nb + 7 * ens)
+    kw = dict(precision=prec, ensemble=ens, reaction=rx, seed=3, steps_per_launch=5)
+    out = []
+    for nmk in (cam.CAM_NMK_FOLDED, cam.CAM_NMK_FOLDED, cam.CAM_NMK_VALU):
+        e = cam.RPMDEngine(surface, nb, ntraj, w["dt"], w["masses"], nm_kernel=nmk, **kw)
+        e.set_state(q, p, beta)
+        if ens == cam.CAM_ENS_CHILD:
+            e.begin_children()
+        e.run(5)
+        out.append(e.get_state() + ((e.kappa(5),) if ens == cam.CAM_ENS_CHILD else ()))
+        e.close()
+    (qa, pa, *ka), (qb, pb, *kb), (qv, pv, *kv) = out
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+    err = np.abs(qa - qv).max() / np.abs(qv).max()
+    perr = np.abs(pa - pv).max() / np.abs(pv).max()
+    print(f"surface={surface} nb={nb} prec={prec} ens={ens}: folded vs unfolded rel dq {err:.1e} dp {perr:.1e}")
+    assert err <= (1e-12 if prec == cam.CAM_PREC_FP64 else 1e-9)
+    assert perr <= (1e-10 if prec == cam.CAM_PREC_FP64 else 1e-6)
+    if ens == cam.CAM_ENS_CHILD:
+        assert ka[0][1] == kv[0][1]
+        np.testing.assert_array_equal(ka[0][0], kb[0][0])
+
+
 def test_folded_rejects_what_it_does_not_cover():
     w = workloads.h3_nve(nbeads=8)
     for kw in (dict(ensemble=cam.CAM_ENS_ANDERSEN, andersen_step=5), dict(remove_com=True),
@@ -146,3 +196,4 @@ def test_folded_rejects_what_it_does_not_cover():
     w = workloads.h3_nve(nbeads=2)
     with pytest.raises(cam.CamError, match="FOLDED"):
         cam.RPMDEngine(cam.CAM_SURFACE_H3, 2, 1, w["dt"], w["masses"], nm_kernel=cam.CAM_NMK_FOLDED)
+
