@@ -36,11 +36,12 @@ sys.path.insert(0, ROOT)
 CH4H_FLOPS_PER_BEAD = 12707
 CH4H_TRANSCENDENTALS_PER_BEAD = 321
 FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
-# HBM bytes of one step-kernel launch at 4096 trajectories x 32 beads, from
-# rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a plain 1 GiB copy
-# (profiles/step_kernel_ch4h_children_fp64_mfma.md); the state is read and
-# written once per launch, so the figure does not depend on steps per launch.
-MEASURED_TRAFFIC = {("mfma", 4096, 32): 117.0e6}
+# HBM bytes of one 50-step launch of the step kernel at 4096 trajectories x
+# 32 beads, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a
+# plain 1 GiB copy (profiles/fold_step_kernel_ch4h_children_fp64.md,
+# profiles/step_kernel_ch4h_children_fp64_mfma.md; the latter was measured
+# before the child centroids moved to a buffer, 29.5 MB per 50 steps).
+MEASURED_TRAFFIC = {("folded", 4096, 32): 145.6e6, ("mfma", 4096, 32): 117.0e6}
 
 
 NMK_NAMES = {1: "valu", 2: "mfma", 3: "folded"}
