@@ -164,13 +164,74 @@ module caracal_amd_mod
          type(cam_tuning), intent(out) :: tuning
          integer(c_int) :: rc
       end function
+      function cam_engine_last_timing(eng, ms_per_launch, steps_per_launch, launches) &
+            bind(C, name="cam_engine_last_timing") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: ms_per_launch
+         integer(c_int), intent(out) :: steps_per_launch, launches
+         integer(c_int) :: rc
+      end function
+      ! ragged batches: one engine per (surface, nbeads) bucket, run concurrently
+      function cam_batch_create(cfgs, nbuckets, batch) bind(C, name="cam_batch_create") result(rc)
+         import :: c_int, c_ptr, cam_config
+         type(cam_config), intent(in) :: cfgs(*)
+         integer(c_int), value :: nbuckets
+         type(c_ptr), intent(out) :: batch
+         integer(c_int) :: rc
+      end function
+      function cam_batch_destroy(batch) bind(C, name="cam_batch_destroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int) :: rc
+      end function
+      function cam_batch_engine(batch, bucket, eng) bind(C, name="cam_batch_engine") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int), value :: bucket
+         type(c_ptr), intent(out) :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_batch_autotune(batch, trial_steps) bind(C, name="cam_batch_autotune") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int), value :: trial_steps
+         integer(c_int) :: rc
+      end function
+      function cam_batch_run(batch, nsteps) bind(C, name="cam_batch_run") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int), value :: nsteps
+         integer(c_int) :: rc
+      end function
+      function cam_batch_sync(batch, ms) bind(C, name="cam_batch_sync") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: batch
+         real(c_double), intent(out) :: ms
+         integer(c_int) :: rc
+      end function
+      ! the reference's water model as a force field (src/egrad_water.f90)
+      function cam_water_create(nwater, periodic, box, zahn, rcut, ff) bind(C, name="cam_water_create") result(rc)
+         import :: c_int, c_ptr, c_double
+         integer(c_int), value :: nwater, periodic, zahn
+         real(c_double), value :: box, rcut
+         type(c_ptr), intent(out) :: ff
+         integer(c_int) :: rc
+      end function
+      function cam_ff_destroy(ff) bind(C, name="cam_ff_destroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ff
+         integer(c_int) :: rc
+      end function
    end interface
 
    public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_run, cam_engine_sync, cam_engine_kappa
-   public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune
+   public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
+   public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
+   public :: cam_batch_sync, cam_water_create, cam_ff_destroy
    public :: egrad_h3_gpu, egrad_ch4h_gpu, cam_error_message
 
 contains

@@ -60,3 +60,14 @@ def test_argument_errors_are_reported_not_aborted():
     cfg.surface, cfg.natoms, cfg.nbeads, cfg.ntraj, cfg.dt = 1, 3, 12, 1, 1.0
     assert lib.cam_engine_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
     assert b"nbeads" in lib.cam_last_error()
+
+
+def test_fortran_module_binds_declared_names_only():
+    """Every bind(C, name=...) of the Fortran host module is an entry point
+    the header declares, and the module binds the engine and batch calls."""
+    text = open(os.path.join(ROOT, "caracal_amd", "fortran", "caracal_amd_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(cam_\w+)"\)', text))
+    assert bound <= set(declared_functions()), bound - set(declared_functions())
+    for n in ("cam_engine_create", "cam_engine_run", "cam_engine_kappa", "cam_engine_autotune",
+              "cam_batch_create", "cam_batch_run", "cam_batch_sync", "cam_water_create"):
+        assert n in bound, n
