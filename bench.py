@@ -251,9 +251,42 @@ def cpu_baseline(seconds=15.0):
         res = pool.map(_cpu_worker, [(i, seconds) for i in range(cores)])
     steps = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
-    return dict(value=steps * 32 / wall, unit="bead-force-evals/s", cores=cores, kind="port",
-                sample=f"{cores} processes x {int(steps / cores)} child steps of one 32-bead CH4H trajectory "
-                       f"(oracle C port of verlet + egrad_ch4h, gcc -O2), {wall:.1f} s")
+    out = dict(value=steps * 32 / wall, unit="bead-force-evals/s", cores=cores, kind="port",
+               sample=f"{cores} processes x {int(steps / cores)} child steps of one 32-bead CH4H trajectory "
+                      f"(oracle C port of verlet + egrad_ch4h, gcc -O2), {wall:.1f} s")
+    ratio = reference_surface_ratio()
+    if ratio is not None:
+        out["reference_surface"] = ratio
+    return out
+
+
+def reference_surface_ratio(seconds=3.0):
+    """The reference's own egrad_ch4h (its Fortran sources compiled with
+    amdflang at the reference's -O1, oracle/_ref) against the port's C
+    surface, one core each, on the same 2048 child-workload beads: relates the
+    port baseline to the reference (the surface is >95% of a child step's CPU
+    time).  None when the compiled reference is not present."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from caracal_amd import workloads
+    try:
+        if ol.ref() is None:
+            return None
+    except OSError:
+        return None
+    q = workloads.ch4h_children(ntraj=64, nbeads=32, seed=0)["q"].reshape(-1, 6, 3)
+    rates = {}
+    for name, fn in (("reference_fortran", ol.ref_egrad_ch4h), ("port_c", ol.egrad_ch4h)):
+        fn(q)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            fn(q)
+            n += q.shape[0]
+        rates[name] = n / (time.perf_counter() - t0)
+    return {"reference_fortran_bead_evals_per_s_1core": rates["reference_fortran"],
+            "port_c_bead_evals_per_s_1core": rates["port_c"],
+            "reference_over_port": rates["reference_fortran"] / rates["port_c"],
+            "sample": f"egrad_ch4h on 2048 beads, {seconds:.0f} s each, one core"}
 
 
 def _cpu_worker(args):
