@@ -59,7 +59,9 @@ __device__ __forceinline__ double sum_lane_groups(double v)
 }
 
 // Folded normal-mode step of the block's columns, in place in xs
-// ([column][bead 0..H], column = 2*(tl*NC + coord) + (0: p, 1: q)).
+// ([column][bead 0..H], column = 2*(tl*NC + coord) + (0: p, 1: q)); tf is
+// the block's LDS copy of M with row stride H + 1 (the 16 rows a wave reads
+// per k step fall in distinct banks).
 // Operand layout of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16] and receives D[l/16 + 4r][l%16] in accumulator element r.
 template <class S, int NB, bool CENT>
@@ -84,7 +86,7 @@ __device__ __forceinline__ void nm_fold(double *xs, const double *prop, const do
 #pragma unroll
         for (int kk = 0; kk < KK; ++kk) {
             const int row = 16 * b + li, c = 4 * kk + lk;
-            av[b][kk] = row < H && c < H ? tf[row * H + c] : 0.0;
+            av[b][kk] = row < H && c < H ? tf[row * (H + 1) + c] : 0.0;
         }
 #pragma unroll 1
     for (int gi = 0; gi < NG; ++gi) {
@@ -136,14 +138,14 @@ __device__ __forceinline__ void nm_fold(double *xs, const double *prop, const do
                     const int mode = min(16 * b + lk + 4 * r, H);
                     const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(acc[t][b][r]);
                     const double *pr = pt + mode * PW;
-                    const double cw = pr[0], p2 = pr[1 + 2 * at], p3 = pr[2 + 2 * at];
-                    acc[t][b][r] = isq ? other * p3 + acc[t][b][r] * cw : acc[t][b][r] * cw + other * p2;
+                    const double cw = pr[0], cx = pr[1 + 2 * at + (isq ? 1 : 0)];
+                    acc[t][b][r] = isq ? other * cx + acc[t][b][r] * cw : acc[t][b][r] * cw + other * cx;
                 }
             {
                 const double other = quad::dpp<quad::perm(1, 0, 3, 2)>(yh[t]);
                 const double *pr = pt + H * PW;
-                const double cw = pr[0], p2 = pr[1 + 2 * at], p3 = pr[2 + 2 * at];
-                yh[t] = isq ? other * p3 + yh[t] * cw : yh[t] * cw + other * p2;
+                const double cw = pr[0], cx = pr[1 + 2 * at + (isq ? 1 : 0)];
+                yh[t] = isq ? other * cx + yh[t] * cw : yh[t] * cw + other * cx;
             }
             // The centroid of the propagated beads is mode 0 of q over sqrt(N).
             if (CENT && isq && ok[t] && lk == 0) cent[pair] = acc[t][0][0] * s;
@@ -202,7 +204,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS, H = F::H, NH = F::NH, TPB = F::TPB;
     constexpr int PW = PropTab<S::NA>::W;
     constexpr bool CHILD = ENS == CAM_ENS_CHILD;
-    __shared__ __align__(16) double X[F::NCOL * NH];
+    // Two column buffers, used by alternate steps: a step's staging writes
+    // cannot reach the buffer the previous step's read-back still reads, so
+    // the step needs no barrier before its staging.
+    __shared__ __align__(16) double X[2][F::NCOL * NH];
+    __shared__ double tfs[H * (H + 1)];       // M, row stride H + 1
     __shared__ __align__(16) double cent[CHILD ? TPB * NC : 1];
     __shared__ double prop[TPB * NH * PW];
 
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
     const int traj_raw = blockIdx.x * TPB + tl;
     const bool act = live && traj_raw < a.ntraj;
     const int traj = traj_raw < a.ntraj ? traj_raw : a.ntraj - 1;
-    double *xs = X + tl * 2 * NC * NH;
+    const int xoff = tl * 2 * NC * NH;        // this trajectory's columns in a column buffer
 
     int cc[SL];
     double q[SL], p[SL], g[SL];
@@ -250,15 +256,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
             }
         }
     }
+    for (int i = tid; i < H * H; i += kBlock) tfs[(i / H) * (H + 1) + i % H] = a.tfold[i];
     const double hdt = 0.5 * a.dt;
     double vb = 0.0;
     for (int it = 0; it < a.nsteps; ++it) {
+        double *xb = X[it & 1];
+        double *xs = xb + xoff;
         // half kick (src/verlet.f90:216-218)
 #pragma unroll
         for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
-        // bead -> modes, propagate, back (src/verlet.f90:377-463); the first
-        // barrier also orders the table before the first step
-        __syncthreads();
+        // bead -> modes, propagate, back (src/verlet.f90:377-463)
         if (live) {
 #pragma unroll
             for (int s = 0; s < SL; ++s)
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
                 }
         }
         __syncthreads();
-        nm_fold<S, NB, CHILD>(X, prop, a.tfold, cent);
+        nm_fold<S, NB, CHILD>(xb, prop, tfs, cent);
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
             p[s] = xs[(2 * cc[s]) * NH + j];
