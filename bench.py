@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 # are counted separately and not included in the flop figure.
 CH4H_FLOPS_PER_BEAD = 12707
 CH4H_TRANSCENDENTALS_PER_BEAD = 321
+# the box force field (oracle/ff_box.c through the same counter): one
+# intermolecular pair inside / outside the cutoff (energy and gradient, the
+# minimum image included), one bond, one angle
+BOX_FLOPS = {"pair_in": 95, "pair_out": 21, "bond": 49, "angle": 116}
 FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
 # HBM bytes of one 50-step launch of the step kernel at 4096 trajectories x
 # 32 beads, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a
@@ -60,6 +64,16 @@ def step_flops(natoms, nbeads, folded=False):
     h = nbeads // 2
     nc = 3 * natoms * (h + 1)
     return 4 * nc + 12 * natoms * (2 * h * h + 4 * h) + 6 * nc + (h + 1) * CH4H_FLOPS_PER_BEAD
+
+
+def step_flops_parts(natoms, nbeads, folded=False):
+    """The same work split into the surface, the normal-mode transform pair
+    and the rest (kicks and the per-mode propagator); the three run fused in
+    one kernel, so they share one time."""
+    n = nbeads // 2 + 1 if folded else nbeads
+    surface = n * CH4H_FLOPS_PER_BEAD
+    rest = 10 * 3 * natoms * n
+    return {"surface": surface, "normal_modes": step_flops(natoms, nbeads, folded) - surface - rest, "rest": rest}
 
 
 def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0,
@@ -202,6 +216,34 @@ def box_nvt(cam, steps=500):
         out[name] = {"bead_force_evals_per_s": nb * steps / wall, "ms_per_step": wall / steps * 1e3, "ms_gpu": ms}
     out["atoms"], out["nbeads"], out["steps"] = len(s["type"]), nb, steps
     ff.close()
+    # algorithmic work of one step: every intermolecular pair tested (the
+    # reference's all-pairs loop, src/ff_nonb.f90:198-332), the pairs inside
+    # the cutoff evaluated (counted at the start geometry), bonds, angles, and
+    # the normal-mode step of the box (24 N n^2)
+    x = s["x"]
+    n = len(x)
+    box = s["box"]
+    inter = 0
+    inside = 0
+    rc = 10.0 / 0.52917721092
+    for i in range(n - 1):
+        d = x[i + 1:] - x[i]
+        d -= box * np.rint(d / box)
+        r2 = (d * d).sum(axis=1)
+        other = s["mol"][i + 1:] != s["mol"][i]
+        inter += int(other.sum())
+        inside += int((other & (r2 < rc * rc)).sum())
+    per_bead = ((inter - inside) * BOX_FLOPS["pair_out"] + inside * BOX_FLOPS["pair_in"]
+                + len(s["bonds"]) * BOX_FLOPS["bond"] + len(s["angles"]) * BOX_FLOPS["angle"])
+    flops = nb * per_bead + 24 * n * nb * nb
+    out["pairs_per_bead"] = {"intermolecular": inter, "inside_cutoff": inside}
+    out["flops_per_step"] = flops
+    out["roofline"] = {name: {"achieved": flops / (out[name]["ms_per_step"] * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s",
+                              "frac": flops / (out[name]["ms_per_step"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+                       for name in ("fp64", "mixed")}
+    out["roofline"]["note"] = ("vector ALU bound; the binary32 pair arithmetic runs without packed FP32, whose "
+                               "peak then equals the FP64 one")
     return out
 
 
@@ -404,6 +446,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get((nmk_name, ntraj, nb)),
                      "kernel": kernel, "ms_per_launch": ms_launch, "flops_per_launch": flops_launch,
+                     "flops_split_per_trajectory_step": step_flops_parts(6, nb, folded),
                      "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
     }
     if not args.skip_extras:
