@@ -12,6 +12,7 @@ CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
 CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD = 0, 1, 2
 CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA, CAM_NMK_FOLDED = 0, 1, 2, 3
+CAM_FF_PAIRS_LIST, CAM_FF_PAIRS_ALL = 0, 1
 CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT = 0, 1, 2
 CAM_MAX_ATOMS, CAM_MAX_BONDS, CAM_MAX_REAC = 16, 4, 4
 
@@ -107,6 +108,7 @@ PROTOTYPES = {
     "cam_batch_sync": (ctypes.c_int, [_vp, _dp]),
     "cam_ff_create": (ctypes.c_int, [ctypes.POINTER(FFDesc), ctypes.POINTER(_vp)]),
     "cam_ff_destroy": (ctypes.c_int, [_vp]),
+    "cam_ff_set_pair_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_ff_energy": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
     "cam_ff_forces": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "cam_box_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_int,

@@ -73,12 +73,90 @@ __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 
+// The j-range of split s of the nonbonded pair loop: every kernel that walks
+// pairs by split uses the same ranges.
+__host__ __device__ __forceinline__ int split_chunk(int n, int nsplit)
+{
+    return ((n + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
+}
+
+// Per atom: the centroid of its beads (the mean of the minimum-image offsets
+// from bead 0) and the largest distance of one of its beads from it.
+__global__ void centroid_dev_kernel(int n, int nbeads, double box, const double *__restrict__ x,
+                                    double *__restrict__ cen, double *__restrict__ dev)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x0[3], s[3] = { 0.0, 0.0, 0.0 };
+    for (int c = 0; c < 3; ++c) x0[c] = x[3 * i + c];
+    for (int b = 0; b < nbeads; ++b)
+        for (int c = 0; c < 3; ++c) s[c] += min_image(x[((size_t)b * n + i) * 3 + c] - x0[c], box);
+    double cc[3];
+    for (int c = 0; c < 3; ++c) cc[c] = x0[c] + s[c] / nbeads;
+    double m = 0.0;
+    for (int b = 0; b < nbeads; ++b) {
+        double r2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double d = min_image(x[((size_t)b * n + i) * 3 + c] - cc[c], box);
+            r2 += d * d;
+        }
+        m = fmax(m, r2);
+    }
+    for (int c = 0; c < 3; ++c) cen[3 * i + c] = cc[c];
+    dev[i] = sqrt(m);
+}
+
+// Neighbour list shared by all beads: atom i keeps every atom j of another
+// molecule whose centroid lies within rc + dev_i + dev_j + margin of its own.
+// For any bead, |d_ij| >= |c_i - c_j| - dev_i - dev_j, so no pair inside the
+// cutoff in any bead is left out; margin covers the binary32 rounding of the
+// mixed kernel's distances.  Row i lists j in increasing order; nofs[i][s] is
+// where split s's j-range starts in it, so the pair loop meets the same pairs
+// in the same order as the all-pairs loop.  One wave per atom.
+__global__ __launch_bounds__(256) void nlist_kernel(FFDev f, int nsplit, double margin, const double *__restrict__ cen,
+                                                    const double *__restrict__ dev, int *__restrict__ nbr,
+                                                    int *__restrict__ nofs)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int n = f.natoms;
+    if (i >= n) return;                    // the whole wave
+    const double c0 = cen[3 * i], c1 = cen[3 * i + 1], c2 = cen[3 * i + 2], di = dev[i];
+    const int mi = f.mol[i];
+    const int chunk = split_chunk(n, nsplit);
+    int *row = nbr + (size_t)i * n;
+    int *ofs = nofs + (size_t)i * (nsplit + 1);
+    int cnt = 0;
+    for (int sp = 0; sp < nsplit; ++sp) {
+        if (lane == 0) ofs[sp] = cnt;
+        const int j0 = sp * chunk, j1 = min(n, j0 + chunk);
+        for (int jb = j0; jb < j1; jb += 64) {
+            const int j = jb + lane;
+            bool keep = false;
+            if (j < j1 && f.mol[j] != mi) {
+                const double d0 = min_image(c0 - cen[3 * j], f.box), d1 = min_image(c1 - cen[3 * j + 1], f.box),
+                             d2 = min_image(c2 - cen[3 * j + 2], f.box);
+                const double lim = f.rc + di + dev[j] + margin;
+                keep = d0 * d0 + d1 * d1 + d2 * d2 < lim * lim;
+            }
+            const unsigned long long m = __ballot(keep);
+            if (keep) row[cnt + __popcll(m & ((1ull << lane) - 1ull))] = j;
+            cnt += __popcll(m);
+        }
+    }
+    if (lane == 0) ofs[nsplit] = cnt;
+}
+
 // Nonbonded pairs of i against a j-range; GRAD: gradient of i, PAR: parameter
-// derivatives (and with v, the force-matching contraction).
-template <typename T, bool GRAD, bool PAR>
+// derivatives (and with v, the force-matching contraction).  LIST: the pairs
+// come from the neighbour list (nbr, nofs) instead of all atoms of the range;
+// the pairs left out are beyond the cutoff in every bead, so both forms add
+// the same terms in the same order.
+template <typename T, bool GRAD, bool PAR, bool LIST>
 __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, const double *__restrict__ x,
                                                           const double *__restrict__ v, int nsplit,
-                                                          double *__restrict__ gpart, double *__restrict__ epart)
+                                                          double *__restrict__ gpart, double *__restrict__ epart,
+                                                          const int *__restrict__ nbr, const int *__restrict__ nofs)
 {
     __shared__ T sx[kTile][3];
     __shared__ T sv[kTile][3];
@@ -117,39 +195,13 @@ __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, con
     // pairs OO/OH/HH; dv* the same for the force-matching contraction
     double dq[2] = { 0.0, 0.0 }, dc[3] = { 0.0, 0.0, 0.0 }, dvq[2] = { 0.0, 0.0 }, dvc[3] = { 0.0, 0.0, 0.0 };
 
-    const int chunk = ((n + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
-    const int j0 = split * chunk, j1 = min(n, j0 + chunk);
-    for (int jt = j0; jt < j1; jt += kTile) {
-        __syncthreads();
-        const int j = jt + tid;
-        if (j < j1) {
-            sx[tid][0] = (T)xb[3 * j];
-            sx[tid][1] = (T)xb[3 * j + 1];
-            sx[tid][2] = (T)xb[3 * j + 2];
-            st[tid] = f.type[j];
-            sm[tid] = f.mol[j];
-            if (PAR && vb) {
-                sv[tid][0] = (T)vb[3 * j];
-                sv[tid][1] = (T)vb[3 * j + 1];
-                sv[tid][2] = (T)vb[3 * j + 2];
-            }
-        } else {
-            sm[tid] = mi;            // padding: excluded as intramolecular
-            st[tid] = 0;
-            sx[tid][0] = sx[tid][1] = sx[tid][2] = 0;
-            if (PAR) sv[tid][0] = sv[tid][1] = sv[tid][2] = 0;
-        }
-        __syncthreads();
-        if (!act) continue;
-        const int nt = min(kTile, j1 - jt);
-        for (int jj = 0; jj < nt; ++jj) {
-            if (sm[jj] == mi) continue;
-            const T d0 = min_image(xi0 - sx[jj][0], box);
-            const T d1 = min_image(xi1 - sx[jj][1], box);
-            const T d2 = min_image(xi2 - sx[jj][2], box);
+    // one pair (i, j): j's position, type and direction vector
+    auto pair = [&](T xj0, T xj1, T xj2, int tj, T vj0, T vj1, T vj2) {
+            const T d0 = min_image(xi0 - xj0, box);
+            const T d1 = min_image(xi1 - xj1, box);
+            const T d2 = min_image(xi2 - xj2, box);
             const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (r2 > rc2) continue;
-            const int tj = st[jj];
+            if (r2 > rc2) return;
             const bool jo = tj == 0;
             const T r = sqrt_t(r2);
             const T rinv = (T)1 / r;
@@ -185,7 +237,7 @@ __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, con
                     const double hd = 0.5 * (double)dispu;
                     if (cp == 0) dc[0] += hd; else if (cp == 1) dc[1] += hd; else dc[2] += hd;
                     if (vb) {
-                        const T proj = ((vi0 - sv[jj][0]) * d0 + (vi1 - sv[jj][1]) * d1 + (vi2 - sv[jj][2]) * d2) * rinv;
+                        const T proj = ((vi0 - vj0) * d0 + (vi1 - vj1) * d1 + (vi2 - vj2) * d2) * rinv;
                         const double vj_ = 0.5 * (double)(qj * dphi * proj), vi_ = 0.5 * (double)(qi * dphi * proj);
                         if (ti == 0) dvq[0] += vj_; else dvq[1] += vj_;
                         if (jo) dvq[0] += vi_; else dvq[1] += vi_;
@@ -193,6 +245,53 @@ __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, con
                         if (cp == 0) dvc[0] += vd; else if (cp == 1) dvc[1] += vd; else dvc[2] += vd;
                     }
                 }
+            }
+    };
+    const int chunk = split_chunk(n, nsplit);
+    const int j0 = split * chunk, j1 = min(n, j0 + chunk);
+    if constexpr (LIST) {
+        if (act) {
+            const int *row = nbr + (size_t)i * n;
+            const int k0 = nofs[(size_t)i * (nsplit + 1) + split], k1 = nofs[(size_t)i * (nsplit + 1) + split + 1];
+            for (int k = k0; k < k1; ++k) {
+                const int j = row[k];
+                T vj0 = 0, vj1 = 0, vj2 = 0;
+                if (PAR && vb) {
+                    vj0 = (T)vb[3 * j];
+                    vj1 = (T)vb[3 * j + 1];
+                    vj2 = (T)vb[3 * j + 2];
+                }
+                pair((T)xb[3 * j], (T)xb[3 * j + 1], (T)xb[3 * j + 2], f.type[j], vj0, vj1, vj2);
+            }
+        }
+    } else {
+        for (int jt = j0; jt < j1; jt += kTile) {
+            __syncthreads();
+            const int j = jt + tid;
+            if (j < j1) {
+                sx[tid][0] = (T)xb[3 * j];
+                sx[tid][1] = (T)xb[3 * j + 1];
+                sx[tid][2] = (T)xb[3 * j + 2];
+                st[tid] = f.type[j];
+                sm[tid] = f.mol[j];
+                if (PAR && vb) {
+                    sv[tid][0] = (T)vb[3 * j];
+                    sv[tid][1] = (T)vb[3 * j + 1];
+                    sv[tid][2] = (T)vb[3 * j + 2];
+                }
+            } else {
+                sm[tid] = mi;            // padding: excluded as intramolecular
+                st[tid] = 0;
+                sx[tid][0] = sx[tid][1] = sx[tid][2] = 0;
+                if (PAR) sv[tid][0] = sv[tid][1] = sv[tid][2] = 0;
+            }
+            __syncthreads();
+            if (!act) continue;
+            const int nt = min(kTile, j1 - jt);
+            for (int jj = 0; jj < nt; ++jj) {
+                if (sm[jj] == mi) continue;
+                pair(sx[jj][0], sx[jj][1], sx[jj][2], st[jj], PAR ? sv[jj][0] : T(0), PAR ? sv[jj][1] : T(0),
+                     PAR ? sv[jj][2] : T(0));
             }
         }
     }
@@ -586,12 +685,31 @@ struct cam_ff {
     int *type = nullptr, *mol = nullptr, *bonds = nullptr, *angles = nullptr, *csr_ptr = nullptr, *csr_term = nullptr;
     double *gpart = nullptr, *epart = nullptr;
     int cap_beads = 0, nsplit = 4;
+    int pair_mode = CAM_FF_PAIRS_LIST;
+    int *nbr = nullptr, *nofs = nullptr;   // neighbour list [atom][atom], split offsets [atom][nsplit + 1]
+    double *cen = nullptr, *dev = nullptr; // bead centroids and spreads of the last call
 };
 
 namespace {
 
+// Largest box for which the neighbour list is kept: its rows hold up to
+// natoms entries each (natoms^2 ints, 1 GiB at this size).
+constexpr int kListMaxAtoms = 16384;
+
+bool ff_uses_list(const cam_ff *ff)
+{
+    return ff->kind == 0 && ff->pair_mode == CAM_FF_PAIRS_LIST && ff->natoms <= kListMaxAtoms;
+}
+
 int ff_ensure(cam_ff *ff, int nbeads)
 {
+    if (ff_uses_list(ff) && !ff->nbr) {
+        const size_t n = ff->natoms;
+        CAM_HIP(hipMalloc(&ff->nbr, n * n * sizeof(int)));
+        CAM_HIP(hipMalloc(&ff->nofs, n * (ff->nsplit + 1) * sizeof(int)));
+        CAM_HIP(hipMalloc(&ff->cen, n * 3 * sizeof(double)));
+        CAM_HIP(hipMalloc(&ff->dev, n * sizeof(double)));
+    }
     if (nbeads <= ff->cap_beads) return CAM_OK;
     if (ff->gpart) CAM_HIP(hipFree(ff->gpart));
     if (ff->epart) CAM_HIP(hipFree(ff->epart));
@@ -637,18 +755,41 @@ int ff_launch(cam_ff *ff, const Theta &th, const double *x, int nbeads, double *
         CAM_LAUNCH_CHECK();
         return CAM_OK;
     }
-    if (grad && par)
-        hipLaunchKernelGGL((nonbonded_kernel<T, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v, ff->nsplit,
-                           ff->gpart, ff->epart);
-    else if (grad)
-        hipLaunchKernelGGL((nonbonded_kernel<T, true, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, nullptr,
-                           ff->nsplit, ff->gpart, ff->epart);
-    else if (par)
-        hipLaunchKernelGGL((nonbonded_kernel<T, false, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v, ff->nsplit,
-                           ff->gpart, ff->epart);
-    else
-        hipLaunchKernelGGL((nonbonded_kernel<T, false, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, nullptr,
-                           ff->nsplit, ff->gpart, ff->epart);
+    if (ff_uses_list(ff)) {
+        // the list of this call's beads: centroids and spreads, then one wave per atom
+        hipLaunchKernelGGL(centroid_dev_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, nbeads, ff->d.box, x,
+                           ff->cen, ff->dev);
+        CAM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(nlist_kernel, dim3((n + 3) / 4), dim3(256), 0, s, ff->d, ff->nsplit, 1e-3, ff->cen,
+                           ff->dev, ff->nbr, ff->nofs);
+        CAM_LAUNCH_CHECK();
+        if (grad && par)
+            hipLaunchKernelGGL((nonbonded_kernel<T, true, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
+                               ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
+        else if (grad)
+            hipLaunchKernelGGL((nonbonded_kernel<T, true, false, true>), gnb, dim3(kTile), 0, s, ff->d, th, x,
+                               nullptr, ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
+        else if (par)
+            hipLaunchKernelGGL((nonbonded_kernel<T, false, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
+                               ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
+        else
+            hipLaunchKernelGGL((nonbonded_kernel<T, false, false, true>), gnb, dim3(kTile), 0, s, ff->d, th, x,
+                               nullptr, ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
+    } else {
+        const int *nl = nullptr;
+        if (grad && par)
+            hipLaunchKernelGGL((nonbonded_kernel<T, true, true, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
+                               ff->nsplit, ff->gpart, ff->epart, nl, nl);
+        else if (grad)
+            hipLaunchKernelGGL((nonbonded_kernel<T, true, false, false>), gnb, dim3(kTile), 0, s, ff->d, th, x,
+                               nullptr, ff->nsplit, ff->gpart, ff->epart, nl, nl);
+        else if (par)
+            hipLaunchKernelGGL((nonbonded_kernel<T, false, true, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
+                               ff->nsplit, ff->gpart, ff->epart, nl, nl);
+        else
+            hipLaunchKernelGGL((nonbonded_kernel<T, false, false, false>), gnb, dim3(kTile), 0, s, ff->d, th, x,
+                               nullptr, ff->nsplit, ff->gpart, ff->epart, nl, nl);
+    }
     CAM_LAUNCH_CHECK();
     dim3 gb(nbx, nbeads);
     if (grad && par)
@@ -788,7 +929,21 @@ int cam_ff_destroy(cam_ff *ff)
         if (p) (void)hipFree(p);
     if (ff->gpart) (void)hipFree(ff->gpart);
     if (ff->epart) (void)hipFree(ff->epart);
+    int *lb[] = { ff->nbr, ff->nofs };
+    for (int *p : lb)
+        if (p) (void)hipFree(p);
+    if (ff->cen) (void)hipFree(ff->cen);
+    if (ff->dev) (void)hipFree(ff->dev);
     delete ff;
+    return CAM_OK;
+}
+
+int cam_ff_set_pair_mode(cam_ff *ff, int mode)
+{
+    CAM_CHECK_ARG(ff, "null force field");
+    CAM_CHECK_ARG(mode == CAM_FF_PAIRS_LIST || mode == CAM_FF_PAIRS_ALL, "bad pair mode %d", mode);
+    clear_error();
+    ff->pair_mode = mode;
     return CAM_OK;
 }
 

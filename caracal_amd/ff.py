@@ -48,6 +48,11 @@ class ForceField:
         self._h, self.natoms, self._keep = h, 3 * nwater, []
         return self
 
+    def set_pair_mode(self, mode):
+        """_abi.CAM_FF_PAIRS_LIST (default: a neighbour list from the bead
+        centroids, the same pairs and order as all pairs) or CAM_FF_PAIRS_ALL."""
+        check("cam_ff_set_pair_mode", lib().cam_ff_set_pair_mode(self._h, mode))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().cam_ff_destroy(self._h)

@@ -22,6 +22,7 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, &
                                             CAM_NMK_FOLDED = 3
    integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
+   integer(c_int32_t), parameter, public :: CAM_FF_PAIRS_LIST = 0, CAM_FF_PAIRS_ALL = 1
    integer, parameter, public :: CAM_MAX_ATOMS = 16, CAM_MAX_BONDS = 4, CAM_MAX_REAC = 4
 
    ! cam_reaction: the BIMOLEC reaction coordinate (MECHA section keys)
@@ -223,6 +224,12 @@ module caracal_amd_mod
          type(c_ptr), value :: ff
          integer(c_int) :: rc
       end function
+      function cam_ff_set_pair_mode(ff, mode) bind(C, name="cam_ff_set_pair_mode") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ff
+         integer(c_int), value :: mode
+         integer(c_int) :: rc
+      end function
    end interface
 
    public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
@@ -231,7 +238,7 @@ module caracal_amd_mod
    public :: cam_engine_begin_children, cam_engine_run, cam_engine_sync, cam_engine_kappa
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
-   public :: cam_batch_sync, cam_water_create, cam_ff_destroy
+   public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode
    public :: egrad_h3_gpu, egrad_ch4h_gpu, cam_error_message
 
 contains

@@ -291,6 +291,19 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out);
  * parameter derivatives are not available. */
 int cam_water_create(int nwater, int periodic, double box, int zahn, double rcut, cam_ff **out);
 int cam_ff_destroy(cam_ff *ff);
+/* How the QMDFF-form nonbonded pairs are found.  LIST (default): a
+ * neighbour list rebuilt on every call from the bead centroids, holding every
+ * pair of different molecules whose centroids are within rc + d_i + d_j
+ * (d = the largest bead distance from its atom's centroid), so that no pair
+ * inside the cutoff in any bead is missed; the pairs are then evaluated in
+ * the all-pairs loop's order and the results are the same bit for bit.  Up
+ * to 16384 atoms (the list takes natoms^2 ints); larger boxes use ALL.  ALL:
+ * every pair tested, as the reference's loop (src/ff_nonb.f90:198-332).  The
+ * mode applies to work enqueued afterwards (a box engine's captured graphs
+ * keep the mode they were captured with).  No counterpart in the reference,
+ * which has no neighbour list. */
+enum { CAM_FF_PAIRS_LIST = 0, CAM_FF_PAIRS_ALL = 1 };
+int cam_ff_set_pair_mode(cam_ff *ff, int mode);
 /* Energy of nbeads replicas: x device [nbeads][natoms][3] bohr, e device
  * [nbeads] hartree.  theta is a host array of 10.  Asynchronous on stream. */
 int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, int precision,
