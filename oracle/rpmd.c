@@ -454,3 +454,42 @@ double oracle_verlet(const oracle_sys *s, const oracle_rx *rx, double *q, double
     if (remove_com && ensemble != 2) oracle_transrot(s, q, p);
     return xi;
 }
+
+/* Start of a recrossing parent (mdinit with bias_mode 2, src/mdinit.f90:79-95):
+ * the constraint direction dxi is the gradient of the mode-1 reaction
+ * coordinate s0/(s0 - s1) at the centroid (umbrella mode 0 -> calc_xi mode 1,
+ * src/umbrella.f90:104-106).  The bias k (xi - xi_ideal) dxi that umbrella
+ * mode 0 adds to the forces vanishes at a centroid on the dividing surface
+ * (mode-1 xi = 1 = xi_ideal there) and is left out. */
+void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *dxi)
+{
+    double c[3 * ORACLE_MAX_ATOMS];
+    oracle_centroid(s, q, c);
+    (void)oracle_calc_xi(s, rx, c, dxi, 1);
+}
+
+/* One step of a recrossing parent (verlet with constrain = 1,
+ * src/verlet.f90): p -= dt/2 g; free ring polymer; centroid c; constrain_q
+ * with the constraint direction dxi of the previous step (:747); forces;
+ * dxi <- gradient of xi (mode 2) at c, the centroid from before the
+ * constraint (umbrella mode 1, :1048); p -= dt/2 g; constrain_p(dxi)
+ * (:1081); Andersen every andersen_step steps (:1159-1165).  Returns the
+ * constraint status (0 = converged), as the reference, which then gives the
+ * step an energy penalty and goes on. */
+int oracle_verlet_parent(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g, double *vbead,
+                         double *dxi, double beta, long istep, int andersen_step, uint64_t seed, uint32_t traj)
+{
+    const int na = s->natoms, nb = s->nbeads;
+    const size_t n = (size_t)3 * na * nb;
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    oracle_free_rp(s, q, p, beta);
+    double c[3 * ORACLE_MAX_ATOMS];
+    oracle_centroid(s, q, c);
+    const int status = oracle_constrain_q(s, rx, c, dxi, q, p);
+    oracle_forces(s, q, vbead, g);
+    (void)oracle_calc_xi(s, rx, c, dxi, 2);
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    oracle_constrain_p(s, dxi, p);
+    if (andersen_step > 0 && istep % andersen_step == 0) oracle_andersen(s, p, beta, seed, traj, (uint64_t)istep);
+    return status;
+}

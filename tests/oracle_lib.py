@@ -137,6 +137,11 @@ def _setup_rpmd():
     o.oracle_constrain_q.restype = ctypes.c_int
     o.oracle_constrain_q.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp]
     o.oracle_constrain_p.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
+    o.oracle_parent_begin.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp]
+    o.oracle_verlet_parent.restype = ctypes.c_int
+    o.oracle_verlet_parent.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp, _dp,
+                                       ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.c_uint32]
     o.oracle_free_rp.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp, ctypes.c_double]
     o.oracle_rfft.argtypes = [_dp, ctypes.c_int]
     o.oracle_transrot.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
@@ -179,9 +184,18 @@ class OracleTraj:
 
     def step(self):
         self.istep += 1
+        if self.ensemble == 3:
+            return rpmd().oracle_verlet_parent(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q),
+                                               _ptr(self.p), _ptr(self.g), _ptr(self.v), _ptr(self.dxi), self.beta,
+                                               self.istep, self.andersen_step, self.seed, self.traj)
         return rpmd().oracle_verlet(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.p),
                                     _ptr(self.g), _ptr(self.v), self.beta, self.ensemble, int(self.remove_com),
                                     self.istep, self.andersen_step, self.seed, self.traj)
+
+    def parent_begin(self):
+        """Recrossing parent (ensemble 3): the constraint direction of mdinit."""
+        self.dxi = np.zeros((self.q.shape[1], 3))
+        rpmd().oracle_parent_begin(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.dxi))
 
 
 def calc_xi(sys, rx, c, mode=2):
