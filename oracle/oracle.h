@@ -71,7 +71,11 @@ void oracle_free_rp(const oracle_sys *s, double *q, double *p, double beta);
 void oracle_centroid(const oracle_sys *s, const double *q, double *c);
 void oracle_transrot(const oracle_sys *s, const double *q, double *p);
 double oracle_calc_xi(const oracle_sys *s, const oracle_rx *rx, const double *c, double *dxi, int mode);
-void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *dxi);
+double oracle_calc_xi_hess(const oracle_sys *s, const oracle_rx *rx, const double *c, double *dxi, double *h, int mode);
+double oracle_umbrella(const oracle_sys *s, const oracle_rx *rx, const double *c, double beta, double k_force,
+                       double *dxi, double *g, double *xi_real);
+void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *g, double *dxi,
+                         double beta, double k_force);
 int oracle_verlet_parent(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g, double *vbead,
                          double *dxi, double beta, long istep, int andersen_step, uint64_t seed, uint32_t traj);
 int oracle_constrain_q(const oracle_sys *s, const oracle_rx *rx, const double *centroid, const double *dxi_act,

@@ -461,11 +461,153 @@ double oracle_verlet(const oracle_sys *s, const oracle_rx *rx, double *q, double
  * src/umbrella.f90:104-106).  The bias k (xi - xi_ideal) dxi that umbrella
  * mode 0 adds to the forces vanishes at a centroid on the dividing surface
  * (mode-1 xi = 1 = xi_ideal there) and is left out. */
-void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *dxi)
+/* Hessian of xi (src/calc_xi.f90:283-502) at coords c(3,natoms), mode 1 or
+ * 2: h[(3*a+d)*3*natoms + 3*b+e] = d2xi(d,a,e,b), atom-major as c.  s1, s0
+ * and their gradients come from oracle_calc_xi mode 2 with xi_ideal 1 and 0,
+ * which returns each of them exactly.  Returns xi; dxi as oracle_calc_xi. */
+double oracle_calc_xi_hess(const oracle_sys *s, const oracle_rx *rx, const double *c, double *dxi, double *h, int mode)
+{
+    const int na = s->natoms, n = 3 * na;
+    oracle_rx r1 = *rx, r0 = *rx;
+    r1.xi_ideal = 1.0;
+    r0.xi_ideal = 0.0;
+    double ds1[3 * ORACLE_MAX_ATOMS], ds0[3 * ORACLE_MAX_ATOMS];
+    const double s1 = oracle_calc_xi(s, &r1, c, ds1, 2), s0 = oracle_calc_xi(s, &r0, c, ds0, 2);
+    const double xi = oracle_calc_xi(s, rx, c, dxi, mode);
+    static double h1[3 * ORACLE_MAX_ATOMS][3 * ORACLE_MAX_ATOMS], h0[3 * ORACLE_MAX_ATOMS][3 * ORACLE_MAX_ATOMS];
+    memset(h1, 0, sizeof h1);
+    memset(h0, 0, sizeof h0);
+    /* d2 of a bond length: (|R|^2 delta_de - R_d R_e)/|R|^3, the sign of the
+     * term it enters s1 with (forming -, breaking +), over the bond count */
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nbond = pass == 0 ? rx->n_form : rx->n_break;
+        const double sg = pass == 0 ? -1.0 : 1.0;
+        for (int i = 0; i < nbond; ++i) {
+            const int a = (pass == 0 ? rx->bond_form[i][0] : rx->bond_break[i][0]) - 1;
+            const int b = (pass == 0 ? rx->bond_form[i][1] : rx->bond_break[i][1]) - 1;
+            double R[3];
+            for (int d = 0; d < 3; ++d) R[d] = c[3 * a + d] - c[3 * b + d];
+            const double rinv = 1.0 / sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+            const double r3 = rinv * rinv * rinv;
+            double blk[3][3];
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 3; ++e) {
+                    if (d == e) blk[d][e] = sg * (R[(d + 1) % 3] * R[(d + 1) % 3] + R[(d + 2) % 3] * R[(d + 2) % 3]) * r3;
+                    else blk[d][e] = -sg * R[d < e ? d : e] * R[d < e ? e : d] * r3;
+                    blk[d][e] = blk[d][e] / (double)(float)nbond;
+                }
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 3; ++e) {
+                    h1[3 * a + d][3 * a + e] += blk[d][e];
+                    h1[3 * a + d][3 * b + e] -= blk[d][e];
+                    h1[3 * b + d][3 * a + e] -= blk[d][e];
+                    h1[3 * b + d][3 * b + e] += blk[d][e];
+                }
+        }
+    }
+    /* reactant centre-of-mass distances: s0 = r_inf - |R|, mass-weighted */
+    double com[4][3], mreac[4];
+    for (int k = 0; k < rx->n_reac; ++k) {
+        mreac[k] = 0.0;
+        for (int i = 0; i < rx->reac_natoms[k]; ++i) mreac[k] += s->mass[rx->reac_atoms[k][i] - 1];
+        com[k][0] = com[k][1] = com[k][2] = 0.0;
+    }
+    for (int k = 0; k < rx->n_reac; ++k)
+        for (int i = 0; i < rx->reac_natoms[k]; ++i) {
+            const int at = rx->reac_atoms[k][i] - 1;
+            for (int j = 0; j < 3; ++j) com[k][j] = com[k][j] + s->mass[at] * c[3 * at + j] / mreac[k];
+        }
+    const int terms = (rx->n_reac * rx->n_reac - rx->n_reac) / 2;
+    for (int i = 0; i < rx->n_reac; ++i)
+        for (int j = i + 1; j < rx->n_reac; ++j) {
+            double R[3];
+            for (int d = 0; d < 3; ++d) R[d] = com[j][d] - com[i][d];
+            const double rinv = 1.0 / sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+            const double r3 = rinv * rinv * rinv;
+            double blk[3][3];
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 3; ++e)
+                    blk[d][e] = d == e ? -(R[(d + 1) % 3] * R[(d + 1) % 3] + R[(d + 2) % 3] * R[(d + 2) % 3]) * r3
+                                       : R[d < e ? d : e] * R[d < e ? e : d] * r3;
+            const int grp[2] = { i, j };
+            for (int g1 = 0; g1 < 2; ++g1)
+                for (int k = 0; k < rx->reac_natoms[grp[g1]]; ++k) {
+                    const int at1 = rx->reac_atoms[grp[g1]][k] - 1;
+                    for (int g2 = 0; g2 < 2; ++g2)
+                        for (int l = 0; l < rx->reac_natoms[grp[g2]]; ++l) {
+                            const int at2 = rx->reac_atoms[grp[g2]][l] - 1;
+                            const double mf = s->mass[at1] / mreac[grp[g1]] * s->mass[at2] / mreac[grp[g2]];
+                            const double sg = g1 == g2 ? 1.0 : -1.0;
+                            for (int d = 0; d < 3; ++d)
+                                for (int e = 0; e < 3; ++e)
+                                    h0[3 * at1 + d][3 * at2 + e] += sg * (blk[d][e] * mf / (double)(float)terms);
+                        }
+                }
+        }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            double v;
+            if (mode == 1)
+                v = ((s0 * h1[a][b] + ds0[b] * ds1[a] - ds1[b] * ds0[a] - s1 * h0[a][b]) * (s0 - s1) -
+                     2.0 * (s0 * ds1[a] - s1 * ds0[a]) * (ds0[b] - ds1[b])) /
+                    ((s0 - s1) * (s0 - s1) * (s0 - s1));
+            else
+                v = rx->xi_ideal * h1[a][b] + (1 - rx->xi_ideal) * h0[a][b];
+            h[(size_t)a * n + b] = v;
+        }
+    return xi;
+}
+
+/* umbrella mode 0 (src/umbrella.f90:109-173) at the centroid c: xi (mode 1)
+ * and its gradient into xi_real and dxi; the bias gradient
+ * k_force (xi - xi_ideal) dxi, then the metric term dhams, added to every
+ * bead of g (bead-major, as the reference adds them).  Returns the bias
+ * energy 0.5 k_force (xi - xi_ideal)^2 - log(fs) / beta. */
+double oracle_umbrella(const oracle_sys *s, const oracle_rx *rx, const double *c, double beta, double k_force,
+                       double *dxi, double *g, double *xi_real)
+{
+    const int na = s->natoms, nb = s->nbeads, n = 3 * na;
+    static double h[9 * ORACLE_MAX_ATOMS * ORACLE_MAX_ATOMS];
+    const double xi = oracle_calc_xi_hess(s, rx, c, dxi, h, 1);
+    if (xi_real) *xi_real = xi;
+    const double delta = xi - rx->xi_ideal;
+    double v_add = 0.5 * k_force * delta * delta;
+    for (int k = 0; k < nb; ++k)
+        for (int j = 0; j < na; ++j)
+            for (int i = 0; i < 3; ++i) g[IDX(k, j, i, na)] = g[IDX(k, j, i, na)] + k_force * delta * dxi[3 * j + i];
+    const double pi = 3.1415926535897932384;
+    double fs2 = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < na; ++j) fs2 = fs2 + dxi[3 * j + i] * dxi[3 * j + i] / s->mass[j];
+    const double coeff1 = 2.0 * pi * beta;
+    fs2 = fs2 / coeff1;
+    const double fs = sqrt(fs2);
+    const double log_fs = log(fs);
+    const double coeff2 = -1.0 / beta;
+    v_add = v_add + coeff2 * log_fs;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < na; ++j) {
+            double dhams = 0.0;
+            for (int i2 = 0; i2 < 3; ++i2)
+                for (int j2 = 0; j2 < na; ++j2)
+                    dhams = dhams + h[(size_t)(3 * j2 + i2) * n + 3 * j + i] * dxi[3 * j2 + i2] / s->mass[j2];
+            dhams = dhams * coeff2 / (coeff1 * fs2);
+            for (int k = 0; k < nb; ++k) g[IDX(k, j, i, na)] = g[IDX(k, j, i, na)] + dhams;
+        }
+    return v_add;
+}
+
+/* mdinit with bias_mode 2 (src/mdinit.f90:79-98), after the caller's force
+ * evaluation g: umbrella mode 0 at the centroid of q adds its bias gradient
+ * to g and leaves the mode-1 gradient of xi in dxi, the constraint direction
+ * of the parent's first step.  The momenta draw that follows in mdinit is the
+ * caller's (oracle_andersen). */
+void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *g, double *dxi,
+                         double beta, double k_force)
 {
     double c[3 * ORACLE_MAX_ATOMS];
     oracle_centroid(s, q, c);
-    (void)oracle_calc_xi(s, rx, c, dxi, 1);
+    (void)oracle_umbrella(s, rx, c, beta, k_force, dxi, g, NULL);
 }
 
 /* One step of a recrossing parent (verlet with constrain = 1,

@@ -137,7 +137,13 @@ def _setup_rpmd():
     o.oracle_constrain_q.restype = ctypes.c_int
     o.oracle_constrain_q.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp]
     o.oracle_constrain_p.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp]
-    o.oracle_parent_begin.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp]
+    o.oracle_parent_begin.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp,
+                                      ctypes.c_double, ctypes.c_double]
+    o.oracle_calc_xi_hess.restype = ctypes.c_double
+    o.oracle_calc_xi_hess.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, ctypes.c_int]
+    o.oracle_umbrella.restype = ctypes.c_double
+    o.oracle_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, ctypes.c_double,
+                                  ctypes.c_double, _dp, _dp, _dp]
     o.oracle_verlet_parent.restype = ctypes.c_int
     o.oracle_verlet_parent.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp, _dp,
                                        ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_uint64,
@@ -192,10 +198,13 @@ class OracleTraj:
                                     _ptr(self.g), _ptr(self.v), self.beta, self.ensemble, int(self.remove_com),
                                     self.istep, self.andersen_step, self.seed, self.traj)
 
-    def parent_begin(self):
-        """Recrossing parent (ensemble 3): the constraint direction of mdinit."""
+    def parent_begin(self, k_force=0.0):
+        """Recrossing parent (ensemble 3): mdinit with bias_mode 2 -- fresh
+        forces plus the umbrella bias gradient, and the constraint direction."""
+        self.v, self.g = forces(self.sys, self.q)
         self.dxi = np.zeros((self.q.shape[1], 3))
-        rpmd().oracle_parent_begin(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.dxi))
+        rpmd().oracle_parent_begin(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.g),
+                                   _ptr(self.dxi), self.beta, k_force)
 
 
 def calc_xi(sys, rx, c, mode=2):
@@ -203,6 +212,25 @@ def calc_xi(sys, rx, c, mode=2):
     d = np.zeros_like(c)
     xi = rpmd().oracle_calc_xi(ctypes.byref(sys), ctypes.byref(rx), _ptr(c), _ptr(d), mode)
     return xi, d
+
+
+def calc_xi_hess(sys, rx, c, mode=1):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    d = np.zeros_like(c)
+    h = np.zeros((c.size, c.size))
+    xi = rpmd().oracle_calc_xi_hess(ctypes.byref(sys), ctypes.byref(rx), _ptr(c), _ptr(d), _ptr(h), mode)
+    return xi, d, h
+
+
+def umbrella(sys, rx, c, beta, k_force):
+    """umbrella mode 0 at centroid c: (bias energy, xi, dxi, bias gradient of each of sys.nbeads beads)."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    d = np.zeros_like(c)
+    g = np.zeros((sys.nbeads,) + c.shape)
+    xi = ctypes.c_double()
+    v = rpmd().oracle_umbrella(ctypes.byref(sys), ctypes.byref(rx), _ptr(c), beta, k_force, _ptr(d), _ptr(g),
+                               ctypes.byref(xi))
+    return v, xi.value, d, g
 
 
 def andersen_normal(seed, traj, draw, coord, bead):
