@@ -10,6 +10,8 @@
 // ch4h::egrad_quad).  One launch advances its trajectories by several steps
 // with the state held in registers; the normal-mode step goes through LDS.
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -71,12 +73,15 @@ __device__ double calc_xi_value(const Rx &rx, const double (*wcom)[NA], const do
     return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
 }
 
-// xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213),
-// c: centroid (3*natoms, atom-major).  Optionally the gradient dxi (3*natoms).
-template <int NA>
-__device__ double calc_xi(const Rx &rx, const double *mass, const double *c, double *dxi)
+// The two dividing surfaces of calc_xi BIMOLEC (src/calc_xi.f90:63-282) at
+// c (3*natoms, atom-major): s1 from the forming and breaking bonds, s0 from
+// the distances between the reactant centres of mass, and with GRAD their
+// gradients.  Both modes of xi combine these.
+template <int NA, bool GRAD>
+__device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, double &s0, double &s1, double *ds0,
+                            double *ds1)
 {
-    double s1 = 0.0;
+    s1 = 0.0;
     double rb[CAM_MAX_BONDS], rf[CAM_MAX_BONDS];
     for (int i = 0; i < rx.n_break; ++i) {
         const int a = rx.brk[i][0], b = rx.brk[i][1];
@@ -98,7 +103,7 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
             for (int j = 0; j < 3; ++j) com[k][j] = com[k][j] + mass[at] * c[3 * at + j] / rx.reac_mass[k];
         }
     }
-    double s0 = 0.0;
+    s0 = 0.0;
     const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
     for (int i = 0; i < rx.n_reac; ++i)
         for (int j = i + 1; j < rx.n_reac; ++j) {
@@ -106,9 +111,7 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
             s0 = s0 + (rx.r_inf - sqrt(dx * dx + dy * dy + dz * dz));
         }
     s0 = s0 / (double)terms;
-    const double xi = rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
-    if (dxi) {
-        double ds1[3 * CAM_MAX_ATOMS], ds0[3 * CAM_MAX_ATOMS];
+    if constexpr (GRAD) {
         for (int i = 0; i < 3 * NA; ++i) ds1[i] = ds0[i] = 0.0;
         for (int i = 0; i < rx.n_form; ++i) {
             const int a = rx.form[i][0], b = rx.form[i][1];
@@ -144,9 +147,131 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
                         ds0[3 * at + d] = ds0[3 * at + d] - red[d] * rinv * mass[at] / rx.reac_mass[j] / (double)terms;
                 }
             }
-        for (int i = 0; i < 3 * NA; ++i) dxi[i] = rx.xi_ideal * ds1[i] + (1.0 - rx.xi_ideal) * ds0[i];
     }
-    return xi;
+}
+
+// xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:209-211,
+// :280-281), c: centroid (3*natoms, atom-major).  Optionally the gradient
+// dxi (3*natoms).
+template <int NA>
+__device__ double calc_xi(const Rx &rx, const double *mass, const double *c, double *dxi)
+{
+    double s0, s1;
+    if (!dxi) {
+        xi_surfaces<NA, false>(rx, mass, c, s0, s1, nullptr, nullptr);
+        return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
+    }
+    double ds0[3 * NA], ds1[3 * NA];
+    xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
+    for (int i = 0; i < 3 * NA; ++i) dxi[i] = rx.xi_ideal * ds1[i] + (1.0 - rx.xi_ideal) * ds0[i];
+    return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
+}
+
+// Product of the Hessians of s1 and s0 with a vector v (3*natoms): the
+// reference assembles both matrices (src/calc_xi.f90:283-480); their blocks
+// are those of a distance, (|R|^2 I - R R^T) / |R|^3, so the product needs
+// only the differences of v across each bond and across each pair of
+// fragment centres of mass (v weighted as the centre of mass weights
+// positions).
+template <int NA>
+__device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const double *c, const double *v, double *h0v,
+                                    double *h1v)
+{
+    for (int i = 0; i < 3 * NA; ++i) h0v[i] = h1v[i] = 0.0;
+    auto dist_block = [](const double *R, const double *u, double sg, double *w) {
+        const double rr = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+        const double rinv = 1.0 / sqrt(rr);
+        const double r3 = rinv * rinv * rinv;
+        const double ru = R[0] * u[0] + R[1] * u[1] + R[2] * u[2];
+        for (int d = 0; d < 3; ++d) w[d] = sg * (rr * u[d] - R[d] * ru) * r3;
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nbond = pass == 0 ? rx.n_form : rx.n_break;
+        const double sg = (pass == 0 ? -1.0 : 1.0) / (double)nbond;   // forming bonds enter s1 with -|R|
+        for (int i = 0; i < nbond; ++i) {
+            const int a = pass == 0 ? rx.form[i][0] : rx.brk[i][0], b = pass == 0 ? rx.form[i][1] : rx.brk[i][1];
+            double R[3], u[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                R[d] = c[3 * a + d] - c[3 * b + d];
+                u[d] = v[3 * a + d] - v[3 * b + d];
+            }
+            dist_block(R, u, sg, w);
+            for (int d = 0; d < 3; ++d) {
+                h1v[3 * a + d] += w[d];
+                h1v[3 * b + d] -= w[d];
+            }
+        }
+    }
+    double com[CAM_MAX_REAC][3], vcom[CAM_MAX_REAC][3];
+    for (int k = 0; k < rx.n_reac; ++k)
+        for (int d = 0; d < 3; ++d) {
+            com[k][d] = vcom[k][d] = 0.0;
+            for (int i = 0; i < rx.reac_n[k]; ++i) {
+                const int at = rx.reac_at[k][i];
+                com[k][d] += mass[at] * c[3 * at + d] / rx.reac_mass[k];
+                vcom[k][d] += mass[at] * v[3 * at + d] / rx.reac_mass[k];
+            }
+        }
+    const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
+    for (int i = 0; i < rx.n_reac; ++i)
+        for (int j = i + 1; j < rx.n_reac; ++j) {
+            double R[3], u[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                R[d] = com[j][d] - com[i][d];
+                u[d] = vcom[i][d] - vcom[j][d];
+            }
+            dist_block(R, u, -1.0 / (double)terms, w);                 // s0 = r_inf - |R|
+            for (int k = 0; k < rx.reac_n[i]; ++k) {
+                const int at = rx.reac_at[i][k];
+                for (int d = 0; d < 3; ++d) h0v[3 * at + d] += mass[at] / rx.reac_mass[i] * w[d];
+            }
+            for (int k = 0; k < rx.reac_n[j]; ++k) {
+                const int at = rx.reac_at[j][k];
+                for (int d = 0; d < 3; ++d) h0v[3 * at + d] -= mass[at] / rx.reac_mass[j] * w[d];
+            }
+        }
+}
+
+// umbrella mode 0 (src/umbrella.f90:109-173) at centroid c: xi of mode 1,
+// s0 / (s0 - s1) (src/calc_xi.f90:204-206, :275-278), and its gradient into
+// dxi; the bias gradient every bead receives, in the reference's two parts:
+// gk = k_force (xi - xi_ideal) dxi and the metric term
+// gh = -(1/beta) H M^-1 dxi / (dxi^T M^-1 dxi), H the Hessian of xi, which
+// the reference contracts from the full matrix (:148-170) and which is
+// formed here as a Hessian-vector product.  Returns the bias energy
+// 0.5 k_force (xi - xi_ideal)^2 - log(fs) / beta.
+template <int NA>
+__device__ double umbrella_bias(const Rx &rx, const double *mass, const double *c, double beta, double k_force,
+                                double *dxi, double *gk, double *gh)
+{
+    constexpr int N = 3 * NA;
+    double s0, s1, ds0[N], ds1[N];
+    xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
+    const double D = s0 - s1;
+    const double xi = s0 / D;
+    for (int i = 0; i < N; ++i) dxi[i] = (s0 * ds1[i] - s1 * ds0[i]) / (D * D);
+    const double delta = xi - rx.xi_ideal;
+    for (int i = 0; i < N; ++i) gk[i] = k_force * delta * dxi[i];
+    double v[N], fs2 = 0.0;
+    for (int i = 0; i < N; ++i) v[i] = dxi[i] / mass[i / 3];
+    for (int d = 0; d < 3; ++d)
+        for (int at = 0; at < NA; ++at) fs2 = fs2 + dxi[3 * at + d] * dxi[3 * at + d] / mass[at];
+    const double coeff1 = 2.0 * 3.1415926535897932384 * beta, coeff2 = -1.0 / beta;
+    fs2 = fs2 / coeff1;
+    const double vadd = 0.5 * k_force * delta * delta + coeff2 * log(sqrt(fs2));
+    double h0v[N], h1v[N];
+    xi_surface_hess_vec<NA>(rx, mass, c, v, h0v, h1v);
+    double ds0v = 0.0, ds1v = 0.0;
+    for (int i = 0; i < N; ++i) {
+        ds0v += ds0[i] * v[i];
+        ds1v += ds1[i] * v[i];
+    }
+    const double cross = s0 * ds1v - s1 * ds0v;
+    const double scale = coeff2 / (coeff1 * fs2);
+    for (int i = 0; i < N; ++i)
+        gh[i] = ((s0 * h1v[i] + ds0[i] * ds1v - ds1[i] * ds0v - s1 * h0v[i]) * D - 2.0 * cross * (ds0[i] - ds1[i])) /
+                (D * D * D) * scale;
+    return vadd;
 }
 
 // ---------------------------------------------------------------------------
@@ -839,6 +964,175 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
         for (int k = 0; k < NB; ++k) p[(size_t)i * NB + k] = p[(size_t)i * NB + k] + lam * dxi_act[i];
 }
 
+// ---------------------------------------------------------------------------
+// Recrossing parent (verlet with constrain = 1).  One wave per trajectory:
+// its lanes cover the beads of each coordinate, and the scalar work of the
+// constraint (xi, Newton's iteration) is repeated by every lane on the same
+// values, so no lane waits for a broadcast.  pdxi[traj][3*natoms] carries the
+// constraint direction from one step to the next; pfail holds per trajectory
+// kPfConstraint (a constraint did not converge since the start), kPfCheck
+// (the per-step trajectory check failed) and kPfThisStep (this step's
+// constraint did not converge).
+// ---------------------------------------------------------------------------
+constexpr int kParentWaves = 4;              // trajectories per block
+constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
+
+// Centroid of trajectory t in every lane: lane i < 3*natoms sums coordinate
+// i over the beads in bead order (get_centroid, src/get_centroid.f90:66-85).
+template <int NA>
+__device__ void wave_centroid(const double *q, int nb, int lane, double *c)
+{
+    double own = 0.0;
+    if (lane < 3 * NA) {
+        for (int k = 0; k < nb; ++k) own = own + q[(size_t)lane * nb + k];
+        own = own / nb;
+    }
+#pragma unroll
+    for (int i = 0; i < 3 * NA; ++i) c[i] = __shfl(own, i, 64);
+}
+
+// mdinit with bias_mode 2 (src/mdinit.f90:79-98), after the forces: the
+// umbrella mode-0 bias at the centroid is added to every bead's gradient,
+// first its force-constant part, then its metric part, and the gradient of
+// xi (mode 1) becomes the constraint direction of the first step.
+template <int NA>
+__global__ __launch_bounds__(64 * kParentWaves) void parent_begin_kernel(TrajArgs a, double *g, double k_force,
+                                                                         double *pdxi, int *pfail)
+{
+    constexpr int NC = 3 * NA;
+    const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads;
+    double c[NC], dxi[NC], gk[NC], gh[NC];
+    wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
+    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], k_force, dxi, gk, gh);
+    double *gt = g + (size_t)t * NC * NB;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        for (int k = lane; k < NB; k += 64) gt[(size_t)i * NB + k] = (gt[(size_t)i * NB + k] + gk[i]) + gh[i];
+        if (lane == i) pdxi[(size_t)t * NC + i] = dxi[i];
+    }
+    if (lane == 0) pfail[t] = 0;
+}
+
+// Between the free ring-polymer step and the forces (src/verlet.f90:649,
+// :744-756): constrain_q (src/constrain_q.f90:35-113) on the centroid c of
+// the propagated beads with the previous step's direction, then the next
+// direction, the gradient of xi (mode 2) at the same c (umbrella mode 1,
+// src/verlet.f90:1048, src/umbrella.f90:117-122).  A constraint that does not
+// converge in 200 iterations leaves the beads as they are and is counted,
+// where the reference prints a warning and gives the step an energy penalty.
+template <int NA>
+__global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(TrajArgs a, double *pdxi, int *pfail)
+{
+    constexpr int NC = 3 * NA;
+    const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads;
+    double *q = a.q + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
+    double c[NC], dprev[NC], qt[NC], dxn[NC];
+    wave_centroid<NA>(q, NB, lane, c);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) dprev[i] = pdxi[(size_t)t * NC + i];
+    double mult = 0.0, coeff = 0.0;
+    bool good = true;
+    for (int iter = 1; iter <= 200; ++iter) {
+        coeff = mult * a.dt * a.dt / NB;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dprev[i] / a.mass[i / 3];
+        const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
+        double dsigma = 0.0;
+        for (int d = 0; d < 3; ++d)
+            for (int at = 0; at < NA; ++at)
+                dsigma = dsigma + dxn[3 * at + d] * a.dt * a.dt * dprev[3 * at + d] / (a.mass[at] * NB);
+        const double dx = sigma / dsigma;
+        mult = mult - dx;
+        // single-precision literals 1.0E-8 and 1.0E-10 of the reference
+        if (fabs(dx) < (double)1.0e-8f || fabs(sigma) < (double)1.0e-10f) break;
+        if (iter == 200) good = false;
+    }
+    if (good) {
+        // the shift uses the coefficient of the last iteration, computed
+        // before its multiplier update, as the reference does
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const double dq = coeff / a.mass[i / 3] * dprev[i], dp = mult * a.dt / NB * dprev[i];
+            for (int k = lane; k < NB; k += 64) {
+                q[(size_t)i * NB + k] = q[(size_t)i * NB + k] + dq;
+                p[(size_t)i * NB + k] = p[(size_t)i * NB + k] + dp;
+            }
+        }
+    }
+    if (lane == 0) pfail[t] = good ? (pfail[t] & ~kPfThisStep) : (pfail[t] | kPfConstraint | kPfThisStep);
+    double dnew[NC];
+    (void)calc_xi<NA>(a.rx, a.mass, c, dnew);
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+        if (lane == i) pdxi[(size_t)t * NC + i] = dnew[i];
+}
+
+// The end of a parent step (src/verlet.f90:1060-1083, :1159-1165): the
+// second half kick, constrain_p (src/constrain_p.f90:36-76) with this step's
+// direction, and the Andersen draw when the step number istep (counted from
+// the engine's creation, first step 1) is a multiple of andersen_step.
+// Then rpmd_check as the recrossing driver calls it for a parent
+// (src/rpmd_check.f90:74-98, traj_type 1 at src/recross_serial.f90:125-135
+// and :290-299), from step check_after + 1 after the start on (never if
+// check_after < 0): a position that is NaN or above HUGE, or an energy (the
+// sum over the beads, plus verlet's 1e5 penalty after a failed constraint,
+// src/verlet.f90:749-755) that is NaN, above HUGE or above epot_max, sets
+// kPfCheck.  step_base[1] counts the steps since the start.
+template <int NA>
+__global__ __launch_bounds__(64 * kParentWaves) void parent_end_kernel(TrajArgs a, const double *pdxi, int *pfail,
+                                                                       const uint64_t *step_base, int step_off,
+                                                                       int andersen_step, uint64_t seed,
+                                                                       double epot_max, int check_after)
+{
+    constexpr int NC = 3 * NA;
+    const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads;
+    double *p = a.p + (size_t)t * NC * NB;
+    const double *g = a.g + (size_t)t * NC * NB;
+    double dxi[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) dxi[i] = pdxi[(size_t)t * NC + i];
+    double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+        for (int k = lane; k < NB; k += 64) {
+            const double pn = p[(size_t)i * NB + k] - 0.5 * a.dt * g[(size_t)i * NB + k];
+            p[(size_t)i * NB + k] = pn;
+            c1 = c1 + dxi[i] * pn / a.mass[i / 3];
+        }
+    for (int off = 32; off > 0; off >>= 1) c1 += __shfl_xor(c1, off, 64);
+    for (int d = 0; d < 3; ++d)
+        for (int at = 0; at < NA; ++at) c2 = c2 + dxi[3 * at + d] * dxi[3 * at + d] / a.mass[at];
+    const double lam = -c1 / c2 / NB;
+    const uint64_t istep = *step_base + (uint64_t)step_off + 1;
+    const bool draw = andersen_step > 0 && istep % (uint64_t)andersen_step == 0;
+    const double beta = a.beta[t];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const double sd = sqrt(a.mass[i / 3] / (beta / NB));
+        for (int k = lane; k < NB; k += 64)
+            p[(size_t)i * NB + k] = draw ? andersen_normal(seed, (uint32_t)t, istep, i, k) * sd
+                                         : p[(size_t)i * NB + k] + lam * dxi[i];
+    }
+    if (check_after >= 0 && step_base[1] + (uint64_t)step_off + 1 > (uint64_t)check_after) {
+        const double *q = a.q + (size_t)t * NC * NB;
+        bool bad = false;
+        for (int i = lane; i < NC * NB; i += 64) {
+            const double v = q[i];
+            bad = bad || v != v || v > DBL_MAX;
+        }
+        double ep = (pfail[t] & kPfThisStep) ? 1.0e5 : 0.0;
+        for (int k = 0; k < NB; ++k) ep = ep + a.vbead[(size_t)t * NB + k];
+        bad = bad || ep != ep || ep > DBL_MAX || ep > epot_max;
+        if (__any(bad) && lane == 0) pfail[t] = pfail[t] | kPfCheck;
+    }
+}
+
 // Flux weight vs/fs of each child and the kappa denominator contribution
 // (src/recross_serial.f90:189-233).
 template <int NA>
@@ -922,8 +1216,8 @@ __global__ void draw_kernel(double *p, const double *beta, int ntraj, int natoms
 }
 
 // Observables per trajectory.
-__global__ void observe_kernel(TrajArgs a, const uint8_t *, double *epot, double *ekin, double *espring,
-                               double *xi, int *flags, int child)
+__global__ void observe_kernel(TrajArgs a, const int *pfail, double *epot, double *ekin, double *espring,
+                               double *xi, int *flags, int with_xi)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntraj) return;
@@ -945,11 +1239,12 @@ __global__ void observe_kernel(TrajArgs a, const uint8_t *, double *epot, double
             }
         }
     }
+    if (pfail) fl |= (pfail[t] & 3) << 1;
     epot[t] = ep;
     ekin[t] = ek;
     espring[t] = es;
     flags[t] = fl;
-    if (child) {
+    if (with_xi) {
         double c[3 * CAM_MAX_ATOMS];
         centroid_of(a, t, c);
         xi[t] = a.natoms == 3 ? calc_xi<3>(a.rx, a.mass, c, nullptr) : calc_xi<6>(a.rx, a.mass, c, nullptr);
@@ -1007,6 +1302,11 @@ struct cam_engine {
     double *weight = nullptr, *den_part = nullptr, *den = nullptr, *num = nullptr;
     uint8_t *xi_flag = nullptr;
     double *cbuf = nullptr;           // CHILD: centroids of one fused launch
+    double *pdxi = nullptr;           // PARENT: constraint directions [traj][3*natoms]
+    int *pfail = nullptr;             // PARENT: non-converged constraints per trajectory
+    bool parent_ready = false;        // PARENT: begin_parent done since the last set_state
+    double pcheck_max = 0.0;          // PARENT: trajectory check (begin_parent)
+    int pcheck_after = -1;
     uint64_t *step_base = nullptr;    // [0] steps since creation, [1] steps since begin_children
     double *scratch_io = nullptr;     // staging for layout conversion
     int num_cap = 0;                  // doubles in num (child steps)
@@ -1151,7 +1451,10 @@ bool fold_available(const cam_config *cfg)
 bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
-    const int nb = cfg->nbeads, ens = cfg->ensemble;
+    const int nb = cfg->nbeads;
+    // a parent's normal-mode kernel is the NVE one: its constraint and the
+    // end of its step are kernels of their own
+    const int ens = cfg->ensemble == CAM_ENS_PARENT ? CAM_ENS_NVE : cfg->ensemble;
     const bool mfma = nmk == CAM_NMK_MFMA;
     bool ok;
     if (cfg->surface == CAM_SURFACE_H3)
@@ -1246,6 +1549,8 @@ StepArgs step_args(cam_engine *e)
     a.tfold = e->tfold;
     a.xi_flag = e->xi_flag;
     a.cbuf = e->cbuf;
+    a.pdxi = e->pdxi;
+    a.pfail = e->pfail;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
     a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
@@ -1264,6 +1569,39 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
 {
     const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
     hipError_t err = hipSuccess;
+    if (e->cfg.ensemble == CAM_ENS_PARENT) {
+        // per step: kick and free ring polymer; constraint; forces; second
+        // half kick, constrain_p and Andersen
+        TrajArgs ta = traj_args(e);
+        ta.q = a.q;
+        ta.p = a.p;
+        ta.g = a.g;
+        const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
+        const bool h3 = e->cfg.natoms == 3;
+        for (int j = 0; j < nsteps && err == hipSuccess; ++j) {
+            a.nsteps = 1;
+            a.step_off = 0;                // no previous step to finish inside the kernel
+            hipLaunchKernelGGL(kn.nm, dim3(grid), dim3(kBlock), 0, e->stream, a);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            if (h3) hipLaunchKernelGGL(parent_constrain_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail);
+            else hipLaunchKernelGGL(parent_constrain_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            hipLaunchKernelGGL(kn.fsplit, dim3(grid), dim3(kBlock), 0, e->stream, a.q, a.g, a.vbead, a.ntraj);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            const int as = e->cfg.andersen_step;
+            if (h3)
+                hipLaunchKernelGGL(parent_end_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi,
+                                   a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
+            else
+                hipLaunchKernelGGL(parent_end_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi,
+                                   a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
+            err = hipGetLastError();
+        }
+        return err;
+    }
     if (split) {
         for (int j = 0; j < nsteps && err == hipSuccess; ++j) {
             a.nsteps = 1;
@@ -1334,7 +1672,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
-    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_CHILD, "bad ensemble");
+    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_PARENT, "bad ensemble");
     CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_FOLDED, "bad nm_kernel");
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
                   "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
@@ -1350,12 +1688,15 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->andersen_step >= 0, "andersen_step must be >= 0");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->schedule != CAM_SCHED_FUSED,
+                  "ensemble PARENT runs on the split schedule (its constraint sits between propagation and forces)");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
-    CAM_CHECK_ARG(!(cfg->remove_com && cfg->ensemble == CAM_ENS_CHILD),
+    CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || cfg->ensemble == CAM_ENS_PARENT)),
                   "remove_com applies to dynamic.x trajectories (constrain <= 0) only");
     const cam_reaction &rr = cfg->reaction;
     Rx rx{};
-    if (cfg->ensemble == CAM_ENS_CHILD) {
+    if (cfg->ensemble == CAM_ENS_CHILD || cfg->ensemble == CAM_ENS_PARENT) {
         CAM_CHECK_ARG(rr.n_form >= 0 && rr.n_form <= CAM_MAX_BONDS && rr.n_break >= 0 && rr.n_break <= CAM_MAX_BONDS &&
                           rr.n_form + rr.n_break > 0,
                       "reaction needs 1..%d forming and breaking bonds", CAM_MAX_BONDS);
@@ -1403,7 +1744,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     else e->nmk = fold_available(cfg) ? CAM_NMK_FOLDED : CAM_NMK_VALU;
     e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
                      e->nmk != CAM_NMK_FOLDED ? 2 : 1;
-    e->split = cfg->schedule == CAM_SCHED_SPLIT;
+    e->split = cfg->schedule == CAM_SCHED_SPLIT || cfg->ensemble == CAM_ENS_PARENT;
     const bool ok = select_kernels(e.get(), e->wpe, e->nmk, e->kern);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
@@ -1421,6 +1762,12 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
         // centroids of one fused launch: the longest schedule the autotuner tries, or the configured one
         const size_t steps = (size_t)std::max(e->spl, 100);
         CAM_HIP(hipMalloc(&e->cbuf, steps * cfg->ntraj * 3 * na * sizeof(double)));
+    }
+    if (cfg->ensemble == CAM_ENS_PARENT) {
+        CAM_HIP(hipMalloc(&e->pdxi, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
+        CAM_HIP(hipMalloc(&e->pfail, (size_t)cfg->ntraj * sizeof(int)));
+        CAM_HIP(hipMemset(e->pdxi, 0, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
+        CAM_HIP(hipMemset(e->pfail, 0, (size_t)cfg->ntraj * sizeof(int)));
     }
     CAM_HIP(hipMalloc(&e->weight, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den_part, (size_t)cfg->ntraj * sizeof(double)));
@@ -1453,9 +1800,10 @@ int cam_engine_destroy(cam_engine *e)
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->tfold, e->weight, e->den_part,
-                       e->den, e->num, e->scratch_io, e->cbuf };
+                       e->den, e->num, e->scratch_io, e->cbuf, e->pdxi };
     for (double *b : bufs)
         if (b) (void)hipFree(b);
+    if (e->pfail) (void)hipFree(e->pfail);
     if (e->xi_flag) (void)hipFree(e->xi_flag);
     if (e->step_base) (void)hipFree(e->step_base);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1489,6 +1837,7 @@ int cam_engine_set_state(cam_engine *e, const double *q, const double *p, const 
     if (rc != CAM_OK) return rc;
     CAM_HIP(hipStreamSynchronize(e->stream));
     e->have_state = true;
+    e->parent_ready = false;
     return CAM_OK;
 }
 
@@ -1597,6 +1946,35 @@ int cam_engine_begin_children(cam_engine *e)
     return CAM_OK;
 }
 
+int cam_engine_begin_parent(cam_engine *e, double k_force, double epot_max, int check_after)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_PARENT, "begin_parent needs ensemble PARENT");
+    CAM_CHECK_ARG(std::isfinite(k_force), "k_force must be finite");
+    CAM_CHECK_ARG(!std::isnan(epot_max), "epot_max must not be NaN");
+    clear_error();
+    e->pcheck_max = epot_max;
+    e->pcheck_after = check_after;
+    drop_graphs(e);                   // the check parameters are arguments of the captured launches
+    int rc = run_forces(e);
+    if (rc != CAM_OK) return rc;
+    const TrajArgs ta = traj_args(e);
+    const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
+    if (e->cfg.natoms == 3)
+        hipLaunchKernelGGL(parent_begin_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
+                           k_force, e->pdxi, e->pfail);
+    else
+        hipLaunchKernelGGL(parent_begin_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
+                           k_force, e->pdxi, e->pfail);
+    CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);   // steps since the start
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    e->parent_ready = true;
+    return CAM_OK;
+}
+
 static int get_graph(cam_engine *e, int nsteps, hipGraphExec_t *out)
 {
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
@@ -1639,6 +2017,8 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_CHECK_ARG(e, "null engine");
     CAM_CHECK_ARG(nsteps >= 0, "nsteps must be >= 0");
     CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_PARENT || e->parent_ready,
+                  "a PARENT engine needs cam_engine_begin_parent after cam_engine_set_state");
     clear_error();
     if (nsteps == 0) return CAM_OK;
     hipGraphExec_t g;
@@ -1690,8 +2070,9 @@ int cam_engine_observables(cam_engine *e, double *epot, double *ekin, double *es
     hipError_t err = hipMalloc(&fl, (size_t)nt * sizeof(int));
     if (err == hipSuccess) {
         const TrajArgs ta = traj_args(e);
-        hipLaunchKernelGGL(observe_kernel, dim3((nt + 127) / 128), dim3(128), 0, e->stream, ta, (const uint8_t *)nullptr,
-                           buf, buf + nt, buf + 2 * nt, buf + 3 * nt, fl, e->cfg.ensemble == CAM_ENS_CHILD ? 1 : 0);
+        const int with_xi = e->cfg.ensemble == CAM_ENS_CHILD || e->cfg.ensemble == CAM_ENS_PARENT;
+        hipLaunchKernelGGL(observe_kernel, dim3((nt + 127) / 128), dim3(128), 0, e->stream, ta, e->pfail,
+                           buf, buf + nt, buf + 2 * nt, buf + 3 * nt, fl, with_xi);
         err = hipGetLastError();
     }
     if (err == hipSuccess) err = hipMemcpyAsync(epot, buf, nt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
@@ -1721,6 +2102,8 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     CAM_CHECK_ARG(e, "null engine");
     CAM_CHECK_ARG(e->have_state, "no state uploaded");
     CAM_CHECK_ARG(trial_steps >= 1 && trial_steps <= 100000, "trial_steps must be in 1..100000");
+    CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_PARENT || e->parent_ready,
+                  "a PARENT engine needs cam_engine_begin_parent before autotune");
     clear_error();
     const cam_config &c = e->cfg;
     const int nb = c.nbeads;
@@ -1747,7 +2130,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     int rc = CAM_OK;
     struct Trial { int nmk, wpe, spl, split; double us; };
     std::vector<Trial> cand, trials;
-    const bool try_fused = c.schedule != CAM_SCHED_SPLIT, try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
+    const bool parent = c.ensemble == CAM_ENS_PARENT;
+    const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent;
+    const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
     for (int nmk : nmks) {
         if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
             for (int spl : spls) cand.push_back({ nmk, 1, spl, 0, 0.0 });
@@ -1762,7 +2147,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         set_error("HIP error in autotune: %s", hipGetErrorString(err));
         rc = CAM_ERR_DEVICE;
     };
-    hipError_t err = hipMalloc(&buf, (3 * nstate + (size_t)c.ntraj * nb) * sizeof(double));
+    // PARENT: a copy of the constraint directions and a scratch failure count
+    const size_t npd = parent ? (size_t)c.ntraj * 3 * c.natoms : 0;
+    hipError_t err = hipMalloc(&buf, (3 * nstate + (size_t)c.ntraj * nb + npd + (size_t)c.ntraj) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&flags, (size_t)trial_steps * c.ntraj);
     if (err == hipSuccess) err = hipMalloc(&ctr, 2 * sizeof(uint64_t));
     if (err == hipSuccess) err = hipEventCreate(&t0);
@@ -1779,7 +2166,13 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         a.vbead = buf + 3 * nstate;
         a.xi_flag = flags;
         a.step_base = ctr;
+        if (parent) {
+            a.pdxi = buf + 3 * nstate + (size_t)c.ntraj * nb;
+            a.pfail = reinterpret_cast<int *>(a.pdxi + npd);
+        }
         err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
+        if (err == hipSuccess && parent)
+            err = hipMemcpyAsync(a.pdxi, e->pdxi, npd * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.q, e->q, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);

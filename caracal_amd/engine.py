@@ -154,6 +154,15 @@ class RPMDEngine:
     def begin_children(self):
         check("cam_engine_begin_children", lib().cam_engine_begin_children(self._h))
 
+    def begin_parent(self, k_force=0.0, epot_max=float("inf"), check_after=-1):
+        """Start recrossing parents (ensemble PARENT) as mdinit with bias_mode 2:
+        fresh forces plus the umbrella bias of the centroid, and the first
+        constraint direction.  Needed after every set_state.  Arms the
+        per-step trajectory check from step check_after + 1 on (off if < 0);
+        its result is bit 2 of observables()["flags"]."""
+        check("cam_engine_begin_parent",
+              lib().cam_engine_begin_parent(self._h, float(k_force), float(epot_max), int(check_after)))
+
     def run(self, nsteps, sync=True):
         check("cam_engine_run", lib().cam_engine_run(self._h, nsteps))
         if sync:

@@ -18,7 +18,8 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
-   integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2
+   integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, &
+                                             CAM_ENS_PARENT = 3
    integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, &
                                             CAM_NMK_FOLDED = 3
    integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
@@ -124,6 +125,14 @@ module caracal_amd_mod
       function cam_engine_begin_children(eng) bind(C, name="cam_engine_begin_children") result(rc)
          import :: c_int, c_ptr
          type(c_ptr), value :: eng
+         integer(c_int) :: rc
+      end function
+      function cam_engine_begin_parent(eng, k_force, epot_max, check_after) &
+            bind(C, name="cam_engine_begin_parent") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), value :: k_force, epot_max
+         integer(c_int), value :: check_after
          integer(c_int) :: rc
       end function
       function cam_engine_run(eng, nsteps) bind(C, name="cam_engine_run") result(rc)
@@ -235,7 +244,7 @@ module caracal_amd_mod
    public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
-   public :: cam_engine_begin_children, cam_engine_run, cam_engine_sync, cam_engine_kappa
+   public :: cam_engine_begin_children, cam_engine_begin_parent, cam_engine_run, cam_engine_sync, cam_engine_kappa
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode

@@ -79,8 +79,15 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
  *              all bead momenta redrawn every andersen_step steps;
  *  CHILD    -- recrossing child trajectory (verlet constrain = 2): the
  *              reaction coordinate of the centroid is evaluated each step and
- *              the recrossing-factor numerator is accumulated. */
-enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2 };
+ *              the recrossing-factor numerator is accumulated;
+ *  PARENT   -- recrossing parent trajectory (verlet constrain = 1,
+ *              src/recross_serial.f90:121-133, :284-299): the centroid is held
+ *              on the dividing surface (constrain_q before the forces,
+ *              constrain_p after the second half kick) and, if
+ *              andersen_step > 0, the momenta are redrawn every andersen_step
+ *              steps.  Runs on the split schedule; start it with
+ *              cam_engine_begin_parent. */
+enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
  * nbeads a multiple of 16).  FOLDED: the reference transform (rfft/irfft, the
@@ -131,7 +138,7 @@ typedef struct {
     int32_t ensemble;         /* CAM_ENS_* */
     int32_t remove_com;       /* 1: remove centre-of-mass translation and rotation after every
                                  step as the reference's transrot does (src/transrot.f90) */
-    int32_t andersen_step;    /* ANDERSEN: resample every andersen_step steps */
+    int32_t andersen_step;    /* ANDERSEN, PARENT: resample every andersen_step steps (PARENT: 0 = never) */
     int32_t steps_per_launch; /* steps fused into one kernel launch; 0 = pick automatically */
     int32_t nm_kernel;        /* CAM_NMK_* */
     int32_t waves_per_eu;     /* register budget of the step kernel: 1 = up to 512 registers per lane,
@@ -141,7 +148,7 @@ typedef struct {
     double dt;                /* time step, atomic time units */
     double mass[CAM_MAX_ATOMS];   /* electron masses */
     uint64_t seed;            /* counter-based RNG key for Andersen draws */
-    cam_reaction reaction;    /* used by CAM_ENS_CHILD and cam_engine_constrain */
+    cam_reaction reaction;    /* used by CAM_ENS_CHILD, CAM_ENS_PARENT and cam_engine_constrain */
 } cam_config;
 
 typedef struct cam_engine cam_engine;
@@ -182,9 +189,29 @@ int cam_engine_constrain(cam_engine *eng, int project_p, int *status);
  * accumulating over successive child batches of the same engine. */
 int cam_engine_begin_children(cam_engine *eng);
 
+/* Start (or restart) recrossing parents, as mdinit with bias_mode 2 does
+ * (src/mdinit.f90:79-98, called at src/recross_serial.f90:118 and :270):
+ * fresh forces of every bead, plus the umbrella bias gradient of the
+ * centroid (src/umbrella.f90:136-173: k_force (xi - xi_ideal) dxi with xi of
+ * mode 1, and the Hessian metric term) on every bead; the gradient of xi
+ * (mode 1) becomes the constraint direction of the first step.  Needed after
+ * every cam_engine_set_state of a PARENT engine before it runs.  mdinit's
+ * momentum draw is cam_engine_draw_momenta.
+ * It also arms the per-step trajectory check of the recrossing driver
+ * (rpmd_check, src/rpmd_check.f90:74-98, as src/recross_serial.f90:125-135
+ * and :290-299 call it): from step check_after + 1 after this call on (never
+ * if check_after < 0), a trajectory whose positions hold a NaN or a value
+ * above HUGE, or whose energy (sum over the beads, plus the reference's 1e5
+ * penalty in a step whose constraint failed) is NaN, above HUGE or above
+ * epot_max, is flagged (cam_engine_observables).  The reference's limit is
+ * (ts_energy + energy_tol) * nbeads; INFINITY keeps only the NaN and HUGE
+ * checks.  Clears the flags. */
+int cam_engine_begin_parent(cam_engine *eng, double k_force, double epot_max, int check_after);
+
 /* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
  * engine's stream; all steps are captured in one HIP graph on first use and
- * replayed afterwards.  Asynchronous: call cam_engine_sync to wait. */
+ * replayed afterwards.  Asynchronous: call cam_engine_sync to wait.  A PARENT
+ * step is istep = (steps run since creation) + 1 for its Andersen schedule. */
 int cam_engine_run(cam_engine *eng, int nsteps);
 /* Capture and instantiate the graph of an nsteps run without running it, so
  * a later cam_engine_run(nsteps) only replays it. */
@@ -202,8 +229,10 @@ int cam_engine_kappa(cam_engine *eng, double *num, double *den, int nsteps);
 /* Per-trajectory observables of the current state (host arrays of ntraj):
  * epot = sum over beads of V (as verlet returns it), ekin = sum p^2/2m over
  * beads, espring = ring-polymer spring energy sum 1/2 m w_n^2 (q_k - q_k+1)^2
- * with w_n = nbeads/beta, xi = reaction coordinate of the centroid (CHILD
- * only, else 0), flags = bit 0: non-finite position seen. */
+ * with w_n = nbeads/beta, xi = reaction coordinate of the centroid (mode 2;
+ * CHILD and PARENT only, else 0), flags = bit 0: non-finite position now;
+ * PARENT, since cam_engine_begin_parent: bit 1 a constraint did not
+ * converge, bit 2 the trajectory check failed. */
 int cam_engine_observables(cam_engine *eng, double *epot, double *ekin, double *espring,
                            double *xi, int *flags);
 
