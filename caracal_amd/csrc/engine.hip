@@ -1073,8 +1073,11 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(Tra
 
 // The end of a parent step (src/verlet.f90:1060-1083, :1159-1165): the
 // second half kick, constrain_p (src/constrain_p.f90:36-76) with this step's
-// direction, and the Andersen draw when the step number istep (counted from
-// the engine's creation, first step 1) is a multiple of andersen_step.
+// direction, and the Andersen draw when the step number istep is a multiple
+// of andersen_step.  istep counts from the start (cam_engine_begin_parent,
+// first step 1), as the reference's verlet loops restart after each mdinit;
+// the draw is keyed by the steps since the engine's creation, so restarts
+// draw fresh momenta.
 // Then rpmd_check as the recrossing driver calls it for a parent
 // (src/rpmd_check.f90:74-98, traj_type 1 at src/recross_serial.f90:125-135
 // and :290-299), from step check_after + 1 after the start on (never if
@@ -1109,14 +1112,15 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_end_kernel(TrajArgs 
     for (int d = 0; d < 3; ++d)
         for (int at = 0; at < NA; ++at) c2 = c2 + dxi[3 * at + d] * dxi[3 * at + d] / a.mass[at];
     const double lam = -c1 / c2 / NB;
-    const uint64_t istep = *step_base + (uint64_t)step_off + 1;
+    const uint64_t istep = step_base[1] + (uint64_t)step_off + 1;
+    const uint64_t key = step_base[0] + (uint64_t)step_off + 1;
     const bool draw = andersen_step > 0 && istep % (uint64_t)andersen_step == 0;
     const double beta = a.beta[t];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const double sd = sqrt(a.mass[i / 3] / (beta / NB));
         for (int k = lane; k < NB; k += 64)
-            p[(size_t)i * NB + k] = draw ? andersen_normal(seed, (uint32_t)t, istep, i, k) * sd
+            p[(size_t)i * NB + k] = draw ? andersen_normal(seed, (uint32_t)t, key, i, k) * sd
                                          : p[(size_t)i * NB + k] + lam * dxi[i];
     }
     if (check_after >= 0 && step_base[1] + (uint64_t)step_off + 1 > (uint64_t)check_after) {

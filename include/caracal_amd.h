@@ -210,8 +210,10 @@ int cam_engine_begin_parent(cam_engine *eng, double k_force, double epot_max, in
 
 /* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
  * engine's stream; all steps are captured in one HIP graph on first use and
- * replayed afterwards.  Asynchronous: call cam_engine_sync to wait.  A PARENT
- * step is istep = (steps run since creation) + 1 for its Andersen schedule. */
+ * replayed afterwards.  Asynchronous: call cam_engine_sync to wait.  The
+ * Andersen schedule of a PARENT counts steps from cam_engine_begin_parent
+ * (first step 1), as the reference's verlet loop restarts after mdinit; its
+ * draws are keyed by the steps since creation (first step 1). */
 int cam_engine_run(cam_engine *eng, int nsteps);
 /* Capture and instantiate the graph of an nsteps run without running it, so
  * a later cam_engine_run(nsteps) only replays it. */
