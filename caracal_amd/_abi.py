@@ -10,7 +10,7 @@ CAM_OK = 0
 CAM_SURFACE_H3, CAM_SURFACE_CH4H = 1, 2
 CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
-CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_PARENT = 0, 1, 2, 3
+CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_PARENT, CAM_ENS_UMBRELLA = 0, 1, 2, 3, 4
 CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA, CAM_NMK_FOLDED = 0, 1, 2, 3
 CAM_FF_PAIRS_LIST, CAM_FF_PAIRS_ALL = 0, 1
 CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT = 0, 1, 2
@@ -90,6 +90,9 @@ PROTOTYPES = {
     "cam_engine_constrain": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "cam_engine_begin_children": (ctypes.c_int, [_vp]),
     "cam_engine_begin_parent": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "cam_engine_set_windows": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_engine_begin_umbrella": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "cam_engine_umbrella_stats": (ctypes.c_int, [_vp, _dp, _dp]),
     "cam_engine_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_engine_prepare": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_engine_sync": (ctypes.c_int, [_vp]),

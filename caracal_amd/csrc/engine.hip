@@ -232,9 +232,10 @@ __device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const doub
         }
 }
 
-// umbrella mode 0 (src/umbrella.f90:109-173) at centroid c: xi of mode 1,
-// s0 / (s0 - s1) (src/calc_xi.f90:204-206, :275-278), and its gradient into
-// dxi; the bias gradient every bead receives, in the reference's two parts:
+// umbrella mode 0 (src/umbrella.f90:109-173) at centroid c for the window
+// xi_ideal: xi of mode 1, s0 / (s0 - s1) (src/calc_xi.f90:204-206,
+// :275-278), returned in xi1, and its gradient into dxi; the bias gradient
+// every bead receives, in the reference's two parts:
 // gk = k_force (xi - xi_ideal) dxi and the metric term
 // gh = -(1/beta) H M^-1 dxi / (dxi^T M^-1 dxi), H the Hessian of xi, which
 // the reference contracts from the full matrix (:148-170) and which is
@@ -242,15 +243,16 @@ __device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const doub
 // 0.5 k_force (xi - xi_ideal)^2 - log(fs) / beta.
 template <int NA>
 __device__ double umbrella_bias(const Rx &rx, const double *mass, const double *c, double beta, double k_force,
-                                double *dxi, double *gk, double *gh)
+                                double xi_ideal, double &xi1, double *dxi, double *gk, double *gh)
 {
     constexpr int N = 3 * NA;
     double s0, s1, ds0[N], ds1[N];
     xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
     const double D = s0 - s1;
     const double xi = s0 / D;
+    xi1 = xi;
     for (int i = 0; i < N; ++i) dxi[i] = (s0 * ds1[i] - s1 * ds0[i]) / (D * D);
-    const double delta = xi - rx.xi_ideal;
+    const double delta = xi - xi_ideal;
     for (int i = 0; i < N; ++i) gk[i] = k_force * delta * dxi[i];
     double v[N], fs2 = 0.0;
     for (int i = 0; i < N; ++i) v[i] = dxi[i] / mass[i / 3];
@@ -991,6 +993,25 @@ __device__ void wave_centroid(const double *q, int nb, int lane, double *c)
     for (int i = 0; i < 3 * NA; ++i) c[i] = __shfl(own, i, 64);
 }
 
+// rpmd_check's structure and energy conditions (src/rpmd_check.f90:74-98)
+// for trajectory t, the same in every lane of its wave: a position that is
+// NaN or above HUGE, or an energy (sum over the beads plus penalty) that is
+// NaN, above HUGE or above epot_max.
+__device__ bool trajectory_fails(const TrajArgs &a, int t, int lane, double penalty, double epot_max)
+{
+    const int NB = a.nbeads, NQ = 3 * a.natoms * NB;
+    const double *q = a.q + (size_t)t * NQ;
+    bool bad = false;
+    for (int i = lane; i < NQ; i += 64) {
+        const double v = q[i];
+        bad = bad || v != v || v > DBL_MAX;
+    }
+    double ep = penalty;
+    for (int k = 0; k < NB; ++k) ep = ep + a.vbead[(size_t)t * NB + k];
+    bad = bad || ep != ep || ep > DBL_MAX || ep > epot_max;
+    return __any(bad);
+}
+
 // mdinit with bias_mode 2 (src/mdinit.f90:79-98), after the forces: the
 // umbrella mode-0 bias at the centroid is added to every bead's gradient,
 // first its force-constant part, then its metric part, and the gradient of
@@ -1005,7 +1026,8 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_begin_kernel(TrajArg
     const int NB = a.nbeads;
     double c[NC], dxi[NC], gk[NC], gh[NC];
     wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
-    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], k_force, dxi, gk, gh);
+    double xi1;
+    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], k_force, a.rx.xi_ideal, xi1, dxi, gk, gh);
     double *gt = g + (size_t)t * NC * NB;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
@@ -1124,16 +1146,92 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_end_kernel(TrajArgs 
                                          : p[(size_t)i * NB + k] + lam * dxi[i];
     }
     if (check_after >= 0 && step_base[1] + (uint64_t)step_off + 1 > (uint64_t)check_after) {
-        const double *q = a.q + (size_t)t * NC * NB;
-        bool bad = false;
-        for (int i = lane; i < NC * NB; i += 64) {
-            const double v = q[i];
-            bad = bad || v != v || v > DBL_MAX;
+        const double penalty = (pfail[t] & kPfThisStep) ? 1.0e5 : 0.0;
+        if (trajectory_fails(a, t, lane, penalty, epot_max) && lane == 0) pfail[t] = pfail[t] | kPfCheck;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Umbrella-sampling windows (verlet with constrain = 0 and the umbrella bias,
+// src/calc_rate.f90:1556-1645), one window per trajectory: uwin[traj] =
+// (xi_ideal, k_force) of its window, ustat[traj] = running sums of xi and xi^2
+// (mode 1) since the start (src/calc_rate.f90:1639-1640).  Same wave layout
+// as the parent kernels.
+// ---------------------------------------------------------------------------
+
+// Start of a window segment: with bias, the umbrella bias that mdinit adds
+// to the fresh forces (src/mdinit.f90:90-96); without, the unbiased forces
+// the sampling phase starts from (src/calc_rate.f90:1610-1618).  Clears the
+// sums and the flags.
+template <int NA>
+__global__ __launch_bounds__(64 * kParentWaves) void umbrella_begin_kernel(TrajArgs a, double *g, const double *uwin,
+                                                                           double *ustat, int *pfail, int bias)
+{
+    constexpr int NC = 3 * NA;
+    const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads;
+    if (bias) {
+        double c[NC], dxi[NC], gk[NC], gh[NC], xi1;
+        wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
+        (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], uwin[2 * t + 1], uwin[2 * t], xi1, dxi, gk, gh);
+        double *gt = g + (size_t)t * NC * NB;
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            for (int k = lane; k < NB; k += 64) gt[(size_t)i * NB + k] = (gt[(size_t)i * NB + k] + gk[i]) + gh[i];
+    }
+    if (lane == 0) {
+        ustat[2 * t] = 0.0;
+        ustat[2 * t + 1] = 0.0;
+        pfail[t] = 0;
+    }
+}
+
+// The end of a window step (src/verlet.f90:1040-1062, :1159-1165): the
+// umbrella bias at the centroid of this step's positions added to every
+// bead's gradient (umbrella mode 0, src/verlet.f90:1046), the running sums of
+// xi, the second half kick, and the Andersen draw every andersen_step steps
+// (counted from the start; draws keyed by the steps since creation).  Then the
+// check of the umbrella loops (rpmd_check with traj_type 2 and 3,
+// src/rpmd_check.f90:74-112): the parent's conditions, without the penalty,
+// plus |xi - xi_ideal| > xi_tol.
+template <int NA>
+__global__ __launch_bounds__(64 * kParentWaves) void umbrella_end_kernel(TrajArgs a, double *g, const double *uwin,
+                                                                         double *ustat, int *pfail,
+                                                                         const uint64_t *step_base, int step_off,
+                                                                         int andersen_step, uint64_t seed,
+                                                                         double epot_max, double xi_tol, int check_after)
+{
+    constexpr int NC = 3 * NA;
+    const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
+    if (t >= a.ntraj) return;
+    const int NB = a.nbeads;
+    double c[NC], dxi[NC], gk[NC], gh[NC], xi1;
+    wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
+    const double xi_ideal = uwin[2 * t];
+    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], uwin[2 * t + 1], xi_ideal, xi1, dxi, gk, gh);
+    const uint64_t istep = step_base[1] + (uint64_t)step_off + 1;
+    const uint64_t key = step_base[0] + (uint64_t)step_off + 1;
+    const bool draw = andersen_step > 0 && istep % (uint64_t)andersen_step == 0;
+    const double beta = a.beta[t];
+    double *gt = g + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const double sd = sqrt(a.mass[i / 3] / (beta / NB));
+        for (int k = lane; k < NB; k += 64) {
+            const size_t o = (size_t)i * NB + k;
+            const double gn = (gt[o] + gk[i]) + gh[i];
+            gt[o] = gn;
+            p[o] = draw ? andersen_normal(seed, (uint32_t)t, key, i, k) * sd : p[o] - 0.5 * a.dt * gn;
         }
-        double ep = (pfail[t] & kPfThisStep) ? 1.0e5 : 0.0;
-        for (int k = 0; k < NB; ++k) ep = ep + a.vbead[(size_t)t * NB + k];
-        bad = bad || ep != ep || ep > DBL_MAX || ep > epot_max;
-        if (__any(bad) && lane == 0) pfail[t] = pfail[t] | kPfCheck;
+    }
+    if (lane == 0) {
+        ustat[2 * t] = ustat[2 * t] + xi1;
+        ustat[2 * t + 1] = ustat[2 * t + 1] + xi1 * xi1;
+    }
+    if (check_after >= 0 && istep > (uint64_t)check_after) {
+        const bool bad = trajectory_fails(a, t, lane, 0.0, epot_max) || fabs(xi1 - xi_ideal) > xi_tol;
+        if (bad && lane == 0) pfail[t] = pfail[t] | kPfCheck;
     }
 }
 
@@ -1307,9 +1405,13 @@ struct cam_engine {
     uint8_t *xi_flag = nullptr;
     double *cbuf = nullptr;           // CHILD: centroids of one fused launch
     double *pdxi = nullptr;           // PARENT: constraint directions [traj][3*natoms]
-    int *pfail = nullptr;             // PARENT: non-converged constraints per trajectory
-    bool parent_ready = false;        // PARENT: begin_parent done since the last set_state
-    double pcheck_max = 0.0;          // PARENT: trajectory check (begin_parent)
+    int *pfail = nullptr;             // PARENT, UMBRELLA: trajectory flags (kPf*)
+    double *uwin = nullptr;           // UMBRELLA: (xi_ideal, k_force) per trajectory
+    double *ustat = nullptr;          // UMBRELLA: running sums of xi, xi^2 per trajectory
+    long ustat_steps = 0;             // UMBRELLA: steps in those sums
+    bool parent_ready = false;        // PARENT, UMBRELLA: begun since the last set_state
+    double pcheck_max = 0.0;          // PARENT, UMBRELLA: trajectory check
+    double pcheck_xi_tol = 0.0;
     int pcheck_after = -1;
     uint64_t *step_base = nullptr;    // [0] steps since creation, [1] steps since begin_children
     double *scratch_io = nullptr;     // staging for layout conversion
@@ -1456,9 +1558,9 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
     const int nb = cfg->nbeads;
-    // a parent's normal-mode kernel is the NVE one: its constraint and the
-    // end of its step are kernels of their own
-    const int ens = cfg->ensemble == CAM_ENS_PARENT ? CAM_ENS_NVE : cfg->ensemble;
+    // a parent's or window's normal-mode kernel is the NVE one: its
+    // constraint or bias and the end of its step are kernels of their own
+    const int ens = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA ? CAM_ENS_NVE : cfg->ensemble;
     const bool mfma = nmk == CAM_NMK_MFMA;
     bool ok;
     if (cfg->surface == CAM_SURFACE_H3)
@@ -1555,6 +1657,7 @@ StepArgs step_args(cam_engine *e)
     a.cbuf = e->cbuf;
     a.pdxi = e->pdxi;
     a.pfail = e->pfail;
+    a.ustat = e->ustat;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
     a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
@@ -1602,6 +1705,35 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
             else
                 hipLaunchKernelGGL(parent_end_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi,
                                    a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
+            err = hipGetLastError();
+        }
+        return err;
+    }
+    if (e->cfg.ensemble == CAM_ENS_UMBRELLA) {
+        // per step: kick and free ring polymer; forces; bias, sums, second half kick, Andersen
+        TrajArgs ta = traj_args(e);
+        ta.q = a.q;
+        ta.p = a.p;
+        ta.g = a.g;
+        const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
+        const int as = e->cfg.andersen_step;
+        for (int j = 0; j < nsteps && err == hipSuccess; ++j) {
+            a.nsteps = 1;
+            a.step_off = 0;
+            hipLaunchKernelGGL(kn.nm, dim3(grid), dim3(kBlock), 0, e->stream, a);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            hipLaunchKernelGGL(kn.fsplit, dim3(grid), dim3(kBlock), 0, e->stream, a.q, a.g, a.vbead, a.ntraj);
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+            if (e->cfg.natoms == 3)
+                hipLaunchKernelGGL(umbrella_end_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g,
+                                   e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max,
+                                   e->pcheck_xi_tol, e->pcheck_after);
+            else
+                hipLaunchKernelGGL(umbrella_end_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g,
+                                   e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max,
+                                   e->pcheck_xi_tol, e->pcheck_after);
             err = hipGetLastError();
         }
         return err;
@@ -1676,7 +1808,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
-    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_PARENT, "bad ensemble");
+    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_UMBRELLA, "bad ensemble");
+    const bool steered = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA;
     CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_FOLDED, "bad nm_kernel");
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
                   "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
@@ -1692,15 +1825,16 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
-    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->andersen_step >= 0, "andersen_step must be >= 0");
-    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->schedule != CAM_SCHED_FUSED,
-                  "ensemble PARENT runs on the split schedule (its constraint sits between propagation and forces)");
+    CAM_CHECK_ARG(!steered || cfg->andersen_step >= 0, "andersen_step must be >= 0");
+    CAM_CHECK_ARG(!steered || cfg->schedule != CAM_SCHED_FUSED,
+                  "ensembles PARENT and UMBRELLA run on the split schedule (their constraint or bias sits between "
+                  "the force kernel and the second half kick)");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
-    CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || cfg->ensemble == CAM_ENS_PARENT)),
+    CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || steered)),
                   "remove_com applies to dynamic.x trajectories (constrain <= 0) only");
     const cam_reaction &rr = cfg->reaction;
     Rx rx{};
-    if (cfg->ensemble == CAM_ENS_CHILD || cfg->ensemble == CAM_ENS_PARENT) {
+    if (cfg->ensemble == CAM_ENS_CHILD || steered) {
         CAM_CHECK_ARG(rr.n_form >= 0 && rr.n_form <= CAM_MAX_BONDS && rr.n_break >= 0 && rr.n_break <= CAM_MAX_BONDS &&
                           rr.n_form + rr.n_break > 0,
                       "reaction needs 1..%d forming and breaking bonds", CAM_MAX_BONDS);
@@ -1748,7 +1882,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     else e->nmk = fold_available(cfg) ? CAM_NMK_FOLDED : CAM_NMK_VALU;
     e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
                      e->nmk != CAM_NMK_FOLDED ? 2 : 1;
-    e->split = cfg->schedule == CAM_SCHED_SPLIT || cfg->ensemble == CAM_ENS_PARENT;
+    e->split = cfg->schedule == CAM_SCHED_SPLIT || steered;
     const bool ok = select_kernels(e.get(), e->wpe, e->nmk, e->kern);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
@@ -1767,11 +1901,22 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
         const size_t steps = (size_t)std::max(e->spl, 100);
         CAM_HIP(hipMalloc(&e->cbuf, steps * cfg->ntraj * 3 * na * sizeof(double)));
     }
+    if (steered) {
+        CAM_HIP(hipMalloc(&e->pfail, (size_t)cfg->ntraj * sizeof(int)));
+        CAM_HIP(hipMemset(e->pfail, 0, (size_t)cfg->ntraj * sizeof(int)));
+    }
     if (cfg->ensemble == CAM_ENS_PARENT) {
         CAM_HIP(hipMalloc(&e->pdxi, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
-        CAM_HIP(hipMalloc(&e->pfail, (size_t)cfg->ntraj * sizeof(int)));
         CAM_HIP(hipMemset(e->pdxi, 0, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
-        CAM_HIP(hipMemset(e->pfail, 0, (size_t)cfg->ntraj * sizeof(int)));
+    }
+    if (cfg->ensemble == CAM_ENS_UMBRELLA) {
+        // every window at the reaction's xi_ideal with no force constant until cam_engine_set_windows
+        std::vector<double> w(2 * (size_t)cfg->ntraj, 0.0);
+        for (int t = 0; t < cfg->ntraj; ++t) w[2 * (size_t)t] = rx.xi_ideal;
+        CAM_HIP(hipMalloc(&e->uwin, w.size() * sizeof(double)));
+        CAM_HIP(hipMemcpy(e->uwin, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+        CAM_HIP(hipMalloc(&e->ustat, w.size() * sizeof(double)));
+        CAM_HIP(hipMemset(e->ustat, 0, w.size() * sizeof(double)));
     }
     CAM_HIP(hipMalloc(&e->weight, (size_t)cfg->ntraj * sizeof(double)));
     CAM_HIP(hipMalloc(&e->den_part, (size_t)cfg->ntraj * sizeof(double)));
@@ -1804,7 +1949,7 @@ int cam_engine_destroy(cam_engine *e)
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->tfold, e->weight, e->den_part,
-                       e->den, e->num, e->scratch_io, e->cbuf, e->pdxi };
+                       e->den, e->num, e->scratch_io, e->cbuf, e->pdxi, e->uwin, e->ustat };
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (e->pfail) (void)hipFree(e->pfail);
@@ -1979,6 +2124,73 @@ int cam_engine_begin_parent(cam_engine *e, double k_force, double epot_max, int 
     return CAM_OK;
 }
 
+int cam_engine_set_windows(cam_engine *e, const double *xi_ideal, const double *k_force)
+{
+    CAM_CHECK_ARG(e && xi_ideal && k_force, "null argument");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_UMBRELLA, "set_windows needs ensemble UMBRELLA");
+    const int nt = e->cfg.ntraj;
+    std::vector<double> w(2 * (size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+        CAM_CHECK_ARG(std::isfinite(xi_ideal[t]) && std::isfinite(k_force[t]), "window %d is not finite", t);
+        w[2 * (size_t)t] = xi_ideal[t];
+        w[2 * (size_t)t + 1] = k_force[t];
+    }
+    clear_error();
+    CAM_HIP(hipMemcpyAsync(e->uwin, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    e->parent_ready = false;          // the start's bias depends on the windows
+    return CAM_OK;
+}
+
+int cam_engine_begin_umbrella(cam_engine *e, int bias, double epot_max, double xi_tol, int check_after)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->have_state, "no state uploaded");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_UMBRELLA, "begin_umbrella needs ensemble UMBRELLA");
+    CAM_CHECK_ARG(!std::isnan(epot_max) && !std::isnan(xi_tol), "epot_max and xi_tol must not be NaN");
+    clear_error();
+    e->pcheck_max = epot_max;
+    e->pcheck_xi_tol = xi_tol;
+    e->pcheck_after = check_after;
+    drop_graphs(e);                   // the check parameters are arguments of the captured launches
+    int rc = run_forces(e);
+    if (rc != CAM_OK) return rc;
+    const TrajArgs ta = traj_args(e);
+    const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
+    if (e->cfg.natoms == 3)
+        hipLaunchKernelGGL(umbrella_begin_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
+                           e->uwin, e->ustat, e->pfail, bias);
+    else
+        hipLaunchKernelGGL(umbrella_begin_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
+                           e->uwin, e->ustat, e->pfail, bias);
+    CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);   // steps since the start
+    CAM_LAUNCH_CHECK();
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    e->ustat_steps = 0;
+    e->parent_ready = true;
+    return CAM_OK;
+}
+
+int cam_engine_umbrella_stats(cam_engine *e, double *average, double *variance)
+{
+    CAM_CHECK_ARG(e && average && variance, "null argument");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_UMBRELLA, "umbrella_stats needs ensemble UMBRELLA");
+    CAM_CHECK_ARG(e->parent_ready && e->ustat_steps > 0, "no window steps since cam_engine_begin_umbrella");
+    const int nt = e->cfg.ntraj;
+    std::vector<double> s(2 * (size_t)nt);
+    clear_error();
+    CAM_HIP(hipMemcpyAsync(s.data(), e->ustat, s.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    const double n = (double)e->ustat_steps;
+    for (int t = 0; t < nt; ++t) {
+        // src/calc_rate.f90:1655-1657
+        average[t] = s[2 * (size_t)t] / n;
+        variance[t] = s[2 * (size_t)t + 1] / n - average[t] * average[t];
+    }
+    return CAM_OK;
+}
+
 static int get_graph(cam_engine *e, int nsteps, hipGraphExec_t *out)
 {
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
@@ -2023,6 +2235,8 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_CHECK_ARG(e->have_state, "no state uploaded");
     CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_PARENT || e->parent_ready,
                   "a PARENT engine needs cam_engine_begin_parent after cam_engine_set_state");
+    CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_UMBRELLA || e->parent_ready,
+                  "an UMBRELLA engine needs cam_engine_begin_umbrella after cam_engine_set_state");
     clear_error();
     if (nsteps == 0) return CAM_OK;
     hipGraphExec_t g;
@@ -2032,6 +2246,7 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_HIP(hipGraphLaunch(g, e->stream));
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
     e->child_steps += nsteps;
+    e->ustat_steps += nsteps;
     e->last_launches = e->split ? nsteps : (nsteps + e->spl - 1) / e->spl;
     e->last_ms = -1.0;
     return CAM_OK;
@@ -2074,7 +2289,7 @@ int cam_engine_observables(cam_engine *e, double *epot, double *ekin, double *es
     hipError_t err = hipMalloc(&fl, (size_t)nt * sizeof(int));
     if (err == hipSuccess) {
         const TrajArgs ta = traj_args(e);
-        const int with_xi = e->cfg.ensemble == CAM_ENS_CHILD || e->cfg.ensemble == CAM_ENS_PARENT;
+        const int with_xi = e->cfg.ensemble != CAM_ENS_NVE && e->cfg.ensemble != CAM_ENS_ANDERSEN;
         hipLaunchKernelGGL(observe_kernel, dim3((nt + 127) / 128), dim3(128), 0, e->stream, ta, e->pfail,
                            buf, buf + nt, buf + 2 * nt, buf + 3 * nt, fl, with_xi);
         err = hipGetLastError();
@@ -2108,6 +2323,8 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     CAM_CHECK_ARG(trial_steps >= 1 && trial_steps <= 100000, "trial_steps must be in 1..100000");
     CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_PARENT || e->parent_ready,
                   "a PARENT engine needs cam_engine_begin_parent before autotune");
+    CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_UMBRELLA || e->parent_ready,
+                  "an UMBRELLA engine needs cam_engine_begin_umbrella before autotune");
     clear_error();
     const cam_config &c = e->cfg;
     const int nb = c.nbeads;
@@ -2134,8 +2351,8 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     int rc = CAM_OK;
     struct Trial { int nmk, wpe, spl, split; double us; };
     std::vector<Trial> cand, trials;
-    const bool parent = c.ensemble == CAM_ENS_PARENT;
-    const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent;
+    const bool parent = c.ensemble == CAM_ENS_PARENT, window = c.ensemble == CAM_ENS_UMBRELLA;
+    const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent && !window;
     const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
     for (int nmk : nmks) {
         if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
@@ -2151,8 +2368,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         set_error("HIP error in autotune: %s", hipGetErrorString(err));
         rc = CAM_ERR_DEVICE;
     };
-    // PARENT: a copy of the constraint directions and a scratch failure count
-    const size_t npd = parent ? (size_t)c.ntraj * 3 * c.natoms : 0;
+    // PARENT: a copy of the constraint directions; UMBRELLA: scratch sums;
+    // both: scratch flags
+    const size_t npd = parent ? (size_t)c.ntraj * 3 * c.natoms : window ? 2 * (size_t)c.ntraj : 0;
     hipError_t err = hipMalloc(&buf, (3 * nstate + (size_t)c.ntraj * nb + npd + (size_t)c.ntraj) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&flags, (size_t)trial_steps * c.ntraj);
     if (err == hipSuccess) err = hipMalloc(&ctr, 2 * sizeof(uint64_t));
@@ -2173,6 +2391,10 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         if (parent) {
             a.pdxi = buf + 3 * nstate + (size_t)c.ntraj * nb;
             a.pfail = reinterpret_cast<int *>(a.pdxi + npd);
+        }
+        if (window) {
+            a.ustat = buf + 3 * nstate + (size_t)c.ntraj * nb;
+            a.pfail = reinterpret_cast<int *>(a.ustat + npd);
         }
         err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
         if (err == hipSuccess && parent)

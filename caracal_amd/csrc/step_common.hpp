@@ -61,7 +61,8 @@ struct StepArgs {
     uint8_t *xi_flag;            // CHILD: [nsteps_run][ntraj]
     double *cbuf;                // CHILD, fused: centroids of this launch's steps [step][traj][3*natoms]
     double *pdxi;                // PARENT: constraint direction of each trajectory [traj][3*natoms]
-    int *pfail;                  // PARENT: non-converged constraints of each trajectory
+    int *pfail;                  // PARENT, UMBRELLA: trajectory flags
+    double *ustat;               // UMBRELLA: running sums of xi and xi^2 [traj][2]
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
     int andersen_step;

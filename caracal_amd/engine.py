@@ -163,6 +163,26 @@ class RPMDEngine:
         check("cam_engine_begin_parent",
               lib().cam_engine_begin_parent(self._h, float(k_force), float(epot_max), int(check_after)))
 
+    def set_windows(self, xi_ideal, k_force):
+        """Ensemble UMBRELLA: window centre (xi, mode 1) and force constant of every trajectory."""
+        x = np.ascontiguousarray(np.broadcast_to(xi_ideal, (self.ntraj,)), dtype=np.float64)
+        k = np.ascontiguousarray(np.broadcast_to(k_force, (self.ntraj,)), dtype=np.float64)
+        check("cam_engine_set_windows", lib().cam_engine_set_windows(self._h, _ptr(x), _ptr(k)))
+
+    def begin_umbrella(self, bias=True, epot_max=float("inf"), xi_tol=float("inf"), check_after=-1):
+        """Ensemble UMBRELLA: start a segment (bias=True: mdinit's biased
+        forces; False: unbiased forces, as the sampling phase starts) and arm
+        the trajectory check from step check_after + 1 on (off if < 0)."""
+        check("cam_engine_begin_umbrella", lib().cam_engine_begin_umbrella(
+            self._h, int(bool(bias)), float(epot_max), float(xi_tol), int(check_after)))
+
+    def umbrella_stats(self):
+        """Ensemble UMBRELLA: (average, variance) of xi per trajectory since begin_umbrella."""
+        avg = np.empty(self.ntraj)
+        var = np.empty(self.ntraj)
+        check("cam_engine_umbrella_stats", lib().cam_engine_umbrella_stats(self._h, _ptr(avg), _ptr(var)))
+        return avg, var
+
     def run(self, nsteps, sync=True):
         check("cam_engine_run", lib().cam_engine_run(self._h, nsteps))
         if sync:

@@ -19,7 +19,7 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, &
-                                             CAM_ENS_PARENT = 3
+                                             CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4
    integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, &
                                             CAM_NMK_FOLDED = 3
    integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
@@ -135,6 +135,26 @@ module caracal_amd_mod
          integer(c_int), value :: check_after
          integer(c_int) :: rc
       end function
+      function cam_engine_set_windows(eng, xi_ideal, k_force) bind(C, name="cam_engine_set_windows") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(in) :: xi_ideal(*), k_force(*)
+         integer(c_int) :: rc
+      end function
+      function cam_engine_begin_umbrella(eng, bias, epot_max, xi_tol, check_after) &
+            bind(C, name="cam_engine_begin_umbrella") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int), value :: bias, check_after
+         real(c_double), value :: epot_max, xi_tol
+         integer(c_int) :: rc
+      end function
+      function cam_engine_umbrella_stats(eng, average, variance) bind(C, name="cam_engine_umbrella_stats") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: average(*), variance(*)
+         integer(c_int) :: rc
+      end function
       function cam_engine_run(eng, nsteps) bind(C, name="cam_engine_run") result(rc)
          import :: c_int, c_ptr
          type(c_ptr), value :: eng
@@ -245,6 +265,7 @@ module caracal_amd_mod
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_begin_parent, cam_engine_run, cam_engine_sync, cam_engine_kappa
+   public :: cam_engine_set_windows, cam_engine_begin_umbrella, cam_engine_umbrella_stats
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode

@@ -86,8 +86,17 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
  *              constrain_p after the second half kick) and, if
  *              andersen_step > 0, the momenta are redrawn every andersen_step
  *              steps.  Runs on the split schedule; start it with
- *              cam_engine_begin_parent. */
-enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3 };
+ *              cam_engine_begin_parent;
+ *  UMBRELLA -- umbrella-sampling window trajectories (verlet with constrain
+ *              = 0 in the sampling loops of src/calc_rate.f90:1556-1645):
+ *              after the forces, the umbrella bias of the centroid
+ *              (umbrella mode 0, src/umbrella.f90:136-173) is added to every
+ *              bead's gradient, with each trajectory's own window
+ *              (cam_engine_set_windows); the running sums of xi (mode 1)
+ *              give each trajectory's average and variance
+ *              (cam_engine_umbrella_stats).  Andersen as PARENT.  Runs on the
+ *              split schedule; start it with cam_engine_begin_umbrella. */
+enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
  * nbeads a multiple of 16).  FOLDED: the reference transform (rfft/irfft, the
@@ -208,6 +217,28 @@ int cam_engine_begin_children(cam_engine *eng);
  * checks.  Clears the flags. */
 int cam_engine_begin_parent(cam_engine *eng, double k_force, double epot_max, int check_after);
 
+/* UMBRELLA: the window of every trajectory, xi_ideal[t] (the xi_wins value,
+ * mode 1) and k_force[t] (hartree), host arrays of ntraj.  Until called,
+ * every window sits at the reaction's xi_ideal with k_force 0.  Needs a new
+ * cam_engine_begin_umbrella. */
+int cam_engine_set_windows(cam_engine *eng, const double *xi_ideal, const double *k_force);
+
+/* UMBRELLA: start a segment of window steps.  bias != 0 is mdinit's start
+ * (src/mdinit.f90:79-98): fresh forces plus the umbrella bias of each
+ * window, as before the equilibration (src/calc_rate.f90:1549); bias = 0
+ * gives the unbiased forces the reference starts its sampling from
+ * (src/calc_rate.f90:1610-1618).  Clears the running sums of xi, the flags
+ * and the step count of the Andersen schedule.  Arms the check of the
+ * umbrella loops (rpmd_check with traj_type 2 and 3, src/rpmd_check.f90:74-112)
+ * as cam_engine_begin_parent does, with xi_tol added: |xi - xi_ideal| > xi_tol
+ * fails (INFINITY disables that part). */
+int cam_engine_begin_umbrella(cam_engine *eng, int bias, double epot_max, double xi_tol, int check_after);
+
+/* UMBRELLA: average and variance of xi (mode 1) of every trajectory over the
+ * steps run since cam_engine_begin_umbrella, as src/calc_rate.f90:1639-1657
+ * forms them (sum/n, sum of squares/n - average^2).  Host arrays of ntraj. */
+int cam_engine_umbrella_stats(cam_engine *eng, double *average, double *variance);
+
 /* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
  * engine's stream; all steps are captured in one HIP graph on first use and
  * replayed afterwards.  Asynchronous: call cam_engine_sync to wait.  The
@@ -232,9 +263,9 @@ int cam_engine_kappa(cam_engine *eng, double *num, double *den, int nsteps);
  * epot = sum over beads of V (as verlet returns it), ekin = sum p^2/2m over
  * beads, espring = ring-polymer spring energy sum 1/2 m w_n^2 (q_k - q_k+1)^2
  * with w_n = nbeads/beta, xi = reaction coordinate of the centroid (mode 2;
- * CHILD and PARENT only, else 0), flags = bit 0: non-finite position now;
- * PARENT, since cam_engine_begin_parent: bit 1 a constraint did not
- * converge, bit 2 the trajectory check failed. */
+ * CHILD, PARENT and UMBRELLA, else 0), flags = bit 0: non-finite position
+ * now; since cam_engine_begin_parent / _begin_umbrella: bit 1 (PARENT) a
+ * constraint did not converge, bit 2 the trajectory check failed. */
 int cam_engine_observables(cam_engine *eng, double *epot, double *ekin, double *espring,
                            double *xi, int *flags);
 

@@ -597,6 +597,30 @@ double oracle_umbrella(const oracle_sys *s, const oracle_rx *rx, const double *c
     return v_add;
 }
 
+/* One step of an umbrella-sampling window trajectory (verlet with
+ * constrain = 0 in the sampling loops, src/calc_rate.f90:1569 and :1626):
+ * p -= dt/2 g; free ring polymer; forces; umbrella mode 0 at the centroid of
+ * the new positions, for the window rx->xi_ideal with k_force, adds its bias
+ * to every bead's g (src/verlet.f90:1046); p -= dt/2 g; Andersen when istep
+ * is a multiple of andersen_step, with the draw id 'draw'.  Returns xi of
+ * mode 1 at that centroid (the xi_real the sampling loop sums). */
+double oracle_verlet_umbrella(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g,
+                              double *vbead, double beta, double k_force, long istep, int andersen_step,
+                              uint64_t seed, uint32_t traj, uint64_t draw)
+{
+    const int na = s->natoms, nb = s->nbeads;
+    const size_t n = (size_t)3 * na * nb;
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    oracle_free_rp(s, q, p, beta);
+    double c[3 * ORACLE_MAX_ATOMS], dxi[3 * ORACLE_MAX_ATOMS], xi = 0.0;
+    oracle_centroid(s, q, c);
+    oracle_forces(s, q, vbead, g);
+    (void)oracle_umbrella(s, rx, c, beta, k_force, dxi, g, &xi);
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    if (andersen_step > 0 && istep % andersen_step == 0) oracle_andersen(s, p, beta, seed, traj, draw);
+    return xi;
+}
+
 /* mdinit with bias_mode 2 (src/mdinit.f90:79-98), after the caller's force
  * evaluation g: umbrella mode 0 at the centroid of q adds its bias gradient
  * to g and leaves the mode-1 gradient of xi in dxi, the constraint direction

@@ -144,6 +144,10 @@ def _setup_rpmd():
     o.oracle_umbrella.restype = ctypes.c_double
     o.oracle_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, ctypes.c_double,
                                   ctypes.c_double, _dp, _dp, _dp]
+    o.oracle_verlet_umbrella.restype = ctypes.c_double
+    o.oracle_verlet_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_long, ctypes.c_int,
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
     o.oracle_verlet_parent.restype = ctypes.c_int
     o.oracle_verlet_parent.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp, _dp,
                                        ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_uint64,
@@ -190,6 +194,14 @@ class OracleTraj:
 
     def step(self):
         self.istep += 1
+        if self.ensemble == 4:
+            xi = rpmd().oracle_verlet_umbrella(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q),
+                                               _ptr(self.p), _ptr(self.g), _ptr(self.v), self.beta, self.k_force,
+                                               self.istep, self.andersen_step, self.seed, self.traj,
+                                               self.key_base + self.istep)
+            self.xi_sum += xi
+            self.xi_sq += xi * xi
+            return xi
         if self.ensemble == 3:
             return rpmd().oracle_verlet_parent(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q),
                                                _ptr(self.p), _ptr(self.g), _ptr(self.v), _ptr(self.dxi), self.beta,
@@ -197,6 +209,24 @@ class OracleTraj:
         return rpmd().oracle_verlet(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.p),
                                     _ptr(self.g), _ptr(self.v), self.beta, self.ensemble, int(self.remove_com),
                                     self.istep, self.andersen_step, self.seed, self.traj)
+
+    def umbrella_begin(self, k_force, bias=True):
+        """Umbrella window (ensemble 4; the window centre is self.rx.xi_ideal):
+        fresh forces, plus the bias if bias (mdinit's start), and a new
+        segment: sums cleared, the Andersen schedule from step 1, draw ids
+        continuing."""
+        self.k_force = k_force
+        self.v, self.g = forces(self.sys, self.q)
+        if bias:
+            c = np.ascontiguousarray(self.q.mean(axis=0))
+            c[:] = 0.0
+            rpmd().oracle_centroid(ctypes.byref(self.sys), _ptr(self.q), _ptr(c))
+            d = np.zeros_like(c)
+            rpmd().oracle_umbrella(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(c), self.beta, k_force,
+                                   _ptr(d), _ptr(self.g), None)
+        self.key_base = getattr(self, "key_base", 0) + self.istep
+        self.istep = 0
+        self.xi_sum = self.xi_sq = 0.0
 
     def parent_begin(self, k_force=0.0):
         """Recrossing parent (ensemble 3): mdinit with bias_mode 2 -- fresh

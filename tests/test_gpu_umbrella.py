@@ -1,0 +1,103 @@
+"""GPU parity of umbrella-sampling windows (ensemble UMBRELLA): biased
+equilibration then unbiased-start sampling, one window per trajectory, with
+Andersen draws, against the oracle; the xi statistics; the window check."""
+import numpy as np
+import pytest
+
+import caracal_amd as cam
+import oracle_lib as ol
+from caracal_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _windows(surface, nb, nwin, seed):
+    if surface == cam.CAM_SURFACE_H3:
+        w = workloads.h3_nve(nbeads=nb)
+        rng = np.random.default_rng(seed)
+        q = np.stack([w["q"]] * nwin) + rng.normal(0, 0.01, (nwin,) + w["q"].shape)
+        beta = np.full(nwin, w["beta"])
+        rx = workloads.h3_reaction()
+    else:
+        w = workloads.ch4h_children(ntraj=nwin, nbeads=nb, seed=seed)
+        q, beta, rx = w["q"], w["beta"], w["reaction"]
+        rng = np.random.default_rng(seed)
+    p = rng.normal(size=q.shape) * np.sqrt(w["masses"][None, None, :, None] * nb / beta[:, None, None, None])
+    xi_w = np.linspace(0.9, 1.06, nwin)
+    k_w = np.linspace(0.02, 0.2, nwin)
+    return w, q, p, beta, rx, xi_w, k_w
+
+
+@pytest.mark.parametrize("surface,nb,nmk", [(cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU),
+                                            (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA),
+                                            (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU)])
+def test_umbrella_windows_match_oracle(surface, nb, nmk):
+    nwin, ast, seed, n_equi, n_samp = 6, 5, 21, 20, 30
+    w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb)
+    e = cam.RPMDEngine(surface, nb, nwin, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
+                       andersen_step=ast, seed=seed, nm_kernel=nmk)
+    e.set_state(q, p, beta)
+    e.set_windows(xi_w, k_w)
+    e.begin_umbrella(bias=True)
+    s = ol.make_sys(surface, nb, w["dt"], w["masses"])
+    refs = []
+    for t in range(nwin):
+        orx = ol.make_rx(rx)
+        orx.xi_ideal = xi_w[t]
+        tr = ol.OracleTraj(s, q[t], p[t], float(beta[t]), rx=orx, ensemble=4, andersen_step=ast, seed=seed, traj=t)
+        tr.umbrella_begin(k_w[t], bias=True)
+        refs.append(tr)
+    _, g0 = e.get_forces()
+    for t in range(nwin):
+        np.testing.assert_allclose(g0[t], refs[t].g, rtol=0, atol=1e-12 * np.abs(refs[t].g).max())
+    e.run(n_equi)
+    for tr in refs:
+        for _ in range(n_equi):
+            tr.step()
+    e.begin_umbrella(bias=False)
+    for t, tr in enumerate(refs):
+        tr.umbrella_begin(k_w[t], bias=False)
+    e.run(n_samp)
+    for tr in refs:
+        for _ in range(n_samp):
+            tr.step()
+    qe, pe = e.get_state()
+    avg, var = e.umbrella_stats()
+    for t, tr in enumerate(refs):
+        err = np.abs(qe[t] - tr.q).max() / np.abs(tr.q).max()
+        perr = np.abs(pe[t] - tr.p).max() / np.abs(tr.p).max()
+        a_ref = tr.xi_sum / n_samp
+        v_ref = tr.xi_sq / n_samp - a_ref * a_ref
+        print(f"surface={surface} nb={nb} window {xi_w[t]:.3f}: rel dq {err:.1e} dp {perr:.1e}; "
+              f"<xi> {avg[t]:.6f} (oracle {a_ref:.6f}) var {var[t]:.3e}")
+        assert err <= 1e-10 and perr <= 1e-9
+        assert abs(avg[t] - a_ref) <= 1e-12
+        assert abs(var[t] - v_ref) <= 1e-12
+    assert not e.observables()["flags"].any()
+    e.close()
+
+
+def test_umbrella_window_check_and_errors():
+    w, q, p, beta, rx, xi_w, k_w = _windows(cam.CAM_SURFACE_H3, 8, 4, seed=5)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 4, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx)
+    e.set_state(q, p, beta)
+    with pytest.raises(cam.CamError, match="begin_umbrella"):
+        e.run(1)
+    e.set_windows(np.full(4, 1.0), np.full(4, 0.1))
+    e.begin_umbrella(bias=True)
+    with pytest.raises(cam.CamError, match="no window steps"):
+        e.umbrella_stats()
+    # windows at xi = 1 from structures near xi = 1: a tolerance of 0.5 passes, 1e-6 fails from step 3 on
+    e.begin_umbrella(bias=True, xi_tol=0.5, check_after=0)
+    e.run(5)
+    assert not e.observables()["flags"].any()
+    e.begin_umbrella(bias=True, xi_tol=1e-6, check_after=2)
+    e.run(2)
+    assert not e.observables()["flags"].any()
+    e.run(1)
+    assert (e.observables()["flags"] == 4).all()
+    e.close()
+    for kw, msg in ((dict(schedule=cam.CAM_SCHED_FUSED), "split schedule"), (dict(remove_com=True), "remove_com")):
+        with pytest.raises(cam.CamError, match=msg):
+            cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA,
+                           reaction=rx, **kw)
