@@ -97,4 +97,13 @@ double oracle_andersen_normal(uint64_t seed, uint32_t traj, uint64_t draw, int c
 #ifdef __cplusplus
 }
 #endif
+/* rate theory after the umbrella samplings (rate.c) */
+void oracle_umbrella_integration(int nwin, const double *xi_wins, const double *average, const double *variance,
+                                 const double *k_force, double beta, double xi_min, double xi_max, int nbins,
+                                 int umbr_traj, int umbr_step, double *dA, double *bin_coord, double *pmf);
+void oracle_pmf_extrema(const double *pmf, int nbins, double xi_min, double xi_max, int minloc_zero, int *imax,
+                        int *imin);
+double oracle_rate_bimolecular(int npaths, double kappa, double r_inf, double beta, double m1, double m2,
+                               double pmf_max, double pmf_min);
+
 #endif

@@ -144,6 +144,13 @@ def _setup_rpmd():
     o.oracle_umbrella.restype = ctypes.c_double
     o.oracle_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, ctypes.c_double,
                                   ctypes.c_double, _dp, _dp, _dp]
+    o.oracle_umbrella_integration.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                              _dp]
+    o.oracle_pmf_extrema.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, _ip]
+    o.oracle_rate_bimolecular.restype = ctypes.c_double
+    o.oracle_rate_bimolecular.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     o.oracle_verlet_umbrella.restype = ctypes.c_double
     o.oracle_verlet_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_long, ctypes.c_int,
@@ -261,6 +268,26 @@ def umbrella(sys, rx, c, beta, k_force):
     v = rpmd().oracle_umbrella(ctypes.byref(sys), ctypes.byref(rx), _ptr(c), beta, k_force, _ptr(d), _ptr(g),
                                ctypes.byref(xi))
     return v, xi.value, d, g
+
+
+def umbrella_integration(xi_wins, average, variance, k_force, beta, xi_min, xi_max, nbins, umbr_traj=1, umbr_step=1):
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (xi_wins, average, variance, k_force)]
+    dA, bc, pmf = np.zeros(nbins), np.zeros(nbins - 1), np.zeros(nbins)
+    rpmd().oracle_umbrella_integration(len(a[0]), *[_ptr(x) for x in a], beta, xi_min, xi_max, nbins, umbr_traj,
+                                       umbr_step, _ptr(dA), _ptr(bc), _ptr(pmf))
+    return dA, bc, pmf
+
+
+def pmf_extrema(pmf, xi_min, xi_max, minloc_zero=True):
+    pmf = np.ascontiguousarray(pmf, dtype=np.float64)
+    imax, imin = ctypes.c_int(), ctypes.c_int()
+    rpmd().oracle_pmf_extrema(_ptr(pmf), len(pmf), xi_min, xi_max, int(minloc_zero), ctypes.byref(imax),
+                              ctypes.byref(imin))
+    return imax.value, imin.value
+
+
+def rate_bimolecular(npaths, kappa, r_inf, beta, m1, m2, pmf_max, pmf_min):
+    return rpmd().oracle_rate_bimolecular(npaths, kappa, r_inf, beta, m1, m2, pmf_max, pmf_min)
 
 
 def andersen_normal(seed, traj, draw, coord, bead):
