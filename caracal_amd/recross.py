@@ -5,9 +5,10 @@ trajectories.
 Each parent follows the reference's sequence:
 
 1. mdinit (fresh forces, the umbrella bias of the centroid, momenta drawn;
-   src/mdinit.f90:79-130) and recr_equi constrained parent steps
-   (src/recross_serial.f90:118-136), checked from step 1001 on; a parent that
-   fails the check starts again from its start structure (goto 112);
+   src/mdinit.f90:79-130) and recr_equi constrained parent steps with the
+   Andersen interval int(sqrt(recr_equi)) (src/recross_serial.f90:91-136),
+   checked from step 1001 on; a parent that fails the check starts again
+   from its start structure (goto 112);
 2. child_times cycles (:156-305): from the parent's configuration,
    child_point/2 pairs of children with fresh momenta +p and -p, each run
    child_evol steps while kappa's numerator and denominator accumulate; then
@@ -46,14 +47,16 @@ class RecrossingError(RuntimeError):
 class Recrossing:
     """One recrossing calculation for nparents independent parents.
 
-    beta: scalar or (nparents,).  check: None (recross_check off) or a dict
-    with energy_tol (hartree per bead), err_max and ts_energy (hartree, the
-    surface's energy at the transition state); a parent fails its check when
-    its summed bead energy exceeds (ts_energy + energy_tol) * nbeads or its
-    positions are not finite (src/rpmd_check.f90:74-98)."""
+    beta: scalar or (nparents,).  andersen_step: the equilibration's Andersen
+    interval, int(sqrt(recr_equi)) as the reference sets it unless given.
+    check: None (recross_check off) or a dict with energy_tol (hartree per
+    bead), err_max and ts_energy (hartree, the surface's energy at the
+    transition state); a parent fails its check when its summed bead energy
+    exceeds (ts_energy + energy_tol) * nbeads or its positions are not finite
+    (src/rpmd_check.f90:74-98)."""
 
     def __init__(self, surface, nbeads, dt, masses, reaction, beta, *, child_point, child_evol, child_interv,
-                 child_times, recr_equi, andersen_step, k_force=0.0, check=None, precision=_abi.CAM_PREC_FP64,
+                 child_times, recr_equi, andersen_step=None, k_force=0.0, check=None, precision=_abi.CAM_PREC_FP64,
                  seed=0, andersen="reference", engine_factory=RPMDEngine):
         if child_point < 2 or child_point % 2:
             raise ValueError("child_point must be a positive even number (children come in +-p pairs)")
@@ -67,7 +70,9 @@ class Recrossing:
         self.masses = np.asarray(masses, dtype=np.float64)
         self.reaction, self.beta0 = reaction, beta
         self.child_point, self.child_evol, self.child_interv = child_point, child_evol, child_interv
-        self.child_times, self.recr_equi, self.andersen_step = child_times, recr_equi, andersen_step
+        # the reference resets andersen_step to int(sqrt(recr_equi)) for the equilibration
+        self.child_times, self.recr_equi = child_times, recr_equi
+        self.andersen_step = int(math.sqrt(recr_equi)) if andersen_step is None else andersen_step
         self.k_force, self.precision, self.seed, self.andersen = k_force, precision, seed, andersen
         self.check = check
         if check is not None:
