@@ -977,6 +977,7 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
 // constraint did not converge).
 // ---------------------------------------------------------------------------
 constexpr int kParentWaves = 4;              // trajectories per block
+constexpr int kSplitGraphSteps = 1000;       // steps per captured graph of a split-schedule run
 constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
 
 // Centroid of trajectory t in every lane: lane i < 3*natoms sums coordinate
@@ -2239,11 +2240,22 @@ int cam_engine_run(cam_engine *e, int nsteps)
                   "an UMBRELLA engine needs cam_engine_begin_umbrella after cam_engine_set_state");
     clear_error();
     if (nsteps == 0) return CAM_OK;
-    hipGraphExec_t g;
-    int rc = get_graph(e, nsteps, &g);
+    // A split-schedule step is three or four launches: long runs replay a
+    // graph of at most kSplitGraphSteps steps (and one of the remainder)
+    // instead of capturing every step into one graph.
+    const int chunk = e->split ? std::min(nsteps, kSplitGraphSteps) : nsteps;
+    const int reps = nsteps / chunk, tail = nsteps % chunk;
+    if (e->cfg.ensemble == CAM_ENS_CHILD) {
+        int rc = ensure_child_buffers(e, chunk, e->child_steps + nsteps);
+        if (rc != CAM_OK) return rc;
+    }
+    hipGraphExec_t g = nullptr, gt = nullptr;
+    int rc = get_graph(e, chunk, &g);
+    if (rc == CAM_OK && tail) rc = get_graph(e, tail, &gt);
     if (rc != CAM_OK) return rc;
     CAM_HIP(hipEventRecord(e->ev0, e->stream));
-    CAM_HIP(hipGraphLaunch(g, e->stream));
+    for (int r = 0; r < reps; ++r) CAM_HIP(hipGraphLaunch(g, e->stream));
+    if (tail) CAM_HIP(hipGraphLaunch(gt, e->stream));
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
     e->child_steps += nsteps;
     e->ustat_steps += nsteps;
