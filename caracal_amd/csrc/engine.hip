@@ -73,81 +73,146 @@ __device__ double calc_xi_value(const Rx &rx, const double (*wcom)[NA], const do
     return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
 }
 
+// The helpers below run on values in registers: every index into a private
+// array is a compile-time constant (loops over atoms, bonds and fragments are
+// unrolled to their maxima under run-time guards), so nothing goes to scratch
+// memory.  An atom named by the reaction is read or updated by blending over
+// all atoms with a 0/1 weight: as arithmetic, the compiler cannot fold it back
+// into an indexed access of the array (a conditional copy it can, and then
+// the array lives in scratch).  The blends are exact for finite values.
+
+// v[3*a .. 3*a+2] for a run-time atom index a.
+template <int NA>
+__device__ inline void gather3(const double *v, int a, double *out)
+{
+    out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+    for (int at = 0; at < NA; ++at) {
+        const double f = at == a ? 1.0 : 0.0;
+        out[0] = fma(f, v[3 * at], out[0]);
+        out[1] = fma(f, v[3 * at + 1], out[1]);
+        out[2] = fma(f, v[3 * at + 2], out[2]);
+    }
+}
+
+// v[3*a + d] += w[d] for a run-time atom index a.
+template <int NA>
+__device__ inline void scatter_add3(double *v, int a, const double *w)
+{
+#pragma unroll
+    for (int at = 0; at < NA; ++at) {
+        const double f = at == a ? 1.0 : 0.0;
+        v[3 * at] = fma(f, w[0], v[3 * at]);
+        v[3 * at + 1] = fma(f, w[1], v[3 * at + 1]);
+        v[3 * at + 2] = fma(f, w[2], v[3 * at + 2]);
+    }
+}
+
+// x[lane] for lane < N (0 beyond), by selects over the values.
+template <int N>
+__device__ inline double pick_lane(const double *x, int lane)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v = lane == i ? x[i] : v;
+    return v;
+}
+
+// Bit at of frag[k]: atom at belongs to reactant k.
+template <int NA>
+__device__ inline void fragment_masks(const Rx &rx, unsigned *frag)
+{
+#pragma unroll
+    for (int k = 0; k < CAM_MAX_REAC; ++k) {
+        frag[k] = 0u;
+#pragma unroll
+        for (int i = 0; i < CAM_MAX_ATOMS; ++i)
+            if (k < rx.n_reac && i < rx.reac_n[k]) frag[k] |= 1u << rx.reac_at[k][i];
+    }
+}
+
 // The two dividing surfaces of calc_xi BIMOLEC (src/calc_xi.f90:63-282) at
 // c (3*natoms, atom-major): s1 from the forming and breaking bonds, s0 from
 // the distances between the reactant centres of mass, and with GRAD their
-// gradients.  Both modes of xi combine these.
+// gradients.  Both modes of xi combine these.  The centres of mass sum their
+// atoms in atom order (the reference in list order: rounding only).
 template <int NA, bool GRAD>
 __device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, double &s0, double &s1, double *ds0,
                             double *ds1)
 {
+    if constexpr (GRAD) {
+#pragma unroll
+        for (int i = 0; i < 3 * NA; ++i) ds1[i] = ds0[i] = 0.0;
+    }
     s1 = 0.0;
-    double rb[CAM_MAX_BONDS], rf[CAM_MAX_BONDS];
-    for (int i = 0; i < rx.n_break; ++i) {
-        const int a = rx.brk[i][0], b = rx.brk[i][1];
-        const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
-        rb[i] = sqrt(dx * dx + dy * dy + dz * dz);
-        s1 = s1 + (rb[i] - rx.break_ref[i]) / (double)rx.n_break;
-    }
-    for (int i = 0; i < rx.n_form; ++i) {
-        const int a = rx.form[i][0], b = rx.form[i][1];
-        const double dx = c[3 * a] - c[3 * b], dy = c[3 * a + 1] - c[3 * b + 1], dz = c[3 * a + 2] - c[3 * b + 2];
-        rf[i] = sqrt(dx * dx + dy * dy + dz * dz);
-        s1 = s1 - (rf[i] - rx.form_ref[i]) / (double)rx.n_form;
-    }
-    double com[CAM_MAX_REAC][3];
-    for (int k = 0; k < rx.n_reac; ++k) {
-        com[k][0] = com[k][1] = com[k][2] = 0.0;
-        for (int i = 0; i < rx.reac_n[k]; ++i) {
-            const int at = rx.reac_at[k][i];
-            for (int j = 0; j < 3; ++j) com[k][j] = com[k][j] + mass[at] * c[3 * at + j] / rx.reac_mass[k];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {                  // breaking bonds (+), then forming bonds (-)
+#pragma unroll
+        for (int i = 0; i < CAM_MAX_BONDS; ++i) {
+            const int nbond = pass == 0 ? rx.n_break : rx.n_form;
+            if (i >= nbond) continue;
+            const int a = pass == 0 ? rx.brk[i][0] : rx.form[i][0], b = pass == 0 ? rx.brk[i][1] : rx.form[i][1];
+            double ca[3], cb[3], R[3];
+            gather3<NA>(c, a, ca);
+            gather3<NA>(c, b, cb);
+            #pragma unroll
+            for (int d = 0; d < 3; ++d) R[d] = ca[d] - cb[d];
+            const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+            if (pass == 0) s1 = s1 + (r - rx.break_ref[i]) / (double)nbond;
+            else s1 = s1 - (r - rx.form_ref[i]) / (double)nbond;
+            if constexpr (GRAD) {
+                const double rinv = 1.0 / r;
+                double w[3], mw[3];
+                #pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    w[d] = R[d] * rinv / (double)nbond;
+                    if (pass == 1) w[d] = -w[d];
+                    mw[d] = -w[d];
+                }
+                scatter_add3<NA>(ds1, a, w);
+                scatter_add3<NA>(ds1, b, mw);
+            }
         }
     }
+    unsigned frag[CAM_MAX_REAC];
+    fragment_masks<NA>(rx, frag);
+    double com[CAM_MAX_REAC][3];
+#pragma unroll
+    for (int k = 0; k < CAM_MAX_REAC; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int at = 0; at < NA; ++at)
+                if ((frag[k] >> at) & 1u) acc = acc + mass[at] * c[3 * at + j] / rx.reac_mass[k];
+            com[k][j] = acc;
+        }
     s0 = 0.0;
     const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
-    for (int i = 0; i < rx.n_reac; ++i)
-        for (int j = i + 1; j < rx.n_reac; ++j) {
-            const double dx = com[j][0] - com[i][0], dy = com[j][1] - com[i][1], dz = com[j][2] - com[i][2];
-            s0 = s0 + (rx.r_inf - sqrt(dx * dx + dy * dy + dz * dz));
+#pragma unroll
+    for (int i = 0; i < CAM_MAX_REAC; ++i)
+#pragma unroll
+        for (int j = i + 1; j < CAM_MAX_REAC; ++j) {
+            if (j >= rx.n_reac) continue;
+            double red[3];
+            #pragma unroll
+            for (int d = 0; d < 3; ++d) red[d] = com[j][d] - com[i][d];
+            const double r = sqrt(red[0] * red[0] + red[1] * red[1] + red[2] * red[2]);
+            s0 = s0 + (rx.r_inf - r);
+            if constexpr (GRAD) {
+                const double rinv = 1.0 / r;
+#pragma unroll
+                for (int at = 0; at < NA; ++at)
+                    #pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        if ((frag[i] >> at) & 1u)
+                            ds0[3 * at + d] = ds0[3 * at + d] + red[d] * rinv * mass[at] / rx.reac_mass[i] / (double)terms;
+                        if ((frag[j] >> at) & 1u)
+                            ds0[3 * at + d] = ds0[3 * at + d] - red[d] * rinv * mass[at] / rx.reac_mass[j] / (double)terms;
+                    }
+            }
         }
     s0 = s0 / (double)terms;
-    if constexpr (GRAD) {
-        for (int i = 0; i < 3 * NA; ++i) ds1[i] = ds0[i] = 0.0;
-        for (int i = 0; i < rx.n_form; ++i) {
-            const int a = rx.form[i][0], b = rx.form[i][1];
-            const double rinv = 1.0 / rf[i];
-            for (int d = 0; d < 3; ++d) {
-                const double r = c[3 * a + d] - c[3 * b + d];
-                ds1[3 * a + d] = ds1[3 * a + d] - r * rinv / (double)rx.n_form;
-                ds1[3 * b + d] = ds1[3 * b + d] + r * rinv / (double)rx.n_form;
-            }
-        }
-        for (int i = 0; i < rx.n_break; ++i) {
-            const int a = rx.brk[i][0], b = rx.brk[i][1];
-            const double rinv = 1.0 / rb[i];
-            for (int d = 0; d < 3; ++d) {
-                const double r = c[3 * a + d] - c[3 * b + d];
-                ds1[3 * a + d] = ds1[3 * a + d] + r * rinv / (double)rx.n_break;
-                ds1[3 * b + d] = ds1[3 * b + d] - r * rinv / (double)rx.n_break;
-            }
-        }
-        for (int i = 0; i < rx.n_reac; ++i)
-            for (int j = i + 1; j < rx.n_reac; ++j) {
-                double red[3];
-                for (int d = 0; d < 3; ++d) red[d] = com[j][d] - com[i][d];
-                const double rinv = 1.0 / sqrt(red[0] * red[0] + red[1] * red[1] + red[2] * red[2]);
-                for (int k = 0; k < rx.reac_n[i]; ++k) {
-                    const int at = rx.reac_at[i][k];
-                    for (int d = 0; d < 3; ++d)
-                        ds0[3 * at + d] = ds0[3 * at + d] + red[d] * rinv * mass[at] / rx.reac_mass[i] / (double)terms;
-                }
-                for (int k = 0; k < rx.reac_n[j]; ++k) {
-                    const int at = rx.reac_at[j][k];
-                    for (int d = 0; d < 3; ++d)
-                        ds0[3 * at + d] = ds0[3 * at + d] - red[d] * rinv * mass[at] / rx.reac_mass[j] / (double)terms;
-                }
-            }
-    }
 }
 
 // xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:209-211,
@@ -163,6 +228,7 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
     }
     double ds0[3 * NA], ds1[3 * NA];
     xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
+#pragma unroll
     for (int i = 0; i < 3 * NA; ++i) dxi[i] = rx.xi_ideal * ds1[i] + (1.0 - rx.xi_ideal) * ds0[i];
     return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
 }
@@ -177,58 +243,76 @@ template <int NA>
 __device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const double *c, const double *v, double *h0v,
                                     double *h1v)
 {
+#pragma unroll
     for (int i = 0; i < 3 * NA; ++i) h0v[i] = h1v[i] = 0.0;
     auto dist_block = [](const double *R, const double *u, double sg, double *w) {
         const double rr = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
         const double rinv = 1.0 / sqrt(rr);
         const double r3 = rinv * rinv * rinv;
         const double ru = R[0] * u[0] + R[1] * u[1] + R[2] * u[2];
+        #pragma unroll
         for (int d = 0; d < 3; ++d) w[d] = sg * (rr * u[d] - R[d] * ru) * r3;
     };
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-        const int nbond = pass == 0 ? rx.n_form : rx.n_break;
-        const double sg = (pass == 0 ? -1.0 : 1.0) / (double)nbond;   // forming bonds enter s1 with -|R|
-        for (int i = 0; i < nbond; ++i) {
+#pragma unroll
+        for (int i = 0; i < CAM_MAX_BONDS; ++i) {
+            const int nbond = pass == 0 ? rx.n_form : rx.n_break;
+            if (i >= nbond) continue;
+            const double sg = (pass == 0 ? -1.0 : 1.0) / (double)nbond;   // forming bonds enter s1 with -|R|
             const int a = pass == 0 ? rx.form[i][0] : rx.brk[i][0], b = pass == 0 ? rx.form[i][1] : rx.brk[i][1];
-            double R[3], u[3], w[3];
+            double ca[3], cb[3], va[3], vb[3], R[3], u[3], w[3], mw[3];
+            gather3<NA>(c, a, ca);
+            gather3<NA>(c, b, cb);
+            gather3<NA>(v, a, va);
+            gather3<NA>(v, b, vb);
+            #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                R[d] = c[3 * a + d] - c[3 * b + d];
-                u[d] = v[3 * a + d] - v[3 * b + d];
+                R[d] = ca[d] - cb[d];
+                u[d] = va[d] - vb[d];
             }
             dist_block(R, u, sg, w);
-            for (int d = 0; d < 3; ++d) {
-                h1v[3 * a + d] += w[d];
-                h1v[3 * b + d] -= w[d];
-            }
+            #pragma unroll
+            for (int d = 0; d < 3; ++d) mw[d] = -w[d];
+            scatter_add3<NA>(h1v, a, w);
+            scatter_add3<NA>(h1v, b, mw);
         }
     }
+    unsigned frag[CAM_MAX_REAC];
+    fragment_masks<NA>(rx, frag);
     double com[CAM_MAX_REAC][3], vcom[CAM_MAX_REAC][3];
-    for (int k = 0; k < rx.n_reac; ++k)
+#pragma unroll
+    for (int k = 0; k < CAM_MAX_REAC; ++k)
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             com[k][d] = vcom[k][d] = 0.0;
-            for (int i = 0; i < rx.reac_n[k]; ++i) {
-                const int at = rx.reac_at[k][i];
-                com[k][d] += mass[at] * c[3 * at + d] / rx.reac_mass[k];
-                vcom[k][d] += mass[at] * v[3 * at + d] / rx.reac_mass[k];
-            }
+#pragma unroll
+            for (int at = 0; at < NA; ++at)
+                if ((frag[k] >> at) & 1u) {
+                    com[k][d] += mass[at] * c[3 * at + d] / rx.reac_mass[k];
+                    vcom[k][d] += mass[at] * v[3 * at + d] / rx.reac_mass[k];
+                }
         }
     const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
-    for (int i = 0; i < rx.n_reac; ++i)
-        for (int j = i + 1; j < rx.n_reac; ++j) {
+#pragma unroll
+    for (int i = 0; i < CAM_MAX_REAC; ++i)
+#pragma unroll
+        for (int j = i + 1; j < CAM_MAX_REAC; ++j) {
+            if (j >= rx.n_reac) continue;
             double R[3], u[3], w[3];
+            #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R[d] = com[j][d] - com[i][d];
                 u[d] = vcom[i][d] - vcom[j][d];
             }
             dist_block(R, u, -1.0 / (double)terms, w);                 // s0 = r_inf - |R|
-            for (int k = 0; k < rx.reac_n[i]; ++k) {
-                const int at = rx.reac_at[i][k];
-                for (int d = 0; d < 3; ++d) h0v[3 * at + d] += mass[at] / rx.reac_mass[i] * w[d];
-            }
-            for (int k = 0; k < rx.reac_n[j]; ++k) {
-                const int at = rx.reac_at[j][k];
-                for (int d = 0; d < 3; ++d) h0v[3 * at + d] -= mass[at] / rx.reac_mass[j] * w[d];
-            }
+#pragma unroll
+            for (int at = 0; at < NA; ++at)
+                #pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if ((frag[i] >> at) & 1u) h0v[3 * at + d] += mass[at] / rx.reac_mass[i] * w[d];
+                    if ((frag[j] >> at) & 1u) h0v[3 * at + d] -= mass[at] / rx.reac_mass[j] * w[d];
+                }
         }
 }
 
@@ -251,12 +335,17 @@ __device__ double umbrella_bias(const Rx &rx, const double *mass, const double *
     const double D = s0 - s1;
     const double xi = s0 / D;
     xi1 = xi;
+    #pragma unroll
     for (int i = 0; i < N; ++i) dxi[i] = (s0 * ds1[i] - s1 * ds0[i]) / (D * D);
     const double delta = xi - xi_ideal;
+    #pragma unroll
     for (int i = 0; i < N; ++i) gk[i] = k_force * delta * dxi[i];
     double v[N], fs2 = 0.0;
+    #pragma unroll
     for (int i = 0; i < N; ++i) v[i] = dxi[i] / mass[i / 3];
+    #pragma unroll
     for (int d = 0; d < 3; ++d)
+        #pragma unroll
         for (int at = 0; at < NA; ++at) fs2 = fs2 + dxi[3 * at + d] * dxi[3 * at + d] / mass[at];
     const double coeff1 = 2.0 * 3.1415926535897932384 * beta, coeff2 = -1.0 / beta;
     fs2 = fs2 / coeff1;
@@ -264,12 +353,14 @@ __device__ double umbrella_bias(const Rx &rx, const double *mass, const double *
     double h0v[N], h1v[N];
     xi_surface_hess_vec<NA>(rx, mass, c, v, h0v, h1v);
     double ds0v = 0.0, ds1v = 0.0;
+    #pragma unroll
     for (int i = 0; i < N; ++i) {
         ds0v += ds0[i] * v[i];
         ds1v += ds1[i] * v[i];
     }
     const double cross = s0 * ds1v - s1 * ds0v;
     const double scale = coeff2 / (coeff1 * fs2);
+    #pragma unroll
     for (int i = 0; i < N; ++i)
         gh[i] = ((s0 * h1v[i] + ds0[i] * ds1v - ds1[i] * ds0v - s1 * h0v[i]) * D - 2.0 * cross * (ds0[i] - ds1[i])) /
                 (D * D * D) * scale;
@@ -911,6 +1002,7 @@ __device__ void centroid_of(const TrajArgs &a, int t, double *c)
 {
     const int NB = a.nbeads, NC = 3 * a.natoms;
     const double *q = a.q + (size_t)t * NC * NB;
+    #pragma unroll
     for (int i = 0; i < NC; ++i) {
         double s = 0.0;
         for (int k = 0; k < NB; ++k) s = s + q[(size_t)i * NB + k];
@@ -933,9 +1025,11 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
     int good = 0;
     for (int iter = 1; iter <= 200; ++iter) {
         coeff = mult * a.dt * a.dt / NB;
+        #pragma unroll
         for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dxi_act[i] / a.mass[i / 3];
         const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
         double dsigma = 0.0;
+        #pragma unroll
         for (int i = 0; i < NC; ++i) dsigma = dsigma + dxn[i] * a.dt * a.dt * dxi_act[i] / (a.mass[i / 3] * NB);
         const double dx = sigma / dsigma;
         mult = mult - dx;
@@ -951,6 +1045,7 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
     double *q = a.q + (size_t)t * NC * NB, *p = a.p + (size_t)t * NC * NB;
     // note: the shift uses the coefficient of the last iteration, computed
     // before its multiplier update, as the reference does
+    #pragma unroll
     for (int i = 0; i < NC; ++i)
         for (int k = 0; k < NB; ++k) {
             q[(size_t)i * NB + k] = q[(size_t)i * NB + k] + coeff / a.mass[i / 3] * dxi_act[i];
@@ -958,10 +1053,13 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
         }
     if (!project_p) return;
     double c1 = 0.0, c2 = 0.0;
+    #pragma unroll
     for (int i = 0; i < NC; ++i)
         for (int k = 0; k < NB; ++k) c1 = c1 + dxi_act[i] * p[(size_t)i * NB + k] / a.mass[i / 3];
+    #pragma unroll
     for (int i = 0; i < NC; ++i) c2 = c2 + dxi_act[i] * dxi_act[i] / a.mass[i / 3];
     const double lam = -c1 / c2 / NB;
+    #pragma unroll
     for (int i = 0; i < NC; ++i)
         for (int k = 0; k < NB; ++k) p[(size_t)i * NB + k] = p[(size_t)i * NB + k] + lam * dxi_act[i];
 }
@@ -1031,10 +1129,10 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_begin_kernel(TrajArg
     (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], k_force, a.rx.xi_ideal, xi1, dxi, gk, gh);
     double *gt = g + (size_t)t * NC * NB;
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
+    for (int i = 0; i < NC; ++i)
         for (int k = lane; k < NB; k += 64) gt[(size_t)i * NB + k] = (gt[(size_t)i * NB + k] + gk[i]) + gh[i];
-        if (lane == i) pdxi[(size_t)t * NC + i] = dxi[i];
-    }
+    const double mine = pick_lane<NC>(dxi, lane);
+    if (lane < NC) pdxi[(size_t)t * NC + lane] = mine;
     if (lane == 0) pfail[t] = 0;
 }
 
@@ -1065,7 +1163,9 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(Tra
         for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dprev[i] / a.mass[i / 3];
         const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
         double dsigma = 0.0;
+        #pragma unroll
         for (int d = 0; d < 3; ++d)
+            #pragma unroll
             for (int at = 0; at < NA; ++at)
                 dsigma = dsigma + dxn[3 * at + d] * a.dt * a.dt * dprev[3 * at + d] / (a.mass[at] * NB);
         const double dx = sigma / dsigma;
@@ -1089,9 +1189,8 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(Tra
     if (lane == 0) pfail[t] = good ? (pfail[t] & ~kPfThisStep) : (pfail[t] | kPfConstraint | kPfThisStep);
     double dnew[NC];
     (void)calc_xi<NA>(a.rx, a.mass, c, dnew);
-#pragma unroll
-    for (int i = 0; i < NC; ++i)
-        if (lane == i) pdxi[(size_t)t * NC + i] = dnew[i];
+    const double mine = pick_lane<NC>(dnew, lane);
+    if (lane < NC) pdxi[(size_t)t * NC + lane] = mine;
 }
 
 // The end of a parent step (src/verlet.f90:1060-1083, :1159-1165): the
@@ -1132,7 +1231,9 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_end_kernel(TrajArgs 
             c1 = c1 + dxi[i] * pn / a.mass[i / 3];
         }
     for (int off = 32; off > 0; off >>= 1) c1 += __shfl_xor(c1, off, 64);
+    #pragma unroll
     for (int d = 0; d < 3; ++d)
+        #pragma unroll
         for (int at = 0; at < NA; ++at) c2 = c2 + dxi[3 * at + d] * dxi[3 * at + d] / a.mass[at];
     const double lam = -c1 / c2 / NB;
     const uint64_t istep = step_base[1] + (uint64_t)step_off + 1;
