@@ -24,6 +24,15 @@
 namespace cam {
 namespace {
 
+// Recrossing parents and umbrella windows: trajectories per block of their
+// per-trajectory kernels, steps per captured graph of a split-schedule run,
+// and the bits of the per-trajectory flags (a constraint did not converge
+// since the start; the trajectory check failed; this step's constraint did
+// not converge).
+constexpr int kParentWaves = 4;
+constexpr int kSplitGraphSteps = 1000;
+constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
+
 // xi of the centroid without its gradient, for the per-step child flags
 // (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213).  One lane evaluates it
 // while its block waits, so everything it reads is in LDS or registers:
@@ -456,9 +465,11 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
 }
 
 // ---------------------------------------------------------------------------
-// The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN or CAM_ENS_CHILD.
-// One step: p -= dt/2 g; free ring-polymer propagation in normal-mode space;
-// [CHILD: xi of the centroid]; forces; p -= dt/2 g; [ANDERSEN: resample].
+// The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD or
+// CAM_ENS_UMBRELLA.  One step: p -= dt/2 g; free ring-polymer propagation in
+// normal-mode space; [CHILD: xi of the centroid]; forces; [UMBRELLA: the
+// window's bias at the centroid, added to every bead]; p -= dt/2 g;
+// [ANDERSEN, UMBRELLA: resample].
 //
 // SPLIT: the same step without its force phase, one step per launch, for the
 // split schedule (forces in force_kernel at its own register budget).  The
@@ -488,6 +499,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     __shared__ double smass[TAB ? S::NA : 1];
     __shared__ double stash[STASH ? 2 * SL * kBlock : 1];
     __shared__ double wcom[ENS == CAM_ENS_CHILD ? CAM_MAX_REAC : 1][S::NA];
+    // UMBRELLA: the two parts of each trajectory's bias gradient, its bead
+    // energies and a flag for positions that are not finite
+    constexpr bool UMB = ENS == CAM_ENS_UMBRELLA;
+    constexpr bool CENT = ENS == CAM_ENS_CHILD || UMB;
+    __shared__ double ubias[UMB ? TPB * 2 * NC : 1];
+    __shared__ double uvb[UMB ? TPB * NB : 1];
+    __shared__ int ubad[UMB ? TPB : 1];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -536,6 +554,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     const double hdt = 0.5 * a.dt;
     double vb = 0.0;
+    double usum = 0.0, usq = 0.0;            // UMBRELLA: this launch's sums of xi, xi^2
+    bool ucheck = false;                     // UMBRELLA: the window check failed in this launch
     if constexpr (ENS == CAM_ENS_CHILD) {
         // fragment weights of the centre-of-mass sums (calc_com, src/calc_com.f90:36-59)
         if (threadIdx.x < CAM_MAX_REAC * S::NA) {
@@ -575,6 +595,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
     }
 
+    if constexpr (UMB) {
+        for (int i = tid; i < TPB; i += kBlock) ubad[i] = 0;
+    }
     if constexpr (SPLIT) {
         if (a.step_off > 0) {
             // second half kick of the previous step (src/verlet.f90:1060-1062) and its Andersen draw
@@ -617,7 +640,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     xm[(2 * cc[s] + 1) * nm_stride<NB>() + k] = q[s];
                 }
             __syncthreads();
-            nm_mfma<NB, TPB, NC, ENS == CAM_ENS_CHILD>(X, Y, prop, a.tfwd, a.tinv, cent);
+            nm_mfma<NB, TPB, NC, CENT>(X, Y, prop, a.tfwd, a.tinv, cent);
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
                 p[s] = xm[(2 * cc[s]) * nm_stride<NB>() + k];
@@ -672,7 +695,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     yp[s] = pn;
                 }
             }
-            if constexpr (ENS == CAM_ENS_CHILD) {
+            if constexpr (CENT) {
                 // centroid of the propagated beads = mode 0 of q / sqrt(NB)
                 if (k == 0) {
 #pragma unroll
@@ -735,6 +758,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
         }
 
+        if constexpr (UMB && NB == 1) {
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < SL; ++s)
+                if (S::valid(s, r)) cent[tl * NC + cc[s]] = q[s];
+        }
         if constexpr (SPLIT) continue;     // one step per launch; forces come next
         // forces of every bead (src/verlet.f90:772-777)
         {
@@ -757,10 +786,63 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 }
             }
         }
+        if constexpr (UMB) {
+            // umbrella mode 0 at the centroid of this step's positions
+            // (src/verlet.f90:1046, src/umbrella.f90:109-173): one lane per
+            // trajectory evaluates it; every bead adds it, the force-constant
+            // part first.  That lane also sums xi and runs the window check
+            // (rpmd_check traj_type 2/3, src/rpmd_check.f90:74-112).
+            bool qbad = false;
+#pragma unroll
+            for (int s = 0; s < SL; ++s) qbad = qbad || (S::valid(s, r) && (q[s] != q[s] || q[s] > DBL_MAX));
+            if (r == 0) uvb[tl * NB + k] = vb;
+            if (qbad) ubad[tl] = 1;
+            __syncthreads();
+            if (k == 0 && r == 0) {
+                double dxi[NC], gk[NC], gh[NC], xi1;
+                const double xw = a.uwin[2 * traj], kw = a.uwin[2 * traj + 1];
+                (void)umbrella_bias<S::NA>(a.rx, a.mass, cent + tl * NC, beta, kw, xw, xi1, dxi, gk, gh);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    ubias[(2 * tl) * NC + i] = gk[i];
+                    ubias[(2 * tl + 1) * NC + i] = gh[i];
+                }
+                usum += xi1;
+                usq += xi1 * xi1;
+                const uint64_t istep = a.step_base[1] + (uint64_t)(a.step_off + it) + 1;
+                if (a.check_after >= 0 && istep > (uint64_t)a.check_after) {
+                    double ep = 0.0;
+                    for (int kk = 0; kk < NB; ++kk) ep = ep + uvb[tl * NB + kk];
+                    const bool bad = ubad[tl] != 0 || ep != ep || ep > DBL_MAX || ep > a.epot_max ||
+                                     fabs(xi1 - xw) > a.xi_tol;
+                    if (bad) ucheck = true;
+                }
+                ubad[tl] = 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < SL; ++s)
+                if (S::valid(s, r)) g[s] = (g[s] + ubias[(2 * tl) * NC + cc[s]]) + ubias[(2 * tl + 1) * NC + cc[s]];
+        }
         // second half kick (src/verlet.f90:1060-1062)
 #pragma unroll
         for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
 
+        if constexpr (UMB) {
+            // Andersen every andersen_step steps counted from the segment's
+            // start, the draw keyed by the steps since the engine's creation
+            if (a.andersen_step > 0) {
+                const uint64_t istep = a.step_base[1] + (uint64_t)(a.step_off + it) + 1;
+                const uint64_t key = a.step_base[0] + (uint64_t)(a.step_off + it) + 1;
+                if (istep % (uint64_t)a.andersen_step == 0) {
+#pragma unroll
+                    for (int s = 0; s < SL; ++s) {
+                        const double sd = sqrt((TAB ? smass[cc[s] / 3] : m[s]) / (beta / NB));
+                        p[s] = andersen_normal(a.seed, (uint32_t)traj, key, cc[s], k) * sd;
+                    }
+                }
+            }
+        }
         if constexpr (ENS == CAM_ENS_ANDERSEN) {
             // Andersen: all bead momenta redrawn every andersen_step steps
             // (src/verlet.f90:1159-1165, src/andersen.f90:37-74)
@@ -784,6 +866,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 if (!SPLIT) a.g[o] = g[s];
             }
         if (r == 0 && !SPLIT) a.vbead[(size_t)traj * NB + k] = vb;
+        if constexpr (UMB) {
+            if (k == 0 && r == 0) {
+                a.ustat[2 * traj] = a.ustat[2 * traj] + usum;
+                a.ustat[2 * traj + 1] = a.ustat[2 * traj + 1] + usq;
+                if (ucheck) a.pfail[traj] = a.pfail[traj] | kPfCheck;
+            }
+        }
     }
 }
 
@@ -1074,9 +1163,6 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
 // (the per-step trajectory check failed) and kPfThisStep (this step's
 // constraint did not converge).
 // ---------------------------------------------------------------------------
-constexpr int kParentWaves = 4;              // trajectories per block
-constexpr int kSplitGraphSteps = 1000;       // steps per captured graph of a split-schedule run
-constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
 
 // Centroid of trajectory t in every lane: lane i < 3*natoms sums coordinate
 // i over the beads in bead order (get_centroid, src/get_centroid.f90:66-85).
@@ -1539,6 +1625,9 @@ template <class S, int NB, typename T, int WPE, bool SPLIT = false>
 step_fn pick_step_w(int ens, bool mfma)
 {
     constexpr bool MF = NB >= 16 && NB % 16 == 0;
+    // umbrella windows: the fused step only, with the vector-ALU transform
+    if constexpr (!SPLIT && WPE == 1)
+        if (ens == CAM_ENS_UMBRELLA) return step_kernel<S, NB, T, CAM_ENS_UMBRELLA, 1, false, false>;
     if (mfma && MF) {
         if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, MF, SPLIT>;
         if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, MF, SPLIT>;
@@ -1556,7 +1645,7 @@ template <class S, int NB, typename T>
 step_fn pick_step(int ens, int wpe, bool mfma)
 {
     if constexpr (S::L == 4 && std::is_same<T, double>::value)
-        if (wpe == 2) return pick_step_w<S, NB, T, 2>(ens, mfma);
+        if (wpe == 2 && ens != CAM_ENS_UMBRELLA) return pick_step_w<S, NB, T, 2>(ens, mfma);
     return pick_step_w<S, NB, T, 1>(ens, mfma);
 }
 
@@ -1570,14 +1659,16 @@ force_fn pick_force_split()
 }
 
 template <class S, typename T>
-bool pick_nb(int nb, int ens, int wpe, bool mfma, step_fn &st, force_fn &fo, step_fn &nm, force_fn &fs)
+// ens: the fused step's ensemble; ens_nm: that of the split schedule's
+// normal-mode kernel.
+bool pick_nb(int nb, int ens, int ens_nm, int wpe, bool mfma, step_fn &st, force_fn &fo, step_fn &nm, force_fn &fs)
 {
     switch (nb) {
 #define CAM_NB_CASE(N)                                                   \
     case N:                                                              \
         st = pick_step<S, N, T>(ens, wpe, mfma);                         \
         fo = force_kernel<S, N, T, 1>;                                   \
-        nm = pick_step_w<S, N, double, 1, true>(ens, mfma);              \
+        nm = pick_step_w<S, N, double, 1, true>(ens_nm, mfma);           \
         fs = pick_force_split<S, N, T>();                                \
         return true;
         CAM_NB_CASE(1)
@@ -1660,17 +1751,22 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
 {
     const cam_config *cfg = &e->cfg;
     const int nb = cfg->nbeads;
-    // a parent's or window's normal-mode kernel is the NVE one: its
-    // constraint or bias and the end of its step are kernels of their own
-    const int ens = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA ? CAM_ENS_NVE : cfg->ensemble;
+    // a parent's or window's split-schedule normal-mode kernel is the NVE
+    // one: its constraint or bias and the end of its step are kernels of
+    // their own.  Windows also have a fused step; parents do not.
+    const bool steered = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA;
+    const int ens_nm = steered ? CAM_ENS_NVE : cfg->ensemble;
+    const int ens = cfg->ensemble == CAM_ENS_PARENT ? CAM_ENS_NVE : cfg->ensemble;
     const bool mfma = nmk == CAM_NMK_MFMA;
     bool ok;
     if (cfg->surface == CAM_SURFACE_H3)
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfH3, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
-                                             : pick_nb<SurfH3, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+        ok = cfg->precision == CAM_PREC_FP64
+                 ? pick_nb<SurfH3, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                 : pick_nb<SurfH3, float>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
     else
-        ok = cfg->precision == CAM_PREC_FP64 ? pick_nb<SurfCH4H, double>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
-                                             : pick_nb<SurfCH4H, float>(nb, ens, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+        ok = cfg->precision == CAM_PREC_FP64
+                 ? pick_nb<SurfCH4H, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
+                 : pick_nb<SurfCH4H, float>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
     k.step_tpb = e->tpb;
     if (ok && nmk == CAM_NMK_FOLDED) {
         int tpb = 0;
@@ -1760,6 +1856,10 @@ StepArgs step_args(cam_engine *e)
     a.pdxi = e->pdxi;
     a.pfail = e->pfail;
     a.ustat = e->ustat;
+    a.uwin = e->uwin;
+    a.epot_max = e->pcheck_max;
+    a.xi_tol = e->pcheck_xi_tol;
+    a.check_after = e->pcheck_after;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
     a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
@@ -1811,7 +1911,7 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
         }
         return err;
     }
-    if (e->cfg.ensemble == CAM_ENS_UMBRELLA) {
+    if (e->cfg.ensemble == CAM_ENS_UMBRELLA && split) {
         // per step: kick and free ring polymer; forces; bias, sums, second half kick, Andersen
         TrajArgs ta = traj_args(e);
         ta.q = a.q;
@@ -1928,9 +2028,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_ANDERSEN || cfg->andersen_step >= 1, "andersen_step must be >= 1");
     CAM_CHECK_ARG(!steered || cfg->andersen_step >= 0, "andersen_step must be >= 0");
-    CAM_CHECK_ARG(!steered || cfg->schedule != CAM_SCHED_FUSED,
-                  "ensembles PARENT and UMBRELLA run on the split schedule (their constraint or bias sits between "
-                  "the force kernel and the second half kick)");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->schedule != CAM_SCHED_FUSED,
+                  "ensemble PARENT runs on the split schedule (its constraint sits between propagation and forces)");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
     CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || steered)),
                   "remove_com applies to dynamic.x trajectories (constrain <= 0) only");
@@ -1984,7 +2083,10 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     else e->nmk = fold_available(cfg) ? CAM_NMK_FOLDED : CAM_NMK_VALU;
     e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
                      e->nmk != CAM_NMK_FOLDED ? 2 : 1;
-    e->split = cfg->schedule == CAM_SCHED_SPLIT || steered;
+    // windows run fused unless split is asked for or MFMA normal modes are
+    // (their fused step has the vector-ALU transform only)
+    e->split = cfg->schedule == CAM_SCHED_SPLIT || cfg->ensemble == CAM_ENS_PARENT ||
+               (cfg->ensemble == CAM_ENS_UMBRELLA && e->nmk == CAM_NMK_MFMA);
     const bool ok = select_kernels(e.get(), e->wpe, e->nmk, e->kern);
     if (!ok) {
         set_error("nbeads %d not supported", nb);
@@ -2465,16 +2567,17 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     struct Trial { int nmk, wpe, spl, split; double us; };
     std::vector<Trial> cand, trials;
     const bool parent = c.ensemble == CAM_ENS_PARENT, window = c.ensemble == CAM_ENS_UMBRELLA;
-    const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent && !window;
+    const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent;
     const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
     for (int nmk : nmks) {
         if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
             for (int spl : spls) cand.push_back({ nmk, 1, spl, 0, 0.0 });
             continue;
         }
-        if (try_fused)
+        if (try_fused && !(window && nmk != CAM_NMK_VALU))      // a window's fused step: VALU, one budget
             for (int wpe : wpes)
-                for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
+                if (!window || wpe == 1)
+                    for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
         if (try_split) cand.push_back({ nmk, 1, 1, 1, 0.0 });
     }
     auto fail = [&](hipError_t err) {

@@ -63,6 +63,9 @@ struct StepArgs {
     double *pdxi;                // PARENT: constraint direction of each trajectory [traj][3*natoms]
     int *pfail;                  // PARENT, UMBRELLA: trajectory flags
     double *ustat;               // UMBRELLA: running sums of xi and xi^2 [traj][2]
+    const double *uwin;          // UMBRELLA: (xi_ideal, k_force) of each trajectory's window
+    double epot_max, xi_tol;     // PARENT, UMBRELLA: trajectory check
+    int check_after;             //   armed after this many steps (never if < 0)
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
     int andersen_step;

@@ -94,8 +94,11 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
  *              bead's gradient, with each trajectory's own window
  *              (cam_engine_set_windows); the running sums of xi (mode 1)
  *              give each trajectory's average and variance
- *              (cam_engine_umbrella_stats).  Andersen as PARENT.  Runs on the
- *              split schedule; start it with cam_engine_begin_umbrella. */
+ *              (cam_engine_umbrella_stats).  Andersen as PARENT.  The fused
+ *              schedule evaluates the bias inside the step kernel (one lane
+ *              per trajectory, vector-ALU normal modes); the split schedule
+ *              (or nm_kernel MFMA) in a kernel of its own after the force
+ *              kernel.  Start it with cam_engine_begin_umbrella. */
 enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,

@@ -28,14 +28,16 @@ def _windows(surface, nb, nwin, seed):
     return w, q, p, beta, rx, xi_w, k_w
 
 
-@pytest.mark.parametrize("surface,nb,nmk", [(cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU),
-                                            (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA),
-                                            (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU)])
-def test_umbrella_windows_match_oracle(surface, nb, nmk):
+@pytest.mark.parametrize("surface,nb,nmk,schedule", [
+    (cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),      # fused: bias inside the step kernel
+    (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA, cam.CAM_SCHED_AUTO),     # MFMA normal modes: split schedule
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT)])
+def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule):
     nwin, ast, seed, n_equi, n_samp = 6, 5, 21, 20, 30
     w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb)
     e = cam.RPMDEngine(surface, nb, nwin, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
-                       andersen_step=ast, seed=seed, nm_kernel=nmk)
+                       andersen_step=ast, seed=seed, nm_kernel=nmk, schedule=schedule, steps_per_launch=7)
     e.set_state(q, p, beta)
     e.set_windows(xi_w, k_w)
     e.begin_umbrella(bias=True)
@@ -97,7 +99,6 @@ def test_umbrella_window_check_and_errors():
     e.run(1)
     assert (e.observables()["flags"] == 4).all()
     e.close()
-    for kw, msg in ((dict(schedule=cam.CAM_SCHED_FUSED), "split schedule"), (dict(remove_com=True), "remove_com")):
-        with pytest.raises(cam.CamError, match=msg):
-            cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA,
-                           reaction=rx, **kw)
+    with pytest.raises(cam.CamError, match="remove_com"):
+        cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
+                       remove_com=True)
