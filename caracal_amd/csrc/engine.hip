@@ -127,27 +127,16 @@ __device__ inline double pick_lane(const double *x, int lane)
     return v;
 }
 
-// Bit at of frag[k]: atom at belongs to reactant k.
-template <int NA>
-__device__ inline void fragment_masks(const Rx &rx, unsigned *frag)
-{
-#pragma unroll
-    for (int k = 0; k < CAM_MAX_REAC; ++k) {
-        frag[k] = 0u;
-#pragma unroll
-        for (int i = 0; i < CAM_MAX_ATOMS; ++i)
-            if (k < rx.n_reac && i < rx.reac_n[k]) frag[k] |= 1u << rx.reac_at[k][i];
-    }
-}
-
 // The two dividing surfaces of calc_xi BIMOLEC (src/calc_xi.f90:63-282) at
 // c (3*natoms, atom-major): s1 from the forming and breaking bonds, s0 from
 // the distances between the reactant centres of mass, and with GRAD their
-// gradients.  Both modes of xi combine these.  The centres of mass sum their
-// atoms in atom order (the reference in list order: rounding only).
+// gradients.  Both modes of xi combine these.  Divisions by bond counts,
+// term counts and reactant masses are multiplications by the reciprocals and
+// fragment weights the reaction carries, and the centres of mass sum over
+// all atoms with those weights (rounding differences from the reference's
+// sums only).
 template <int NA, bool GRAD>
-__device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, double &s0, double &s1, double *ds0,
-                            double *ds1)
+__device__ void xi_surfaces(const Rx &rx, const double *c, double &s0, double &s1, double *ds0, double *ds1)
 {
     if constexpr (GRAD) {
 #pragma unroll
@@ -161,21 +150,21 @@ __device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, d
             const int nbond = pass == 0 ? rx.n_break : rx.n_form;
             if (i >= nbond) continue;
             const int a = pass == 0 ? rx.brk[i][0] : rx.form[i][0], b = pass == 0 ? rx.brk[i][1] : rx.form[i][1];
+            const double inv_n = pass == 0 ? rx.inv_nbreak : rx.inv_nform;
             double ca[3], cb[3], R[3];
             gather3<NA>(c, a, ca);
             gather3<NA>(c, b, cb);
-            #pragma unroll
+#pragma unroll
             for (int d = 0; d < 3; ++d) R[d] = ca[d] - cb[d];
             const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
-            if (pass == 0) s1 = s1 + (r - rx.break_ref[i]) / (double)nbond;
-            else s1 = s1 - (r - rx.form_ref[i]) / (double)nbond;
+            if (pass == 0) s1 = s1 + (r - rx.break_ref[i]) * inv_n;
+            else s1 = s1 - (r - rx.form_ref[i]) * inv_n;
             if constexpr (GRAD) {
-                const double rinv = 1.0 / r;
+                const double f = (pass == 0 ? 1.0 : -1.0) * inv_n / r;
                 double w[3], mw[3];
-                #pragma unroll
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    w[d] = R[d] * rinv / (double)nbond;
-                    if (pass == 1) w[d] = -w[d];
+                    w[d] = R[d] * f;
                     mw[d] = -w[d];
                 }
                 scatter_add3<NA>(ds1, a, w);
@@ -183,8 +172,6 @@ __device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, d
             }
         }
     }
-    unsigned frag[CAM_MAX_REAC];
-    fragment_masks<NA>(rx, frag);
     double com[CAM_MAX_REAC][3];
 #pragma unroll
     for (int k = 0; k < CAM_MAX_REAC; ++k)
@@ -192,51 +179,46 @@ __device__ void xi_surfaces(const Rx &rx, const double *mass, const double *c, d
         for (int j = 0; j < 3; ++j) {
             double acc = 0.0;
 #pragma unroll
-            for (int at = 0; at < NA; ++at)
-                if ((frag[k] >> at) & 1u) acc = acc + mass[at] * c[3 * at + j] / rx.reac_mass[k];
+            for (int at = 0; at < NA; ++at) acc = fma(rx.fw[k][at], c[3 * at + j], acc);
             com[k][j] = acc;
         }
     s0 = 0.0;
-    const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
 #pragma unroll
     for (int i = 0; i < CAM_MAX_REAC; ++i)
 #pragma unroll
         for (int j = i + 1; j < CAM_MAX_REAC; ++j) {
             if (j >= rx.n_reac) continue;
             double red[3];
-            #pragma unroll
+#pragma unroll
             for (int d = 0; d < 3; ++d) red[d] = com[j][d] - com[i][d];
             const double r = sqrt(red[0] * red[0] + red[1] * red[1] + red[2] * red[2]);
             s0 = s0 + (rx.r_inf - r);
             if constexpr (GRAD) {
-                const double rinv = 1.0 / r;
+                const double f = rx.inv_terms / r;
 #pragma unroll
-                for (int at = 0; at < NA; ++at)
-                    #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        if ((frag[i] >> at) & 1u)
-                            ds0[3 * at + d] = ds0[3 * at + d] + red[d] * rinv * mass[at] / rx.reac_mass[i] / (double)terms;
-                        if ((frag[j] >> at) & 1u)
-                            ds0[3 * at + d] = ds0[3 * at + d] - red[d] * rinv * mass[at] / rx.reac_mass[j] / (double)terms;
-                    }
+                for (int at = 0; at < NA; ++at) {
+                    const double wij = rx.fw[i][at] - rx.fw[j][at];       // an atom is in one reactant at most
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) ds0[3 * at + d] = fma(red[d] * f, wij, ds0[3 * at + d]);
+                }
             }
         }
-    s0 = s0 / (double)terms;
+    s0 = s0 * rx.inv_terms;
 }
 
 // xi of the centroid (calc_xi BIMOLEC mode 2, src/calc_xi.f90:209-211,
 // :280-281), c: centroid (3*natoms, atom-major).  Optionally the gradient
 // dxi (3*natoms).
 template <int NA>
-__device__ double calc_xi(const Rx &rx, const double *mass, const double *c, double *dxi)
+__device__ double calc_xi(const Rx &rx, const double *c, double *dxi)
 {
     double s0, s1;
     if (!dxi) {
-        xi_surfaces<NA, false>(rx, mass, c, s0, s1, nullptr, nullptr);
+        xi_surfaces<NA, false>(rx, c, s0, s1, nullptr, nullptr);
         return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
     }
     double ds0[3 * NA], ds1[3 * NA];
-    xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
+    xi_surfaces<NA, true>(rx, c, s0, s1, ds0, ds1);
 #pragma unroll
     for (int i = 0; i < 3 * NA; ++i) dxi[i] = rx.xi_ideal * ds1[i] + (1.0 - rx.xi_ideal) * ds0[i];
     return rx.xi_ideal * s1 + (1.0 - rx.xi_ideal) * s0;
@@ -249,18 +231,17 @@ __device__ double calc_xi(const Rx &rx, const double *mass, const double *c, dou
 // fragment centres of mass (v weighted as the centre of mass weights
 // positions).
 template <int NA>
-__device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const double *c, const double *v, double *h0v,
-                                    double *h1v)
+__device__ void xi_surface_hess_vec(const Rx &rx, const double *c, const double *v, double *h0v, double *h1v)
 {
 #pragma unroll
     for (int i = 0; i < 3 * NA; ++i) h0v[i] = h1v[i] = 0.0;
     auto dist_block = [](const double *R, const double *u, double sg, double *w) {
         const double rr = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
         const double rinv = 1.0 / sqrt(rr);
-        const double r3 = rinv * rinv * rinv;
+        const double f = sg * rinv * rinv * rinv;
         const double ru = R[0] * u[0] + R[1] * u[1] + R[2] * u[2];
-        #pragma unroll
-        for (int d = 0; d < 3; ++d) w[d] = sg * (rr * u[d] - R[d] * ru) * r3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w[d] = (rr * u[d] - R[d] * ru) * f;
     };
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
@@ -268,60 +249,57 @@ __device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const doub
         for (int i = 0; i < CAM_MAX_BONDS; ++i) {
             const int nbond = pass == 0 ? rx.n_form : rx.n_break;
             if (i >= nbond) continue;
-            const double sg = (pass == 0 ? -1.0 : 1.0) / (double)nbond;   // forming bonds enter s1 with -|R|
+            const double sg = pass == 0 ? -rx.inv_nform : rx.inv_nbreak;   // forming bonds enter s1 with -|R|
             const int a = pass == 0 ? rx.form[i][0] : rx.brk[i][0], b = pass == 0 ? rx.form[i][1] : rx.brk[i][1];
             double ca[3], cb[3], va[3], vb[3], R[3], u[3], w[3], mw[3];
             gather3<NA>(c, a, ca);
             gather3<NA>(c, b, cb);
             gather3<NA>(v, a, va);
             gather3<NA>(v, b, vb);
-            #pragma unroll
+#pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R[d] = ca[d] - cb[d];
                 u[d] = va[d] - vb[d];
             }
             dist_block(R, u, sg, w);
-            #pragma unroll
+#pragma unroll
             for (int d = 0; d < 3; ++d) mw[d] = -w[d];
             scatter_add3<NA>(h1v, a, w);
             scatter_add3<NA>(h1v, b, mw);
         }
     }
-    unsigned frag[CAM_MAX_REAC];
-    fragment_masks<NA>(rx, frag);
     double com[CAM_MAX_REAC][3], vcom[CAM_MAX_REAC][3];
 #pragma unroll
     for (int k = 0; k < CAM_MAX_REAC; ++k)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            com[k][d] = vcom[k][d] = 0.0;
+            double cc = 0.0, vc = 0.0;
 #pragma unroll
-            for (int at = 0; at < NA; ++at)
-                if ((frag[k] >> at) & 1u) {
-                    com[k][d] += mass[at] * c[3 * at + d] / rx.reac_mass[k];
-                    vcom[k][d] += mass[at] * v[3 * at + d] / rx.reac_mass[k];
-                }
+            for (int at = 0; at < NA; ++at) {
+                cc = fma(rx.fw[k][at], c[3 * at + d], cc);
+                vc = fma(rx.fw[k][at], v[3 * at + d], vc);
+            }
+            com[k][d] = cc;
+            vcom[k][d] = vc;
         }
-    const int terms = (rx.n_reac * rx.n_reac - rx.n_reac) / 2;
 #pragma unroll
     for (int i = 0; i < CAM_MAX_REAC; ++i)
 #pragma unroll
         for (int j = i + 1; j < CAM_MAX_REAC; ++j) {
             if (j >= rx.n_reac) continue;
             double R[3], u[3], w[3];
-            #pragma unroll
+#pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R[d] = com[j][d] - com[i][d];
                 u[d] = vcom[i][d] - vcom[j][d];
             }
-            dist_block(R, u, -1.0 / (double)terms, w);                 // s0 = r_inf - |R|
+            dist_block(R, u, -rx.inv_terms, w);                         // s0 = r_inf - |R|
 #pragma unroll
-            for (int at = 0; at < NA; ++at)
-                #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if ((frag[i] >> at) & 1u) h0v[3 * at + d] += mass[at] / rx.reac_mass[i] * w[d];
-                    if ((frag[j] >> at) & 1u) h0v[3 * at + d] -= mass[at] / rx.reac_mass[j] * w[d];
-                }
+            for (int at = 0; at < NA; ++at) {
+                const double wij = rx.fw[i][at] - rx.fw[j][at];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) h0v[3 * at + d] = fma(wij, w[d], h0v[3 * at + d]);
+            }
         }
 }
 
@@ -335,44 +313,45 @@ __device__ void xi_surface_hess_vec(const Rx &rx, const double *mass, const doub
 // formed here as a Hessian-vector product.  Returns the bias energy
 // 0.5 k_force (xi - xi_ideal)^2 - log(fs) / beta.
 template <int NA>
-__device__ double umbrella_bias(const Rx &rx, const double *mass, const double *c, double beta, double k_force,
-                                double xi_ideal, double &xi1, double *dxi, double *gk, double *gh)
+__device__ double umbrella_bias(const Rx &rx, const double *c, double beta, double k_force, double xi_ideal,
+                                double &xi1, double *dxi, double *gk, double *gh)
 {
     constexpr int N = 3 * NA;
     double s0, s1, ds0[N], ds1[N];
-    xi_surfaces<NA, true>(rx, mass, c, s0, s1, ds0, ds1);
+    xi_surfaces<NA, true>(rx, c, s0, s1, ds0, ds1);
     const double D = s0 - s1;
-    const double xi = s0 / D;
+    const double inv_D = 1.0 / D;
+    const double inv_D2 = inv_D * inv_D;
+    const double xi = s0 * inv_D;
     xi1 = xi;
-    #pragma unroll
-    for (int i = 0; i < N; ++i) dxi[i] = (s0 * ds1[i] - s1 * ds0[i]) / (D * D);
+#pragma unroll
+    for (int i = 0; i < N; ++i) dxi[i] = (s0 * ds1[i] - s1 * ds0[i]) * inv_D2;
     const double delta = xi - xi_ideal;
-    #pragma unroll
+#pragma unroll
     for (int i = 0; i < N; ++i) gk[i] = k_force * delta * dxi[i];
     double v[N], fs2 = 0.0;
-    #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = dxi[i] / mass[i / 3];
-    #pragma unroll
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = dxi[i] * rx.inv_mass[i / 3];
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-        #pragma unroll
-        for (int at = 0; at < NA; ++at) fs2 = fs2 + dxi[3 * at + d] * dxi[3 * at + d] / mass[at];
+#pragma unroll
+        for (int at = 0; at < NA; ++at) fs2 = fs2 + dxi[3 * at + d] * v[3 * at + d];
     const double coeff1 = 2.0 * 3.1415926535897932384 * beta, coeff2 = -1.0 / beta;
     fs2 = fs2 / coeff1;
     const double vadd = 0.5 * k_force * delta * delta + coeff2 * log(sqrt(fs2));
     double h0v[N], h1v[N];
-    xi_surface_hess_vec<NA>(rx, mass, c, v, h0v, h1v);
+    xi_surface_hess_vec<NA>(rx, c, v, h0v, h1v);
     double ds0v = 0.0, ds1v = 0.0;
-    #pragma unroll
+#pragma unroll
     for (int i = 0; i < N; ++i) {
         ds0v += ds0[i] * v[i];
         ds1v += ds1[i] * v[i];
     }
     const double cross = s0 * ds1v - s1 * ds0v;
-    const double scale = coeff2 / (coeff1 * fs2);
-    #pragma unroll
+    const double f = coeff2 / (coeff1 * fs2) * inv_D2 * inv_D;
+#pragma unroll
     for (int i = 0; i < N; ++i)
-        gh[i] = ((s0 * h1v[i] + ds0[i] * ds1v - ds1[i] * ds0v - s1 * h0v[i]) * D - 2.0 * cross * (ds0[i] - ds1[i])) /
-                (D * D * D) * scale;
+        gh[i] = ((s0 * h1v[i] + ds0[i] * ds1v - ds1[i] * ds0v - s1 * h0v[i]) * D - 2.0 * cross * (ds0[i] - ds1[i])) * f;
     return vadd;
 }
 
@@ -801,7 +780,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (k == 0 && r == 0) {
                 double dxi[NC], gk[NC], gh[NC], xi1;
                 const double xw = a.uwin[2 * traj], kw = a.uwin[2 * traj + 1];
-                (void)umbrella_bias<S::NA>(a.rx, a.mass, cent + tl * NC, beta, kw, xw, xi1, dxi, gk, gh);
+                (void)umbrella_bias<S::NA>(a.rx, cent + tl * NC, beta, kw, xw, xi1, dxi, gk, gh);
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     ubias[(2 * tl) * NC + i] = gk[i];
@@ -1109,14 +1088,14 @@ __global__ void constrain_kernel(TrajArgs a, int project_p, int *status)
     const int NB = a.nbeads, NC = 3 * NA;
     double c[3 * NA], dxi_act[3 * NA], qt[3 * NA], dxn[3 * NA];
     centroid_of(a, t, c);
-    (void)calc_xi<NA>(a.rx, a.mass, c, dxi_act);
+    (void)calc_xi<NA>(a.rx, c, dxi_act);
     double mult = 0.0, coeff = 0.0;
     int good = 0;
     for (int iter = 1; iter <= 200; ++iter) {
         coeff = mult * a.dt * a.dt / NB;
         #pragma unroll
         for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dxi_act[i] / a.mass[i / 3];
-        const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
+        const double sigma = calc_xi<NA>(a.rx, qt, dxn);
         double dsigma = 0.0;
         #pragma unroll
         for (int i = 0; i < NC; ++i) dsigma = dsigma + dxn[i] * a.dt * a.dt * dxi_act[i] / (a.mass[i / 3] * NB);
@@ -1212,7 +1191,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_begin_kernel(TrajArg
     double c[NC], dxi[NC], gk[NC], gh[NC];
     wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
     double xi1;
-    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], k_force, a.rx.xi_ideal, xi1, dxi, gk, gh);
+    (void)umbrella_bias<NA>(a.rx, c, a.beta[t], k_force, a.rx.xi_ideal, xi1, dxi, gk, gh);
     double *gt = g + (size_t)t * NC * NB;
 #pragma unroll
     for (int i = 0; i < NC; ++i)
@@ -1247,7 +1226,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(Tra
         coeff = mult * a.dt * a.dt / NB;
 #pragma unroll
         for (int i = 0; i < NC; ++i) qt[i] = c[i] + coeff * dprev[i] / a.mass[i / 3];
-        const double sigma = calc_xi<NA>(a.rx, a.mass, qt, dxn);
+        const double sigma = calc_xi<NA>(a.rx, qt, dxn);
         double dsigma = 0.0;
         #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -1274,7 +1253,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void parent_constrain_kernel(Tra
     }
     if (lane == 0) pfail[t] = good ? (pfail[t] & ~kPfThisStep) : (pfail[t] | kPfConstraint | kPfThisStep);
     double dnew[NC];
-    (void)calc_xi<NA>(a.rx, a.mass, c, dnew);
+    (void)calc_xi<NA>(a.rx, c, dnew);
     const double mine = pick_lane<NC>(dnew, lane);
     if (lane < NC) pdxi[(size_t)t * NC + lane] = mine;
 }
@@ -1362,7 +1341,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void umbrella_begin_kernel(TrajA
     if (bias) {
         double c[NC], dxi[NC], gk[NC], gh[NC], xi1;
         wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
-        (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], uwin[2 * t + 1], uwin[2 * t], xi1, dxi, gk, gh);
+        (void)umbrella_bias<NA>(a.rx, c, a.beta[t], uwin[2 * t + 1], uwin[2 * t], xi1, dxi, gk, gh);
         double *gt = g + (size_t)t * NC * NB;
 #pragma unroll
         for (int i = 0; i < NC; ++i)
@@ -1397,7 +1376,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void umbrella_end_kernel(TrajArg
     double c[NC], dxi[NC], gk[NC], gh[NC], xi1;
     wave_centroid<NA>(a.q + (size_t)t * NC * NB, NB, lane, c);
     const double xi_ideal = uwin[2 * t];
-    (void)umbrella_bias<NA>(a.rx, a.mass, c, a.beta[t], uwin[2 * t + 1], xi_ideal, xi1, dxi, gk, gh);
+    (void)umbrella_bias<NA>(a.rx, c, a.beta[t], uwin[2 * t + 1], xi_ideal, xi1, dxi, gk, gh);
     const uint64_t istep = step_base[1] + (uint64_t)step_off + 1;
     const uint64_t key = step_base[0] + (uint64_t)step_off + 1;
     const bool draw = andersen_step > 0 && istep % (uint64_t)andersen_step == 0;
@@ -1433,7 +1412,7 @@ __global__ void child_begin_kernel(TrajArgs a, double *weight, double *den_part)
     const int NB = a.nbeads, NC = 3 * NA;
     double c[3 * NA], dxi[3 * NA];
     centroid_of(a, t, c);
-    (void)calc_xi<NA>(a.rx, a.mass, c, dxi);
+    (void)calc_xi<NA>(a.rx, c, dxi);
     const double *p = a.p + (size_t)t * NC * NB;
     double vs = 0.0;
     for (int i = 0; i < NC; ++i)
@@ -1537,7 +1516,7 @@ __global__ void observe_kernel(TrajArgs a, const int *pfail, double *epot, doubl
     if (with_xi) {
         double c[3 * CAM_MAX_ATOMS];
         centroid_of(a, t, c);
-        xi[t] = a.natoms == 3 ? calc_xi<3>(a.rx, a.mass, c, nullptr) : calc_xi<6>(a.rx, a.mass, c, nullptr);
+        xi[t] = a.natoms == 3 ? calc_xi<3>(a.rx, c, nullptr) : calc_xi<6>(a.rx, c, nullptr);
     } else {
         xi[t] = 0.0;
     }
@@ -2070,6 +2049,13 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
             rx.reac_mass[k] += cfg->mass[rx.reac_at[k][i]];
         }
     }
+    for (int k = 0; k < rr.n_reac && k < CAM_MAX_REAC; ++k)
+        for (int i = 0; i < rx.reac_n[k]; ++i)
+            rx.fw[k][rx.reac_at[k][i]] = cfg->mass[rx.reac_at[k][i]] / rx.reac_mass[k];
+    for (int i = 0; i < na; ++i) rx.inv_mass[i] = 1.0 / cfg->mass[i];
+    rx.inv_nform = rx.n_form > 0 ? 1.0 / rx.n_form : 0.0;
+    rx.inv_nbreak = rx.n_break > 0 ? 1.0 / rx.n_break : 0.0;
+    rx.inv_terms = rx.n_reac >= 2 ? 2.0 / (rx.n_reac * (rx.n_reac - 1)) : 0.0;
     clear_error();
     std::unique_ptr<cam_engine> e(new cam_engine());
     e->cfg = *cfg;

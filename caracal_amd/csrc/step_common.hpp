@@ -51,6 +51,11 @@ struct Rx {                                  // reaction coordinate, 0-based ind
     int reac_n[CAM_MAX_REAC], reac_at[CAM_MAX_REAC][CAM_MAX_ATOMS];
     double reac_mass[CAM_MAX_REAC];
     double r_inf, xi_ideal;
+    // derived on the host: fw[k][at] = mass[at] / reac_mass[k] for the atoms
+    // of reactant k (0 for the others), reciprocals of the masses and counts
+    double fw[CAM_MAX_REAC][CAM_MAX_ATOMS];
+    double inv_mass[CAM_MAX_ATOMS];
+    double inv_nform, inv_nbreak, inv_terms;
 };
 
 struct StepArgs {

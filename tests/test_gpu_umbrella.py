@@ -32,7 +32,9 @@ def _windows(surface, nb, nwin, seed):
     (cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),      # fused: bias inside the step kernel
     (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA, cam.CAM_SCHED_AUTO),     # MFMA normal modes: split schedule
     (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),
-    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT)])
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT),
+    (cam.CAM_SURFACE_H3, 1, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),        # one bead: its own centroid
+    (cam.CAM_SURFACE_CH4H, 4, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO)])
 def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule):
     nwin, ast, seed, n_equi, n_samp = 6, 5, 21, 20, 30
     w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb)
