@@ -17,7 +17,8 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import (CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_NVE, CAM_ENS_PARENT, CAM_ENS_UMBRELLA,  # noqa: F401
+from ._abi import (CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_NHC, CAM_ENS_NVE, CAM_ENS_PARENT,  # noqa: F401
+                   CAM_ENS_UMBRELLA,
                    CAM_NM_ORTHOGONAL,
                    CAM_NM_REFERENCE, CAM_NMK_AUTO, CAM_NMK_FOLDED, CAM_NMK_MFMA, CAM_NMK_VALU, CAM_PREC_FP64,
                    CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT, CAM_PREC_MIXED, CAM_SURFACE_CH4H, CAM_SURFACE_H3,

@@ -10,7 +10,7 @@ CAM_OK = 0
 CAM_SURFACE_H3, CAM_SURFACE_CH4H = 1, 2
 CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
-CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_PARENT, CAM_ENS_UMBRELLA = 0, 1, 2, 3, 4
+CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_PARENT, CAM_ENS_UMBRELLA, CAM_ENS_NHC = 0, 1, 2, 3, 4, 5
 CAM_NMK_AUTO, CAM_NMK_VALU, CAM_NMK_MFMA, CAM_NMK_FOLDED = 0, 1, 2, 3
 CAM_FF_PAIRS_LIST, CAM_FF_PAIRS_ALL = 0, 1
 CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT = 0, 1, 2
@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("mass", ctypes.c_double * CAM_MAX_ATOMS),
         ("seed", ctypes.c_uint64),
         ("reaction", Reaction),
+        ("nose_q", ctypes.c_double),
     ]
 
 

@@ -443,9 +443,55 @@ __device__ __forceinline__ void nm_mfma(double *xs, double *ys, const double *pr
     }
 }
 
+// The Nose-Hoover chain of one trajectory over half a time step
+// (src/nhc.f90:35-169): four thermostats with velocities vnh and masses qnh,
+// nc = 5 substeps of the 3-stage Suzuki-Yoshida factorisation, on the
+// centroid-momenta kinetic energy eksum.  Returns the factor that scales
+// every momentum.  ekt is the chain's kT.
+__device__ double nhc_chain(double *vnh, const double *qnh, double eksum, double ekt, int nfree, double dt)
+{
+    const double w0 = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0));
+    const double dtc = dt / 5.0;
+    double scale = 1.0, gnh[4];
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double dts = (j == 1 ? 1.0 - 2.0 * w0 : w0) * dtc;
+            const double dt2 = 0.5 * dts, dt4 = 0.25 * dts, dt8 = 0.125 * dts;
+            double e;
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dt4;
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - ekt) / qnh[2];
+            e = exp(-vnh[3] * dt8);
+            vnh[2] = e * (vnh[2] * e + gnh[2] * dt4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - ekt) / qnh[1];
+            e = exp(-vnh[2] * dt8);
+            vnh[1] = e * (vnh[1] * e + gnh[1] * dt4);
+            gnh[0] = (2.0 * eksum - (double)nfree * ekt) / qnh[0];
+            e = exp(-vnh[1] * dt8);
+            vnh[0] = e * (vnh[0] * e + gnh[0] * dt4);
+            e = exp(-vnh[0] * dt2);
+            scale = scale * e;
+            eksum = eksum * e * e;
+            gnh[0] = (2.0 * eksum - (double)nfree * ekt) / qnh[0];
+            e = exp(-vnh[1] * dt8);
+            vnh[0] = e * (vnh[0] * e + gnh[0] * dt4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - ekt) / qnh[1];
+            e = exp(-vnh[2] * dt8);
+            vnh[1] = e * (vnh[1] * e + gnh[1] * dt4);
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - ekt) / qnh[2];
+            e = exp(-vnh[3] * dt8);
+            vnh[2] = e * (vnh[2] * e + gnh[2] * dt4);
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dt4;
+        }
+    return scale;
+}
+
 // ---------------------------------------------------------------------------
-// The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD or
-// CAM_ENS_UMBRELLA.  One step: p -= dt/2 g; free ring-polymer propagation in
+// The step kernel.  ENS: CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD,
+// CAM_ENS_UMBRELLA or CAM_ENS_NHC (the chain before the first and after the
+// second half kick).  One step: p -= dt/2 g; free ring-polymer propagation in
 // normal-mode space; [CHILD: xi of the centroid]; forces; [UMBRELLA: the
 // window's bias at the centroid, added to every bead]; p -= dt/2 g;
 // [ANDERSEN, UMBRELLA: resample].
@@ -485,6 +531,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     __shared__ double ubias[UMB ? TPB * 2 * NC : 1];
     __shared__ double uvb[UMB ? TPB * NB : 1];
     __shared__ int ubad[UMB ? TPB : 1];
+    // NHC: the lanes' kinetic-energy parts and each trajectory's scale factor
+    constexpr bool NHC = ENS == CAM_ENS_NHC;
+    __shared__ double nred[NHC ? kBlock : 1];
+    __shared__ double nscale[NHC ? TPB : 1];
 
     const int tid = threadIdx.x;
     const int r = tid % L;
@@ -535,6 +585,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     double vb = 0.0;
     double usum = 0.0, usq = 0.0;            // UMBRELLA: this launch's sums of xi, xi^2
     bool ucheck = false;                     // UMBRELLA: the window check failed in this launch
+    // NHC: the chain of this lane's trajectory lives in the registers of its
+    // bead-0 lane for the launch; masses and kT as mdinit and nhc set them
+    // (src/mdinit.f90:139-146; src/nhc.f90:53, a default-REAL quotient)
+    double vnh[4] = { 0.0, 0.0, 0.0, 0.0 }, qnh[4] = { 0.0, 0.0, 0.0, 0.0 }, ekt_chain = 0.0;
+    if constexpr (NHC) {
+        const double kelvin = 1.0 / (beta * 0.316679e-5);
+        const double qterm = 0.316679e-5 * kelvin * a.nose_q * a.nose_q;
+        qnh[0] = (double)NC * qterm;
+        qnh[1] = qnh[2] = qnh[3] = qterm;
+        ekt_chain = (double)(1.380649E-23f / 4.3597447E-18f) * kelvin;
+        if (k == 0 && r == 0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vnh[j] = a.nhc[4 * (size_t)traj + j];
+    }
+    // one application of the chain to this block's trajectories
+    auto nhc_apply = [&]() {
+        double part = 0.0;
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+            if (S::valid(s, r)) part += p[s] * p[s] / (2.0 * (TAB ? smass[cc[s] / 3] : m[s]));
+        nred[tid] = part;
+        __syncthreads();
+        if (k == 0 && r == 0) {
+            double ek = 0.0;
+            for (int i = 0; i < NB * L; ++i) ek += nred[tl * NB * L + i];
+            nscale[tl] = nhc_chain(vnh, qnh, ek / NB / NB, ekt_chain, NC, a.dt);
+        }
+        __syncthreads();
+        const double f = nscale[tl];
+#pragma unroll
+        for (int s = 0; s < SL; ++s) p[s] = f * p[s];
+    };
     if constexpr (ENS == CAM_ENS_CHILD) {
         // fragment weights of the centre-of-mass sums (calc_com, src/calc_com.f90:36-59)
         if (threadIdx.x < CAM_MAX_REAC * S::NA) {
@@ -596,6 +678,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
     }
     for (int it = 0; it < a.nsteps; ++it) {
+        if constexpr (NHC) {
+            if constexpr (TAB) __syncthreads();       // the mass table is written before the first step
+            nhc_apply();                              // src/verlet.f90:185-206
+        }
         // half kick (src/verlet.f90:216-218)
 #pragma unroll
         for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
@@ -807,6 +893,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 #pragma unroll
         for (int s = 0; s < SL; ++s) p[s] = p[s] - hdt * g[s];
 
+        if constexpr (NHC) nhc_apply();             // src/verlet.f90:1086-1106
         if constexpr (UMB) {
             // Andersen every andersen_step steps counted from the segment's
             // start, the draw keyed by the steps since the engine's creation
@@ -845,6 +932,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 if (!SPLIT) a.g[o] = g[s];
             }
         if (r == 0 && !SPLIT) a.vbead[(size_t)traj * NB + k] = vb;
+        if constexpr (NHC) {
+            if (k == 0 && r == 0)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a.nhc[4 * (size_t)traj + j] = vnh[j];
+        }
         if constexpr (UMB) {
             if (k == 0 && r == 0) {
                 a.ustat[2 * traj] = a.ustat[2 * traj] + usum;
@@ -1575,6 +1667,7 @@ struct cam_engine {
     int *pfail = nullptr;             // PARENT, UMBRELLA: trajectory flags (kPf*)
     double *uwin = nullptr;           // UMBRELLA: (xi_ideal, k_force) per trajectory
     double *ustat = nullptr;          // UMBRELLA: running sums of xi, xi^2 per trajectory
+    double *nhc = nullptr;            // NHC: thermostat velocities [traj][4]
     long ustat_steps = 0;             // UMBRELLA: steps in those sums
     bool parent_ready = false;        // PARENT, UMBRELLA: begun since the last set_state
     double pcheck_max = 0.0;          // PARENT, UMBRELLA: trajectory check
@@ -1604,9 +1697,12 @@ template <class S, int NB, typename T, int WPE, bool SPLIT = false>
 step_fn pick_step_w(int ens, bool mfma)
 {
     constexpr bool MF = NB >= 16 && NB % 16 == 0;
-    // umbrella windows: the fused step only, with the vector-ALU transform
-    if constexpr (!SPLIT && WPE == 1)
+    // umbrella windows and the Nose-Hoover chain: the fused step only, with
+    // the vector-ALU transform
+    if constexpr (!SPLIT && WPE == 1) {
         if (ens == CAM_ENS_UMBRELLA) return step_kernel<S, NB, T, CAM_ENS_UMBRELLA, 1, false, false>;
+        if (ens == CAM_ENS_NHC) return step_kernel<S, NB, T, CAM_ENS_NHC, 1, false, false>;
+    }
     if (mfma && MF) {
         if (ens == CAM_ENS_NVE) return step_kernel<S, NB, T, CAM_ENS_NVE, WPE, MF, SPLIT>;
         if (ens == CAM_ENS_ANDERSEN) return step_kernel<S, NB, T, CAM_ENS_ANDERSEN, WPE, MF, SPLIT>;
@@ -1624,7 +1720,7 @@ template <class S, int NB, typename T>
 step_fn pick_step(int ens, int wpe, bool mfma)
 {
     if constexpr (S::L == 4 && std::is_same<T, double>::value)
-        if (wpe == 2 && ens != CAM_ENS_UMBRELLA) return pick_step_w<S, NB, T, 2>(ens, mfma);
+        if (wpe == 2 && ens != CAM_ENS_UMBRELLA && ens != CAM_ENS_NHC) return pick_step_w<S, NB, T, 2>(ens, mfma);
     return pick_step_w<S, NB, T, 1>(ens, mfma);
 }
 
@@ -1734,7 +1830,7 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
     // one: its constraint or bias and the end of its step are kernels of
     // their own.  Windows also have a fused step; parents do not.
     const bool steered = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA;
-    const int ens_nm = steered ? CAM_ENS_NVE : cfg->ensemble;
+    const int ens_nm = steered || cfg->ensemble == CAM_ENS_NHC ? CAM_ENS_NVE : cfg->ensemble;
     const int ens = cfg->ensemble == CAM_ENS_PARENT ? CAM_ENS_NVE : cfg->ensemble;
     const bool mfma = nmk == CAM_NMK_MFMA;
     bool ok;
@@ -1836,6 +1932,8 @@ StepArgs step_args(cam_engine *e)
     a.pfail = e->pfail;
     a.ustat = e->ustat;
     a.uwin = e->uwin;
+    a.nhc = e->nhc;
+    a.nose_q = e->cfg.nose_q;
     a.epot_max = e->pcheck_max;
     a.xi_tol = e->pcheck_xi_tol;
     a.check_after = e->pcheck_after;
@@ -1989,7 +2087,11 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
-    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_UMBRELLA, "bad ensemble");
+    CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_NHC, "bad ensemble");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_NHC || cfg->nose_q > 0.0, "ensemble NHC needs nose_q > 0");
+    CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_NHC ||
+                      (cfg->schedule != CAM_SCHED_SPLIT && (cfg->nm_kernel == CAM_NMK_AUTO || cfg->nm_kernel == CAM_NMK_VALU)),
+                  "ensemble NHC runs on the fused schedule with the vector-ALU transform");
     const bool steered = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA;
     CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_FOLDED, "bad nm_kernel");
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
@@ -2099,6 +2201,10 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
         CAM_HIP(hipMalloc(&e->pdxi, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
         CAM_HIP(hipMemset(e->pdxi, 0, (size_t)cfg->ntraj * 3 * na * sizeof(double)));
     }
+    if (cfg->ensemble == CAM_ENS_NHC) {
+        CAM_HIP(hipMalloc(&e->nhc, (size_t)cfg->ntraj * 4 * sizeof(double)));
+        CAM_HIP(hipMemset(e->nhc, 0, (size_t)cfg->ntraj * 4 * sizeof(double)));
+    }
     if (cfg->ensemble == CAM_ENS_UMBRELLA) {
         // every window at the reaction's xi_ideal with no force constant until cam_engine_set_windows
         std::vector<double> w(2 * (size_t)cfg->ntraj, 0.0);
@@ -2139,7 +2245,7 @@ int cam_engine_destroy(cam_engine *e)
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     double *bufs[] = { e->q, e->p, e->g, e->vbead, e->beta, e->tfwd, e->tinv, e->tfold, e->weight, e->den_part,
-                       e->den, e->num, e->scratch_io, e->cbuf, e->pdxi, e->uwin, e->ustat };
+                       e->den, e->num, e->scratch_io, e->cbuf, e->pdxi, e->uwin, e->ustat, e->nhc };
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (e->pfail) (void)hipFree(e->pfail);
@@ -2172,6 +2278,8 @@ int cam_engine_set_state(cam_engine *e, const double *q, const double *p, const 
     hipLaunchKernelGGL(to_device_layout, dim3(grid), dim3(256), 0, e->stream, e->scratch_io, e->p, nt, nc, nb);
     CAM_LAUNCH_CHECK();
     CAM_HIP(hipMemcpyAsync(e->beta, hb.data(), nt * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    // a new state starts its Nose-Hoover chains at rest, as mdinit does (src/mdinit.f90:141-145)
+    if (e->nhc) CAM_HIP(hipMemsetAsync(e->nhc, 0, (size_t)nt * 4 * sizeof(double), e->stream));
     int rc = run_forces(e);
     if (rc != CAM_OK) return rc;
     CAM_HIP(hipStreamSynchronize(e->stream));
@@ -2553,16 +2661,18 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     struct Trial { int nmk, wpe, spl, split; double us; };
     std::vector<Trial> cand, trials;
     const bool parent = c.ensemble == CAM_ENS_PARENT, window = c.ensemble == CAM_ENS_UMBRELLA;
+    const bool nhc = c.ensemble == CAM_ENS_NHC;
+    const bool vfused = window || nhc;          // a fused step with the VALU transform and one budget only
     const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent;
-    const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com;
+    const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com && !nhc;
     for (int nmk : nmks) {
         if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
             for (int spl : spls) cand.push_back({ nmk, 1, spl, 0, 0.0 });
             continue;
         }
-        if (try_fused && !(window && nmk != CAM_NMK_VALU))      // a window's fused step: VALU, one budget
+        if (try_fused && !(vfused && nmk != CAM_NMK_VALU))
             for (int wpe : wpes)
-                if (!window || wpe == 1)
+                if (!vfused || wpe == 1)
                     for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
         if (try_split) cand.push_back({ nmk, 1, 1, 1, 0.0 });
     }
@@ -2571,8 +2681,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         rc = CAM_ERR_DEVICE;
     };
     // PARENT: a copy of the constraint directions; UMBRELLA: scratch sums;
-    // both: scratch flags
-    const size_t npd = parent ? (size_t)c.ntraj * 3 * c.natoms : window ? 2 * (size_t)c.ntraj : 0;
+    // NHC: scratch chains; all: scratch flags
+    const size_t npd = parent ? (size_t)c.ntraj * 3 * c.natoms
+                              : window ? 2 * (size_t)c.ntraj : nhc ? 4 * (size_t)c.ntraj : 0;
     hipError_t err = hipMalloc(&buf, (3 * nstate + (size_t)c.ntraj * nb + npd + (size_t)c.ntraj) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&flags, (size_t)trial_steps * c.ntraj);
     if (err == hipSuccess) err = hipMalloc(&ctr, 2 * sizeof(uint64_t));
@@ -2598,9 +2709,12 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
             a.ustat = buf + 3 * nstate + (size_t)c.ntraj * nb;
             a.pfail = reinterpret_cast<int *>(a.ustat + npd);
         }
+        if (nhc) a.nhc = buf + 3 * nstate + (size_t)c.ntraj * nb;
         err = hipMemsetAsync(ctr, 0, 2 * sizeof(uint64_t), e->stream);
         if (err == hipSuccess && parent)
             err = hipMemcpyAsync(a.pdxi, e->pdxi, npd * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
+        if (err == hipSuccess && nhc)
+            err = hipMemcpyAsync(a.nhc, e->nhc, npd * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.q, e->q, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);

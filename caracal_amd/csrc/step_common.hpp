@@ -71,6 +71,8 @@ struct StepArgs {
     const double *uwin;          // UMBRELLA: (xi_ideal, k_force) of each trajectory's window
     double epot_max, xi_tol;     // PARENT, UMBRELLA: trajectory check
     int check_after;             //   armed after this many steps (never if < 0)
+    double *nhc;                 // NHC: thermostat velocities [traj][4]
+    double nose_q;               // NHC: damping time (atomic time units)
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
     int andersen_step;

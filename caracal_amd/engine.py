@@ -48,7 +48,7 @@ class Reaction:
 
 def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64, nm_mode=_abi.CAM_NM_REFERENCE,
                 ensemble=_abi.CAM_ENS_NVE, remove_com=False, andersen_step=0, steps_per_launch=0, seed=0,
-                reaction=None, nm_kernel=_abi.CAM_NMK_AUTO, waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO):
+                reaction=None, nm_kernel=_abi.CAM_NMK_AUTO, waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO, nose_q=0.0):
     """cam_config of one engine (or of one bucket of a batch)."""
     natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
     masses = np.asarray(masses, dtype=np.float64)
@@ -67,6 +67,7 @@ def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_F
     c.seed = seed
     if reaction is not None:
         c.reaction = reaction.to_c()
+    c.nose_q = nose_q
     return c
 
 
@@ -77,11 +78,11 @@ class RPMDEngine:
     def __init__(self, surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_FP64,
                  nm_mode=_abi.CAM_NM_REFERENCE, ensemble=_abi.CAM_ENS_NVE, remove_com=False,
                  andersen_step=0, steps_per_launch=0, seed=0, reaction=None, nm_kernel=_abi.CAM_NMK_AUTO,
-                 waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO):
+                 waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO, nose_q=0.0):
         c = make_config(surface, nbeads, ntraj, dt, masses, precision=precision, nm_mode=nm_mode,
                         ensemble=ensemble, remove_com=remove_com, andersen_step=andersen_step,
                         steps_per_launch=steps_per_launch, seed=seed, reaction=reaction, nm_kernel=nm_kernel,
-                        waves_per_eu=waves_per_eu, schedule=schedule)
+                        waves_per_eu=waves_per_eu, schedule=schedule, nose_q=nose_q)
         h = ctypes.c_void_p()
         check("cam_engine_create", lib().cam_engine_create(ctypes.byref(c), ctypes.byref(h)))
         self._attach(c, h, owned=True)

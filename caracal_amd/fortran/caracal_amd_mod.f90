@@ -19,7 +19,7 @@ module caracal_amd_mod
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, &
-                                             CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4
+                                             CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4, CAM_ENS_NHC = 5
    integer(c_int32_t), parameter, public :: CAM_NMK_AUTO = 0, CAM_NMK_VALU = 1, CAM_NMK_MFMA = 2, &
                                             CAM_NMK_FOLDED = 3
    integer(c_int32_t), parameter, public :: CAM_SCHED_AUTO = 0, CAM_SCHED_FUSED = 1, CAM_SCHED_SPLIT = 2
@@ -48,6 +48,7 @@ module caracal_amd_mod
       real(c_double) :: mass(CAM_MAX_ATOMS) = 0.d0
       integer(c_int64_t) :: seed = 0
       type(cam_reaction) :: reaction
+      real(c_double) :: nose_q = 0.0_c_double
    end type cam_config
 
    integer, parameter, public :: CAM_MAX_TRIALS = 32

@@ -98,8 +98,15 @@ enum { CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1 };
  *              schedule evaluates the bias inside the step kernel (one lane
  *              per trajectory, vector-ALU normal modes); the split schedule
  *              (or nm_kernel MFMA) in a kernel of its own after the force
- *              kernel.  Start it with cam_engine_begin_umbrella. */
-enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4 };
+ *              kernel.  Start it with cam_engine_begin_umbrella;
+ *  NHC      -- dynamic.x with the Nose-Hoover chain thermostat (thermostat
+ *              = 2, src/nhc.f90:35-169): four thermostats per trajectory,
+ *              applied to all bead momenta before the first and after the
+ *              second half kick (src/verlet.f90:185-206, :1086-1106), with
+ *              the damping time nose_q; cam_engine_set_state resets them as
+ *              mdinit does.  Fused schedule, vector-ALU normal modes. */
+enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT = 3, CAM_ENS_UMBRELLA = 4,
+       CAM_ENS_NHC = 5 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
  * nbeads a multiple of 16).  FOLDED: the reference transform (rfft/irfft, the
@@ -161,6 +168,8 @@ typedef struct {
     double mass[CAM_MAX_ATOMS];   /* electron masses */
     uint64_t seed;            /* counter-based RNG key for Andersen draws */
     cam_reaction reaction;    /* used by CAM_ENS_CHILD, CAM_ENS_PARENT and cam_engine_constrain */
+    double nose_q;            /* NHC: thermostat damping time in atomic time units (the key file's
+                                 NOSE_DAMP times the time step, src/dynamic.f90:578) */
 } cam_config;
 
 typedef struct cam_engine cam_engine;

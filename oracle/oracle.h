@@ -76,6 +76,9 @@ double oracle_umbrella(const oracle_sys *s, const oracle_rx *rx, const double *c
                        double *dxi, double *g, double *xi_real);
 void oracle_parent_begin(const oracle_sys *s, const oracle_rx *rx, const double *q, double *g, double *dxi,
                          double beta, double k_force);
+void oracle_nhc(const oracle_sys *s, double *p, double *vnh, const double *qnh, double kelvin, double *eta);
+void oracle_verlet_nhc(const oracle_sys *s, double *q, double *p, double *g, double *vbead, double beta,
+                       double *vnh, const double *qnh, double *eta);
 double oracle_verlet_umbrella(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g,
                               double *vbead, double beta, double k_force, long istep, int andersen_step,
                               uint64_t seed, uint32_t traj, uint64_t draw);

@@ -597,6 +597,84 @@ double oracle_umbrella(const oracle_sys *s, const oracle_rx *rx, const double *c
     return v_add;
 }
 
+/* Nose-Hoover chain thermostat, four thermostats (src/nhc.f90:35-169): the
+ * kinetic energy in the centroid-momenta form sum p^2/(2m)/nbeads^2, then
+ * nc = 5 substeps of the 3-stage Suzuki-Yoshida factorisation of the chain,
+ * and all momenta scaled.  vnh[4]: thermostat velocities, qnh[4]: masses
+ * (mdinit: nfree ekT Q^2 for the first, ekT Q^2 for the others, with
+ * k_B = 0.316679e-5); the chain's own kT is the default-REAL quotient
+ * 1.380649E-23/4.3597447E-18 times the temperature.  If eta is not NULL, the
+ * thermostat positions are advanced as well (not part of the reference,
+ * which does not need them; a test uses them for the conserved energy). */
+void oracle_nhc(const oracle_sys *s, double *p, double *vnh, const double *qnh, double kelvin, double *eta)
+{
+    const int na = s->natoms, nb = s->nbeads, nfree = 3 * na;
+    const double ekt = (double)(1.380649E-23f / 4.3597447E-18f) * kelvin;
+    const int nc = 5, ns = 3;
+    const double dtc = s->dt / (double)nc;
+    double w[3];
+    w[0] = 1.0 / (2.0 - pow(2.0, 1.0 / 3.0));
+    w[1] = 1.0 - 2.0 * w[0];
+    w[2] = w[0];
+    double scale = 1.0, eksum = 0.0, gnh[4];
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j) {
+            const double *pp = &p[IDX(j, i, 0, na)];
+            eksum = eksum + (pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]) / (2.0 * s->mass[i]) / nb / nb;
+        }
+    for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < ns; ++j) {
+            const double dts = w[j] * dtc, dt2 = 0.5 * dts, dt4 = 0.25 * dts, dt8 = 0.125 * dts;
+            double e;
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dt4;
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - ekt) / qnh[2];
+            e = exp(-vnh[3] * dt8);
+            vnh[2] = e * (vnh[2] * e + gnh[2] * dt4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - ekt) / qnh[1];
+            e = exp(-vnh[2] * dt8);
+            vnh[1] = e * (vnh[1] * e + gnh[1] * dt4);
+            gnh[0] = (2.0 * eksum - (double)nfree * ekt) / qnh[0];
+            e = exp(-vnh[1] * dt8);
+            vnh[0] = e * (vnh[0] * e + gnh[0] * dt4);
+            e = exp(-vnh[0] * dt2);
+            scale = scale * e;
+            eksum = eksum * e * e;
+            if (eta)
+                for (int k = 0; k < 4; ++k) eta[k] = eta[k] + vnh[k] * dt2;
+            gnh[0] = (2.0 * eksum - (double)nfree * ekt) / qnh[0];
+            e = exp(-vnh[1] * dt8);
+            vnh[0] = e * (vnh[0] * e + gnh[0] * dt4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - ekt) / qnh[1];
+            e = exp(-vnh[2] * dt8);
+            vnh[1] = e * (vnh[1] * e + gnh[1] * dt4);
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - ekt) / qnh[2];
+            e = exp(-vnh[3] * dt8);
+            vnh[2] = e * (vnh[2] * e + gnh[2] * dt4);
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dt4;
+        }
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j)
+            for (int k = 0; k < 3; ++k) p[IDX(j, i, k, na)] = scale * p[IDX(j, i, k, na)];
+}
+
+/* One verlet step with the Nose-Hoover chain (thermostat 2, src/verlet.f90:
+ * 185-206 and :1086-1106): the chain, p -= dt/2 g, free ring polymer,
+ * forces, p -= dt/2 g, the chain again. */
+void oracle_verlet_nhc(const oracle_sys *s, double *q, double *p, double *g, double *vbead, double beta,
+                       double *vnh, const double *qnh, double *eta)
+{
+    const size_t n = (size_t)3 * s->natoms * s->nbeads;
+    const double kelvin = 1.0 / (beta * 0.316679e-5);
+    oracle_nhc(s, p, vnh, qnh, kelvin, eta);
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    oracle_free_rp(s, q, p, beta);
+    oracle_forces(s, q, vbead, g);
+    for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
+    oracle_nhc(s, p, vnh, qnh, kelvin, eta);
+}
+
 /* One step of an umbrella-sampling window trajectory (verlet with
  * constrain = 0 in the sampling loops, src/calc_rate.f90:1569 and :1626):
  * p -= dt/2 g; free ring polymer; forces; umbrella mode 0 at the centroid of

@@ -151,6 +151,8 @@ def _setup_rpmd():
     o.oracle_rate_bimolecular.restype = ctypes.c_double
     o.oracle_rate_bimolecular.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    o.oracle_nhc.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp, _dp, ctypes.c_double, _dp]
+    o.oracle_verlet_nhc.argtypes = [ctypes.POINTER(OracleSys), _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
     o.oracle_verlet_umbrella.restype = ctypes.c_double
     o.oracle_verlet_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_long, ctypes.c_int,
@@ -201,6 +203,10 @@ class OracleTraj:
 
     def step(self):
         self.istep += 1
+        if self.ensemble == 5:
+            rpmd().oracle_verlet_nhc(ctypes.byref(self.sys), _ptr(self.q), _ptr(self.p), _ptr(self.g), _ptr(self.v),
+                                     self.beta, _ptr(self.vnh), _ptr(self.qnh), _ptr(self.eta))
+            return 0.0
         if self.ensemble == 4:
             xi = rpmd().oracle_verlet_umbrella(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q),
                                                _ptr(self.p), _ptr(self.g), _ptr(self.v), self.beta, self.k_force,
@@ -216,6 +222,18 @@ class OracleTraj:
         return rpmd().oracle_verlet(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q), _ptr(self.p),
                                     _ptr(self.g), _ptr(self.v), self.beta, self.ensemble, int(self.remove_com),
                                     self.istep, self.andersen_step, self.seed, self.traj)
+
+    def nhc_begin(self, nose_q):
+        """Nose-Hoover chain (ensemble 5), as mdinit sets it up
+        (src/mdinit.f90:132-146): thermostat velocities 0, masses
+        nfree kT Q^2 and kT Q^2 with kT = 0.316679e-5 T; positions 0 (tracked
+        for the conserved energy only)."""
+        kelvin = 1.0 / (self.beta * 0.316679e-5)
+        qterm = 0.316679e-5 * kelvin * nose_q * nose_q
+        nfree = 3 * self.q.shape[1]
+        self.qnh = np.array([nfree * qterm, qterm, qterm, qterm])
+        self.vnh = np.zeros(4)
+        self.eta = np.zeros(4)
 
     def umbrella_begin(self, k_force, bias=True):
         """Umbrella window (ensemble 4; the window centre is self.rx.xi_ideal):
