@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "nhc.hpp"
 #include "philox.hpp"
 
 namespace cam {
@@ -1108,6 +1109,54 @@ __global__ void box_energy_kernel(const double *q, const double *p, const double
     }
 }
 
+// Nose-Hoover chain of the box (thermostat 2, src/nhc.f90:35-169): the
+// centroid-momenta kinetic energy sum p^2/(2m)/nbeads^2 in per-block parts
+// (a fixed order, so runs repeat bit for bit), the chain in one thread, then
+// every momentum scaled.  st[0..3]: thermostat velocities, st[4]: the scale.
+constexpr int kBoxNhcBlocks = 256;
+
+__global__ __launch_bounds__(256) void box_ke_kernel(const double *p, const double *mass, int natoms, int nb,
+                                                     double *part)
+{
+    __shared__ double red[256];
+    const size_t n = (size_t)3 * natoms * nb;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v = p[i];
+        acc += v * v / (2.0 * mass[(i % (3 * (size_t)natoms)) / 3]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void box_nhc_kernel(const double *part, int nparts, double *st, double beta, double nose_q, int natoms,
+                               int nb, double dt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double ek = 0.0;
+    for (int i = 0; i < nparts; ++i) ek += part[i];
+    const double kelvin = 1.0 / (beta * 0.316679e-5);
+    const double qterm = 0.316679e-5 * kelvin * nose_q * nose_q;    // mdinit (src/mdinit.f90:139-146)
+    const int nfree = 3 * natoms;
+    const double qnh[4] = { (double)nfree * qterm, qterm, qterm, qterm };
+    const double ekt = (double)(1.380649E-23f / 4.3597447E-18f) * kelvin;
+    double vnh[4] = { st[0], st[1], st[2], st[3] };
+    const double scale = nhc_chain(vnh, qnh, ek / nb / nb, ekt, nfree, dt);
+    for (int j = 0; j < 4; ++j) st[j] = vnh[j];
+    st[4] = scale;
+}
+
+__global__ void box_scale_kernel(double *p, size_t n, const double *st)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = st[4] * p[i];
+}
+
 }  // namespace
 }  // namespace cam
 
@@ -1121,6 +1170,9 @@ struct cam_box {
     double *q = nullptr, *p = nullptr, *g = nullptr, *e = nullptr, *mass = nullptr, *tfwd = nullptr, *tinv = nullptr,
            *ek = nullptr;
     uint64_t *step_base = nullptr;
+    double nose_q = 0.0;               // Nose-Hoover chain on when > 0
+    double *nhc = nullptr;             // chain state: 4 velocities and the last scale factor
+    double *nhc_part = nullptr;        // kinetic-energy parts (kBoxNhcBlocks)
     bool have_state = false;
     std::map<int, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evf0 = nullptr, evf1 = nullptr;
@@ -1141,7 +1193,22 @@ int box_enqueue(cam_box *bx, int nsteps)
     Theta th;
     std::memcpy(th.p, bx->theta, sizeof th.p);
     const size_t n = (size_t)ncol * bx->nb;
+    const unsigned egrid = (unsigned)((n + 255) / 256);
+    auto chain = [&]() -> int {                        // src/verlet.f90:185-206, :1086-1106
+        if (bx->nose_q <= 0.0) return CAM_OK;
+        hipLaunchKernelGGL(box_ke_kernel, dim3(kBoxNhcBlocks), dim3(256), 0, bx->stream, bx->p, bx->mass, bx->natoms,
+                           bx->nb, bx->nhc_part);
+        CAM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(box_nhc_kernel, dim3(1), dim3(64), 0, bx->stream, bx->nhc_part, kBoxNhcBlocks, bx->nhc,
+                           bx->beta, bx->nose_q, bx->natoms, bx->nb, bx->dt);
+        CAM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(box_scale_kernel, dim3(egrid), dim3(256), 0, bx->stream, bx->p, n, bx->nhc);
+        CAM_LAUNCH_CHECK();
+        return CAM_OK;
+    };
     for (int s = 0; s < nsteps; ++s) {
+        int rc0 = chain();
+        if (rc0 != CAM_OK) return rc0;
         hipLaunchKernelGGL(box_nm_kernel, dim3((ncol + cpb - 1) / cpb), dim3(256), 2 * 256 * sizeof(double), bx->stream,
                            a);
         CAM_LAUNCH_CHECK();
@@ -1152,6 +1219,8 @@ int box_enqueue(cam_box *bx, int nsteps)
         hipLaunchKernelGGL(box_kick_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, bx->stream, bx->p, bx->g,
                            bx->mass, bx->natoms, bx->nb, bx->dt, bx->beta, bx->andersen_step, bx->seed, bx->step_base, s);
         CAM_LAUNCH_CHECK();
+        rc = chain();
+        if (rc != CAM_OK) return rc;
     }
     hipLaunchKernelGGL(box_advance_kernel, dim3(1), dim3(1), 0, bx->stream, bx->step_base, nsteps);
     CAM_LAUNCH_CHECK();
@@ -1215,6 +1284,8 @@ int cam_box_destroy(cam_box *bx)
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (bx->step_base) (void)hipFree(bx->step_base);
+    if (bx->nhc) (void)hipFree(bx->nhc);
+    if (bx->nhc_part) (void)hipFree(bx->nhc_part);
     if (bx->ev0) (void)hipEventDestroy(bx->ev0);
     if (bx->ev1) (void)hipEventDestroy(bx->ev1);
     if (bx->stream) (void)hipStreamDestroy(bx->stream);
@@ -1249,7 +1320,27 @@ int cam_box_set_state(cam_box *bx, const double *q, const double *p)
     CAM_HIP(hipMemcpyAsync(bx->q, q, n * sizeof(double), hipMemcpyHostToDevice, bx->stream));
     CAM_HIP(hipMemcpyAsync(bx->p, p, n * sizeof(double), hipMemcpyHostToDevice, bx->stream));
     bx->have_state = true;
+    // a new state starts the Nose-Hoover chain at rest, as mdinit does
+    if (bx->nhc) CAM_HIP(hipMemsetAsync(bx->nhc, 0, 5 * sizeof(double), bx->stream));
     return cam_box_set_params(bx, bx->theta);
+}
+
+int cam_box_set_nhc(cam_box *bx, double nose_q)
+{
+    CAM_CHECK_ARG(bx, "null box");
+    CAM_CHECK_ARG(nose_q >= 0.0 && std::isfinite(nose_q), "nose_q must be >= 0");
+    CAM_CHECK_ARG(nose_q == 0.0 || bx->andersen_step == 0, "the Nose-Hoover chain replaces Andersen: andersen_step must be 0");
+    clear_error();
+    if (nose_q > 0.0 && !bx->nhc) {
+        CAM_HIP(hipMalloc(&bx->nhc, 5 * sizeof(double)));
+        CAM_HIP(hipMalloc(&bx->nhc_part, kBoxNhcBlocks * sizeof(double)));
+    }
+    if (bx->nhc) CAM_HIP(hipMemsetAsync(bx->nhc, 0, 5 * sizeof(double), bx->stream));
+    CAM_HIP(hipStreamSynchronize(bx->stream));
+    bx->nose_q = nose_q;
+    for (auto &kv : bx->graphs) (void)hipGraphExecDestroy(kv.second);   // the chain is part of the captured step
+    bx->graphs.clear();
+    return CAM_OK;
 }
 
 int cam_box_get_state(cam_box *bx, double *q, double *p)

@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
+#include "nhc.hpp"
 
 namespace cam {
 

@@ -171,6 +171,11 @@ class BoxEngine:
     def prepare(self, n):
         check("cam_box_prepare", lib().cam_box_prepare(self._h, n))
 
+    def set_nhc(self, nose_q):
+        """Nose-Hoover chain thermostat with damping time nose_q (atomic time
+        units; 0 turns it off).  Needs andersen_step 0."""
+        check("cam_box_set_nhc", lib().cam_box_set_nhc(self._h, float(nose_q)))
+
     def run(self, n, sync=True):
         check("cam_box_run", lib().cam_box_run(self._h, n))
         if sync:

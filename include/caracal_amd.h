@@ -402,6 +402,13 @@ int cam_box_set_state(cam_box *box, const double *q, const double *p);
 int cam_box_get_state(cam_box *box, double *q, double *p);
 int cam_box_prepare(cam_box *box, int nsteps);
 int cam_box_run(cam_box *box, int nsteps);
+/* Nose-Hoover chain thermostat for the box (thermostat 2, src/nhc.f90:35-169,
+ * applied before the first and after the second half kick of every step as
+ * src/verlet.f90:185-206 and :1086-1106 do): nose_q is the damping time in
+ * atomic time units (NOSE_DAMP x dt, src/dynamic.f90:578), 0 turns it off.
+ * Needs andersen_step 0.  The chain starts at rest, and again after every
+ * cam_box_set_state (mdinit). */
+int cam_box_set_nhc(cam_box *box, double nose_q);
 /* Waits for the last run; ms (may be NULL) receives its duration measured
  * with HIP events on the box's stream. */
 int cam_box_sync(cam_box *box, double *ms);

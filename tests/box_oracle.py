@@ -74,12 +74,60 @@ def free_rp(q, p, dt, masses, beta, orthogonal=False):
     return np.einsum("kn,naj->kaj", inv, qn), np.einsum("kn,naj->kaj", inv, pn)
 
 
+def nhc(p, masses, vnh, nose_q, beta, dt):
+    """Nose-Hoover chain over half a step (src/nhc.f90:35-169), numpy
+    restatement for the box: centroid-momenta kinetic energy, four thermostats
+    (masses as mdinit sets them), nc = 5 Suzuki-Yoshida substeps; returns the
+    scaled momenta and updates vnh in place.  Pinned against the C oracle's
+    oracle_nhc on a small system (tests/test_box_oracle.py)."""
+    nb, na = p.shape[0], p.shape[1]
+    nfree = 3 * na
+    kelvin = 1.0 / (beta * 0.316679e-5)
+    qterm = 0.316679e-5 * kelvin * nose_q * nose_q
+    qnh = [nfree * qterm, qterm, qterm, qterm]
+    ekt = float(np.float32(np.float32(1.380649e-23) / np.float32(4.3597447e-18))) * kelvin
+    w0 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    ws = (w0, 1.0 - 2.0 * w0, w0)
+    dtc = dt / 5.0
+    eksum = float(np.sum(p * p / (2.0 * masses[None, :, None]))) / nb / nb
+    scale = 1.0
+    g = [0.0] * 4
+    for _ in range(5):
+        for wj in ws:
+            dts = wj * dtc
+            dt2, dt4, dt8 = 0.5 * dts, 0.25 * dts, 0.125 * dts
+            g[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3]
+            vnh[3] = vnh[3] + g[3] * dt4
+            for j in (2, 1):
+                g[j] = (qnh[j - 1] * vnh[j - 1] * vnh[j - 1] - ekt) / qnh[j]
+                e = np.exp(-vnh[j + 1] * dt8)
+                vnh[j] = e * (vnh[j] * e + g[j] * dt4)
+            g[0] = (2.0 * eksum - nfree * ekt) / qnh[0]
+            e = np.exp(-vnh[1] * dt8)
+            vnh[0] = e * (vnh[0] * e + g[0] * dt4)
+            e = np.exp(-vnh[0] * dt2)
+            scale *= e
+            eksum *= e * e
+            g[0] = (2.0 * eksum - nfree * ekt) / qnh[0]
+            e = np.exp(-vnh[1] * dt8)
+            vnh[0] = e * (vnh[0] * e + g[0] * dt4)
+            for j in (1, 2):
+                g[j] = (qnh[j - 1] * vnh[j - 1] * vnh[j - 1] - ekt) / qnh[j]
+                e = np.exp(-vnh[j + 1] * dt8)
+                vnh[j] = e * (vnh[j] * e + g[j] * dt4)
+            g[3] = (qnh[2] * vnh[2] * vnh[2] - ekt) / qnh[3]
+            vnh[3] = vnh[3] + g[3] * dt4
+    return scale * p
+
+
 class BoxRPMD:
     """The oracle's ring polymer of one box; arrays (nbeads, natoms, 3), bohr
-    and atomic units."""
+    and atomic units.  nose_q > 0 adds the Nose-Hoover chain before the first
+    and after the second half kick (src/verlet.f90:185-206, :1086-1106)."""
 
-    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False):
+    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False, nose_q=0.0):
         self.orthogonal = orthogonal
+        self.nose_q, self.vnh = nose_q, [0.0, 0.0, 0.0, 0.0]
         self.ffo, self.theta = ffo, np.asarray(theta, dtype=np.float64)
         self.q, self.p = np.array(q, dtype=np.float64), np.array(p, dtype=np.float64)
         self.dt, self.m, self.beta = dt, np.asarray(masses, dtype=np.float64), beta
@@ -95,10 +143,14 @@ class BoxRPMD:
 
     def step(self):
         hdt = 0.5 * self.dt
+        if self.nose_q > 0:
+            self.p = nhc(self.p, self.m, self.vnh, self.nose_q, self.beta, self.dt)
         self.p = self.p - hdt * self.g
         self.q, self.p = free_rp(self.q, self.p, self.dt, self.m, self.beta, self.orthogonal)
         self.v, self.g = self.forces(self.q)
         self.p = self.p - hdt * self.g
+        if self.nose_q > 0:
+            self.p = nhc(self.p, self.m, self.vnh, self.nose_q, self.beta, self.dt)
 
     def energies(self):
         """Per-bead potential, kinetic energy and ring-polymer spring energy

@@ -1,11 +1,14 @@
 """The numpy ring-polymer step of the box oracle (tests/box_oracle.py) is
 pinned to the C oracle's free ring-polymer step (oracle_free_rp, itself pinned
 by its closed form) for the reference transform, and to the exact
-conservation of the free ring-polymer energy for the orthogonal basis."""
+conservation of the free ring-polymer energy for the orthogonal basis; its
+Nose-Hoover chain to the C oracle's (itself pinned by the extended-system
+energy)."""
 import numpy as np
 
 import box_oracle as bo
 import oracle_lib as ol
+from caracal_amd import workloads
 
 
 def test_numpy_free_rp_matches_c_oracle_reference_mode():
@@ -42,3 +45,24 @@ def test_orthogonal_free_rp_conserves_the_free_ring_polymer_energy():
         assert abs(h(q, p) - h0) < 1e-10 * h0
         u = bo.orthogonal_matrix(nb)
         np.testing.assert_allclose(u @ u.T, np.eye(nb), atol=1e-14)
+
+
+def test_numpy_nhc_matches_c_oracle():
+    """The box oracle's chain (numpy) against the C oracle's oracle_nhc on a
+    three-atom ring polymer, over 30 successive applications."""
+    import box_oracle as bo
+    w = workloads.h3_nve(nbeads=8)
+    s = ol.make_sys(1, 8, w["dt"], w["masses"])
+    p_c = np.ascontiguousarray(w["p"] * 1.4)
+    p_n = p_c.copy()
+    v_c, v_n = np.zeros(4), [0.0] * 4
+    nose_q = 15.0 * w["dt"]
+    kelvin = 1.0 / (w["beta"] * 0.316679e-5)
+    qterm = 0.316679e-5 * kelvin * nose_q * nose_q
+    qnh = np.array([9 * qterm, qterm, qterm, qterm])
+    for _ in range(30):
+        ol.rpmd().oracle_nhc(ol.ctypes.byref(s), ol._ptr(p_c), ol._ptr(v_c), ol._ptr(qnh), kelvin, None)
+        p_n = bo.nhc(p_n, np.asarray(w["masses"]), v_n, nose_q, w["beta"], w["dt"])
+    np.testing.assert_allclose(p_n, p_c, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(v_n, v_c, rtol=1e-12, atol=1e-30)
+    assert abs(v_c[0]) > 0

@@ -263,3 +263,37 @@ def test_neighbour_list_equals_all_pairs_bitwise(prec, sigma):
         eng.close()
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
     ff.close()
+
+
+def test_box_engine_nhc_matches_oracle():
+    """The box with the Nose-Hoover chain (thermostat 2): 125 molecules,
+    8 beads, momenta 1.5 times too hot; 30 steps against the oracle's ring
+    polymer with the same chain; set_state restarts the chain."""
+    import box_oracle as bo
+    nb = 8
+    s, m, beta, q, p = _box_start(125, 15.5, 8, nb)
+    p = 1.5 * p
+    ff = cff.ForceField(s, rc_angstrom=7.5)
+    dt = 0.1 * cam.FS_TO_AU
+    nose_q = 20.0 * dt
+    eng = cff.BoxEngine(ff, nb, dt, m, beta, ffbox.theta_ref(), precision=cam.CAM_PREC_FP64)
+    eng.set_nhc(nose_q)
+    eng.set_state(q, p)
+    t = bo.BoxRPMD(ol.FFOracle(s, rc_angstrom=7.5), ffbox.theta_ref(), q, p, dt, m, beta, nose_q=nose_q)
+    eng.run(30)
+    for _ in range(30):
+        t.step()
+    qg, pg = eng.get_state()
+    err = np.abs(qg - t.q).max() / np.abs(t.q).max()
+    perr = np.abs(pg - t.p).max() / np.abs(t.p).max()
+    print(f"box NHC: rel dq {err:.1e} dp {perr:.1e}; chain v1 {t.vnh[0]:.3e}")
+    assert err < 1e-10 and perr < 1e-8
+    assert abs(t.vnh[0]) > 0
+    eng.set_state(q, p)
+    eng.run(30)
+    q2, _ = eng.get_state()
+    assert np.array_equal(q2, qg)
+    with pytest.raises(cam.CamError, match="andersen"):
+        e2 = cff.BoxEngine(ff, nb, dt, m, beta, ffbox.theta_ref(), andersen_step=5)
+        e2.set_nhc(nose_q)
+    eng.close()
