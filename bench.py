@@ -355,6 +355,18 @@ def cpu_count_for_sample():
     return 1
 
 
+def rpmd_rate_h_h2(scale=0.1):
+    """The reference's H + H2 rate example (examples/calc_rate/h+h2/rate.key)
+    end to end -- start structures, 110 umbrella windows, PMF, recrossing,
+    k(T) -- with every step count times scale; wall time per phase."""
+    from caracal_amd.rpmdrate import h_h2_example
+    out = h_h2_example(scale=scale)
+    return {"scale": scale, "wall_s": out["wall_s"], "total_wall_s": sum(out["wall_s"].values()),
+            "xi_barrier": out["xi_barrier"], "dW_kcal_mol": (out["pmf_max"] - out["pmf_min"]) * 627.5094743,
+            "kappa": out["kappa"], "k_cm3_molecule_s": out["k_cm3_molecule_s"],
+            "note": "the key file's own settings (scale 1) took 53 s on one MI355X"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +479,7 @@ def main():
         out["ragged"] = ragged_batch(cam, workloads)
         out["box"] = box_nvt(cam)
         out["force_matching"] = force_matching(cam)
+        out["rpmd_rate_h_h2"] = rpmd_rate_h_h2()
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 
