@@ -1,23 +1,35 @@
 // Drop-in analytic surfaces: the bead-batched plugin contract of the
-// reference (egrad_h3, src/egrad_h3.f:29-77; egrad_ch4h, src/egrad_ch4h.f:74-129).
+// reference (egrad_h3, src/egrad_h3.f:29-77; egrad_ch4h, src/egrad_ch4h.f:74-129;
+// egrad_oh3, src/egrad_oh3.f:33-190).
 // One lane per bead.  A block stages its 256 beads through LDS so the global
 // reads and writes are coalesced although each bead's coordinates are one
 // contiguous 3*natoms-double record.
 #include "common.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
+#include "pes_oh3.hpp"
 
 namespace cam {
 namespace {
 
 constexpr int kBlock = 256;
 
-// H3: one lane per bead.
+// Surfaces evaluated one lane per bead.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void egrad_h3_kernel(const double *__restrict__ q, int nbeads,
-                                                          double *__restrict__ v, double *__restrict__ g)
+struct LaneH3 {
+    static constexpr int NC = 9;
+    __device__ static void egrad(const double *x, double &v, double *g) { h3::egrad_bead<T>(x, v, g); }
+};
+struct LaneOH3 {
+    static constexpr int NC = 12;
+    __device__ static void egrad(const double *x, double &v, double *g) { oh3::egrad_bead(x, v, g); }
+};
+
+template <typename P>
+__global__ __launch_bounds__(kBlock) void egrad_lane_kernel(const double *__restrict__ q, int nbeads,
+                                                            double *__restrict__ v, double *__restrict__ g)
 {
-    constexpr int NC = 9;
+    constexpr int NC = P::NC;
     __shared__ double tile[kBlock * NC];
     const long b0 = (long)blockIdx.x * kBlock;
     const int nb = (int)min((long)kBlock, (long)nbeads - b0);
@@ -28,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void egrad_h3_kernel(const double *__restri
     if (act) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) x[c] = tile[threadIdx.x * NC + c];
-        h3::egrad_bead<T>(x, vb, gr);
+        P::egrad(x, vb, gr);
     }
     __syncthreads();
     if (act) {
@@ -83,9 +95,12 @@ int launch_egrad(const double *q, int nbeads, double *v, double *g, int precisio
     if constexpr (NA == 3) {
         const int grid = (nbeads + kBlock - 1) / kBlock;
         if (precision == CAM_PREC_MIXED)
-            hipLaunchKernelGGL(egrad_h3_kernel<float>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+            hipLaunchKernelGGL(egrad_lane_kernel<LaneH3<float>>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
         else
-            hipLaunchKernelGGL(egrad_h3_kernel<double>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+            hipLaunchKernelGGL(egrad_lane_kernel<LaneH3<double>>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+    } else if constexpr (NA == 4) {
+        const int grid = (nbeads + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(egrad_lane_kernel<LaneOH3>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
     } else {
         const int grid = (nbeads + kBlock / 4 - 1) / (kBlock / 4);
         if (precision == CAM_PREC_MIXED)
@@ -104,6 +119,7 @@ int egrad_dev(const double *q, int natoms, int nbeads, double *v, double *g, int
     CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
     CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null device pointer");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision %d", precision);
+    CAM_CHECK_ARG(NA != 4 || precision == CAM_PREC_FP64, "the OH3 surface is evaluated in FP64 only");
     clear_error();
     return launch_egrad<NA>(q, nbeads, v, g, precision, (hipStream_t)stream);
 }
@@ -152,6 +168,14 @@ int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvd
 int cam_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
     return cam::egrad_host<6>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_oh3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    return cam::egrad_host<4>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_oh3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
+{
+    return cam::egrad_dev<4>(q, natoms, nbeads, v, dvdq, precision, stream);
 }
 int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {

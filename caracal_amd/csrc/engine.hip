@@ -1563,7 +1563,9 @@ __global__ void observe_kernel(TrajArgs a, const int *pfail, double *epot, doubl
     if (with_xi) {
         double c[3 * CAM_MAX_ATOMS];
         centroid_of(a, t, c);
-        xi[t] = a.natoms == 3 ? calc_xi<3>(a.rx, c, nullptr) : calc_xi<6>(a.rx, c, nullptr);
+        xi[t] = a.natoms == 3   ? calc_xi<3>(a.rx, c, nullptr)
+                : a.natoms == 4 ? calc_xi<4>(a.rx, c, nullptr)
+                                : calc_xi<6>(a.rx, c, nullptr);
     } else {
         xi[t] = 0.0;
     }
@@ -1688,6 +1690,16 @@ force_fn pick_force_split()
     else return force_kernel<S, NB, T, 1>;
 }
 
+// Calls f with the engine's atom count as a compile-time constant (natoms is
+// 3, 4 or 6 and matches the surface; checked at creation).
+template <typename F>
+void with_natoms(int natoms, F &&f)
+{
+    if (natoms == 3) f(std::integral_constant<int, 3>{});
+    else if (natoms == 4) f(std::integral_constant<int, 4>{});
+    else f(std::integral_constant<int, 6>{});
+}
+
 template <class S, typename T>
 // ens: the fused step's ensemble; ens_nm: that of the split schedule's
 // normal-mode kernel.
@@ -1793,6 +1805,8 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
         ok = cfg->precision == CAM_PREC_FP64
                  ? pick_nb<SurfH3, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
                  : pick_nb<SurfH3, float>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+    else if (cfg->surface == CAM_SURFACE_OH3)
+        ok = pick_nb<SurfOH3, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
     else
         ok = cfg->precision == CAM_PREC_FP64
                  ? pick_nb<SurfCH4H, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
@@ -1918,27 +1932,24 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
         ta.p = a.p;
         ta.g = a.g;
         const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
-        const bool h3 = e->cfg.natoms == 3;
         for (int j = 0; j < nsteps && err == hipSuccess; ++j) {
             a.nsteps = 1;
             a.step_off = 0;                // no previous step to finish inside the kernel
             hipLaunchKernelGGL(kn.nm, dim3(grid), dim3(kBlock), 0, e->stream, a);
             err = hipGetLastError();
             if (err != hipSuccess) break;
-            if (h3) hipLaunchKernelGGL(parent_constrain_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail);
-            else hipLaunchKernelGGL(parent_constrain_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail);
+            with_natoms(e->cfg.natoms, [&](auto na) {
+                hipLaunchKernelGGL(parent_constrain_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail);
+            });
             err = hipGetLastError();
             if (err != hipSuccess) break;
             hipLaunchKernelGGL(kn.fsplit, dim3(grid), dim3(kBlock), 0, e->stream, a.q, a.g, a.vbead, a.ntraj);
             err = hipGetLastError();
             if (err != hipSuccess) break;
             const int as = e->cfg.andersen_step;
-            if (h3)
-                hipLaunchKernelGGL(parent_end_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi,
-                                   a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
-            else
-                hipLaunchKernelGGL(parent_end_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi,
-                                   a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
+            with_natoms(e->cfg.natoms, [&](auto na) {
+                hipLaunchKernelGGL(parent_end_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.pdxi, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_after);
+            });
             err = hipGetLastError();
         }
         return err;
@@ -1960,14 +1971,9 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
             hipLaunchKernelGGL(kn.fsplit, dim3(grid), dim3(kBlock), 0, e->stream, a.q, a.g, a.vbead, a.ntraj);
             err = hipGetLastError();
             if (err != hipSuccess) break;
-            if (e->cfg.natoms == 3)
-                hipLaunchKernelGGL(umbrella_end_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g,
-                                   e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max,
-                                   e->pcheck_xi_tol, e->pcheck_after);
-            else
-                hipLaunchKernelGGL(umbrella_end_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g,
-                                   e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max,
-                                   e->pcheck_xi_tol, e->pcheck_after);
+            with_natoms(e->cfg.natoms, [&](auto na) {
+                hipLaunchKernelGGL(umbrella_end_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g, e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_xi_tol, e->pcheck_after);
+            });
             err = hipGetLastError();
         }
         return err;
@@ -1999,8 +2005,9 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
         err = hipGetLastError();
         if (err == hipSuccess && e->cfg.ensemble == CAM_ENS_CHILD) {
             const unsigned xgrid = (unsigned)(((size_t)a.nsteps * a.ntraj + 255) / 256);
-            if (e->cfg.natoms == 3) hipLaunchKernelGGL(xi_flag_kernel<3>, dim3(xgrid), dim3(256), 0, e->stream, a);
-            else hipLaunchKernelGGL(xi_flag_kernel<6>, dim3(xgrid), dim3(256), 0, e->stream, a);
+            with_natoms(e->cfg.natoms, [&](auto na) {
+                hipLaunchKernelGGL(xi_flag_kernel<decltype(na)::value>, dim3(xgrid), dim3(256), 0, e->stream, a);
+            });
             err = hipGetLastError();
         }
         if (err == hipSuccess && transrot) {
@@ -2034,13 +2041,15 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg && out, "null argument");
     *out = nullptr;
     const int surf = cfg->surface;
-    CAM_CHECK_ARG(surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_CH4H, "unknown surface %d", surf);
-    const int na = surf == CAM_SURFACE_H3 ? 3 : 6;
+    CAM_CHECK_ARG(surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_CH4H || surf == CAM_SURFACE_OH3,
+                  "unknown surface %d", surf);
+    const int na = surf == CAM_SURFACE_H3 ? 3 : surf == CAM_SURFACE_OH3 ? 4 : 6;
     CAM_CHECK_ARG(cfg->natoms == na, "surface %d needs natoms = %d (got %d)", surf, na, cfg->natoms);
     const int nb = cfg->nbeads;
     CAM_CHECK_ARG(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, "nbeads must be a power of two in 1..64 (got %d)", nb);
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
+    CAM_CHECK_ARG(surf != CAM_SURFACE_OH3 || cfg->precision == CAM_PREC_FP64, "the OH3 surface runs in FP64 only");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
     CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_NHC, "bad ensemble");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_NHC || cfg->nose_q > 0.0, "ensemble NHC needs nose_q > 0");
@@ -2117,7 +2126,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     std::unique_ptr<cam_engine> e(new cam_engine());
     e->cfg = *cfg;
     e->rx = rx;
-    const int L = surf == CAM_SURFACE_H3 ? 1 : 4;
+    const int L = surf == CAM_SURFACE_CH4H ? 4 : 1;
     e->tpb = kBlock / (L * nb);
     // steps fused per launch: state stays in registers for this many steps
     e->spl = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 50;
@@ -2310,10 +2319,9 @@ int cam_engine_constrain(cam_engine *e, int project_p, int *status)
     CAM_HIP(hipMalloc(&dstat, (size_t)e->cfg.ntraj * sizeof(int)));
     const TrajArgs ta = traj_args(e);
     const int grid = (e->cfg.ntraj + 127) / 128;
-    if (e->cfg.natoms == 3)
-        hipLaunchKernelGGL(constrain_kernel<3>, dim3(grid), dim3(128), 0, e->stream, ta, project_p, dstat);
-    else
-        hipLaunchKernelGGL(constrain_kernel<6>, dim3(grid), dim3(128), 0, e->stream, ta, project_p, dstat);
+    with_natoms(e->cfg.natoms, [&](auto na) {
+        hipLaunchKernelGGL(constrain_kernel<decltype(na)::value>, dim3(grid), dim3(128), 0, e->stream, ta, project_p, dstat);
+    });
     hipError_t le = hipGetLastError();
     if (le == hipSuccess && status)
         le = hipMemcpyAsync(status, dstat, (size_t)e->cfg.ntraj * sizeof(int), hipMemcpyDeviceToHost, e->stream);
@@ -2334,10 +2342,9 @@ int cam_engine_begin_children(cam_engine *e)
     clear_error();
     const TrajArgs ta = traj_args(e);
     const int grid = (e->cfg.ntraj + 127) / 128;
-    if (e->cfg.natoms == 3)
-        hipLaunchKernelGGL(child_begin_kernel<3>, dim3(grid), dim3(128), 0, e->stream, ta, e->weight, e->den_part);
-    else
-        hipLaunchKernelGGL(child_begin_kernel<6>, dim3(grid), dim3(128), 0, e->stream, ta, e->weight, e->den_part);
+    with_natoms(e->cfg.natoms, [&](auto na) {
+        hipLaunchKernelGGL(child_begin_kernel<decltype(na)::value>, dim3(grid), dim3(128), 0, e->stream, ta, e->weight, e->den_part);
+    });
     CAM_LAUNCH_CHECK();
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, e->stream, e->den_part, e->cfg.ntraj, e->den);
     CAM_LAUNCH_CHECK();
@@ -2363,12 +2370,9 @@ int cam_engine_begin_parent(cam_engine *e, double k_force, double epot_max, int 
     if (rc != CAM_OK) return rc;
     const TrajArgs ta = traj_args(e);
     const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
-    if (e->cfg.natoms == 3)
-        hipLaunchKernelGGL(parent_begin_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
-                           k_force, e->pdxi, e->pfail);
-    else
-        hipLaunchKernelGGL(parent_begin_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
-                           k_force, e->pdxi, e->pfail);
+    with_natoms(e->cfg.natoms, [&](auto na) {
+        hipLaunchKernelGGL(parent_begin_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g, k_force, e->pdxi, e->pfail);
+    });
     CAM_LAUNCH_CHECK();
     hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);   // steps since the start
     CAM_LAUNCH_CHECK();
@@ -2410,12 +2414,9 @@ int cam_engine_begin_umbrella(cam_engine *e, int bias, double epot_max, double x
     if (rc != CAM_OK) return rc;
     const TrajArgs ta = traj_args(e);
     const int pgrid = (e->cfg.ntraj + kParentWaves - 1) / kParentWaves;
-    if (e->cfg.natoms == 3)
-        hipLaunchKernelGGL(umbrella_begin_kernel<3>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
-                           e->uwin, e->ustat, e->pfail, bias);
-    else
-        hipLaunchKernelGGL(umbrella_begin_kernel<6>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g,
-                           e->uwin, e->ustat, e->pfail, bias);
+    with_natoms(e->cfg.natoms, [&](auto na) {
+        hipLaunchKernelGGL(umbrella_begin_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, e->g, e->uwin, e->ustat, e->pfail, bias);
+    });
     CAM_LAUNCH_CHECK();
     hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);   // steps since the start
     CAM_LAUNCH_CHECK();
@@ -2766,7 +2767,9 @@ int cam_batch_create(const cam_config *cfgs, int nbuckets, cam_batch **out)
         }
         b->eng.push_back(e);
         // flops of one step with elementary functions weighted 20 flops each
-        const double bead = cfgs[i].surface == CAM_SURFACE_H3 ? 1438.0 + 20.0 * 22 : 12707.0 + 20.0 * 321;
+        const double bead = cfgs[i].surface == CAM_SURFACE_H3    ? 1438.0 + 20.0 * 22
+                            : cfgs[i].surface == CAM_SURFACE_OH3 ? 420.0 + 20.0 * 26
+                                                                 : 12707.0 + 20.0 * 321;
         work[i] = (double)cfgs[i].ntraj * cfgs[i].nbeads * bead;
     }
     b->order.resize(nbuckets);

@@ -349,6 +349,7 @@ step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensembl
     const bool f64 = precision == CAM_PREC_FP64;
     if (surface == CAM_SURFACE_H3)
         return f64 ? pick_fold<SurfH3, double>(nbeads, ensemble, tpb) : pick_fold<SurfH3, float>(nbeads, ensemble, tpb);
+    if (surface == CAM_SURFACE_OH3) return f64 ? pick_fold<SurfOH3, double>(nbeads, ensemble, tpb) : nullptr;
     return f64 ? pick_fold<SurfCH4H, double>(nbeads, ensemble, tpb) : pick_fold<SurfCH4H, float>(nbeads, ensemble, tpb);
 }
 

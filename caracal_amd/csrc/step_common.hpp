@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
+#include "pes_oh3.hpp"
 #include "nhc.hpp"
 
 namespace cam {
@@ -31,6 +32,15 @@ struct SurfH3 {
     __device__ static bool valid(int, int) { return true; }
     template <typename T>
     __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<double>(own, v, g); }
+};
+
+// OH + H2: one lane per bead, FP64 only (the engine rejects mixed precision).
+struct SurfOH3 {
+    static constexpr int NA = 4, NC = 12, L = 1, SLOTS = 12;
+    __device__ static int coord(int slot, int) { return slot; }
+    __device__ static bool valid(int, int) { return true; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { oh3::egrad_bead(own, v, g); }
 };
 
 struct SurfCH4H {

@@ -50,7 +50,7 @@ def make_config(surface, nbeads, ntraj, dt, masses, *, precision=_abi.CAM_PREC_F
                 ensemble=_abi.CAM_ENS_NVE, remove_com=False, andersen_step=0, steps_per_launch=0, seed=0,
                 reaction=None, nm_kernel=_abi.CAM_NMK_AUTO, waves_per_eu=0, schedule=_abi.CAM_SCHED_AUTO, nose_q=0.0):
     """cam_config of one engine (or of one bucket of a batch)."""
-    natoms = 3 if surface == _abi.CAM_SURFACE_H3 else 6
+    natoms = _abi.NATOMS[surface]
     masses = np.asarray(masses, dtype=np.float64)
     if masses.shape != (natoms,):
         raise ValueError(f"need {natoms} masses")

@@ -15,7 +15,7 @@ module caracal_amd_mod
    private
 
    integer(c_int), parameter, public :: CAM_OK = 0
-   integer(c_int32_t), parameter, public :: CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2
+   integer(c_int32_t), parameter, public :: CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, &
@@ -74,6 +74,14 @@ module caracal_amd_mod
       end function
       ! drop-in for egrad_ch4h(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_ch4h.f:74)
       function cam_egrad_ch4h(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_ch4h") result(rc)
+         import :: c_int, c_double
+         real(c_double), intent(in) :: q(*)
+         integer(c_int), value :: natoms, nbeads
+         real(c_double), intent(out) :: v(*), dvdq(*)
+         integer(c_int) :: rc
+      end function
+      ! drop-in for egrad_oh3(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_oh3.f:33)
+      function cam_egrad_oh3(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_oh3") result(rc)
          import :: c_int, c_double
          real(c_double), intent(in) :: q(*)
          integer(c_int), value :: natoms, nbeads
@@ -262,7 +270,7 @@ module caracal_amd_mod
       end function
    end interface
 
-   public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h
+   public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h, cam_egrad_oh3
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_begin_parent, cam_engine_run, cam_engine_sync, cam_engine_kappa
@@ -270,7 +278,7 @@ module caracal_amd_mod
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode
-   public :: egrad_h3_gpu, egrad_ch4h_gpu, cam_error_message
+   public :: egrad_h3_gpu, egrad_ch4h_gpu, egrad_oh3_gpu, cam_error_message
 
 contains
 
@@ -291,6 +299,14 @@ contains
       integer, intent(out) :: info
       info = cam_egrad_ch4h(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
    end subroutine egrad_ch4h_gpu
+
+   subroutine egrad_oh3_gpu(q, natoms, nbeads, v, dvdq, info)
+      integer, intent(in) :: natoms, nbeads
+      real(c_double), intent(in) :: q(3, natoms, nbeads)
+      real(c_double), intent(out) :: v(nbeads), dvdq(3, natoms, nbeads)
+      integer, intent(out) :: info
+      info = cam_egrad_oh3(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
+   end subroutine egrad_oh3_gpu
 
    function cam_error_message() result(msg)
       character(len=:), allocatable :: msg

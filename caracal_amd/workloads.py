@@ -6,6 +6,8 @@ h3_nve        3 H atoms (1.008 amu) on BKMP2, collinear at R = 1.757 bohr
 ch4h_children CH4 + H recrossing children on the CH5 surface: saddle
               geometry + N(0, 0.02^2) per trajectory, centroid constrained to
               xi = 0 (dividing surface xi_ideal = 1), momenta at 500 K.
+oh3_nve       OH + H2 (O 15.999, H 1.008 amu) on the Schatz-Elgersma surface at
+              its saddle, bead positions + N(0, 0.01^2) bohr, momenta at 300 K.
 ragged        H3 / CH4H half-and-half with nbeads in {4,8,16,32,64} and
               T in U[200, 1000] K, seed 1 (bucketed by surface and nbeads).
 """
@@ -28,6 +30,17 @@ CH4H_TS = np.array([
     [2.4775365649116083, 2.477536564911461, 2.477536564911683],
 ])
 
+# First-order saddle of the OH + H2 surface of src/egrad_oh3.f (bohr; atoms O,
+# H1, H2, H3), planar, V = 6.0909 kcal/mol, one imaginary mode, |dV/dq| ~ 1e-16
+# (tests/golden/make_oh3_saddle.py).
+OH3_TS = np.array([
+    [-1.3546985242473657, -0.2317507462336809, 0.0],
+    [-2.098485218890634, 1.4691079024461797, 0.0],
+    [0.9504565430735537, -0.3544506279978717, 0.0],
+    [2.502727200064447, -0.8829065282146269, 0.0],
+])
+MASSES_OH3_AMU = [15.999, 1.008, 1.008, 1.008]
+
 
 def h3_reaction():
     """H_A H_B + H_C -> H_A + H_B H_C (examples/calc_rate/h+h2/rate.key)."""
@@ -44,6 +57,14 @@ def ch4h_reaction():
                     reactants=[[1, 2, 3, 4, 5], [6]], r_inf=16.0, xi_ideal=1.0)
 
 
+def oh3_reaction():
+    """OH + H2 -> H2O + H: O-H2 forms, H2-H3 breaks, at the saddle's lengths."""
+    t = OH3_TS
+    return Reaction(bond_form=[(1, 3)], bond_break=[(3, 4)],
+                    form_ref=[float(np.linalg.norm(t[0] - t[2]))], break_ref=[float(np.linalg.norm(t[2] - t[3]))],
+                    reactants=[[1, 2], [3, 4]], r_inf=16.0, xi_ideal=1.0)
+
+
 def mb_momenta(rng, masses, nbeads, beta, shape):
     """Maxwell-Boltzmann bead momenta: p ~ N(0, m nbeads / beta) (andersen.f90:50-56)."""
     sd = np.sqrt(np.asarray(masses)[:, None] * nbeads / np.asarray(beta))  # (natoms, 1)
@@ -56,6 +77,15 @@ def h3_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
     beta = float(beta_of(kelvin))
     q = H3_TS[None] + rng.normal(0.0, 0.01, (nbeads, 3, 3))
     p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 3, 3))
+    return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
+
+
+def oh3_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
+    rng = np.random.default_rng(seed)
+    masses = mass_au(MASSES_OH3_AMU)
+    beta = float(beta_of(kelvin))
+    q = OH3_TS[None] + rng.normal(0.0, 0.01, (nbeads, 4, 3))
+    p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 4, 3))
     return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
 
 

@@ -50,6 +50,12 @@ int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvd
  * initialize_ch4h (src/read_pes.f90:551).  natoms must be 6, atom order of the
  * reference surface: h4, C, h1, h2, h3, H_attacking. */
 int cam_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+/* Replaces: egrad_oh3 (src/egrad_oh3.f:33-190, OH + H2 of Schatz and
+ * Elgersma), initialised by initialize_oh3 (src/read_pes.f90:491).  natoms
+ * must be 4, atom order O, H1 (hydroxyl), H2, H3.  Every bead gets its own
+ * gradient (the reference clears the whole array inside its bead loop and is
+ * only called one bead at a time). */
+int cam_egrad_oh3(const double *q, int natoms, int nbeads, double *v, double *dvdq);
 
 /* Device-resident variants: q, v, dvdq are device pointers, stream is a
  * hipStream_t (NULL = default stream); the call is asynchronous on that
@@ -59,13 +65,17 @@ int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double 
                      int precision, void *stream);
 int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
                        int precision, void *stream);
+/* OH3 has no binary32 form: precision must be CAM_PREC_FP64. */
+int cam_egrad_oh3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
+                      int precision, void *stream);
 
 /* ---- (2) trajectory-batch RPMD engine ---- */
-enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2 };
+enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3 };
 /* Precision of the engine.  MIXED: CH4H forces in FP32 arithmetic with FP64
  * positions, momenta, accumulation and normal modes.  The H3 surface is
  * evaluated in FP64 by the engine under either setting (its FP32 correction
- * terms gave wrong forces in multi-step kernels); cam_egrad_h3_dev keeps them. */
+ * terms gave wrong forces in multi-step kernels); cam_egrad_h3_dev keeps them.
+ * The OH3 surface runs in FP64 only (CAM_PREC_MIXED is rejected). */
 enum { CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1 };
 /* Normal-mode transform.  REFERENCE reproduces the reference's rfft/irfft pair
  * (src/rfft.f90:36-62, src/irfft.f90:36-63): both directions are the
@@ -149,7 +159,7 @@ typedef struct {
 
 typedef struct {
     int32_t surface;          /* CAM_SURFACE_* */
-    int32_t natoms;           /* 3 (H3) or 6 (CH4H) */
+    int32_t natoms;           /* 3 (H3), 6 (CH4H) or 4 (OH3) */
     int32_t nbeads;           /* 1, 2, 4, 8, 16, 32 or 64 */
     int32_t ntraj;            /* trajectories in the batch */
     int32_t precision;        /* CAM_PREC_* */

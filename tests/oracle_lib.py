@@ -37,6 +37,7 @@ def ref():
     if _ref is None and os.path.exists(REF_SO):
         _ref = ctypes.CDLL(REF_SO)
         _ref.initialize_ch4h_()
+        _ref.initialize_oh3_()
     return _ref
 
 
@@ -57,6 +58,10 @@ def egrad_ch4h(q):
     return _egrad(oracle().oracle_egrad_ch4h, q)
 
 
+def egrad_oh3(q):
+    return _egrad(oracle().oracle_egrad_oh3, q)
+
+
 def _ref_egrad(fn, q):
     q = np.ascontiguousarray(q, dtype=np.float64)
     nb, na, _ = q.shape
@@ -73,6 +78,14 @@ def ref_egrad_h3(q):
 
 def ref_egrad_ch4h(q):
     return _ref_egrad(ref().egrad_ch4h_, q)
+
+
+def ref_egrad_oh3(q):
+    """One bead per call: the reference's egrad_oh3 clears the whole gradient
+    array inside its bead loop, and gradient.f90 calls it one bead at a time."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = [_ref_egrad(ref().egrad_oh3_, q[b:b + 1]) for b in range(q.shape[0])]
+    return np.concatenate([o[0] for o in out]), np.concatenate([o[1] for o in out])
 
 
 # ---- RPMD step (oracle/rpmd.c) ----
