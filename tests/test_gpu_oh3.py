@@ -2,7 +2,8 @@
 egrad contract and inside the trajectory engine, against the oracle: forces
 over a cloud and the special geometries (the VH2O shared-derivative quirk
 included), NVE trajectories on the three normal-mode kernels, recrossing
-children with kappa, and an umbrella window."""
+children with kappa, an umbrella window, and the trajectory output files of
+an engine run (caracal_amd.trajout) against the oracle's trajectory."""
 import numpy as np
 import pytest
 import torch
@@ -146,3 +147,35 @@ def test_oh3_engine_rejects_mixed_precision():
     w = workloads.oh3_nve(nbeads=4)
     with pytest.raises(cam.CamError, match="FP64 only"):
         cam.RPMDEngine(cam.CAM_SURFACE_OH3, 4, 1, w["dt"], w["masses"], precision=cam.CAM_PREC_MIXED)
+
+
+def test_oh3_trajectory_files_follow_the_oracle(tmp_path):
+    """dynamic.x-style output of an engine trajectory (trajectory.xyz and the
+    verbose gradient and velocity files every 10 steps) reads back equal to
+    the oracle's trajectory at those steps."""
+    from caracal_amd.trajout import BOHR, TrajectoryWriter, read_indexed_frames, read_xyz_frames
+    nb = 8
+    w = workloads.oh3_nve(nbeads=nb)
+    e = cam.RPMDEngine(cam.CAM_SURFACE_OH3, nb, 1, w["dt"], w["masses"])
+    e.set_state(w["q"][None], w["p"][None], w["beta"])
+    t = ol.OracleTraj(ol.make_sys(3, nb, w["dt"], w["masses"]), w["q"], w["p"], w["beta"], ensemble=0)
+    ref = []
+    with TrajectoryWriter(["O", "H", "H", "H"], tmp_path, verbose=True) as out:
+        for _ in range(3):
+            e.run(10)
+            for _ in range(10):
+                t.step()
+            q, p = e.get_state()
+            _, g = e.get_forces()
+            out.write_md_frame(q[0], g[0], p[0], w["masses"])
+            ref.append((t.q.copy(), t.g.copy(), t.p.copy()))
+    xyz = read_xyz_frames(tmp_path / "trajectory.xyz")
+    grads = read_indexed_frames(tmp_path / "md_gradients.dat")
+    vels = read_indexed_frames(tmp_path / "md_velocities.dat")
+    assert len(xyz) == len(grads) == len(vels) == 3
+    for (qr, gr, pr), (_, names, x), gf, vf in zip(ref, xyz, grads, vels):
+        assert names == ["O", "H", "H", "H"] * nb
+        np.testing.assert_allclose(x / BOHR, qr.reshape(-1, 3), rtol=0, atol=1e-10 * np.abs(qr).max())
+        np.testing.assert_allclose(gf[:, 2:], gr.reshape(-1, 3), rtol=0, atol=1e-9 * np.abs(gr).max())
+        v = (pr / w["masses"][None, :, None]).reshape(-1, 3)
+        np.testing.assert_allclose(vf[:, 2:], v, rtol=0, atol=1e-9 * np.abs(v).max())

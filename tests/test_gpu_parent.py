@@ -19,6 +19,12 @@ def _parents(surface, nb, ntraj, seed):
         q = np.stack([w["q"]] * ntraj) + rng.normal(0, 0.01, (ntraj,) + w["q"].shape)
         beta = np.full(ntraj, w["beta"]) * rng.uniform(0.7, 1.4, ntraj)
         rx = workloads.h3_reaction()
+    elif surface == cam.CAM_SURFACE_OH3:
+        w = workloads.oh3_nve(nbeads=nb)
+        rng = np.random.default_rng(seed)
+        q = np.stack([w["q"]] * ntraj) + rng.normal(0, 0.01, (ntraj,) + w["q"].shape)
+        beta = np.full(ntraj, w["beta"]) * rng.uniform(0.7, 1.4, ntraj)
+        rx = workloads.oh3_reaction()
     else:
         w = workloads.ch4h_children(ntraj=ntraj, nbeads=nb, seed=seed)
         q, beta, rx = w["q"], w["beta"], w["reaction"]
@@ -29,7 +35,8 @@ def _parents(surface, nb, ntraj, seed):
 
 @pytest.mark.parametrize("surface,nb,nmk", [(cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU),
                                             (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA),
-                                            (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU)])
+                                            (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU),
+                                            (cam.CAM_SURFACE_OH3, 8, cam.CAM_NMK_VALU)])
 def test_parent_matches_oracle(surface, nb, nmk):
     ntraj, andersen_step, k_force, seed = 5, 7, 0.37, 11
     w, q, p, beta, rx = _parents(surface, nb, ntraj, seed=nb)
