@@ -355,6 +355,28 @@ def cpu_count_for_sample():
     return 1
 
 
+def oh3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
+    """OH + H2 (src/egrad_oh3.f): 4096 NVE trajectories of 32 beads at 300 K
+    around the saddle, FP64, autotuned schedule; bead-force-evals/s."""
+    import torch
+    w = workloads.oh3_nve(nbeads=nb)
+    rng = np.random.default_rng(2)
+    q = workloads.OH3_TS[None, None] + rng.normal(0.0, 0.02, (ntraj, nb, 4, 3))
+    p = workloads.mb_momenta(rng, w["masses"], nb, w["beta"], (ntraj, nb, 4, 3))
+    e = cam.RPMDEngine(cam.CAM_SURFACE_OH3, nb, ntraj, w["dt"], w["masses"])
+    e.set_state(q, p, np.full(ntraj, w["beta"]))
+    _, best = e.autotune(50)
+    e.prepare(steps)
+    e.run(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.run(steps)
+    wall = time.perf_counter() - t0
+    e.close()
+    return {"bead_force_evals_per_s": ntraj * nb * steps / wall, "us_per_step": wall / steps * 1e6,
+            "ntraj": ntraj, "nbeads": nb, "steps": steps, "schedule": best}
+
+
 def rpmd_rate_h_h2(scale=0.1):
     """The reference's H + H2 rate example (examples/calc_rate/h+h2/rate.key)
     end to end -- start structures, 110 umbrella windows, PMF, recrossing,
@@ -477,6 +499,7 @@ def main():
         out["speedup_mixed_vs_plain"] = mixed_value / plain
         out["speedup_fp64_batched_vs_plain"] = value / plain
         out["ragged"] = ragged_batch(cam, workloads)
+        out["oh3"] = oh3_batch(cam, workloads)
         out["box"] = box_nvt(cam)
         out["force_matching"] = force_matching(cam)
         out["rpmd_rate_h_h2"] = rpmd_rate_h_h2()
