@@ -30,7 +30,14 @@ program rpmd_driver
    call get_command_argument(2, fout)
    open(newunit=u, file=trim(fin), access="stream", form="unformatted", status="old")
    read(u) hdr
-   natoms = merge(3, 6, hdr(1) == CAM_SURFACE_H3)
+   select case (hdr(1))
+   case (CAM_SURFACE_H3)
+      natoms = 3
+   case (CAM_SURFACE_OH3)
+      natoms = 4
+   case default
+      natoms = 6
+   end select
    nbeads = hdr(2)
    ntraj = hdr(3)
    nsteps = hdr(6)
@@ -64,6 +71,8 @@ program rpmd_driver
    allocate(v(nbeads*ntraj), g(3,natoms,nbeads,ntraj))
    if (hdr(1) == CAM_SURFACE_H3) then
       call egrad_h3_gpu(q0, natoms, nbeads*ntraj, v, g, info)
+   else if (hdr(1) == CAM_SURFACE_OH3) then
+      call egrad_oh3_gpu(q0, natoms, nbeads*ntraj, v, g, info)
    else
       call egrad_ch4h_gpu(q0, natoms, nbeads*ntraj, v, g, info)
    end if

@@ -1,6 +1,6 @@
 """The Fortran host path: caracal_amd/fortran/rpmd_driver (iso_c_binding
-module caracal_amd_mod) advances the H3 NVE ring polymer on the GPU through
-the C ABI; its output must match the oracle's verlet."""
+module caracal_amd_mod) advances an H3 or OH + H2 NVE ring polymer on the GPU
+through the C ABI; its output must match the oracle's verlet."""
 import os
 import subprocess
 
@@ -15,27 +15,30 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "caracal_amd", "fortran", "rpmd_driver")
 
 
-def test_fortran_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("surface", [1, 3], ids=["H3", "OH3"])
+def test_fortran_driver_matches_oracle(tmp_path, surface):
     if not os.path.exists(DRIVER):
         pytest.fail("caracal_amd/fortran/rpmd_driver is not built")
-    w = workloads.h3_nve(nbeads=16)
+    w = workloads.h3_nve(nbeads=16) if surface == 1 else workloads.oh3_nve(nbeads=16)
+    na = len(w["masses"])
     nsteps = 100
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
-        np.array([1, 16, 1, 0, 1, nsteps], dtype="<i4").tofile(f)
+        np.array([surface, 16, 1, 0, 1, nsteps], dtype="<i4").tofile(f)
         np.array([w["dt"], w["beta"]], dtype="<f8").tofile(f)
         np.asarray(w["masses"], dtype="<f8").tofile(f)
-        np.ascontiguousarray(w["q"], dtype="<f8").tofile(f)     # (nbeads,3,3) C order == q(3,3,nbeads)
+        np.ascontiguousarray(w["q"], dtype="<f8").tofile(f)     # (nbeads,na,3) C order == q(3,na,nbeads)
         np.ascontiguousarray(w["p"], dtype="<f8").tofile(f)
     out = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     data = np.fromfile(fout, dtype="<f8")
-    n = 16 * 9
-    q, p, v = data[:n].reshape(16, 3, 3), data[n:2 * n].reshape(16, 3, 3), data[2 * n:]
-    s = ol.make_sys(1, 16, w["dt"], w["masses"])
+    n = 16 * 3 * na
+    q, p, v = data[:n].reshape(16, na, 3), data[n:2 * n].reshape(16, na, 3), data[2 * n:]
+    s = ol.make_sys(surface, 16, w["dt"], w["masses"])
     t = ol.OracleTraj(s, w["q"], w["p"], w["beta"], ensemble=0, remove_com=True)
     for _ in range(nsteps):
         t.step()
     assert np.abs(q - t.q).max() / np.abs(t.q).max() < 1e-10
-    np.testing.assert_allclose(v, ol.egrad_h3(w["q"])[0], rtol=1e-12)
+    v_ref = (ol.egrad_h3 if surface == 1 else ol.egrad_oh3)(w["q"])[0]
+    np.testing.assert_allclose(v, v_ref, rtol=1e-12, atol=1e-15)
