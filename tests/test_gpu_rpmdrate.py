@@ -1,4 +1,5 @@
-"""The RPMD rate constant end to end on the GPU (caracal_amd.rpmdrate): the
+"""The RPMD rate constant end to end on the GPU (caracal_amd.rpmdrate), on
+H + H2 and on OH + H2 (the second test below).  First the
 H + H2 calculation of examples/calc_rate/h+h2/rate.key at 300 K and 8 beads,
 with every phase shortened and the windows spaced 0.05 (and extended to
 xi = 1.15, so that the barrier has windows on both sides).  Checks what the
@@ -40,6 +41,36 @@ def test_h_plus_h2_rate_end_to_end():
     assert 0.95 <= out["xi_barrier"] <= 1.06
     assert 8.0 <= barrier <= 16.0
     # the biased averages sit between their window centre and the barrier top
+    assert np.abs(out["average"] - out["xi_wins"]).max() < 0.1
+    assert 0.2 < out["kappa"] <= 1.05
+    assert np.isfinite(out["k_cm3_molecule_s"]) and out["k_cm3_molecule_s"] > 0
+
+
+def test_oh_plus_h2_rate_end_to_end():
+    """The same pipeline on the OH + H2 surface (src/egrad_oh3.f) at 300 K:
+    the PMF barrier sits near xi = 1 and is the surface's 6.09 kcal/mol
+    classical barrier plus the translational free energy lost between R_inf
+    and the saddle, less zero-point effects; two equivalent abstraction
+    paths; kappa in (0, 1]; a positive, finite rate."""
+    ts = workloads.OH3_TS
+    ts_energy = float(cam.egrad_oh3(ts[None])[0][0])
+    r = RPMDRate(cam.CAM_SURFACE_OH3, 8, 0.1 * FS_TO_AU, mass_au(workloads.MASSES_OH3_AMU), workloads.oh3_reaction(),
+                 300.0,
+                 umbrella=UmbrellaKeys(bias=0.05, bonds=(-0.05, 1.15), dist=0.05, gen_steps=300, equi_steps=300,
+                                       sample_steps=2000, sample_trajs=2),
+                 pmf=PmfKeys(xi_range=(-0.05, 1.15), bins=601),
+                 recross=RecrossKeys(equi_steps=500, child_total=256, child_interval=100, child_perpoint=64,
+                                     child_steps=300, nparents=4),
+                 andersen_step=80, n_paths=2, seed=3)
+    t0 = time.perf_counter()
+    out = r.run(ts, ts_energy)
+    wall = time.perf_counter() - t0
+    barrier = (out["pmf_max"] - out["pmf_min"]) * 627.5094743
+    print(f"OH + H2: wall {wall:.1f} s; xi_barrier {out['xi_barrier']:.3f}; dW {barrier:.2f} kcal/mol; "
+          f"kappa {out['kappa']:.3f}; k(T) {out['k_cm3_molecule_s']:.3e} cm^3/(molecule s); "
+          f"failed segments {out['errors']}")
+    assert 0.9 <= out["xi_barrier"] <= 1.1
+    assert 5.0 <= barrier <= 15.0
     assert np.abs(out["average"] - out["xi_wins"]).max() < 0.1
     assert 0.2 < out["kappa"] <= 1.05
     assert np.isfinite(out["k_cm3_molecule_s"]) and out["k_cm3_molecule_s"] > 0
