@@ -49,3 +49,37 @@ def test_ragged_mixed_forces_within_1e5_of_fp64_per_bucket():
     print("mixed vs fp64 relative force error per (surface, nbeads):",
           {f"{'H3' if s == cam.CAM_SURFACE_H3 else 'CH4H'}/{n}": f"{v:.2e}" for (s, n), v in errs.items()})
     assert max(errs.values()) <= 1e-5
+
+
+def test_ragged_batch_with_three_surfaces_in_fp64():
+    """H3, CH5 and OH + H2 buckets (4-32 beads) in one FP64 batch, each equal
+    bit for bit to its bucket run alone."""
+    rng = np.random.default_rng(4)
+    buckets = []
+    for surf, nb, n in ((cam.CAM_SURFACE_OH3, 16, 40), (cam.CAM_SURFACE_H3, 8, 30), (cam.CAM_SURFACE_CH4H, 4, 20),
+                        (cam.CAM_SURFACE_OH3, 32, 10)):
+        if surf == cam.CAM_SURFACE_OH3:
+            w = workloads.oh3_nve(nbeads=nb)
+            ts = workloads.OH3_TS
+        elif surf == cam.CAM_SURFACE_H3:
+            w = workloads.h3_nve(nbeads=nb)
+            ts = workloads.H3_TS
+        else:
+            w = workloads.ch4h_children(ntraj=1, nbeads=nb)
+            ts = workloads.CH4H_TS
+        beta = np.full(n, float(np.atleast_1d(w["beta"])[0]))
+        q = ts[None, None] + rng.normal(0.0, 0.02, (n, nb) + ts.shape)
+        p = workloads.mb_momenta(rng, w["masses"], nb, beta[0], (n, nb) + ts.shape)
+        buckets.append((surf, nb, dict(q=q, p=p, beta=beta, masses=w["masses"], dt=w["dt"])))
+    rb = cam.RaggedBatch(buckets, steps_per_launch=10)
+    rb.set_states()
+    rb.run(30)
+    for (surf, nb, d), eb in zip(buckets, rb.engines):
+        alone = cam.RPMDEngine(surf, nb, len(d["beta"]), d["dt"], d["masses"], steps_per_launch=10)
+        alone.set_state(d["q"], d["p"], d["beta"])
+        alone.run(30)
+        qa, pa = alone.get_state()
+        qb, pb = eb.get_state()
+        alone.close()
+        assert np.array_equal(qa, qb) and np.array_equal(pa, pb), (surf, nb)
+    rb.close()
