@@ -1,0 +1,62 @@
+"""Writes golden trajectories of the reference's own RPMD step.
+
+Runs oracle/_ref/ref_step_O1 (src/verlet.f90 and what it calls, compiled
+from the reference's sources by oracle/Makefile.ref) from seeded inputs and
+stores inputs and frames as small .npz fixtures beside this script:
+
+  c1_h3_nve_reference.npz     H + H2 on BKMP2, 16 beads, NVE with transrot
+                              (dynamic.x), 100 steps of 0.1 fs from the seeded
+                              collinear start (caracal_amd.workloads.h3_nve)
+  ch4h_child_reference.npz    one CH4 + H recrossing child (calc_rate,
+                              constrain = 2), 32 beads, 100 steps from a
+                              seeded start near the saddle, xi every step
+  oh3_nve_reference.npz       OH + H2, 8 beads, NVE with transrot, 100 steps
+
+Frames are kept at steps 1, 2, 10, 50 and 100 (q and p), with epot and xi
+at every step.  Run in the container where /root/reference is mounted:
+    python tests/golden/make_reference_trajectories.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import refstep  # noqa: E402
+from caracal_amd import workloads  # noqa: E402
+
+KEEP = np.array([1, 2, 10, 50, 100])
+
+
+def _save(name, surface, q, p, masses, dt, beta, constrain, reaction=None):
+    fr, _ = refstep.run(surface, q, p, masses, dt, beta, int(KEEP[-1]), constrain=constrain, every=1,
+                        reaction=reaction)
+    np.savez(os.path.join(HERE, name), q0=q, p0=p, masses=masses, dt=dt, beta=beta, steps=KEEP,
+             q=fr["q"][KEEP - 1], p=fr["p"][KEEP - 1], epot=fr["epot"], xi=fr["xi"])
+
+
+def child_start():
+    """A CH4 + H child start: the seeded saddle perturbation of the C2
+    workload (first trajectory) with momenta at 500 K from seed 7."""
+    w = workloads.ch4h_children(ntraj=1, nbeads=32)
+    rng = np.random.default_rng(7)
+    p = workloads.mb_momenta(rng, w["masses"], 32, w["beta"][0], (32, 6, 3))
+    return w, w["q"][0], p
+
+
+def main():
+    if not refstep.available():
+        raise SystemExit("oracle/_ref/ref_step_O1 is not built (make -C oracle -f Makefile.ref)")
+    h = workloads.h3_nve(nbeads=16)
+    _save("c1_h3_nve_reference.npz", "h3", h["q"], h["p"], h["masses"], h["dt"], h["beta"], -1)
+    w, q, p = child_start()
+    _save("ch4h_child_reference.npz", "ch4h", q, p, w["masses"], w["dt"], w["beta"][0], 2, w["reaction"])
+    o = workloads.oh3_nve(nbeads=8)
+    _save("oh3_nve_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], -1)
+
+
+if __name__ == "__main__":
+    main()
