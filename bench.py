@@ -40,12 +40,18 @@ CH4H_TRANSCENDENTALS_PER_BEAD = 321
 # minimum image included), one bond, one angle
 BOX_FLOPS = {"pair_in": 95, "pair_out": 21, "bond": 49, "angle": 116}
 FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
-# HBM bytes of one 50-step launch of the step kernel at 4096 trajectories x
-# 32 beads, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters scaled by a
-# plain 1 GiB copy (profiles/fold_step_kernel_ch4h_children_fp64.md,
-# profiles/step_kernel_ch4h_children_fp64_mfma.md; the latter was measured
-# before the child centroids moved to a buffer, 29.5 MB per 50 steps).
-MEASURED_TRAFFIC = {("folded", 4096, 32): 145.6e6, ("mfma", 4096, 32): 117.0e6}
+# HBM bytes of one launch of the folded child step kernel at 4096
+# trajectories x 32 beads, from rocprofv3 FETCH_SIZE and WRITE_SIZE counters
+# scaled by a plain 1 GiB copy (profiles/fold_step_kernel_ch4h_children_fp64.md):
+# a fixed part (q, p and the gradient read and written once per launch) and
+# a part per step (the child centroids of every step, 3 x 6 doubles per
+# trajectory), from launches of 50 and 100 steps.
+MEASURED_TRAFFIC = {("folded", 4096, 32): (113.3e6, 0.646e6)}
+
+
+def launch_traffic(nmk_name, ntraj, nbeads, spl):
+    t = MEASURED_TRAFFIC.get((nmk_name, ntraj, nbeads))
+    return None if t is None else t[0] + spl * t[1]
 
 
 NMK_NAMES = {1: "valu", 2: "mfma", 3: "folded"}
@@ -283,52 +289,80 @@ def plain_fp64(cam, workloads, steps):
     return 2 * 32 * steps / wall
 
 
-def cpu_baseline(seconds=15.0):
-    """The oracle's RPMD child step (a C port of the reference's verlet and
-    egrad_ch4h) on the host cores, one trajectory per process like the
-    reference's MPI task farm; a bounded sample of the same workload."""
-    import multiprocessing as mp
-    cores = min(16, os.cpu_count() or 1)
-    with mp.get_context("fork").Pool(cores) as pool:
-        res = pool.map(_cpu_worker, [(i, seconds) for i in range(cores)])
-    steps = sum(r[0] for r in res)
-    wall = max(r[1] for r in res)
-    out = dict(value=steps * 32 / wall, unit="bead-force-evals/s", cores=cores, kind="port",
-               sample=f"{cores} processes x {int(steps / cores)} child steps of one 32-bead CH4H trajectory "
-                      f"(oracle C port of verlet + egrad_ch4h, gcc -O2), {wall:.1f} s")
-    ratio = reference_surface_ratio()
-    if ratio is not None:
-        out["reference_surface"] = ratio
-    return out
-
-
-def reference_surface_ratio(seconds=3.0):
-    """The reference's own egrad_ch4h (its Fortran sources compiled with
-    amdflang at the reference's -O1, oracle/_ref) against the port's C
-    surface, one core each, on the same 2048 child-workload beads: relates the
-    port baseline to the reference (the surface is >95% of a child step's CPU
-    time).  None when the compiled reference is not present."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    from caracal_amd import workloads
+def cpu_model():
     try:
-        if ol.ref() is None:
-            return None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
-        return None
-    q = workloads.ch4h_children(ntraj=64, nbeads=32, seed=0)["q"].reshape(-1, 6, 3)
-    rates = {}
-    for name, fn in (("reference_fortran", ol.ref_egrad_ch4h), ("port_c", ol.egrad_ch4h)):
-        fn(q)
-        n, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < seconds:
-            fn(q)
-            n += q.shape[0]
-        rates[name] = n / (time.perf_counter() - t0)
-    return {"reference_fortran_bead_evals_per_s_1core": rates["reference_fortran"],
-            "port_c_bead_evals_per_s_1core": rates["port_c"],
-            "reference_over_port": rates["reference_fortran"] / rates["port_c"],
-            "sample": f"egrad_ch4h on 2048 beads, {seconds:.0f} s each, one core"}
+        pass
+    return "unknown"
+
+
+def cpu_baseline(steps=40000):
+    """The CPU path timed on this host before the GPU is touched: the
+    reference's own recrossing-child step (src/verlet.f90 with constrain = 2:
+    FFTW normal modes, calc_xi of the centroid, egrad_ch4h for every bead),
+    compiled from its sources at the reference's -O1 (and at -O2) by
+    oracle/Makefile.ref, one 32-bead CH4H child per process on every core of
+    this process's share (at most 16), as the reference's MPI task farm runs
+    one trajectory per rank.  value = total bead-force-evals/s.  Falls back
+    to the oracle's C port of the same step when the compiled reference is
+    absent (kind "port")."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refstep
+    from caracal_amd import workloads
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = min(16, ncpu)
+    w = workloads.ch4h_children(ntraj=1, nbeads=32)
+    rng = np.random.default_rng(7)
+    p = workloads.mb_momenta(rng, w["masses"], 32, w["beta"][0], (32, 6, 3))
+    res = {"unit": "bead-force-evals/s", "cores": cores, "cpu_model": cpu_model()}
+    if not refstep.available("O1"):
+        res.update(cpu_baseline_port(cores, 10.0))
+        return res
+
+    def farm(opt):
+        with tempfile.TemporaryDirectory() as tmp:
+            fin = os.path.join(tmp, "in.bin")
+            refstep._write_input(fin, "ch4h", w["q"][0], p, w["masses"], w["dt"], w["beta"][0], steps, 2, 0,
+                                 w["reaction"])
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([refstep.binary(opt), fin, os.path.join(tmp, f"o{i}.bin")],
+                                      stdout=subprocess.PIPE, text=True, cwd=tmp) for i in range(cores)]
+            secs = []
+            for pr in procs:
+                so, _ = pr.communicate()
+                if pr.returncode != 0:
+                    raise RuntimeError(f"reference step harness failed ({pr.returncode})")
+                secs.append(float(so.split("seconds")[-1].split()[0]))
+            wall = time.perf_counter() - t0
+        return cores * steps * 32 / max(secs), max(secs), wall
+
+    v1, s1, wall1 = farm("O1")
+    res.update(value=v1, kind="reference",
+               sample=f"{cores} processes x {steps} child steps of one 32-bead CH4H child trajectory "
+                      f"(the reference's verlet + calc_xi + egrad_ch4h, amdflang -O1 as src/Makefile:44), "
+                      f"slowest process {s1:.1f} s of step loop ({wall1:.1f} s wall with start-up)")
+    if refstep.available("O2"):
+        v2, s2, _ = farm("O2")
+        res["O2"] = {"value": v2, "seconds": s2}
+    return res
+
+
+def cpu_baseline_port(cores, seconds):
+    """The oracle's C port of the same child step, one trajectory per process."""
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(cores) as pool:
+        out = pool.map(_cpu_worker, [(i, seconds) for i in range(cores)])
+    steps = sum(r[0] for r in out)
+    wall = max(r[1] for r in out)
+    return dict(value=steps * 32 / wall, kind="port",
+                sample=f"{cores} processes x {int(steps / cores)} child steps of one 32-bead CH4H trajectory "
+                       f"(oracle C port of verlet + egrad_ch4h, gcc -O2), {wall:.1f} s")
 
 
 def _cpu_worker(args):
@@ -336,7 +370,7 @@ def _cpu_worker(args):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     from caracal_amd import workloads
-    w = workloads.ch4h_children(ntraj=cpu_count_for_sample(), nbeads=32, seed=idx)
+    w = workloads.ch4h_children(ntraj=1, nbeads=32, seed=idx)
     s = ol.make_sys(2, 32, w["dt"], w["masses"])
     rx = ol.make_rx(w["reaction"])
     rng = np.random.default_rng(idx)
@@ -349,10 +383,6 @@ def _cpu_worker(args):
             t.step()
         n += 10
     return n, time.perf_counter() - t0
-
-
-def cpu_count_for_sample():
-    return 1
 
 
 def oh3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
@@ -396,7 +426,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--ntraj", type=int, default=4096)
     ap.add_argument("--nbeads", type=int, default=32)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-steps", type=int, default=40000,
+                    help="child steps per host core of the CPU baseline (0: skip it)")
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
     ap.add_argument("--schedule", default=None,
                     help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
@@ -408,6 +439,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard across GPUs "
               "without communication, see the sharding section of the design notes", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline runs first, before anything initialises the GPU, so its
+    # worker processes start from a process without GPU state
+    cpu = cpu_baseline(args.cpu_steps) if args.cpu_steps > 0 and not args.calibrate_copy else None
 
     import torch
     import caracal_amd as cam
@@ -441,8 +476,12 @@ def main():
     value = bead_evals / wall
     steps_per_s = ntraj * args.steps / wall
     folded = best["nm_kernel"] == cam.CAM_NMK_FOLDED
-    flops_launch = step_flops(6, nb, folded) * ntraj * spl
-    achieved = flops_launch / (ms_launch * 1e-3) / 1e12
+    # algorithmic work of the timed run over its GPU time (events around the
+    # graph replays): nl launches of ms_launch each cover exactly args.steps
+    flops_run = step_flops(6, nb, folded) * ntraj * args.steps
+    achieved = flops_run / (ms_launch * nl * 1e-3) / 1e12
+    if achieved > FP64_PEAK_TFLOPS:
+        raise RuntimeError(f"roofline above peak ({achieved:.1f} TFLOP/s): the work count does not match the run")
     nmk_name = NMK_NAMES[best["nm_kernel"]]
     if best.get("split"):
         kernel = (f"split step: step_kernel<SurfCH4H, {nb}, double, CHILD, {nmk_name.upper()} normal modes, "
@@ -477,11 +516,15 @@ def main():
                        f"kernel evaluates {distinct} distinct beads per step and mirrors the rest"),
         "kappa_final": float(num[-1] / den) if den > 0 else None,
         "autotune": trials,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get((nmk_name, ntraj, nb)),
-                     "kernel": kernel, "ms_per_launch": ms_launch, "flops_per_launch": flops_launch,
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": launch_traffic(nmk_name, ntraj, nb, spl),
+                     "kernel": kernel, "ms_per_launch": ms_launch, "launches": nl, "steps_per_launch": spl,
+                     "flops_per_launch": flops_run / nl,
                      "flops_split_per_trajectory_step": step_flops_parts(6, nb, folded),
-                     "note": "FP64 peak is the same for VALU and MFMA on MI355X; elementary functions not counted"},
+                     "note": ("FP64 peak is the same for the vector ALUs and the matrix cores on MI355X; "
+                              f"{100 * step_flops_parts(6, nb, folded)['surface'] / step_flops(6, nb, folded):.0f}% "
+                              "of the counted flops are the surface on the vector ALUs; elementary functions not "
+                              "counted; traffic is the counter-measured HBM bytes of one launch of this shape")},
     }
     if not args.skip_extras:
         e, _ = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_MIXED)
@@ -503,7 +546,8 @@ def main():
         out["box"] = box_nvt(cam)
         out["force_matching"] = force_matching(cam)
         out["rpmd_rate_h_h2"] = rpmd_rate_h_h2()
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

@@ -25,12 +25,13 @@ namespace cam {
 namespace {
 
 // Recrossing parents and umbrella windows: trajectories per block of their
-// per-trajectory kernels, steps per captured graph of a split-schedule run,
-// and the bits of the per-trajectory flags (a constraint did not converge
-// since the start; the trajectory check failed; this step's constraint did
-// not converge).
+// per-trajectory kernels, step launches per captured graph (a split-schedule
+// step is one launch sequence; a fused launch advances steps_per_launch
+// steps), and the bits of the per-trajectory flags (a constraint did not
+// converge since the start; the trajectory check failed; this step's
+// constraint did not converge).
 constexpr int kParentWaves = 4;
-constexpr int kSplitGraphSteps = 1000;
+constexpr int kGraphLaunches = 1000;
 constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
 
 // xi of the centroid without its gradient, for the per-step child flags
@@ -1644,7 +1645,8 @@ struct cam_engine {
     std::map<int, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
-    int last_launches = 0;
+    int last_launches = 0;            // step launches of the last cam_engine_run
+    long last_steps = 0;              // steps of the last cam_engine_run
     int tpb = 1;                      // trajectories per block of the step kernel
 };
 
@@ -1847,8 +1849,13 @@ int run_forces(cam_engine *e)
     return CAM_OK;
 }
 
+// Destroys the cached graphs.  A replay of one may still be in flight on the
+// engine's stream (cam_engine_run is asynchronous), so the stream is drained
+// first: no executable graph is destroyed while the GPU runs it.
 void drop_graphs(cam_engine *e)
 {
+    if (e->graphs.empty()) return;
+    (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     e->graphs.clear();
 }
@@ -1859,15 +1866,19 @@ void drop_graphs(cam_engine *e)
 int ensure_child_buffers(cam_engine *e, int nsteps, long total)
 {
     if (nsteps > e->flag_cap) {
+        // the graphs that write the old buffer go first (drop_graphs waits
+        // for the stream, so nothing still uses it when it is freed)
+        drop_graphs(e);
+        CAM_HIP(hipStreamSynchronize(e->stream));
         if (e->xi_flag) CAM_HIP(hipFree(e->xi_flag));
         e->xi_flag = nullptr;
         CAM_HIP(hipMalloc(&e->xi_flag, (size_t)nsteps * e->cfg.ntraj));
         e->flag_cap = nsteps;
-        drop_graphs(e);
     }
     if (total > e->num_cap) {
         const long cap = std::max<long>(total, std::max<long>(2L * e->num_cap, 8192L));
         CAM_CHECK_ARG(cap <= (1L << 30), "too many child steps (%ld)", total);
+        drop_graphs(e);
         double *nn = nullptr;
         CAM_HIP(hipMalloc(&nn, (size_t)cap * sizeof(double)));
         CAM_HIP(hipMemsetAsync(nn, 0, (size_t)cap * sizeof(double), e->stream));
@@ -1878,7 +1889,6 @@ int ensure_child_buffers(cam_engine *e, int nsteps, long total)
         }
         e->num = nn;
         e->num_cap = (int)cap;
-        drop_graphs(e);
     }
     return CAM_OK;
 }
@@ -1975,6 +1985,13 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
                 hipLaunchKernelGGL(umbrella_end_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g, e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_xi_tol, e->pcheck_after);
             });
             err = hipGetLastError();
+            if (err == hipSuccess && transrot) {
+                TrajArgs tr = *transrot;
+                tr.q = a.q;
+                tr.p = a.p;
+                hipLaunchKernelGGL(transrot_kernel, dim3((e->cfg.ntraj + 127) / 128), dim3(128), 0, e->stream, tr);
+                err = hipGetLastError();
+            }
         }
         return err;
     }
@@ -2067,7 +2084,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
                   "nm_kernel FOLDED has no kernel for this surface, bead count and precision");
     CAM_CHECK_ARG(cfg->waves_per_eu >= 0 && cfg->waves_per_eu <= 2, "waves_per_eu must be 0, 1 or 2");
     CAM_CHECK_ARG(cfg->schedule >= CAM_SCHED_AUTO && cfg->schedule <= CAM_SCHED_SPLIT, "bad schedule");
-    CAM_CHECK_ARG(cfg->schedule != CAM_SCHED_SPLIT || !cfg->remove_com,
+    CAM_CHECK_ARG(cfg->schedule != CAM_SCHED_SPLIT || !cfg->remove_com || cfg->ensemble == CAM_ENS_UMBRELLA,
                   "remove_com needs the fused schedule (transrot after every step)");
     CAM_CHECK_ARG(cfg->reserved == 0, "reserved must be 0");
     CAM_CHECK_ARG(cfg->dt > 0.0, "dt must be > 0");
@@ -2076,8 +2093,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_PARENT || cfg->schedule != CAM_SCHED_FUSED,
                   "ensemble PARENT runs on the split schedule (its constraint sits between propagation and forces)");
     for (int i = 0; i < na; ++i) CAM_CHECK_ARG(cfg->mass[i] > 0.0, "mass[%d] must be > 0", i);
-    CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || steered)),
-                  "remove_com applies to dynamic.x trajectories (constrain <= 0) only");
+    CAM_CHECK_ARG(!(cfg->remove_com && (cfg->ensemble == CAM_ENS_CHILD || cfg->ensemble == CAM_ENS_PARENT)),
+                  "remove_com applies to trajectories with verlet's constrain <= 0 only (dynamic.x, windows)");
     const cam_reaction &rr = cfg->reaction;
     Rx rx{};
     if (cfg->ensemble == CAM_ENS_CHILD || steered) {
@@ -2493,10 +2510,12 @@ int cam_engine_run(cam_engine *e, int nsteps)
                   "an UMBRELLA engine needs cam_engine_begin_umbrella after cam_engine_set_state");
     clear_error();
     if (nsteps == 0) return CAM_OK;
-    // A split-schedule step is three or four launches: long runs replay a
-    // graph of at most kSplitGraphSteps steps (and one of the remainder)
-    // instead of capturing every step into one graph.
-    const int chunk = e->split ? std::min(nsteps, kSplitGraphSteps) : nsteps;
+    // Long runs replay a graph of at most kGraphLaunches step launches (and
+    // one graph of the remainder) instead of capturing every step into one
+    // graph: a split-schedule step is three or four launches, a fused launch
+    // spl steps (one, plus transrot, when the centre of mass is removed).
+    const int steps_per_graph = (e->split ? 1 : e->spl) * kGraphLaunches;
+    const int chunk = std::min(nsteps, steps_per_graph);
     const int reps = nsteps / chunk, tail = nsteps % chunk;
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
         int rc = ensure_child_buffers(e, chunk, e->child_steps + nsteps);
@@ -2512,7 +2531,11 @@ int cam_engine_run(cam_engine *e, int nsteps)
     CAM_HIP(hipEventRecord(e->ev1, e->stream));
     e->child_steps += nsteps;
     e->ustat_steps += nsteps;
-    e->last_launches = e->split ? nsteps : (nsteps + e->spl - 1) / e->spl;
+    // the step launches just enqueued: per graph, ceil(steps / spl) fused
+    // launches (the last one of a graph may be shorter) or one per split step
+    const int per = e->split ? 1 : e->spl;
+    e->last_launches = reps * ((chunk + per - 1) / per) + (tail ? (tail + per - 1) / per : 0);
+    e->last_steps = nsteps;
     e->last_ms = -1.0;
     return CAM_OK;
 }
@@ -2850,7 +2873,7 @@ int cam_engine_last_timing(cam_engine *e, double *ms_per_launch, int *steps_per_
     CAM_CHECK_ARG(e, "null engine");
     CAM_CHECK_ARG(e->last_ms >= 0.0 && e->last_launches > 0, "no completed run (call cam_engine_sync first)");
     if (ms_per_launch) *ms_per_launch = e->last_ms / e->last_launches;
-    if (steps_per_launch) *steps_per_launch = e->spl;
+    if (steps_per_launch) *steps_per_launch = (int)std::min<long>(e->split ? 1 : e->spl, e->last_steps);
     if (launches) *launches = e->last_launches;
     return CAM_OK;
 }

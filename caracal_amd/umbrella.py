@@ -85,7 +85,7 @@ class UmbrellaSampling:
             self._engines[ntraj] = self._factory(
                 self.surface, self.nbeads, ntraj, self.dt, self.masses, precision=self.precision,
                 ensemble=_abi.CAM_ENS_UMBRELLA, reaction=self.reaction, andersen_step=self.andersen_step,
-                seed=self.seed + self._created)
+                seed=self.seed + self._created, remove_com=True)
             self._created += 1
         return self._engines[ntraj]
 

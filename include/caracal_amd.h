@@ -316,9 +316,12 @@ typedef struct {
 } cam_tuning;
 int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);
 
-/* Mean duration in milliseconds of the step kernel launches of the last
- * cam_engine_run, measured with HIP events on the engine's stream, and the
- * number of steps one launch advances. */
+/* Timing of the last cam_engine_run (after cam_engine_sync): the GPU time of
+ * the whole run (HIP events on the engine's stream around its graph
+ * replays, so the small per-launch kernels are included) divided by the
+ * number of step launches it enqueued, the steps one launch advances
+ * (min(steps_per_launch, nsteps) fused, 1 split; a run whose nsteps is not a
+ * multiple ends with a shorter launch) and the number of launches. */
 int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_per_launch, int *launches);
 
 /* Ragged batches (trajectories of different surfaces and bead counts): one

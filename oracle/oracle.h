@@ -84,7 +84,7 @@ void oracle_verlet_nhc(const oracle_sys *s, double *q, double *p, double *g, dou
                        double *vnh, const double *qnh, double *eta);
 double oracle_verlet_umbrella(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g,
                               double *vbead, double beta, double k_force, long istep, int andersen_step,
-                              uint64_t seed, uint32_t traj, uint64_t draw);
+                              uint64_t seed, uint32_t traj, uint64_t draw, int remove_com);
 int oracle_verlet_parent(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g, double *vbead,
                          double *dxi, double beta, long istep, int andersen_step, uint64_t seed, uint32_t traj);
 int oracle_constrain_q(const oracle_sys *s, const oracle_rx *rx, const double *centroid, const double *dxi_act,

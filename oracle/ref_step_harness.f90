@@ -4,19 +4,24 @@
 ! of MKL), transrot, get_centroid, umbrella mode 1 / calc_xi / calc_com and
 ! the analytic surfaces, all compiled from the reference's sources by
 ! Makefile.ref.  It sets the module state the reference's input readers set
-! (src/dynamic.f90, src/calc_rate_read.f90), computes the start forces as
-! mdinit does (src/mdinit.f90:79-83) and calls verlet unchanged.
+! (src/dynamic.f90, src/calc_rate_read.f90), calls the reference's mdinit
+! (start forces, the umbrella bias of a window start, the Nose-Hoover chain
+! masses) with the momenta it is given, and calls verlet unchanged.
 !
 ! Usage: ref_step_harness <input> <output>
 ! Input (stream, native endianness):
-!   int32  natoms, nbeads, nsteps, constrain, surface (1 H3, 2 CH4H, 3 OH3),
-!          every (write a frame after every 'every' steps; 0 = timing only)
+!   int32  natoms, nbeads, nsteps, constrain (-1 dynamic.x, 0 umbrella
+!          window, 2 recrossing child), surface (1 H3, 2 CH4H, 3 OH3),
+!          every (write a frame after every 'every' steps; 0 = timing only),
+!          thermostat (0 none, 1 Andersen, 2 Nose-Hoover chain), andersen_step
 !   real64 dt, beta, mass(natoms), q(3,natoms,nbeads), p(3,natoms,nbeads)
-!   constrain = 2 (recrossing child) adds the BIMOLEC reaction coordinate:
+!   thermostat 2:  real64 kelvin, nose_q
+!   constrain >= 0 adds the BIMOLEC reaction coordinate:
 !   int32  n_form, n_break, bond_form(2,n_form), bond_break(2,n_break)
 !   real64 form_ref(n_form), break_ref(n_break)
 !   int32  n_reac, reac_natoms(n_reac), reac_atoms(maxr,n_reac) (maxr = max count)
 !   real64 r_inf, xi_ideal
+!   constrain = 0: real64 k_force
 ! Output (stream): per frame  real64 q(3,natoms,nbeads), p(3,natoms,nbeads),
 !   epot, xi_real; the step loop's wall time is printed to stdout.
 program ref_step_harness
@@ -27,11 +32,11 @@ program ref_step_harness
    implicit none
    include 'mpif.h'
    integer :: ierr
-   integer :: nat, nb, nsteps, cmode, surf, every, i, k, istep, nf, nbk, nr, maxr
-   integer :: rank, round, info
+   integer :: nat, nb, nsteps, cmode, surf, every, ithermo, astep, i, k, istep, nf, nbk, nr, maxr
+   integer :: rank, round
    integer, allocatable :: bf(:, :), bb(:, :), rn(:), ra(:, :)
-   real(kind=8) :: dt, beta_in, epot, ekin, afm_force, xi_ideal, xi_real, epot1
-   real(kind=8), allocatable :: derivs(:, :, :), dxi_act(:, :), q1(:, :), g1(:, :)
+   real(kind=8) :: dt, beta_in, epot, ekin, afm_force, xi_ideal, xi_real, kf
+   real(kind=8), allocatable :: derivs(:, :, :), dxi_act(:, :), p_in(:, :, :)
    integer(kind=8) :: c0, c1, crate
    character(len=512) :: fin, fout
    logical :: analyze
@@ -42,16 +47,19 @@ program ref_step_harness
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
    open (unit=71, file=trim(fin), access='stream', form='unformatted', status='old')
-   read (71) nat, nb, nsteps, cmode, surf, every
+   read (71) nat, nb, nsteps, cmode, surf, every, ithermo, astep
    read (71) dt, beta_in
    natoms = nat
    nbeads = nb
    read (71) mass(1:nat)
-   allocate (q_i(3, nat, nb), p_i(3, nat, nb), derivs(3, nat, nb), dxi_act(3, nat), q1(3, nat), g1(3, nat))
+   allocate (q_i(3, nat, nb), p_i(3, nat, nb), p_in(3, nat, nb), derivs(3, nat, nb), dxi_act(3, nat))
    read (71) q_i
-   read (71) p_i
+   read (71) p_in
+   kelvin = 1.d0 / (0.316679d-5 * beta_in)
+   nose_q = 0.d0
+   if (ithermo == 2) read (71) kelvin, nose_q
    xi_ideal = 0.d0
-   if (cmode == 2) then
+   if (cmode >= 0) then
       read (71) nf, nbk
       form_num = nf
       break_num = nbk
@@ -83,6 +91,12 @@ program ref_step_harness
       read (71) R_inf, xi_ideal
       umbr_type = "BIMOLEC"
    end if
+   if (cmode == 0) then
+      read (71) kf
+      allocate (k_force(1))
+      k_force(1) = kf
+      um_window_act = 1
+   end if
    close (71)
 
    ! module state of a dynamic.x / calc_rate run with no extras
@@ -98,11 +112,15 @@ program ref_step_harness
       pot_type = "oh3"
       call initialize_oh3()
    end select
-   use_calc_rate = cmode == 2
+   use_calc_rate = cmode >= 0
    use_stick_coeff = .false.
    use_rpmd = .true.
-   thermostat = 0
-   andersen_step = huge(1)
+   thermostat = ithermo
+   andersen_step = max(astep, 1)
+   nve = .true.
+   read_vel = .false.
+   fix_atoms = .false.
+   barostat = 0
    npt = .false.
    periodic = .false.
    box_walls = .false.
@@ -130,13 +148,16 @@ program ref_step_harness
    analyze = .false.
    afm_force = 0.d0
 
-   ! start forces of every bead (src/mdinit.f90:79-83)
-   do k = 1, nb
-      q1 = q_i(:, :, k)
-      call gradient(q1, epot1, g1, k, rank)
-      derivs(:, :, k) = g1
-   end do
-   info = 0
+   ! mdinit: start forces of every bead, the umbrella bias of a window start
+   ! (bias_mode 2, as src/calc_rate.f90:1554 calls it), the chain masses of
+   ! the Nose-Hoover thermostat; its momentum draw is replaced by the given
+   ! momenta (nve makes it set p = 0, or the chain's andersen call draws them)
+   if (cmode == 0) then
+      call mdinit(derivs, xi_ideal, dxi_act, 2, rank)
+   else
+      call mdinit(derivs, xi_ideal, dxi_act, 0, rank)
+   end if
+   p_i = p_in
 
    open (unit=72, file=trim(fout), access='stream', form='unformatted', status='replace')
    call system_clock(c0, crate)

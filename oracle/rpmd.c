@@ -682,11 +682,12 @@ void oracle_verlet_nhc(const oracle_sys *s, double *q, double *p, double *g, dou
  * p -= dt/2 g; free ring polymer; forces; umbrella mode 0 at the centroid of
  * the new positions, for the window rx->xi_ideal with k_force, adds its bias
  * to every bead's g (src/verlet.f90:1046); p -= dt/2 g; Andersen when istep
- * is a multiple of andersen_step, with the draw id 'draw'.  Returns xi of
+ * is a multiple of andersen_step, with the draw id 'draw'; transrot if
+ * remove_com (the reference always applies it to windows).  Returns xi of
  * mode 1 at that centroid (the xi_real the sampling loop sums). */
 double oracle_verlet_umbrella(const oracle_sys *s, const oracle_rx *rx, double *q, double *p, double *g,
                               double *vbead, double beta, double k_force, long istep, int andersen_step,
-                              uint64_t seed, uint32_t traj, uint64_t draw)
+                              uint64_t seed, uint32_t traj, uint64_t draw, int remove_com)
 {
     const int na = s->natoms, nb = s->nbeads;
     const size_t n = (size_t)3 * na * nb;
@@ -698,6 +699,9 @@ double oracle_verlet_umbrella(const oracle_sys *s, const oracle_rx *rx, double *
     (void)oracle_umbrella(s, rx, c, beta, k_force, dxi, g, &xi);
     for (size_t i = 0; i < n; ++i) p[i] = p[i] - 0.5 * s->dt * g[i];
     if (andersen_step > 0 && istep % andersen_step == 0) oracle_andersen(s, p, beta, seed, traj, draw);
+    /* verlet ends every step with constrain <= 0, windows included, with
+     * transrot (src/verlet.f90:1299-1305) */
+    if (remove_com) oracle_transrot(s, q, p);
     return xi;
 }
 

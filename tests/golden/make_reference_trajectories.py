@@ -11,11 +11,20 @@ stores inputs and frames as small .npz fixtures beside this script:
                               constrain = 2), 32 beads, 100 steps from a
                               seeded start near the saddle, xi every step
   oh3_nve_reference.npz       OH + H2, 8 beads, NVE with transrot, 100 steps
+  h3_window_reference.npz     an H + H2 umbrella window (calc_rate, constrain =
+                              0): xi_ideal 0.9, k_force 0.5, 8 beads; mdinit's
+                              biased start forces, bias and transrot every step
+  h3_nhc_reference.npz        H + H2 with the Nose-Hoover chain (thermostat 2),
+                              8 beads, 300 K, damping 20 dt, transrot
+  oh3_child_xi05_reference.npz  an OH + H2 recrossing child with the dividing
+                              surface at xi_ideal = 0.5 (so xi = (s0 + s1)/2
+                              depends on the mass-weighted fragment centres)
 
 Frames are kept at steps 1, 2, 10, 50 and 100 (q and p), with epot and xi
 at every step.  Run in the container where /root/reference is mounted:
     python tests/golden/make_reference_trajectories.py
 """
+import dataclasses
 import os
 import sys
 
@@ -31,11 +40,14 @@ from caracal_amd import workloads  # noqa: E402
 KEEP = np.array([1, 2, 10, 50, 100])
 
 
-def _save(name, surface, q, p, masses, dt, beta, constrain, reaction=None):
+def _save(name, surface, q, p, masses, dt, beta, constrain, reaction=None, **kw):
     fr, _ = refstep.run(surface, q, p, masses, dt, beta, int(KEEP[-1]), constrain=constrain, every=1,
-                        reaction=reaction)
+                        reaction=reaction, **kw)
+    extra = {k: v for k, v in kw.items() if k in ("k_force", "kelvin", "nose_q")}
+    if reaction is not None:
+        extra["xi_ideal"] = reaction.xi_ideal
     np.savez(os.path.join(HERE, name), q0=q, p0=p, masses=masses, dt=dt, beta=beta, steps=KEEP,
-             q=fr["q"][KEEP - 1], p=fr["p"][KEEP - 1], epot=fr["epot"], xi=fr["xi"])
+             q=fr["q"][KEEP - 1], p=fr["p"][KEEP - 1], epot=fr["epot"], xi=fr["xi"], **extra)
 
 
 def child_start():
@@ -56,6 +68,13 @@ def main():
     _save("ch4h_child_reference.npz", "ch4h", q, p, w["masses"], w["dt"], w["beta"][0], 2, w["reaction"])
     o = workloads.oh3_nve(nbeads=8)
     _save("oh3_nve_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], -1)
+    h8 = workloads.h3_nve(nbeads=8)
+    _save("h3_window_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], 0,
+          dataclasses.replace(workloads.h3_reaction(), xi_ideal=0.9), k_force=0.5)
+    _save("h3_nhc_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], -1,
+          thermostat=2, kelvin=1.0 / (0.316679e-5 * h8["beta"]), nose_q=20 * h8["dt"])
+    _save("oh3_child_xi05_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], 2,
+          dataclasses.replace(workloads.oh3_reaction(), xi_ideal=0.5))
 
 
 if __name__ == "__main__":

@@ -169,7 +169,7 @@ def _setup_rpmd():
     o.oracle_verlet_umbrella.restype = ctypes.c_double
     o.oracle_verlet_umbrella.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_long, ctypes.c_int,
-                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
     o.oracle_verlet_parent.restype = ctypes.c_int
     o.oracle_verlet_parent.argtypes = [ctypes.POINTER(OracleSys), ctypes.POINTER(OracleRx), _dp, _dp, _dp, _dp, _dp,
                                        ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_uint64,
@@ -219,12 +219,14 @@ class OracleTraj:
         if self.ensemble == 5:
             rpmd().oracle_verlet_nhc(ctypes.byref(self.sys), _ptr(self.q), _ptr(self.p), _ptr(self.g), _ptr(self.v),
                                      self.beta, _ptr(self.vnh), _ptr(self.qnh), _ptr(self.eta))
+            if self.remove_com:              # dynamic.x ends every step with transrot
+                rpmd().oracle_transrot(ctypes.byref(self.sys), _ptr(self.q), _ptr(self.p))
             return 0.0
         if self.ensemble == 4:
             xi = rpmd().oracle_verlet_umbrella(ctypes.byref(self.sys), ctypes.byref(self.rx), _ptr(self.q),
                                                _ptr(self.p), _ptr(self.g), _ptr(self.v), self.beta, self.k_force,
                                                self.istep, self.andersen_step, self.seed, self.traj,
-                                               self.key_base + self.istep)
+                                               self.key_base + self.istep, int(self.remove_com))
             self.xi_sum += xi
             self.xi_sq += xi * xi
             return xi

@@ -22,15 +22,19 @@ def available(opt="O1"):
     return os.path.exists(binary(opt))
 
 
-def _write_input(path, surface, q, p, masses, dt, beta, nsteps, constrain, every, reaction):
+def _write_input(path, surface, q, p, masses, dt, beta, nsteps, constrain, every, reaction, thermostat=0,
+                 andersen_step=0, kelvin=None, nose_q=0.0, k_force=0.0):
     nbeads, natoms, _ = q.shape
     with open(path, "wb") as f:
-        np.array([natoms, nbeads, nsteps, constrain, SURFACE_CODE[surface], every], dtype=np.int32).tofile(f)
+        np.array([natoms, nbeads, nsteps, constrain, SURFACE_CODE[surface], every, thermostat, andersen_step],
+                 dtype=np.int32).tofile(f)
         np.array([dt, beta], dtype=np.float64).tofile(f)
         np.asarray(masses, dtype=np.float64).tofile(f)
         np.ascontiguousarray(q, dtype=np.float64).tofile(f)
         np.ascontiguousarray(p, dtype=np.float64).tofile(f)
-        if constrain == 2:
+        if thermostat == 2:
+            np.array([kelvin, nose_q], dtype=np.float64).tofile(f)
+        if constrain >= 0:
             r = reaction
             np.array([len(r.bond_form), len(r.bond_break)], dtype=np.int32).tofile(f)
             np.asarray(r.bond_form, dtype=np.int32).reshape(-1).tofile(f)
@@ -46,19 +50,25 @@ def _write_input(path, surface, q, p, masses, dt, beta, nsteps, constrain, every
                 ra[k, :len(atoms)] = atoms
             ra.tofile(f)
             np.array([r.r_inf, r.xi_ideal], dtype=np.float64).tofile(f)
+        if constrain == 0:
+            np.array([k_force], dtype=np.float64).tofile(f)
 
 
-def run(surface, q, p, masses, dt, beta, nsteps, constrain=-1, every=1, reaction=None, opt="O1"):
-    """nsteps reference verlet steps from (q, p), each (nbeads, natoms, 3).
-    constrain -1: dynamic.x NVE (transrot after every step); 2: recrossing
-    child (xi of the centroid every step).  Returns (frames, seconds): frames
-    is a dict of q, p (nframes, nbeads, natoms, 3), epot and xi (nframes)
-    after every 'every' steps."""
+def run(surface, q, p, masses, dt, beta, nsteps, constrain=-1, every=1, reaction=None, opt="O1", **kw):
+    """nsteps reference verlet steps from (q, p), each (nbeads, natoms, 3),
+    after the reference's mdinit.  constrain -1: dynamic.x (transrot after
+    every step); 0: umbrella window (k_force=..., the reaction's xi_ideal;
+    bias in the start forces and every step, transrot); 2: recrossing child
+    (xi of the centroid every step).  thermostat=1 (Andersen every
+    andersen_step steps, the compiler's random stream) or 2 (Nose-Hoover
+    chain at kelvin=..., nose_q=...).  Returns (frames, seconds): frames is a
+    dict of q, p (nframes, nbeads, natoms, 3), epot and xi (nframes) after
+    every 'every' steps."""
     q = np.asarray(q, dtype=np.float64)
     nbeads, natoms, _ = q.shape
     with tempfile.TemporaryDirectory() as tmp:
         fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
-        _write_input(fin, surface, q, p, masses, dt, beta, nsteps, constrain, every, reaction)
+        _write_input(fin, surface, q, p, masses, dt, beta, nsteps, constrain, every, reaction, **kw)
         out = subprocess.run([binary(opt), fin, fout], check=True, capture_output=True, text=True, cwd=tmp)
         seconds = float(out.stdout.split("seconds")[-1].split()[0])
         raw = np.fromfile(fout, dtype=np.float64) if every > 0 else np.zeros(0)

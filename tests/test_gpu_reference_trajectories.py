@@ -6,6 +6,7 @@ NVE with transrot), a CH4 + H recrossing child (32 beads, xi every step) and
 OH + H2 (8 beads).  The engine starts from the same inputs; its positions
 must stay within 1e-10 of the reference's scale over the 100 steps, for each
 normal-mode kernel that applies."""
+import dataclasses
 import os
 
 import numpy as np
@@ -66,3 +67,48 @@ def test_oh3_nve_matches_reference():
         e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
         q, p = _frames(e, f["steps"])
     _check("OH3", q, p, f)
+
+
+@pytest.mark.parametrize("schedule", [cam.CAM_SCHED_FUSED, cam.CAM_SCHED_SPLIT])
+def test_h3_umbrella_window_matches_reference(schedule):
+    """An umbrella window as calc_rate runs it (verlet constrain = 0): mdinit's
+    biased start forces, the bias with its metric term every step, and
+    transrot after every step."""
+    f = dict(np.load(os.path.join(GOLDEN, "h3_window_reference.npz")))
+    nb = f["q0"].shape[0]
+    rx = dataclasses.replace(workloads.h3_reaction(), xi_ideal=float(f["xi_ideal"]))
+    with cam.RPMDEngine(cam.CAM_SURFACE_H3, nb, 1, float(f["dt"]), f["masses"], ensemble=cam.CAM_ENS_UMBRELLA,
+                        reaction=rx, remove_com=True, schedule=schedule) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        e.set_windows([float(f["xi_ideal"])], [float(f["k_force"])])
+        e.begin_umbrella(bias=True)
+        q, p = _frames(e, f["steps"])
+        avg, _ = e.umbrella_stats()
+    _check(f"H3 window schedule={schedule}", q, p, f)
+    assert abs(avg[0] - f["xi"].mean()) < 1e-10
+
+
+def test_h3_nose_hoover_chain_matches_reference():
+    f = dict(np.load(os.path.join(GOLDEN, "h3_nhc_reference.npz")))
+    nb = f["q0"].shape[0]
+    with cam.RPMDEngine(cam.CAM_SURFACE_H3, nb, 1, float(f["dt"]), f["masses"], ensemble=cam.CAM_ENS_NHC,
+                        remove_com=True, nose_q=float(f["nose_q"])) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        q, p = _frames(e, f["steps"])
+    _check("H3 Nose-Hoover chain", q, p, f)
+
+
+def test_oh3_child_at_xi_half_matches_reference():
+    """Dividing surface at xi_ideal = 0.5, so the child's xi mixes s1 with s0
+    of the mass-weighted fragment centres."""
+    f = dict(np.load(os.path.join(GOLDEN, "oh3_child_xi05_reference.npz")))
+    nb = f["q0"].shape[0]
+    rx = dataclasses.replace(workloads.oh3_reaction(), xi_ideal=float(f["xi_ideal"]))
+    with cam.RPMDEngine(cam.CAM_SURFACE_OH3, nb, 1, float(f["dt"]), f["masses"], ensemble=cam.CAM_ENS_CHILD,
+                        reaction=rx) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        e.begin_children()
+        q, p = _frames(e, f["steps"])
+        xi = e.observables()["xi"][0]
+    _check("OH3 child xi_ideal 0.5", q, p, f)
+    assert abs(xi - f["xi"][-1]) < 1e-10

@@ -28,18 +28,23 @@ def _windows(surface, nb, nwin, seed):
     return w, q, p, beta, rx, xi_w, k_w
 
 
-@pytest.mark.parametrize("surface,nb,nmk,schedule", [
-    (cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),      # fused: bias inside the step kernel
-    (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA, cam.CAM_SCHED_AUTO),     # MFMA normal modes: split schedule
-    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),
-    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT),
-    (cam.CAM_SURFACE_H3, 1, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO),        # one bead: its own centroid
-    (cam.CAM_SURFACE_CH4H, 4, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO)])
-def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule):
+@pytest.mark.parametrize("surface,nb,nmk,schedule,remove_com", [
+    (cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),   # fused: bias inside the step kernel
+    (cam.CAM_SURFACE_CH4H, 8, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, True),    # ... with transrot every step
+    (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA, cam.CAM_SCHED_AUTO, False),  # MFMA normal modes: split schedule
+    (cam.CAM_SURFACE_CH4H, 32, cam.CAM_NMK_MFMA, cam.CAM_SCHED_AUTO, True),
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, True),
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT, False),
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT, True),
+    (cam.CAM_SURFACE_H3, 1, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),     # one bead: its own centroid
+    (cam.CAM_SURFACE_CH4H, 4, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False)])
+def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule, remove_com):
     nwin, ast, seed, n_equi, n_samp = 6, 5, 21, 20, 30
     w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb)
     e = cam.RPMDEngine(surface, nb, nwin, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
-                       andersen_step=ast, seed=seed, nm_kernel=nmk, schedule=schedule, steps_per_launch=7)
+                       andersen_step=ast, seed=seed, nm_kernel=nmk, schedule=schedule, steps_per_launch=7,
+                       remove_com=remove_com)
     e.set_state(q, p, beta)
     e.set_windows(xi_w, k_w)
     e.begin_umbrella(bias=True)
@@ -48,7 +53,8 @@ def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule):
     for t in range(nwin):
         orx = ol.make_rx(rx)
         orx.xi_ideal = xi_w[t]
-        tr = ol.OracleTraj(s, q[t], p[t], float(beta[t]), rx=orx, ensemble=4, andersen_step=ast, seed=seed, traj=t)
+        tr = ol.OracleTraj(s, q[t], p[t], float(beta[t]), rx=orx, ensemble=4, andersen_step=ast, seed=seed, traj=t,
+                           remove_com=remove_com)
         tr.umbrella_begin(k_w[t], bias=True)
         refs.append(tr)
     _, g0 = e.get_forces()
@@ -72,7 +78,7 @@ def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule):
         perr = np.abs(pe[t] - tr.p).max() / np.abs(tr.p).max()
         a_ref = tr.xi_sum / n_samp
         v_ref = tr.xi_sq / n_samp - a_ref * a_ref
-        print(f"surface={surface} nb={nb} window {xi_w[t]:.3f}: rel dq {err:.1e} dp {perr:.1e}; "
+        print(f"surface={surface} nb={nb} transrot={remove_com} window {xi_w[t]:.3f}: rel dq {err:.1e} dp {perr:.1e}; "
               f"<xi> {avg[t]:.6f} (oracle {a_ref:.6f}) var {var[t]:.3e}")
         assert err <= 1e-10 and perr <= 1e-9
         assert abs(avg[t] - a_ref) <= 1e-12
@@ -102,5 +108,5 @@ def test_umbrella_window_check_and_errors():
     assert (e.observables()["flags"] == 4).all()
     e.close()
     with pytest.raises(cam.CamError, match="remove_com"):
-        cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
+        cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_PARENT, reaction=rx,
                        remove_com=True)

@@ -8,6 +8,7 @@ its sources (oracle/Makefile.ref) and driven by oracle/ref_step_harness.f90
 them from the same inputs; where the compiled reference is present the
 fixtures are also re-derived from it, and longer runs are compared directly.
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -18,7 +19,13 @@ import refstep
 from caracal_amd import workloads
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = [("c1_h3_nve_reference.npz", 1, -1), ("ch4h_child_reference.npz", 2, 2), ("oh3_nve_reference.npz", 3, -1)]
+# fixture, surface, verlet's constrain, thermostat, reaction of the fixture
+CASES = [("c1_h3_nve_reference.npz", 1, -1, 0, None),
+         ("ch4h_child_reference.npz", 2, 2, 0, workloads.ch4h_reaction),
+         ("oh3_nve_reference.npz", 3, -1, 0, None),
+         ("h3_window_reference.npz", 1, 0, 0, workloads.h3_reaction),
+         ("h3_nhc_reference.npz", 1, -1, 2, None),
+         ("oh3_child_xi05_reference.npz", 3, 2, 0, workloads.oh3_reaction)]
 SURF = {1: "h3", 2: "ch4h", 3: "oh3"}
 
 
@@ -26,14 +33,30 @@ def _fixture(name):
     return dict(np.load(os.path.join(GOLDEN, name)))
 
 
-def _oracle_frames(f, surface, constrain):
+def _reaction(f, reaction):
+    if reaction is None:
+        return None
+    return dataclasses.replace(reaction(), xi_ideal=float(f["xi_ideal"]))
+
+
+def _oracle_frames(f, surface, constrain, thermostat, reaction):
+    """The oracle's trajectory from the fixture's inputs: dynamic.x NVE or
+    Nose-Hoover chain (transrot after every step), an umbrella window (bias
+    start, bias and transrot every step) or a recrossing child."""
     nb = f["q0"].shape[0]
     s = ol.make_sys(surface, nb, float(f["dt"]), f["masses"])
+    rx = _reaction(f, reaction)
+    beta = float(f["beta"])
     if constrain == 2:
-        rx = ol.make_rx(workloads.ch4h_reaction())
-        t = ol.OracleTraj(s, f["q0"], f["p0"], float(f["beta"]), rx=rx, ensemble=2)
+        t = ol.OracleTraj(s, f["q0"], f["p0"], beta, rx=ol.make_rx(rx), ensemble=2)
+    elif constrain == 0:
+        t = ol.OracleTraj(s, f["q0"], f["p0"], beta, rx=ol.make_rx(rx), ensemble=4, remove_com=True)
+        t.umbrella_begin(float(f["k_force"]), bias=True)
+    elif thermostat == 2:
+        t = ol.OracleTraj(s, f["q0"], f["p0"], beta, ensemble=5, remove_com=True)
+        t.nhc_begin(float(f["nose_q"]))
     else:
-        t = ol.OracleTraj(s, f["q0"], f["p0"], float(f["beta"]), ensemble=0, remove_com=True)
+        t = ol.OracleTraj(s, f["q0"], f["p0"], beta, ensemble=0, remove_com=True)
     qs, ps, xis, eps = [], [], [], []
     for k in range(1, int(f["steps"][-1]) + 1):
         xis.append(t.step())
@@ -44,19 +67,20 @@ def _oracle_frames(f, surface, constrain):
     return np.array(qs), np.array(ps), np.array(xis), np.array(eps)
 
 
-@pytest.mark.parametrize("name,surface,constrain", CASES)
-def test_oracle_reproduces_reference_trajectory(name, surface, constrain):
+@pytest.mark.parametrize("name,surface,constrain,thermostat,reaction", CASES)
+def test_oracle_reproduces_reference_trajectory(name, surface, constrain, thermostat, reaction):
     """Positions and momenta within 1e-12 of the reference's scale at steps
-    1, 2, 10, 50, 100; the energy every step within 1e-12 of its scale; the child
-    reaction coordinate every step within 1e-12 absolute."""
+    1, 2, 10, 50, 100; the energy every step within 1e-12 of its scale; the
+    reaction coordinate of a child (mode 2) or window (mode 1) every step
+    within 1e-12 absolute."""
     f = _fixture(name)
-    q, p, xi, ep = _oracle_frames(f, surface, constrain)
+    q, p, xi, ep = _oracle_frames(f, surface, constrain, thermostat, reaction)
     eq = np.abs(q - f["q"]).max() / np.abs(f["q"]).max()
     epm = np.abs(p - f["p"]).max() / np.abs(f["p"]).max()
     print(f"{name}: max rel position error {eq:.2e}, momentum {epm:.2e}")
     assert eq < 1e-12 and epm < 1e-12
     assert np.abs(ep - f["epot"]).max() <= 1e-12 * np.abs(f["epot"]).max()
-    if constrain == 2:
+    if constrain >= 0:
         np.testing.assert_allclose(xi, f["xi"], rtol=0, atol=1e-12)
 
 
@@ -83,12 +107,16 @@ def test_transform_matters_to_the_fixture():
 
 
 @pytest.mark.skipif(not refstep.available(), reason="compiled reference step not built here")
-@pytest.mark.parametrize("name,surface,constrain", CASES)
-def test_fixture_is_the_compiled_reference(name, surface, constrain):
+@pytest.mark.parametrize("name,surface,constrain,thermostat,reaction", CASES)
+def test_fixture_is_the_compiled_reference(name, surface, constrain, thermostat, reaction):
     f = _fixture(name)
-    rx = workloads.ch4h_reaction() if constrain == 2 else None
+    kw = {}
+    if constrain == 0:
+        kw["k_force"] = float(f["k_force"])
+    if thermostat == 2:
+        kw.update(thermostat=2, kelvin=float(f["kelvin"]), nose_q=float(f["nose_q"]))
     fr, _ = refstep.run(SURF[surface], f["q0"], f["p0"], f["masses"], float(f["dt"]), float(f["beta"]),
-                        int(f["steps"][-1]), constrain=constrain, reaction=rx)
+                        int(f["steps"][-1]), constrain=constrain, reaction=_reaction(f, reaction), **kw)
     assert np.array_equal(fr["q"][f["steps"] - 1], f["q"])
     assert np.array_equal(fr["epot"], f["epot"])
 

@@ -19,6 +19,9 @@ class FakeEngine:
 
     def __init__(self, surface, nbeads, ntraj, dt, masses, **kw):
         self.ntraj = ntraj
+        # windows and the structure walk are verlet steps with constrain = 0,
+        # which end with transrot (src/verlet.f90:1299-1305)
+        assert kw.get("remove_com") is True
         self.log.append(("create", ntraj))
 
     def set_windows(self, xi, k):
