@@ -45,8 +45,8 @@ FP64_PEAK_TFLOPS = 78.6            # MI355X dense FP64 (vector and matrix)
 # scaled by a plain 1 GiB copy (profiles/fold_step_kernel_ch4h_children_fp64.md):
 # a fixed part (q, p and the gradient read and written once per launch) and
 # a part per step (the child centroids of every step, 3 x 6 doubles per
-# trajectory), from launches of 50 and 100 steps.
-MEASURED_TRAFFIC = {("folded", 4096, 32): (113.3e6, 0.646e6)}
+# trajectory), from launches of 50 and 100 steps (145.6 and 176.2 MB).
+MEASURED_TRAFFIC = {("folded", 4096, 32): (115.0e6, 0.612e6)}
 
 
 def launch_traffic(nmk_name, ntraj, nbeads, spl):
@@ -300,7 +300,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(steps=40000):
+def cpu_baseline(steps=120000):
     """The CPU path timed on this host before the GPU is touched: the
     reference's own recrossing-child step (src/verlet.f90 with constrain = 2:
     FFTW normal modes, calc_xi of the centroid, egrad_ch4h for every bead),
@@ -426,7 +426,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--ntraj", type=int, default=4096)
     ap.add_argument("--nbeads", type=int, default=32)
-    ap.add_argument("--cpu-steps", type=int, default=40000,
+    ap.add_argument("--cpu-steps", type=int, default=120000,
                     help="child steps per host core of the CPU baseline (0: skip it)")
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
     ap.add_argument("--schedule", default=None,

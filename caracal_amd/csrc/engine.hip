@@ -1918,7 +1918,10 @@ StepArgs step_args(cam_engine *e)
     a.check_after = e->pcheck_after;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
-    a.andersen_step = e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1;
+    // ANDERSEN draws every andersen_step >= 1 steps; a window's 0 means never
+    // (the fused window step tests andersen_step > 0)
+    a.andersen_step = e->cfg.ensemble == CAM_ENS_UMBRELLA ? e->cfg.andersen_step
+                                                          : (e->cfg.andersen_step > 0 ? e->cfg.andersen_step : 1);
     a.seed = e->cfg.seed;
     a.dt = e->cfg.dt;
     std::memcpy(a.mass, e->cfg.mass, sizeof a.mass);
