@@ -58,6 +58,31 @@ double oracle_water_eval(int nwater, const double *x, double *g, int periodic, d
 double oracle_ff_eval(const oracle_ff *f, const double *theta, const double *x, double *g, double *dth,
                       const double *v, double *dvth);
 
+/* The reference's QMDFF (oracle/qmdff.c): the arrays prepare leaves in its
+ * qmdff module, 1-based atom indices; element-pair tables [z_i-1][z_j-1]
+ * (94 x 94); c6xy n x n (symmetric); rad in bohr (as prepare converts it). */
+typedef struct {
+    int n, nbond, nangl, ntors, nhb, nnci, nmols;
+    const int *at;
+    const double *molnum, *q;
+    const int *bond;            /* 2 per bond */
+    const double *vbond;        /* r0, k, exponent */
+    const int *angl;            /* 3 per angle, vertex first */
+    const double *vangl;        /* theta0, k */
+    const int *tors;            /* 6 per torsion: i, j, k, l, terms, kind (2 = out of plane) */
+    const double *vtors;        /* 14 per torsion */
+    const int *hb;              /* 3 per listed hydrogen bond: A, B, H */
+    const double *vhb;          /* 2 per listed hydrogen bond */
+    const int *nci;             /* 3 per noncovalent pair: i, j, kind 1..6 */
+    const double *c6xy, *r094, *sr42, *zab, *r0ab;
+    double eps1[6], eps2[6];
+    const double *rad, *scalehb;
+    int periodic, zahn;
+    double box[3], coul_cut, vdw_cut, cut_low, zahn_a, zahn_par;
+} oracle_qmdff;
+int oracle_qmdff_supported(const oracle_qmdff *f);
+double oracle_qmdff_eval(const oracle_qmdff *f, const double *x, double *g, double *parts);
+
 /* analytic surfaces (src/egrad_h3.f, src/egrad_ch4h.f) */
 void oracle_h3_pote(const double r[3], double *pe, double dpe[3]);
 void oracle_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq);

@@ -415,3 +415,60 @@ def ref_water_eval(x, periodic=False, box=0.0, zahn=False, zahn_a=0.0, zahn_par=
                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     f(len(x), _ptr(x), _ptr(g), ctypes.byref(e), ip, bx, iz, za, zp, ct)
     return e.value, g
+
+
+# ---- QMDFF (oracle/qmdff.c) ----
+class OracleQmdff(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nbond", ctypes.c_int), ("nangl", ctypes.c_int), ("ntors", ctypes.c_int),
+                ("nhb", ctypes.c_int), ("nnci", ctypes.c_int), ("nmols", ctypes.c_int),
+                ("at", _ip), ("molnum", _dp), ("q", _dp), ("bond", _ip), ("vbond", _dp), ("angl", _ip),
+                ("vangl", _dp), ("tors", _ip), ("vtors", _dp), ("hb", _ip), ("vhb", _dp), ("nci", _ip),
+                ("c6xy", _dp), ("r094", _dp), ("sr42", _dp), ("zab", _dp), ("r0ab", _dp),
+                ("eps1", ctypes.c_double * 6), ("eps2", ctypes.c_double * 6), ("rad", _dp), ("scalehb", _dp),
+                ("periodic", ctypes.c_int), ("zahn", ctypes.c_int), ("box", ctypes.c_double * 3),
+                ("coul_cut", ctypes.c_double), ("vdw_cut", ctypes.c_double), ("cut_low", ctypes.c_double),
+                ("zahn_a", ctypes.c_double), ("zahn_par", ctypes.c_double)]
+
+
+class QmdffOracle:
+    """The oracle's QMDFF over a parameter dict in the layout of
+    tests/refqmdff.py (the arrays prepare leaves in the qmdff module)."""
+
+    def __init__(self, p):
+        self.keep = {}
+
+        def arr(name, dtype):
+            a = np.ascontiguousarray(p[name], dtype=dtype)
+            self.keep[name] = a
+            return a.ctypes.data_as(_ip if dtype == np.int32 else _dp)
+
+        s = OracleQmdff()
+        s.n, s.nmols = int(p["natoms"]), int(p["nmols"])
+        s.nbond, s.nangl, s.ntors = len(p["bond"]), len(p["angl"]), len(p["tors"])
+        s.nhb, s.nnci = len(p["hb"]), len(p["nci"])
+        for name in ("at", "bond", "angl", "tors", "hb", "nci"):
+            setattr(s, name, arr(name, np.int32))
+        for name in ("molnum", "q", "vbond", "vangl", "vtors", "vhb", "c6xy", "rad", "scalehb"):
+            setattr(s, name, arr(name, np.float64))
+        for name in ("r094", "sr42", "zab", "r0ab"):
+            setattr(s, name, arr(name, np.float64))
+        for k in range(6):
+            s.eps1[k], s.eps2[k] = p["eps1"][k], p["eps2"][k]
+        s.periodic, s.zahn = int(p["periodic"]), int(p["zahn"])
+        for k in range(3):
+            s.box[k] = p["box"][k]
+        s.coul_cut, s.vdw_cut, s.cut_low = p["coul_cut"], p["vdw_cut"], p["cut_low"]
+        s.zahn_a, s.zahn_par = p["zahn_a"], p["zahn_par"]
+        self.s = s
+        o = oracle()
+        o.oracle_qmdff_eval.restype = ctypes.c_double
+        o.oracle_qmdff_eval.argtypes = [ctypes.POINTER(OracleQmdff), _dp, _dp, _dp]
+        self.fn = o.oracle_qmdff_eval
+
+    def eval(self, x):
+        """x (natoms, 3) bohr -> (energy, gradient (natoms, 3), parts (3,))."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x)
+        parts = np.zeros(3)
+        e = self.fn(ctypes.byref(self.s), _ptr(x), _ptr(g), _ptr(parts))
+        return e, g, parts
