@@ -85,6 +85,7 @@ PROTOTYPES = {
     "cam_egrad_h3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_ch4h_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_oh3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "cam_egrad_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "cam_engine_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
     "cam_engine_destroy": (ctypes.c_int, [_vp]),
     "cam_engine_set_state": (ctypes.c_int, [_vp, _dp, _dp, _dp]),

@@ -4,6 +4,8 @@
 // One lane per bead.  A block stages its 256 beads through LDS so the global
 // reads and writes are coalesced although each bead's coordinates are one
 // contiguous 3*natoms-double record.
+#include <mutex>
+
 #include "common.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
@@ -157,10 +159,64 @@ int egrad_host(const double *q, int natoms, int nbeads, double *v, double *g)
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free host-pointer form.  The staging buffers of egrad_host live
+// in one file-scope device array per GPU instead of per-call hipMalloc'd
+// memory: q, then v, then dV/dq of up to kNoAllocBeads beads of the largest
+// surface (6 atoms).  In host code `scratch` names a host-side stand-in, so
+// its device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and only that address is used.  Calls on one device
+// are serialised by a mutex (the array is shared by every call there); the
+// array keeps the last call's contents, which no call reads before writing.
+constexpr int kNoAllocBeads = 16384;
+constexpr size_t kNoAllocDoubles = (size_t)kNoAllocBeads * (2 * 18 + 1);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex[kMaxDevices];
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+template <int NA>
+int egrad_host_noalloc(const double *q, int natoms, int nbeads, double *v, double *g)
+{
+    CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
+    CAM_CHECK_ARG(nbeads >= 0 && nbeads <= kNoAllocBeads,
+                  "nbeads must be in 0..%d for the allocation-free entry point (got %d)", kNoAllocBeads, nbeads);
+    CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null host pointer");
+    clear_error();
+    if (nbeads == 0) return CAM_OK;
+    int dev = 0;
+    CAM_HIP(hipGetDevice(&dev));
+    CAM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex[dev]);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        CAM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = static_cast<double *>(ptr);
+    }
+    const size_t nq = (size_t)nbeads * 3 * NA;
+    double *dq = g_noalloc_ptr[dev];
+    double *dv = dq + (size_t)kNoAllocBeads * 18;
+    double *dg = dv + kNoAllocBeads;
+    CAM_HIP(hipMemcpy(dq, q, nq * sizeof(double), hipMemcpyHostToDevice));
+    const int r = launch_egrad<NA>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
+    if (r != CAM_OK) return r;
+    CAM_HIP(hipMemcpy(v, dv, (size_t)nbeads * sizeof(double), hipMemcpyDeviceToHost));
+    CAM_HIP(hipMemcpy(g, dg, nq * sizeof(double), hipMemcpyDeviceToHost));
+    return CAM_OK;
+}
+
 }  // namespace
 }  // namespace cam
 
 extern "C" {
+int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    if (surface == CAM_SURFACE_H3) return cam::egrad_host_noalloc<3>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_CH4H) return cam::egrad_host_noalloc<6>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_OH3) return cam::egrad_host_noalloc<4>(q, natoms, nbeads, v, dvdq);
+    CAM_CHECK_ARG(false, "unknown surface %d", surface);
+}
 int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
     return cam::egrad_host<3>(q, natoms, nbeads, v, dvdq);

@@ -69,6 +69,16 @@ int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, doubl
 int cam_egrad_oh3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
                       int precision, void *stream);
 
+/* Allocation-free form of cam_egrad_h3 / cam_egrad_ch4h / cam_egrad_oh3 (the
+ * surface by its CAM_SURFACE_* code; same arguments, host pointers,
+ * synchronous).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory per call (a tight per-step force loop, code
+ * that forbids hipMalloc).  Its staging buffers live in one device array per
+ * GPU, held until the process ends (4.9 MB); nbeads is limited to 16384
+ * (CAM_ERR_ARG beyond).  Calls from several host threads on one GPU are
+ * serialised. */
+int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, double *v, double *dvdq);
+
 /* ---- (2) trajectory-batch RPMD engine ---- */
 enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3 };
 /* Precision of the engine.  MIXED: CH4H forces in FP32 arithmetic with FP64
