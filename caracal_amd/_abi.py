@@ -59,6 +59,26 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class QmdffDesc(ctypes.Structure):
+    """cam_qmdff_desc: the reference's QMDFF module arrays."""
+    _fields_ = [("natoms", ctypes.c_int32), ("nbond", ctypes.c_int32), ("nangl", ctypes.c_int32),
+                ("ntors", ctypes.c_int32), ("nhb", ctypes.c_int32), ("nnci", ctypes.c_int32),
+                ("at", ctypes.POINTER(ctypes.c_int32)), ("molnum", ctypes.POINTER(ctypes.c_double)),
+                ("q", ctypes.POINTER(ctypes.c_double)), ("bond", ctypes.POINTER(ctypes.c_int32)),
+                ("vbond", ctypes.POINTER(ctypes.c_double)), ("angl", ctypes.POINTER(ctypes.c_int32)),
+                ("vangl", ctypes.POINTER(ctypes.c_double)), ("tors", ctypes.POINTER(ctypes.c_int32)),
+                ("vtors", ctypes.POINTER(ctypes.c_double)), ("hb", ctypes.POINTER(ctypes.c_int32)),
+                ("vhb", ctypes.POINTER(ctypes.c_double)), ("nci", ctypes.POINTER(ctypes.c_int32)),
+                ("c6xy", ctypes.POINTER(ctypes.c_double)), ("r094", ctypes.POINTER(ctypes.c_double)),
+                ("sr42", ctypes.POINTER(ctypes.c_double)), ("zab", ctypes.POINTER(ctypes.c_double)),
+                ("r0ab", ctypes.POINTER(ctypes.c_double)), ("eps1", ctypes.c_double * 6),
+                ("eps2", ctypes.c_double * 6), ("rad", ctypes.POINTER(ctypes.c_double)),
+                ("scalehb", ctypes.POINTER(ctypes.c_double)), ("periodic", ctypes.c_int32),
+                ("zahn", ctypes.c_int32), ("box", ctypes.c_double * 3), ("coul_cut", ctypes.c_double),
+                ("vdw_cut", ctypes.c_double), ("cut_low", ctypes.c_double), ("zahn_a", ctypes.c_double),
+                ("zahn_par", ctypes.c_double)]
+
+
 class FFDesc(ctypes.Structure):
     _fields_ = [
         ("natoms", ctypes.c_int32), ("nbonds", ctypes.c_int32), ("nangles", ctypes.c_int32),
@@ -116,6 +136,7 @@ PROTOTYPES = {
     "cam_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_batch_sync": (ctypes.c_int, [_vp, _dp]),
     "cam_ff_create": (ctypes.c_int, [ctypes.POINTER(FFDesc), ctypes.POINTER(_vp)]),
+    "cam_qmdff_create": (ctypes.c_int, [ctypes.POINTER(QmdffDesc), ctypes.POINTER(_vp)]),
     "cam_ff_destroy": (ctypes.c_int, [_vp]),
     "cam_ff_set_pair_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_ff_energy": (ctypes.c_int, [_vp, _dp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),

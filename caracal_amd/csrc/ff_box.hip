@@ -25,6 +25,7 @@
 #include "common.hpp"
 #include "nhc.hpp"
 #include "philox.hpp"
+#include "qmdff.hpp"
 
 namespace cam {
 namespace {
@@ -679,7 +680,8 @@ __global__ __launch_bounds__(kTile) void water_intra_kernel(WaterDev w, const do
 using namespace cam;
 
 struct cam_ff {
-    int kind = 0;                     // 0: QMDFF form, 1: the reference's water model
+    int kind = 0;                     // 0: QMDFF form, 1: the reference's water model, 2: the reference's QMDFF
+    QmdffDev *qm = nullptr;           // kind 2
     WaterDev w{};
     FFDev d{};
     int natoms = 0;
@@ -704,6 +706,7 @@ bool ff_uses_list(const cam_ff *ff)
 
 int ff_ensure(cam_ff *ff, int nbeads)
 {
+    if (ff->kind == 2) return CAM_OK;             // the QMDFF tables hold their own buffers
     if (ff_uses_list(ff) && !ff->nbr) {
         const size_t n = ff->natoms;
         CAM_HIP(hipMalloc(&ff->nbr, n * n * sizeof(int)));
@@ -736,6 +739,7 @@ int ff_launch(cam_ff *ff, const Theta &th, const double *x, int nbeads, double *
     const bool grad = g != nullptr;
     const bool par = dth != nullptr || dvth != nullptr;
     dim3 gnb(nbx, ff->nsplit, nbeads);
+    if (ff->kind == 2) return qmdff_launch<T>(ff->qm, x, nbeads, e, g, s);
     if (ff->kind == 1) {
         if (grad)
             hipLaunchKernelGGL((water_nb_kernel<T, true>), gnb, dim3(kTile), 0, s, ff->w, x, ff->nsplit, ff->gpart,
@@ -922,9 +926,74 @@ int cam_water_create(int nwater, int periodic, double box, int zahn, double rcut
     return CAM_OK;
 }
 
+int cam_qmdff_create(const cam_qmdff_desc *d, cam_ff **out)
+{
+    CAM_CHECK_ARG(d && out, "null argument");
+    *out = nullptr;
+    const int n = d->natoms;
+    CAM_CHECK_ARG(n >= 1 && n <= 65536, "natoms must be in 1..65536");
+    CAM_CHECK_ARG(d->nbond >= 0 && d->nangl >= 0 && d->ntors >= 0 && d->nhb >= 0 && d->nnci >= 0,
+                  "negative term count");
+    CAM_CHECK_ARG(d->at && d->molnum && d->q && d->c6xy && d->r094 && d->sr42 && d->zab && d->r0ab && d->rad &&
+                      d->scalehb,
+                  "missing per-atom or pair tables");
+    CAM_CHECK_ARG((d->nbond == 0 || (d->bond && d->vbond)) && (d->nangl == 0 || (d->angl && d->vangl)) &&
+                      (d->ntors == 0 || (d->tors && d->vtors)) && (d->nhb == 0 || (d->hb && d->vhb)) &&
+                      (d->nnci == 0 || d->nci),
+                  "missing term list");
+    for (int i = 0; i < n; ++i) CAM_CHECK_ARG(d->at[i] >= 1 && d->at[i] <= 94, "at[%d] must be in 1..94", i);
+    auto atom_ok = [&](const int32_t *v, size_t cnt) {
+        for (size_t k = 0; k < cnt; ++k)
+            if (v[k] < 1 || v[k] > n) return false;
+        return true;
+    };
+    CAM_CHECK_ARG(atom_ok(d->bond, 2 * (size_t)d->nbond) && atom_ok(d->angl, 3 * (size_t)d->nangl) &&
+                      atom_ok(d->hb, 3 * (size_t)d->nhb),
+                  "atom index out of 1..natoms");
+    for (int m = 0; m < d->ntors; ++m) {
+        CAM_CHECK_ARG(atom_ok(d->tors + 6 * m, 4), "torsion %d: atom index out of range", m + 1);
+        CAM_CHECK_ARG(d->tors[6 * m + 4] >= 0 && d->tors[6 * m + 4] <= 4, "torsion %d: 0..4 terms", m + 1);
+    }
+    for (int m = 0; m < d->nnci; ++m) {
+        CAM_CHECK_ARG(atom_ok(d->nci + 3 * m, 2), "nci pair %d: atom index out of range", m + 1);
+        CAM_CHECK_ARG(d->nci[3 * m + 2] >= 1 && d->nci[3 * m + 2] <= 6, "nci pair %d: kind must be 1..6", m + 1);
+    }
+    // halogen bonds (eabxag) are not implemented
+    int nmols = 0;
+    for (int i = 0; i < n; ++i) nmols = std::max(nmols, (int)d->molnum[i]);
+    for (int m = 0; m < d->nhb; ++m)
+        if (d->at[d->hb[3 * m + 2] - 1] != 1) {
+            set_error("hydrogen-bond term %d has a halogen bridge (halogen bonds are not implemented)", m + 1);
+            return CAM_ERR_UNSUPPORTED;
+        }
+    auto halogen = [](int z) { return z == 17 || z == 35 || z == 53 || z == 85; };
+    if (nmols > 1)
+        for (int m = 0; m < d->nbond; ++m) {
+            const int a = d->at[d->bond[2 * m] - 1], b = d->at[d->bond[2 * m + 1] - 1];
+            if ((halogen(a) && b != 1) || (halogen(b) && a != 1)) {
+                set_error("bond %d gives intermolecular halogen bonds (not implemented)", m + 1);
+                return CAM_ERR_UNSUPPORTED;
+            }
+        }
+    if (d->periodic) {
+        CAM_CHECK_ARG(d->box[0] > 0.0 && d->box[1] > 0.0 && d->box[2] > 0.0, "box lengths must be > 0");
+    }
+    clear_error();
+    QmdffDev *q = nullptr;
+    const int rc = qmdff_build(d, &q);
+    if (rc != CAM_OK) return rc;
+    cam_ff *ff = new cam_ff();
+    ff->kind = 2;
+    ff->qm = q;
+    ff->natoms = n;
+    *out = ff;
+    return CAM_OK;
+}
+
 int cam_ff_destroy(cam_ff *ff)
 {
     if (!ff) return CAM_OK;
+    qmdff_free(ff->qm);
     int *ib[] = { ff->type, ff->mol, ff->bonds, ff->angles, ff->csr_ptr, ff->csr_term };
     for (int *p : ib)
         if (p) (void)hipFree(p);
@@ -952,7 +1021,7 @@ int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, 
                   void *stream)
 {
     CAM_CHECK_ARG(ff && x && e, "null argument");
-    CAM_CHECK_ARG(theta || ff->kind == 1, "theta is required for this force field");
+    CAM_CHECK_ARG(theta || ff->kind != 0, "theta is required for this force field");
     CAM_CHECK_ARG(nbeads >= 1, "nbeads must be >= 1");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision");
     clear_error();
@@ -967,8 +1036,8 @@ int cam_ff_forces(cam_ff *ff, const double *theta, const double *x, int nbeads, 
                   double *dedtheta, const double *v, double *dvdtheta, int precision, void *stream)
 {
     CAM_CHECK_ARG(ff && x && dedx, "null argument");
-    CAM_CHECK_ARG(theta || ff->kind == 1, "theta is required for this force field");
-    CAM_CHECK_ARG(ff->kind == 0 || (!dedtheta && !dvdtheta), "the water model has no parameter derivatives");
+    CAM_CHECK_ARG(theta || ff->kind != 0, "theta is required for this force field");
+    CAM_CHECK_ARG(ff->kind == 0 || (!dedtheta && !dvdtheta), "this force field has no parameter derivatives");
     CAM_CHECK_ARG(nbeads >= 1, "nbeads must be >= 1");
     CAM_CHECK_ARG(!dvdtheta || v, "dvdtheta needs the direction v");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision");
@@ -1212,6 +1281,10 @@ int box_enqueue(cam_box *bx, int nsteps)
         hipLaunchKernelGGL(box_nm_kernel, dim3((ncol + cpb - 1) / cpb), dim3(256), 2 * 256 * sizeof(double), bx->stream,
                            a);
         CAM_LAUNCH_CHECK();
+        if (bx->ff->kind == 2) {                       // the periodic wrap of src/verlet.f90:540-640
+            const int rw = qmdff_wrap(bx->ff->qm, bx->q, bx->nb, bx->stream);
+            if (rw != CAM_OK) return rw;
+        }
         int rc = bx->precision == CAM_PREC_MIXED
                      ? ff_launch<float>(bx->ff, th, bx->q, bx->nb, bx->e, bx->g, nullptr, nullptr, nullptr, bx->stream)
                      : ff_launch<double>(bx->ff, th, bx->q, bx->nb, bx->e, bx->g, nullptr, nullptr, nullptr, bx->stream);

@@ -48,6 +48,42 @@ class ForceField:
         self._h, self.natoms, self._keep = h, 3 * nwater, []
         return self
 
+    @classmethod
+    def qmdff(cls, p):
+        """The reference's QMDFF (ff_eg + ff_nonb + ff_hb) from its module
+        arrays: a dict as caracal_amd.qmdff.qmdff_params returns (1-based
+        atom indices, element-pair tables [z_a - 1][z_b - 1], bohr).  theta
+        arguments of the methods are ignored."""
+        self = cls.__new__(cls)
+        keep = []
+
+        def arr(v, dtype, ctype):
+            a = np.ascontiguousarray(v, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+        d = _abi.QmdffDesc()
+        d.natoms = int(p["natoms"])
+        d.nbond, d.nangl, d.ntors = len(p["bond"]), len(p["angl"]), len(p["tors"])
+        d.nhb, d.nnci = len(p["hb"]), len(p["nci"])
+        for name in ("at", "bond", "angl", "tors", "hb", "nci"):
+            setattr(d, name, arr(p[name], np.int32, ctypes.c_int32))
+        for name in ("molnum", "q", "vbond", "vangl", "vtors", "vhb", "c6xy", "rad", "scalehb"):
+            setattr(d, name, arr(p[name], np.float64, ctypes.c_double))
+        for name in ("r094", "sr42", "zab", "r0ab"):     # to the module's column-major (94, 94)
+            setattr(d, name, arr(np.asarray(p[name]).T, np.float64, ctypes.c_double))
+        for k in range(6):
+            d.eps1[k], d.eps2[k] = p["eps1"][k], p["eps2"][k]
+        d.periodic, d.zahn = int(p["periodic"]), int(p["zahn"])
+        for k in range(3):
+            d.box[k] = p["box"][k]
+        d.coul_cut, d.vdw_cut, d.cut_low = p["coul_cut"], p["vdw_cut"], p["cut_low"]
+        d.zahn_a, d.zahn_par = p["zahn_a"], p["zahn_par"]
+        h = ctypes.c_void_p()
+        check("cam_qmdff_create", lib().cam_qmdff_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h, self.natoms, self._keep = h, d.natoms, keep
+        return self
+
     def set_pair_mode(self, mode):
         """_abi.CAM_FF_PAIRS_LIST (default: a neighbour list from the bead
         centroids, the same pairs and order as all pairs) or CAM_FF_PAIRS_ALL."""

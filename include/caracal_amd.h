@@ -413,6 +413,47 @@ int cam_ff_energy(cam_ff *ff, const double *theta, const double *x, int nbeads, 
 int cam_ff_forces(cam_ff *ff, const double *theta, const double *x, int nbeads, double *e, double *dedx,
                   double *dedtheta, const double *v, double *dvdtheta, int precision, void *stream);
 
+/* The reference's own QMDFF force field (one QMDFF, src/gradient.f90:341-348:
+ * ff_eg + ff_nonb + ff_hb) as a cam_ff for cam_ff_energy / cam_ff_forces and
+ * cam_box_create.  The descriptor holds what the reference's prepare
+ * (src/prepare.f90:30-196: rdsolvff, setnonb, copyc6, ncoord_qmdff, getc6)
+ * and set_periodic (src/set_periodic.f90) leave in the qmdff, evb_mod and
+ * pbc_mod modules, in their layouts: 1-based atom indices, per-term
+ * parameter columns, 94 x 94 element-pair tables column-major (element
+ * (z_a, z_b) at [(z_b - 1) * 94 + z_a - 1]), lengths in bohr.  Terms: bond
+ * stretch k(1 + (r0/r)^a - 2 (r0/r)^(a/2)), bends k(cos t - cos t0)^2 (or
+ * k(t - t0)^2 for t0 ~ pi) damped by abdamp, torsions and out-of-plane terms
+ * (src/ff_eg.f90:87-619), the nci pair list with eps2/eps1 scaling and the
+ * pairs between molecules (dispersion, repulsion, Coulomb plain, switched or
+ * Zahn; src/ff_nonb.f90:88-509), listed and intermolecular hydrogen bonds
+ * (src/ff_hb.f90:60-276, src/eabhag.f90).  Halogen bonds (eabxag) and Ewald
+ * sums are not implemented (CAM_ERR_UNSUPPORTED).  Energies exclude the
+ * QMDFF shift (E_zero1).  theta is ignored; no parameter derivatives. */
+typedef struct {
+    int32_t natoms, nbond, nangl, ntors, nhb, nnci;
+    const int32_t *at;        /* natoms atomic numbers 1..94 */
+    const double *molnum;     /* natoms molecule numbers (all 0: one molecule) */
+    const double *q;          /* natoms charges */
+    const int32_t *bond;      /* 2 * nbond: bond(1:2, m) */
+    const double *vbond;      /* 3 * nbond: r0, k, exponent */
+    const int32_t *angl;      /* 3 * nangl: vertex, then the outer atoms */
+    const double *vangl;      /* 2 * nangl: theta0, k */
+    const int32_t *tors;      /* 6 * ntors: i, j, k, l, number of terms, kind (2: out of plane) */
+    const double *vtors;      /* 14 * ntors */
+    const int32_t *hb;        /* 3 * nhb: A, B, H */
+    const double *vhb;        /* 2 * nhb */
+    const int32_t *nci;       /* 3 * nnci: i, j, kind 1..6 */
+    const double *c6xy;       /* natoms * natoms, symmetric */
+    const double *r094, *sr42, *zab, *r0ab;   /* 94 x 94 each */
+    double eps1[6], eps2[6];
+    const double *rad;        /* 94 covalent radii, bohr */
+    const double *scalehb;    /* 94 hydrogen-bond scaling factors */
+    int32_t periodic, zahn;
+    double box[3];            /* box lengths, bohr */
+    double coul_cut, vdw_cut, cut_low, zahn_a, zahn_par;
+} cam_qmdff_desc;
+int cam_qmdff_create(const cam_qmdff_desc *desc, cam_ff **out);
+
 /* Ring-polymer dynamics of one box (replaces verlet's per-bead force loop
  * over gradient -> ff_eg/ff_nonb, src/verlet.f90:772-777, for a QMDFF box).
  * andersen_step 0 = NVE.  State arrays are host (3,natoms,nbeads). */

@@ -120,13 +120,40 @@ def nhc(p, masses, vnh, nose_q, beta, dt):
     return scale * p
 
 
+def wrap(q, box):
+    """The periodic wrap of src/verlet.f90:540-640 (orthorhombic box): bead
+    by bead, an atom whose coordinate lies below 0 or above the box length is
+    moved by a box length with all its beads, until inside."""
+    nb, na, _ = q.shape
+    for k in range(nb):
+        for j in range(na):
+            for c in range(3):
+                L = box[c]
+                while q[k, j, c] < 0.0:
+                    q[:, j, c] += L
+                while q[k, j, c] > L:
+                    q[:, j, c] -= L
+
+
+class QmdffFF:
+    """The oracle's QMDFF in the interface BoxRPMD expects."""
+
+    def __init__(self, qo):
+        self.qo = qo
+
+    def eval(self, theta, x):
+        e, g, _ = self.qo.eval(x)
+        return e, g, None, None
+
+
 class BoxRPMD:
     """The oracle's ring polymer of one box; arrays (nbeads, natoms, 3), bohr
     and atomic units.  nose_q > 0 adds the Nose-Hoover chain before the first
     and after the second half kick (src/verlet.f90:185-206, :1086-1106)."""
 
-    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False, nose_q=0.0):
+    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False, nose_q=0.0, wrap_box=None):
         self.orthogonal = orthogonal
+        self.wrap_box = None if wrap_box is None else np.asarray(wrap_box, dtype=np.float64)
         self.nose_q, self.vnh = nose_q, [0.0, 0.0, 0.0, 0.0]
         self.ffo, self.theta = ffo, np.asarray(theta, dtype=np.float64)
         self.q, self.p = np.array(q, dtype=np.float64), np.array(p, dtype=np.float64)
@@ -147,6 +174,8 @@ class BoxRPMD:
             self.p = nhc(self.p, self.m, self.vnh, self.nose_q, self.beta, self.dt)
         self.p = self.p - hdt * self.g
         self.q, self.p = free_rp(self.q, self.p, self.dt, self.m, self.beta, self.orthogonal)
+        if self.wrap_box is not None:
+            wrap(self.q, self.wrap_box)
         self.v, self.g = self.forces(self.q)
         self.p = self.p - hdt * self.g
         if self.nose_q > 0:
