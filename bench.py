@@ -253,6 +253,44 @@ def box_nvt(cam, steps=500):
     return out
 
 
+def qmdff_ethanol_box(cam, steps=200, nb=16):
+    """The reference's examples/dynamic/ethanol_box on the GPU: 125 ethanol
+    molecules (1125 atoms) in the periodic 27 angstrom box with the
+    reference's own QMDFF (every molecule with the parameters of
+    examples/poly_qmdff/ethanol/ethanol.qmdff), 16 beads, the Nose-Hoover
+    chain at 200 K, dt 0.5 fs, the periodic wrap and transrot every step, as
+    sampling.key runs it; FP64 and mixed precision."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import qmdff_fixtures as qf
+    from caracal_amd import ff as cff
+    p, x = qf.ethanol_box()
+    amu = {1: 1.008, 6: 12.011, 8: 15.999}
+    m = cam.mass_au([amu[int(z)] for z in p["at"]])
+    beta = float(cam.beta_of(200.0))
+    dt = 0.5 * cam.FS_TO_AU
+    rng = np.random.default_rng(3)
+    q = x[None] + rng.normal(0.0, 0.02, (nb,) + x.shape)
+    pm = rng.normal(size=q.shape) * np.sqrt(m[None, :, None] * nb / beta)
+    ff = cff.ForceField.qmdff(p)
+    out = {"atoms": int(p["natoms"]), "nbeads": nb, "steps": steps}
+    for name, prec in (("fp64", cam.CAM_PREC_FP64), ("mixed", cam.CAM_PREC_MIXED)):
+        eng = cff.BoxEngine(ff, nb, dt, m, beta, None, precision=prec, nm_mode=cam.CAM_NM_REFERENCE)
+        eng.set_nhc(20 * dt)
+        eng.set_transrot(True)
+        eng.set_state(q, pm)
+        eng.prepare(steps)
+        eng.run(10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms = eng.run(steps)
+        wall = time.perf_counter() - t0
+        eng.close()
+        out[name] = {"bead_force_evals_per_s": nb * steps / wall, "ms_per_step": wall / steps * 1e3, "ms_gpu": ms}
+    ff.close()
+    return out
+
+
 def force_matching(cam):
     """50 Adam steps of force matching on the 3000-atom box with 64 beads,
     from the reference parameters perturbed by 10% (seed 3), gradients from
@@ -432,6 +470,8 @@ def main():
     ap.add_argument("--schedule", default=None,
                     help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
                          "the autotuner (for profiling: every launch then has the same shape)")
+    ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "force_matching"],
+                    help="run only this extra measurement and print its result")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -456,6 +496,13 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 8 * n, "copies": 5}))
+        return
+
+    if args.only:
+        fn = {"qmdff_box": lambda: qmdff_ethanol_box(cam), "box": lambda: box_nvt(cam),
+              "ragged": lambda: ragged_batch(cam, workloads), "oh3": lambda: oh3_batch(cam, workloads),
+              "force_matching": lambda: force_matching(cam)}[args.only]
+        print(json.dumps({args.only: fn()}))
         return
 
     ntraj, nb = args.ntraj, args.nbeads
@@ -544,6 +591,7 @@ def main():
         out["ragged"] = ragged_batch(cam, workloads)
         out["oh3"] = oh3_batch(cam, workloads)
         out["box"] = box_nvt(cam)
+        out["qmdff_ethanol_box"] = qmdff_ethanol_box(cam)
         out["force_matching"] = force_matching(cam)
         out["rpmd_rate_h_h2"] = rpmd_rate_h_h2()
     if cpu is not None:

@@ -149,6 +149,7 @@ PROTOTYPES = {
     "cam_box_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
     "cam_box_prepare": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_box_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cam_box_set_transrot": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_box_set_nhc": (ctypes.c_int, [_vp, ctypes.c_double]),
     "cam_box_sync": (ctypes.c_int, [_vp, _dp]),
     "cam_box_energies": (ctypes.c_int, [_vp, _dp, _dp, _dp]),

@@ -1226,6 +1226,160 @@ __global__ void box_scale_kernel(double *p, size_t n, const double *st)
     if (i < n) p[i] = st[4] * p[i];
 }
 
+// transrot for one box (src/transrot.f90:36-237, with its total mass of
+// nbeads^2 * M): two passes of fixed-order block sums over every (bead,
+// atom) -- mass-weighted velocities and positions, then, about the centre
+// they give, the angular momentum and the inertia tensor -- one thread that
+// inverts the tensor, and the velocity update.  tr[0..2] vtot, [3..5] centre,
+// [6..8] angular velocity.
+constexpr int kTrBlocks = 64;
+
+__device__ void tr_block_sum(double *v, int nv, double *sh, double *out)
+{
+    // v: this thread's nv partial sums -> out[nv] for the block, fixed order
+    for (int c = 0; c < nv; ++c) sh[c * 256 + threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < nv; ++c) sh[c * 256 + threadIdx.x] = sh[c * 256 + threadIdx.x] + sh[c * 256 + threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < nv; ++c) out[c] = sh[c * 256];
+}
+
+__global__ __launch_bounds__(256) void box_tr_pass1(const double *q, const double *p, const double *mass, int natoms,
+                                                    int nb, double *part)
+{
+    __shared__ double sh[6 * 256];
+    double v[6] = { 0, 0, 0, 0, 0, 0 };
+    const size_t n = (size_t)natoms * nb;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)kTrBlocks * 256) {
+        const double w = mass[e % natoms];
+        for (int c = 0; c < 3; ++c) {
+            v[c] = v[c] + (p[3 * e + c] / w) * w;
+            v[3 + c] = v[3 + c] + q[3 * e + c] * w;
+        }
+    }
+    tr_block_sum(v, 6, sh, part + 6 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void box_tr_pass2(const double *q, const double *p, const double *mass, int natoms,
+                                                    int nb, const double *part1, double totmass, double *tr,
+                                                    double *part)
+{
+    __shared__ double sh[9 * 256];
+    __shared__ double ctr[6];
+    if (threadIdx.x == 0) {
+        double s[6] = { 0, 0, 0, 0, 0, 0 };
+        for (int b = 0; b < kTrBlocks; ++b)
+            for (int c = 0; c < 6; ++c) s[c] = s[c] + part1[6 * b + c];
+        for (int c = 0; c < 6; ++c) ctr[c] = s[c] / totmass;
+        if (blockIdx.x == 0)
+            for (int c = 0; c < 6; ++c) tr[c] = ctr[c];
+    }
+    __syncthreads();
+    double v[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    const size_t n = (size_t)natoms * nb;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)kTrBlocks * 256) {
+        const double w = mass[e % natoms];
+        const double *x = q + 3 * e;
+        const double vx = p[3 * e] / w, vy = p[3 * e + 1] / w, vz = p[3 * e + 2] / w;
+        v[0] = v[0] + (x[1] * vz - x[2] * vy) * w;
+        v[1] = v[1] + (x[2] * vx - x[0] * vz) * w;
+        v[2] = v[2] + (x[0] * vy - x[1] * vx) * w;
+        const double xd = x[0] - ctr[3], yd = x[1] - ctr[4], zd = x[2] - ctr[5];
+        v[3] = v[3] + xd * xd * w;
+        v[4] = v[4] + xd * yd * w;
+        v[5] = v[5] + xd * zd * w;
+        v[6] = v[6] + yd * yd * w;
+        v[7] = v[7] + yd * zd * w;
+        v[8] = v[8] + zd * zd * w;
+    }
+    tr_block_sum(v, 9, sh, part + 9 * blockIdx.x);
+}
+
+__global__ void box_tr_solve(const double *part, int natoms, double totmass, double *tr)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    for (int b = 0; b < kTrBlocks; ++b)
+        for (int c = 0; c < 9; ++c) s[c] = s[c] + part[9 * b + c];
+    const double vt[3] = { tr[0], tr[1], tr[2] }, xt = tr[3], yt = tr[4], zt = tr[5];
+    double mang[3];
+    mang[0] = s[0] - (yt * vt[2] - zt * vt[1]) * totmass;
+    mang[1] = s[1] - (zt * vt[0] - xt * vt[2]) * totmass;
+    mang[2] = s[2] - (xt * vt[1] - yt * vt[0]) * totmass;
+    const double xx = s[3], xy = s[4], xz = s[5], yy = s[6], yz = s[7], zz = s[8];
+    double ten[3][3];
+    ten[0][0] = yy + zz; ten[0][1] = -xy;    ten[0][2] = -xz;
+    ten[1][0] = -xy;     ten[1][1] = xx + zz; ten[1][2] = -yz;
+    ten[2][0] = -xz;     ten[2][1] = -yz;    ten[2][2] = xx + yy;
+    if (natoms <= 2) {
+        ten[0][0] += 0.000001;
+        ten[1][1] += 0.000001;
+        ten[2][2] += 0.000001;
+    }
+    // Gauss-Jordan inversion with full pivoting (src/invert.f90)
+    int ipiv[3] = { 0, 0, 0 }, indxr[3], indxc[3];
+    for (int i = 0; i < 3; ++i) {
+        double big = 0.0;
+        int irow = 0, icol = 0;
+        for (int j = 0; j < 3; ++j)
+            if (ipiv[j] != 1)
+                for (int kk = 0; kk < 3; ++kk)
+                    if (ipiv[kk] == 0 && fabs(ten[j][kk]) >= big) {
+                        big = fabs(ten[j][kk]);
+                        irow = j;
+                        icol = kk;
+                    }
+        ipiv[icol] += 1;
+        if (irow != icol)
+            for (int j = 0; j < 3; ++j) {
+                const double tmp = ten[irow][j];
+                ten[irow][j] = ten[icol][j];
+                ten[icol][j] = tmp;
+            }
+        indxr[i] = irow;
+        indxc[i] = icol;
+        const double piv = ten[icol][icol];
+        ten[icol][icol] = 1.0;
+        for (int j = 0; j < 3; ++j) ten[icol][j] = ten[icol][j] / piv;
+        for (int j = 0; j < 3; ++j)
+            if (j != icol) {
+                const double tmp = ten[j][icol];
+                ten[j][icol] = 0.0;
+                for (int kk = 0; kk < 3; ++kk) ten[j][kk] = ten[j][kk] - ten[icol][kk] * tmp;
+            }
+    }
+    for (int i = 2; i >= 0; --i)
+        if (indxr[i] != indxc[i])
+            for (int kk = 0; kk < 3; ++kk) {
+                const double tmp = ten[kk][indxr[i]];
+                ten[kk][indxr[i]] = ten[kk][indxc[i]];
+                ten[kk][indxc[i]] = tmp;
+            }
+    for (int i = 0; i < 3; ++i) {
+        double v = 0.0;
+        for (int j = 0; j < 3; ++j) v = v + ten[i][j] * mang[j];
+        tr[6 + i] = v;
+    }
+}
+
+__global__ void box_tr_apply(const double *q, double *p, const double *mass, int natoms, int nb, const double *tr)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)natoms * nb) return;
+    const double w = mass[e % natoms];
+    double v[3];
+    for (int j = 0; j < 3; ++j) v[j] = p[3 * e + j] / w - tr[j];
+    const double xd = q[3 * e] - tr[3], yd = q[3 * e + 1] - tr[4], zd = q[3 * e + 2] - tr[5];
+    v[0] = v[0] - tr[7] * zd + tr[8] * yd;
+    v[1] = v[1] - tr[8] * xd + tr[6] * zd;
+    v[2] = v[2] - tr[6] * yd + tr[7] * xd;
+    for (int j = 0; j < 3; ++j) p[3 * e + j] = v[j] * w;
+}
+
 }  // namespace
 }  // namespace cam
 
@@ -1242,6 +1396,9 @@ struct cam_box {
     double nose_q = 0.0;               // Nose-Hoover chain on when > 0
     double *nhc = nullptr;             // chain state: 4 velocities and the last scale factor
     double *nhc_part = nullptr;        // kinetic-energy parts (kBoxNhcBlocks)
+    bool transrot = false;             // remove centre-of-mass motion after every step (dynamic.x)
+    double totmass = 0.0;              // transrot's nbeads^2 * M
+    double *tr = nullptr;              // transrot: partial sums and the solved velocities
     bool have_state = false;
     std::map<int, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evf0 = nullptr, evf1 = nullptr;
@@ -1294,6 +1451,21 @@ int box_enqueue(cam_box *bx, int nsteps)
         CAM_LAUNCH_CHECK();
         rc = chain();
         if (rc != CAM_OK) return rc;
+        if (bx->transrot) {                             // src/verlet.f90:1299-1305
+            double *part1 = bx->tr + 16, *part2 = part1 + 6 * kTrBlocks;
+            hipLaunchKernelGGL(box_tr_pass1, dim3(kTrBlocks), dim3(256), 0, bx->stream, bx->q, bx->p, bx->mass,
+                               bx->natoms, bx->nb, part1);
+            CAM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(box_tr_pass2, dim3(kTrBlocks), dim3(256), 0, bx->stream, bx->q, bx->p, bx->mass,
+                               bx->natoms, bx->nb, part1, bx->totmass, bx->tr, part2);
+            CAM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(box_tr_solve, dim3(1), dim3(64), 0, bx->stream, part2, bx->natoms, bx->totmass, bx->tr);
+            CAM_LAUNCH_CHECK();
+            const size_t ne = (size_t)bx->natoms * bx->nb;
+            hipLaunchKernelGGL(box_tr_apply, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, bx->stream, bx->q,
+                               bx->p, bx->mass, bx->natoms, bx->nb, bx->tr);
+            CAM_LAUNCH_CHECK();
+        }
     }
     hipLaunchKernelGGL(box_advance_kernel, dim3(1), dim3(1), 0, bx->stream, bx->step_base, nsteps);
     CAM_LAUNCH_CHECK();
@@ -1359,6 +1531,7 @@ int cam_box_destroy(cam_box *bx)
     if (bx->step_base) (void)hipFree(bx->step_base);
     if (bx->nhc) (void)hipFree(bx->nhc);
     if (bx->nhc_part) (void)hipFree(bx->nhc_part);
+    if (bx->tr) (void)hipFree(bx->tr);
     if (bx->ev0) (void)hipEventDestroy(bx->ev0);
     if (bx->ev1) (void)hipEventDestroy(bx->ev1);
     if (bx->stream) (void)hipStreamDestroy(bx->stream);
@@ -1396,6 +1569,25 @@ int cam_box_set_state(cam_box *bx, const double *q, const double *p)
     // a new state starts the Nose-Hoover chain at rest, as mdinit does
     if (bx->nhc) CAM_HIP(hipMemsetAsync(bx->nhc, 0, 5 * sizeof(double), bx->stream));
     return cam_box_set_params(bx, bx->theta);
+}
+
+int cam_box_set_transrot(cam_box *bx, int on)
+{
+    CAM_CHECK_ARG(bx, "null box");
+    clear_error();
+    if (on && !bx->tr) CAM_HIP(hipMalloc(&bx->tr, (16 + 15 * (size_t)kTrBlocks) * sizeof(double)));
+    // the reference's total mass: every atom counted once per bead, then
+    // times nbeads (src/transrot.f90:62-72)
+    std::vector<double> m(bx->natoms);
+    CAM_HIP(hipMemcpy(m.data(), bx->mass, m.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double tm = 0.0;
+    for (int i = 0; i < bx->natoms; ++i)
+        for (int k = 0; k < bx->nb; ++k) tm = tm + m[i];
+    bx->totmass = tm * bx->nb;
+    bx->transrot = on != 0;
+    for (auto &kv : bx->graphs) (void)hipGraphExecDestroy(kv.second);   // part of the captured step
+    bx->graphs.clear();
+    return CAM_OK;
 }
 
 int cam_box_set_nhc(cam_box *bx, double nose_q)

@@ -207,6 +207,10 @@ class BoxEngine:
     def prepare(self, n):
         check("cam_box_prepare", lib().cam_box_prepare(self._h, n))
 
+    def set_transrot(self, on=True):
+        """transrot after every step, as dynamic.x does (src/verlet.f90:1299-1305)."""
+        check("cam_box_set_transrot", lib().cam_box_set_transrot(self._h, int(on)))
+
     def set_nhc(self, nose_q):
         """Nose-Hoover chain thermostat with damping time nose_q (atomic time
         units; 0 turns it off).  Needs andersen_step 0."""

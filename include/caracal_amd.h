@@ -473,6 +473,11 @@ int cam_box_run(cam_box *box, int nsteps);
  * Needs andersen_step 0.  The chain starts at rest, and again after every
  * cam_box_set_state (mdinit). */
 int cam_box_set_nhc(cam_box *box, double nose_q);
+/* Centre-of-mass translation and rotation removal after every step, as
+ * dynamic.x's verlet ends each step with transrot (src/verlet.f90:1299-1305,
+ * src/transrot.f90:36-237, its total mass nbeads^2 * M included); on != 0
+ * turns it on.  Off by default. */
+int cam_box_set_transrot(cam_box *box, int on);
 /* Waits for the last run; ms (may be NULL) receives its duration measured
  * with HIP events on the box's stream. */
 int cam_box_sync(cam_box *box, double *ms);

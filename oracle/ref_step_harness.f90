@@ -11,11 +11,13 @@
 ! Usage: ref_step_harness <input> <output>
 ! Input (stream, native endianness):
 !   int32  natoms, nbeads, nsteps, constrain (-1 dynamic.x, 0 umbrella
-!          window, 2 recrossing child), surface (1 H3, 2 CH4H, 3 OH3),
+!          window, 2 recrossing child), surface (1 H3, 2 CH4H, 3 OH3, 4 QMDFF),
 !          every (write a frame after every 'every' steps; 0 = timing only),
 !          thermostat (0 none, 1 Andersen, 2 Nose-Hoover chain), andersen_step
 !   real64 dt, beta, mass(natoms), q(3,natoms,nbeads), p(3,natoms,nbeads)
 !   thermostat 2:  real64 kelvin, nose_q
+!   surface 4:     int32 length, then the QMDFF file name; int32 periodic,
+!                  zahn; real64 box, coul_cut, vdw_cut (angstrom, key-file units)
 !   constrain >= 0 adds the BIMOLEC reaction coordinate:
 !   int32  n_form, n_break, bond_form(2,n_form), bond_break(2,n_break)
 !   real64 form_ref(n_form), break_ref(n_break)
@@ -39,6 +41,9 @@ program ref_step_harness
    real(kind=8), allocatable :: derivs(:, :, :), dxi_act(:, :), p_in(:, :, :)
    integer(kind=8) :: c0, c1, crate
    character(len=512) :: fin, fout
+   character(len=70) :: ff1, ff2, ff3
+   integer :: plen, iper, izahn
+   real(kind=8) :: boxa, ccut, vcut
    logical :: analyze
 
    ! dynamic.x runs under MPI (src/dynamic.f90); verlet's dynamic.x branch
@@ -58,6 +63,13 @@ program ref_step_harness
    kelvin = 1.d0 / (0.316679d-5 * beta_in)
    nose_q = 0.d0
    if (ithermo == 2) read (71) kelvin, nose_q
+   if (surf == 4) then
+      read (71) plen
+      ff1 = ""
+      read (71) ff1(1:plen)
+      read (71) iper, izahn
+      read (71) boxa, ccut, vcut
+   end if
    xi_ideal = 0.d0
    if (cmode >= 0) then
       read (71) nf, nbk
@@ -111,6 +123,38 @@ program ref_step_harness
    case (3)
       pot_type = "oh3"
       call initialize_oh3()
+   case (4)
+      ! one QMDFF (src/read_pes.f90:1745-1919, :2729-2736); gradient then
+      ! takes its ff_eg / ff_nonb / ff_hb branch (src/gradient.f90:341-355)
+      pot_ana = .false.
+      pot_type = "qmdff"
+      ewald = .false.
+      zahn = izahn /= 0
+      coul_cut = ccut
+      vdw_cut = vcut
+      eval_cutoff = .false.
+      ff2 = "none"
+      ff3 = "none"
+      call prepare(ff1, ff2, ff3, 1)
+      nqmdff = 1
+      treq = .false.
+      E_zero1 = 0.d0
+      water_spc = .false.
+      gfn_xtb = .false.
+      pgfn_ff = .false.
+      orca = .false.
+      aenet_ann = .false.
+      mace_ase = .false.
+      uma_ase = .false.
+      call_ext = .false.
+      call_cust = .false.
+      periodic = iper /= 0
+      if (periodic) then
+         boxlen_x = boxa / bohr
+         boxlen_y = boxa / bohr
+         boxlen_z = boxa / bohr
+         call set_periodic(periodic, boxlen_x, boxlen_y, boxlen_z, ewald, zahn, ewald_brute, coul_cut, vdw_cut)
+      end if
    end select
    use_calc_rate = cmode >= 0
    use_stick_coeff = .false.
@@ -122,7 +166,7 @@ program ref_step_harness
    fix_atoms = .false.
    barostat = 0
    npt = .false.
-   periodic = .false.
+   if (surf /= 4) periodic = .false.
    box_walls = .false.
    mirrors = .false.
    calc_ekin = .false.

@@ -135,6 +135,32 @@ def wrap(q, box):
                     q[:, j, c] -= L
 
 
+def transrot(q, p, masses):
+    """src/transrot.f90:36-237 over a whole box, numpy restatement of the C
+    oracle's oracle_transrot (which holds at most 16 atoms; pinned against it
+    in tests/test_box_oracle.py): centre-of-mass velocity and centre with the
+    reference's total mass nbeads^2 * M, angular momentum about that centre,
+    inertia tensor, and the removal of both from every bead's velocity."""
+    nb = q.shape[0]
+    m = np.asarray(masses, dtype=np.float64)[None, :, None]
+    vel = p / m
+    totmass = float(np.sum(np.repeat(np.asarray(masses, dtype=np.float64), nb))) * nb
+    vtot = np.sum(vel * m, axis=(0, 1)) / totmass
+    ctr = np.sum(q * m, axis=(0, 1)) / totmass
+    mang = np.sum(np.cross(q, vel) * m, axis=(0, 1)) - np.cross(ctr, vtot) * totmass
+    d = q - ctr
+    w = np.broadcast_to(m, q.shape)[..., 0]
+    xx, yy, zz = (np.sum(d[..., a] * d[..., a] * w) for a in range(3))
+    xy, xz, yz = np.sum(d[..., 0] * d[..., 1] * w), np.sum(d[..., 0] * d[..., 2] * w), np.sum(d[..., 1] * d[..., 2] * w)
+    ten = np.array([[yy + zz, -xy, -xz], [-xy, xx + zz, -yz], [-xz, -yz, xx + yy]])
+    if q.shape[1] <= 2:
+        ten = ten + np.eye(3) * 0.000001
+    vang = np.linalg.solve(ten, mang)
+    v = vel - vtot
+    v = v - np.cross(vang, d)
+    return v * m
+
+
 class QmdffFF:
     """The oracle's QMDFF in the interface BoxRPMD expects."""
 
@@ -151,8 +177,9 @@ class BoxRPMD:
     and atomic units.  nose_q > 0 adds the Nose-Hoover chain before the first
     and after the second half kick (src/verlet.f90:185-206, :1086-1106)."""
 
-    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False, nose_q=0.0, wrap_box=None):
-        self.orthogonal = orthogonal
+    def __init__(self, ffo, theta, q, p, dt, masses, beta, orthogonal=False, nose_q=0.0, wrap_box=None,
+                 remove_com=False):
+        self.orthogonal, self.remove_com = orthogonal, remove_com
         self.wrap_box = None if wrap_box is None else np.asarray(wrap_box, dtype=np.float64)
         self.nose_q, self.vnh = nose_q, [0.0, 0.0, 0.0, 0.0]
         self.ffo, self.theta = ffo, np.asarray(theta, dtype=np.float64)
@@ -180,6 +207,8 @@ class BoxRPMD:
         self.p = self.p - hdt * self.g
         if self.nose_q > 0:
             self.p = nhc(self.p, self.m, self.vnh, self.nose_q, self.beta, self.dt)
+        if self.remove_com:
+            self.p = transrot(self.q, self.p, self.m)
 
     def energies(self):
         """Per-bead potential, kinetic energy and ring-polymer spring energy

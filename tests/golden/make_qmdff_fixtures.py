@@ -18,7 +18,11 @@ structures (the file's, and two seeded perturbations of 0.1 and 0.3 bohr)
 and the reference's running energies after ff_eg, ff_nonb, ff_hb and its
 gradient.  Also qmdff_box_positions.npz: the coordinates and molecule numbers
 of the whole ethanol box (1125 atoms) with its box and cutoffs, from which
-the tests build the 125-molecule box from the ethanol molecule's parameters.
+the tests build the 125-molecule box from the ethanol molecule's parameters;
+and qmdff_ethanol16_nhc_reference.npz: 20 steps of the reference's dynamic.x
+verlet on the 16-molecule box with 4 beads, the Nose-Hoover chain at 200 K
+(damping 20 dt), dt 0.5 fs, the periodic wrap and transrot, from seeded
+positions and momenta (frames at steps 1, 10 and 20).
 Run where /root/reference is mounted:
     python tests/golden/make_qmdff_fixtures.py
 """
@@ -33,7 +37,11 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import refqmdff  # noqa: E402
+import refstep  # noqa: E402
 from caracal_amd import qmdff  # noqa: E402
+from caracal_amd.units import FS_TO_AU, beta_of, mass_au  # noqa: E402
+
+AMU = {1: 1.008, 6: 12.011, 8: 15.999}
 
 EX = "/root/reference/examples"
 NZ = 18
@@ -94,6 +102,21 @@ def main():
         sub = os.path.join(tmp, "ethanol16.qmdff")
         xyz = subset_file(box, 16, sub)
         save("ethanol16", sub, xyz, periodic=True, box=27.0)
+        nb, steps = 4, 20
+        at = qmdff.read_qmdff(sub)["at"]
+        m = mass_au([AMU[int(z)] for z in at])
+        beta = float(beta_of(200.0))
+        dt = 0.5 * FS_TO_AU
+        rng = np.random.default_rng(9)
+        q0 = xyz[None] + rng.normal(0, 0.02, (nb,) + xyz.shape)
+        p0 = rng.normal(size=q0.shape) * np.sqrt(m[None, :, None] * nb / beta)
+        fr, _ = refstep.run("qmdff", q0, p0, m, dt, beta, steps, constrain=-1, every=1, thermostat=2,
+                            kelvin=1.0 / (0.316679e-5 * beta), nose_q=20 * dt,
+                            qmdff=dict(file=sub, periodic=True, box=27.0))
+        keep = np.array([1, 10, 20])
+        np.savez_compressed(os.path.join(HERE, "qmdff_ethanol16_nhc_reference.npz"), q0=q0, p0=p0, masses=m, dt=dt,
+                            beta=beta, nose_q=20 * dt, steps=keep, q=fr["q"][keep - 1], p=fr["p"][keep - 1],
+                            epot=fr["epot"])
     t = qmdff.read_qmdff(box)
     _, _, _, _, p = refqmdff.run(box, t["xyz"][None], dump=True, periodic=True, box=27.0)
     np.savez_compressed(os.path.join(HERE, "qmdff_box_positions.npz"), xyz=t["xyz"], molnum=t["molnum"],

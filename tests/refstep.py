@@ -11,7 +11,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
-SURFACE_CODE = {"h3": 1, "ch4h": 2, "oh3": 3}
+SURFACE_CODE = {"h3": 1, "ch4h": 2, "oh3": 3, "qmdff": 4}
 
 
 def binary(opt="O1"):
@@ -23,7 +23,7 @@ def available(opt="O1"):
 
 
 def _write_input(path, surface, q, p, masses, dt, beta, nsteps, constrain, every, reaction, thermostat=0,
-                 andersen_step=0, kelvin=None, nose_q=0.0, k_force=0.0):
+                 andersen_step=0, kelvin=None, nose_q=0.0, k_force=0.0, qmdff=None):
     nbeads, natoms, _ = q.shape
     with open(path, "wb") as f:
         np.array([natoms, nbeads, nsteps, constrain, SURFACE_CODE[surface], every, thermostat, andersen_step],
@@ -34,6 +34,13 @@ def _write_input(path, surface, q, p, masses, dt, beta, nsteps, constrain, every
         np.ascontiguousarray(p, dtype=np.float64).tofile(f)
         if thermostat == 2:
             np.array([kelvin, nose_q], dtype=np.float64).tofile(f)
+        if surface == "qmdff":
+            name = os.path.abspath(qmdff["file"]).encode()
+            np.array([len(name)], dtype=np.int32).tofile(f)
+            f.write(name)
+            np.array([int(qmdff.get("periodic", False)), int(qmdff.get("zahn", False))], dtype=np.int32).tofile(f)
+            np.array([qmdff.get("box", 0.0), qmdff.get("coul_cut", 50.0), qmdff.get("vdw_cut", 10.0)],
+                     dtype=np.float64).tofile(f)
         if constrain >= 0:
             r = reaction
             np.array([len(r.bond_form), len(r.bond_break)], dtype=np.int32).tofile(f)
@@ -63,7 +70,8 @@ def run(surface, q, p, masses, dt, beta, nsteps, constrain=-1, every=1, reaction
     andersen_step steps, the compiler's random stream) or 2 (Nose-Hoover
     chain at kelvin=..., nose_q=...).  Returns (frames, seconds): frames is a
     dict of q, p (nframes, nbeads, natoms, 3), epot and xi (nframes) after
-    every 'every' steps."""
+    every 'every' steps.  surface "qmdff" takes qmdff=dict(file=..., periodic=...,
+    box=..., coul_cut=..., vdw_cut=..., zahn=...) (angstrom), as the key file."""
     q = np.asarray(q, dtype=np.float64)
     nbeads, natoms, _ = q.shape
     with tempfile.TemporaryDirectory() as tmp:

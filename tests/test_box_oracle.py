@@ -66,3 +66,20 @@ def test_numpy_nhc_matches_c_oracle():
     np.testing.assert_allclose(p_n, p_c, rtol=1e-12, atol=0)
     np.testing.assert_allclose(v_n, v_c, rtol=1e-12, atol=1e-30)
     assert abs(v_c[0]) > 0
+
+
+def test_numpy_transrot_equals_the_c_oracle():
+    """box_oracle.transrot (any size) against oracle_transrot (up to 16 atoms)."""
+    import ctypes
+    import oracle_lib as ol
+    from box_oracle import transrot
+    rng = np.random.default_rng(8)
+    for nb, na in ((1, 5), (4, 7), (8, 16)):
+        m = rng.uniform(1000, 30000, na)
+        q = rng.normal(0, 3, (nb, na, 3)) + 10.0
+        p = rng.normal(0, 5, (nb, na, 3))
+        s = ol.make_sys(1, nb, 1.0, m)
+        pc = np.ascontiguousarray(p.copy())
+        ol.rpmd().oracle_transrot(ctypes.byref(s), np.ascontiguousarray(q).ctypes.data_as(ol._dp),
+                                  pc.ctypes.data_as(ol._dp))
+        assert np.abs(transrot(q, p, m) - pc).max() <= 1e-11 * np.abs(pc).max()

@@ -99,3 +99,33 @@ def test_ethanol_box_ring_polymer_nhc_matches_oracle():
     perr = np.abs(pq - ref.p).max() / np.abs(ref.p).max()
     print(f"ethanol box, {nb} beads, {steps} NHC steps: rel dq {err:.1e}, dp {perr:.1e}")
     assert err <= 1e-11 and perr <= 1e-9
+
+
+def test_ethanol16_box_dynamics_match_reference():
+    """The box engine with the Nose-Hoover chain, the periodic wrap and transrot
+    against 20 steps of the reference's own dynamic.x verlet on 16 molecules
+    of the ethanol box (4 beads, real parameters, fixture from the compiled
+    reference)."""
+    import os
+    f = dict(np.load(os.path.join(qf.GOLDEN, "qmdff_ethanol16_nhc_reference.npz")))
+    p, _, _ = qf.load("ethanol16")
+    nb = f["q0"].shape[0]
+    ff = cff.ForceField.qmdff(p)
+    eng = cff.BoxEngine(ff, nb, float(f["dt"]), f["masses"], float(f["beta"]), None, precision=cam.CAM_PREC_FP64,
+                        nm_mode=cam.CAM_NM_REFERENCE)
+    eng.set_nhc(float(f["nose_q"]))
+    eng.set_transrot(True)
+    eng.set_state(f["q0"], f["p0"])
+    done = 0
+    for k, step in enumerate(f["steps"]):
+        eng.run(int(step) - done)
+        done = int(step)
+        q, pq = eng.get_state()
+        eq = np.abs(q - f["q"][k]).max() / np.abs(f["q"][k]).max()
+        ep = np.abs(pq - f["p"][k]).max() / np.abs(f["p"][k]).max()
+        print(f"step {step}: rel dq {eq:.1e}, dp {ep:.1e}", flush=True)
+        assert eq <= 1e-10 and ep <= 1e-9
+    ep_beads, _, _ = eng.energies()
+    assert abs(ep_beads.sum() - f["epot"][-1]) <= 1e-10 * max(1.0, abs(f["epot"][-1]))
+    eng.close()
+    ff.close()

@@ -98,3 +98,25 @@ def test_box_file_repeats_the_ethanol_molecule():
     for k in ("at", "q", "molnum", "bond", "vbond", "angl", "vangl", "tors", "vtors", "nci"):
         assert np.array_equal(np.asarray(t[k]), np.asarray(p[k])), k
     assert np.array_equal(t["xyz"], x)
+
+
+def test_box_oracle_dynamics_reproduce_the_reference():
+    """The box oracle (Nose-Hoover chain, free ring polymer, the periodic wrap,
+    QMDFF forces, transrot) against 20 steps of the reference's own dynamic.x
+    verlet on the 16-molecule periodic ethanol box with 4 beads."""
+    import os
+    from box_oracle import BoxRPMD, QmdffFF
+    f = dict(np.load(os.path.join(qf.GOLDEN, "qmdff_ethanol16_nhc_reference.npz")))
+    p, _, _ = qf.load("ethanol16")
+    ref = BoxRPMD(QmdffFF(ol.QmdffOracle(p)), np.zeros(10), f["q0"], f["p0"], float(f["dt"]), f["masses"],
+                  float(f["beta"]), nose_q=float(f["nose_q"]), wrap_box=p["box"], remove_com=True)
+    k = 0
+    for step in range(1, int(f["steps"][-1]) + 1):
+        ref.step()
+        assert abs(ref.v.sum() - f["epot"][step - 1]) <= 1e-12 * max(1.0, abs(f["epot"][step - 1]))
+        if step in f["steps"]:
+            eq = np.abs(ref.q - f["q"][k]).max() / np.abs(f["q"][k]).max()
+            ep = np.abs(ref.p - f["p"][k]).max() / np.abs(f["p"][k]).max()
+            print(f"step {step}: rel dq {eq:.1e}, dp {ep:.1e}")
+            assert eq < 1e-12 and ep < 1e-11
+            k += 1
