@@ -55,7 +55,8 @@ struct QDev {
     const double *rad;                  // ne (bohr)
     const int *bond, *angl, *tors, *nci, *hb;      // 0-based atoms
     const double *vbond, *vangl, *vtors, *vhb;
-    const int *rptr, *rcode;            // per atom: the terms it takes part in
+    const int *rptr, *rcode;            // per atom: the term slots (and hydrogen-bond parts) it takes part in
+    int sbase[5], nslot;                // first slot of bonds, bends, torsions, nci pairs, listed H-bonds
     const int *don, *acc;               // donor bonds (A, H) and acceptor atoms
     const double *don_lim, *acc_lim;    // hydrogen-bond distance thresholds
     const double *hbc;                  // n: hbpara(10, 5, q) * scalehb
@@ -69,6 +70,7 @@ struct QmdffDev {
     QDev d{};
     std::vector<void *> allocs;
     double *gpart = nullptr, *epart = nullptr, *gacc = nullptr, *eacc = nullptr, *gdon = nullptr;
+    double *gslot = nullptr, *eslot = nullptr;
     int cap_beads = 0;
 };
 
@@ -657,91 +659,124 @@ __global__ __launch_bounds__(kQTile) void qm_inter_kernel(QDev f, const double *
     }
 }
 
-// acceptor side: gradient of acceptor atom ac and the energy, over all donors
+// Every (term, atom) slot of the bonded terms, nci pairs and listed hydrogen
+// bonds: the term's gradient on that atom, and its energy at the slot of its
+// first atom (block sums in a fixed order).  Slots are contiguous per kind.
+__global__ __launch_bounds__(256) void qm_slot_kernel(QDev f, const double *__restrict__ xall, double *gslot,
+                                                      double *eslot)
+{
+    __shared__ double sred[4];
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const double *x = xall + (size_t)b * f.n * 3;
+    double e = 0.0;
+    if (sl < f.nslot) {
+        double et = 0.0, gv[3] = { 0.0, 0.0, 0.0 };
+        int kind = 4;
+        while (kind > 0 && sl < f.sbase[kind]) --kind;
+        const int roles = kind == 0 ? 2 : kind == 1 ? 3 : kind == 2 ? 4 : kind == 3 ? 2 : 3;
+        const int idx = (sl - f.sbase[kind]) / roles, role = (sl - f.sbase[kind]) % roles;
+        switch (kind) {
+        case kTBond: qbond(f, x, idx, role, et, gv); break;
+        case kTAngle: qangle(f, x, idx, role, et, gv); break;
+        case kTTors: qtors(f, x, idx, role, et, gv); break;
+        case kTNci: qnci(f, x, idx, role, et, gv); break;
+        default: qhblist(f, x, idx, role, et, gv); break;
+        }
+        double *o = gslot + ((size_t)b * f.nslot + sl) * 3;
+        o[0] = gv[0];
+        o[1] = gv[1];
+        o[2] = gv[2];
+        if (role == 0) e = et;
+    }
+    for (int o = 32; o > 0; o >>= 1) e = e + __shfl_xor(e, o);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) eslot[(size_t)b * gridDim.x + blockIdx.x] = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+}
+
+// acceptor side: one wave per acceptor atom, lanes over the donor bonds; the
+// acceptor's gradient and the energy (butterfly sums: a fixed order)
 __global__ __launch_bounds__(256) void qm_hbacc_kernel(QDev f, const double *__restrict__ xall, double *gacc,
                                                        double *eacc)
 {
-    const int ac = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int lane = threadIdx.x & 63, ac = blockIdx.x * 4 + (threadIdx.x >> 6), b = blockIdx.y;
     if (ac >= f.nacc) return;
     const double *x = xall + (size_t)b * f.n * 3;
     double g[3] = { 0.0, 0.0, 0.0 }, e = 0.0, gj[3], ga[3], gh[3];
-    for (int dn = 0; dn < f.ndon; ++dn) {
+    for (int dn = lane; dn < f.ndon; dn += 64) {
         e = e + qhbinter(f, x, dn, ac, gj, ga, gh);
         g[0] = g[0] + gj[0];
         g[1] = g[1] + gj[1];
         g[2] = g[2] + gj[2];
     }
-    double *o = gacc + ((size_t)b * f.nacc + ac) * 3;
-    o[0] = g[0];
-    o[1] = g[1];
-    o[2] = g[2];
-    eacc[(size_t)b * f.nacc + ac] = e;
+    for (int o = 32; o > 0; o >>= 1) {
+        e = e + __shfl_xor(e, o);
+        for (int c = 0; c < 3; ++c) g[c] = g[c] + __shfl_xor(g[c], o);
+    }
+    if (lane == 0) {
+        double *out = gacc + ((size_t)b * f.nacc + ac) * 3;
+        out[0] = g[0];
+        out[1] = g[1];
+        out[2] = g[2];
+        eacc[(size_t)b * f.nacc + ac] = e;
+    }
 }
 
-// donor side: gradients of A and H of donor bond dn, over all acceptors
+// donor side: one wave per donor bond, lanes over the acceptors; the A and H gradients
 __global__ __launch_bounds__(256) void qm_hbdon_kernel(QDev f, const double *__restrict__ xall, double *gdon)
 {
-    const int dn = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int lane = threadIdx.x & 63, dn = blockIdx.x * 4 + (threadIdx.x >> 6), b = blockIdx.y;
     if (dn >= f.ndon) return;
     const double *x = xall + (size_t)b * f.n * 3;
     double sa[3] = { 0.0, 0.0, 0.0 }, sh[3] = { 0.0, 0.0, 0.0 }, gj[3], ga[3], gh[3];
-    for (int ac = 0; ac < f.nacc; ++ac) {
+    for (int ac = lane; ac < f.nacc; ac += 64) {
         (void)qhbinter(f, x, dn, ac, gj, ga, gh);
         for (int c = 0; c < 3; ++c) {
             sa[c] = sa[c] + ga[c];
             sh[c] = sh[c] + gh[c];
         }
     }
-    double *o = gdon + ((size_t)b * f.ndon + dn) * 6;
-    for (int c = 0; c < 3; ++c) {
-        o[c] = sa[c];
-        o[3 + c] = sh[c];
+    for (int o = 32; o > 0; o >>= 1)
+        for (int c = 0; c < 3; ++c) {
+            sa[c] = sa[c] + __shfl_xor(sa[c], o);
+            sh[c] = sh[c] + __shfl_xor(sh[c], o);
+        }
+    if (lane == 0) {
+        double *out = gdon + ((size_t)b * f.ndon + dn) * 6;
+        for (int c = 0; c < 3; ++c) {
+            out[c] = sa[c];
+            out[3 + c] = sh[c];
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void qm_gather_kernel(QDev f, const double *__restrict__ xall, int nsplit,
-                                                        int nbeads, const double *gpart, const double *gacc,
-                                                        const double *eacc, const double *gdon, double *gout,
-                                                        double *egather)
+// one lane per (bead, atom): its term slots, hydrogen-bond parts and
+// intermolecular partials in a fixed order; the acceptors' energies
+__global__ __launch_bounds__(256) void qm_gather_kernel(QDev f, int nsplit, int nbeads, const double *gslot,
+                                                        const double *gpart, const double *gacc, const double *eacc,
+                                                        const double *gdon, double *gout, double *egather)
 {
     __shared__ double sred[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     const bool act = i < f.n;
-    const double *x = xall + (size_t)b * f.n * 3;
     double g[3] = { 0.0, 0.0, 0.0 }, e = 0.0;
     if (act) {
         for (int k = f.rptr[i]; k < f.rptr[i + 1]; ++k) {
             const int code = f.rcode[k];
-            const int kind = code >> 28, role = (code >> 26) & 3, idx = code & 0x03FFFFFF;
-            double et = 0.0, gv[3] = { 0.0, 0.0, 0.0 };
-            bool count = role == 0;
-            switch (kind) {
-            case kTBond: qbond(f, x, idx, role, et, gv); break;
-            case kTAngle: qangle(f, x, idx, role, et, gv); break;
-            case kTTors: qtors(f, x, idx, role, et, gv); break;
-            case kTNci: qnci(f, x, idx, role, et, gv); break;
-            case kTHbList: qhblist(f, x, idx, role, et, gv); break;
-            case kTHbAcc: {
-                const double *o = gacc + ((size_t)b * f.nacc + idx) * 3;
-                gv[0] = o[0];
-                gv[1] = o[1];
-                gv[2] = o[2];
-                et = eacc[(size_t)b * f.nacc + idx];
-                count = true;
-                break;
+            const int kind = code >> 28, payload = code & 0x0FFFFFFF;
+            const double *src;
+            if (kind == kTHbAcc) {
+                src = gacc + ((size_t)b * f.nacc + payload) * 3;
+                e = e + eacc[(size_t)b * f.nacc + payload];
+            } else if (kind == kTHbDon) {
+                const int role = payload >> 26, dn = payload & 0x03FFFFFF;
+                src = gdon + ((size_t)b * f.ndon + dn) * 6 + (role == 1 ? 0 : 3);
+            } else {
+                src = gslot + ((size_t)b * f.nslot + payload) * 3;
             }
-            default: {                  // donor bond: role 1 = A, 2 = H
-                const double *o = gdon + ((size_t)b * f.ndon + idx) * 6 + (role == 1 ? 0 : 3);
-                gv[0] = o[0];
-                gv[1] = o[1];
-                gv[2] = o[2];
-                count = false;
-            }
-            }
-            g[0] = g[0] + gv[0];
-            g[1] = g[1] + gv[1];
-            g[2] = g[2] + gv[2];
-            if (count) e = e + et;
+            g[0] = g[0] + src[0];
+            g[1] = g[1] + src[1];
+            g[2] = g[2] + src[2];
         }
         if (f.inter)
             for (int s = 0; s < nsplit; ++s) {
@@ -764,34 +799,78 @@ __global__ __launch_bounds__(256) void qm_gather_kernel(QDev f, const double *__
     if (threadIdx.x == 0) egather[(size_t)b * gridDim.x + blockIdx.x] = ((sred[0] + sred[1]) + sred[2]) + sred[3];
 }
 
-__global__ void qm_reduce_kernel(const double *epart, int ninter, const double *egather, int ngather, double *e)
+__global__ void qm_reduce_kernel(const double *eslot, int nsl, const double *epart, int ninter, const double *egather,
+                                 int ngather, double *e)
 {
     const int b = blockIdx.x;
     if (threadIdx.x != 0) return;
     double s = 0.0;
+    for (int k = 0; k < nsl; ++k) s = s + eslot[(size_t)b * nsl + k];
     for (int k = 0; k < ngather; ++k) s = s + egather[(size_t)b * ngather + k];
     for (int k = 0; k < ninter; ++k) s = s + epart[(size_t)b * ninter + k];
     e[b] = s;
 }
 
 // src/verlet.f90:540-640 (orthorhombic box): an atom whose coordinate left
-// [0, L] in some bead is moved with all its beads, bead by bead in order
+// [0, L] in some bead is moved with all its beads, bead by bead in order.
+// One lane per (atom, coordinate): the beads are read in chunks of
+// independent loads and the shifts recorded in order; the rare lane that
+// shifts applies them to every bead in that order, as the reference adds them.
+constexpr int kWrapChunk = 16, kWrapMaxShift = 8;
+
 __global__ void qm_wrap_kernel(QDev f, double *x, int nbeads)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 3 * f.n) return;
     const int i = t / 3, c = t % 3;
     const double L = f.box[c];
+    double sh[kWrapMaxShift];
+    int ns = 0;
+    bool over = false;
+    for (int k0 = 0; k0 < nbeads; k0 += kWrapChunk) {
+        double v[kWrapChunk];
+#pragma unroll
+        for (int u = 0; u < kWrapChunk; ++u)
+            v[u] = k0 + u < nbeads ? x[((size_t)(k0 + u) * f.n + i) * 3 + c] : 0.5 * L;
+        for (int u = 0; u < kWrapChunk && k0 + u < nbeads; ++u) {
+            double w = v[u];
+            for (int s = 0; s < ns; ++s) w = w + sh[s];
+            while (w < 0.0 && !over) {
+                if (ns == kWrapMaxShift) over = true;
+                else {
+                    sh[ns++] = L;
+                    w = w + L;
+                }
+            }
+            while (w > L && !over) {
+                if (ns == kWrapMaxShift) over = true;
+                else {
+                    sh[ns++] = -L;
+                    w = w - L;
+                }
+            }
+        }
+    }
+    if (over) {                                 // many shifts: the reference's loop on memory
+        for (int k = 0; k < nbeads; ++k) {
+            int tries = 0;
+            while (x[((size_t)k * f.n + i) * 3 + c] < 0.0 && tries <= 100) {
+                for (int kk = 0; kk < nbeads; ++kk) x[((size_t)kk * f.n + i) * 3 + c] += L;
+                ++tries;
+            }
+            while (x[((size_t)k * f.n + i) * 3 + c] > L && tries <= 100) {
+                for (int kk = 0; kk < nbeads; ++kk) x[((size_t)kk * f.n + i) * 3 + c] -= L;
+                ++tries;
+            }
+        }
+        return;
+    }
+    if (ns == 0) return;
     for (int k = 0; k < nbeads; ++k) {
-        int tries = 0;
-        while (x[((size_t)k * f.n + i) * 3 + c] < 0.0 && tries <= 100) {
-            for (int kk = 0; kk < nbeads; ++kk) x[((size_t)kk * f.n + i) * 3 + c] += L;
-            ++tries;
-        }
-        while (x[((size_t)k * f.n + i) * 3 + c] > L && tries <= 100) {
-            for (int kk = 0; kk < nbeads; ++kk) x[((size_t)kk * f.n + i) * 3 + c] -= L;
-            ++tries;
-        }
+        double *p = x + ((size_t)k * f.n + i) * 3 + c;
+        double w = *p;
+        for (int s = 0; s < ns; ++s) w = w + sh[s];
+        *p = w;
     }
 }
 
@@ -814,7 +893,7 @@ void qmdff_free(QmdffDev *q)
 {
     if (!q) return;
     for (void *p : q->allocs) (void)hipFree(p);
-    double *bufs[] = { q->gpart, q->epart, q->gacc, q->eacc, q->gdon };
+    double *bufs[] = { q->gpart, q->epart, q->gacc, q->eacc, q->gdon, q->gslot, q->eslot };
     for (double *p : bufs)
         if (p) (void)hipFree(p);
     delete q;
@@ -934,19 +1013,31 @@ int qmdff_build(const cam_qmdff_desc *in, QmdffDev **out)
     }
     d.ndon = (int)don_lim.size();
     d.nacc = (int)acc.size();
-    // per-atom term lists
+    // Term slots: one per (term, atom of the term), contiguous per kind;
+    // each atom lists its slots (kind << 28 | slot) and its hydrogen-bond
+    // parts (kind << 28 | role << 26 | index) in the reference's term order.
+    const int64_t nslot64 = 2 * (int64_t)in->nbond + 3 * (int64_t)in->nangl + 4 * (int64_t)in->ntors +
+                            2 * (int64_t)in->nnci + 3 * (int64_t)in->nhb;
+    if (nslot64 >= (int64_t)1 << 28) return CAM_ERR_ARG;
+    d.sbase[0] = 0;
+    d.sbase[1] = 2 * in->nbond;
+    d.sbase[2] = d.sbase[1] + 3 * in->nangl;
+    d.sbase[3] = d.sbase[2] + 4 * in->ntors;
+    d.sbase[4] = d.sbase[3] + 2 * in->nnci;
+    d.nslot = (int)nslot64;
     std::vector<std::vector<int>> roles(n);
     auto code = [](int kind, int role, int idx) { return kind << 28 | role << 26 | idx; };
+    auto slot = [&](int kind, int nroles, int m, int r) { return kind << 28 | (d.sbase[kind] + nroles * m + r); };
     for (int m = 0; m < in->nbond; ++m)
-        for (int r = 0; r < 2; ++r) roles[bond[2 * m + r]].push_back(code(kTBond, r, m));
+        for (int r = 0; r < 2; ++r) roles[bond[2 * m + r]].push_back(slot(kTBond, 2, m, r));
     for (int m = 0; m < in->nangl; ++m)
-        for (int r = 0; r < 3; ++r) roles[angl[3 * m + r]].push_back(code(kTAngle, r, m));
+        for (int r = 0; r < 3; ++r) roles[angl[3 * m + r]].push_back(slot(kTAngle, 3, m, r));
     for (int m = 0; m < in->ntors; ++m)
-        for (int r = 0; r < 4; ++r) roles[tors[6 * m + r]].push_back(code(kTTors, r, m));
+        for (int r = 0; r < 4; ++r) roles[tors[6 * m + r]].push_back(slot(kTTors, 4, m, r));
     for (int m = 0; m < in->nnci; ++m)
-        for (int r = 0; r < 2; ++r) roles[nci[3 * m + r]].push_back(code(kTNci, r, m));
+        for (int r = 0; r < 2; ++r) roles[nci[3 * m + r]].push_back(slot(kTNci, 2, m, r));
     for (int m = 0; m < in->nhb; ++m)
-        for (int r = 0; r < 3; ++r) roles[hb[3 * m + r]].push_back(code(kTHbList, r, m));
+        for (int r = 0; r < 3; ++r) roles[hb[3 * m + r]].push_back(slot(kTHbList, 3, m, r));
     for (int a = 0; a < d.nacc; ++a) roles[acc[a]].push_back(code(kTHbAcc, 0, a));
     for (int dn = 0; dn < d.ndon; ++dn) {
         roles[don[2 * dn]].push_back(code(kTHbDon, 1, dn));
@@ -983,7 +1074,7 @@ namespace {
 int qmdff_ensure(QmdffDev *q, int nbeads)
 {
     if (nbeads <= q->cap_beads) return CAM_OK;
-    double **bufs[] = { &q->gpart, &q->epart, &q->gacc, &q->eacc, &q->gdon };
+    double **bufs[] = { &q->gpart, &q->epart, &q->gacc, &q->eacc, &q->gdon, &q->gslot, &q->eslot };
     for (double **p : bufs)
         if (*p) {
             CAM_HIP(hipFree(*p));
@@ -997,6 +1088,8 @@ int qmdff_ensure(QmdffDev *q, int nbeads)
     CAM_HIP(hipMalloc(&q->gacc, ((size_t)std::max(1, d.nacc) * nbeads * 4) * sizeof(double)));
     CAM_HIP(hipMalloc(&q->eacc, ((size_t)std::max(1, d.nacc) * nbeads) * sizeof(double)));
     CAM_HIP(hipMalloc(&q->gdon, ((size_t)std::max(1, d.ndon) * nbeads * 6) * sizeof(double)));
+    CAM_HIP(hipMalloc(&q->gslot, ((size_t)std::max(1, d.nslot) * nbeads * 3) * sizeof(double)));
+    CAM_HIP(hipMalloc(&q->eslot, ((size_t)std::max(1, (d.nslot + 255) / 256) * nbeads) * sizeof(double)));
     q->cap_beads = nbeads;
     return CAM_OK;
 }
@@ -1017,17 +1110,21 @@ int qmdff_launch(QmdffDev *q, const double *x, int nbeads, double *e, double *g,
         CAM_LAUNCH_CHECK();
     }
     if (d.nacc > 0 && d.ndon > 0) {
-        hipLaunchKernelGGL(qm_hbacc_kernel, dim3((d.nacc + 255) / 256, nbeads), dim3(256), 0, s, d, x, q->gacc,
-                           q->eacc);
+        hipLaunchKernelGGL(qm_hbacc_kernel, dim3((d.nacc + 3) / 4, nbeads), dim3(256), 0, s, d, x, q->gacc, q->eacc);
         CAM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(qm_hbdon_kernel, dim3((d.ndon + 255) / 256, nbeads), dim3(256), 0, s, d, x, q->gdon);
+        hipLaunchKernelGGL(qm_hbdon_kernel, dim3((d.ndon + 3) / 4, nbeads), dim3(256), 0, s, d, x, q->gdon);
         CAM_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(qm_gather_kernel, dim3(ngx, nbeads), dim3(256), 0, s, d, x, kQSplit, nbeads, q->gpart, q->gacc,
-                       q->eacc, q->gdon, g, egather);
+    const int nsx = (d.nslot + 255) / 256;
+    if (nsx > 0) {
+        hipLaunchKernelGGL(qm_slot_kernel, dim3(nsx, nbeads), dim3(256), 0, s, d, x, q->gslot, q->eslot);
+        CAM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(qm_gather_kernel, dim3(ngx, nbeads), dim3(256), 0, s, d, kQSplit, nbeads, q->gslot, q->gpart,
+                       q->gacc, q->eacc, q->gdon, g, egather);
     CAM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(qm_reduce_kernel, dim3(nbeads), dim3(64), 0, s, q->epart, d.inter ? nbx * kQSplit : 0, egather,
-                       ngx, e);
+    hipLaunchKernelGGL(qm_reduce_kernel, dim3(nbeads), dim3(64), 0, s, q->eslot, nsx, q->epart,
+                       d.inter ? nbx * kQSplit : 0, egather, ngx, e);
     CAM_LAUNCH_CHECK();
     return CAM_OK;
 }
