@@ -34,9 +34,15 @@ constexpr int kNP = 10;       // parameters
 constexpr int kNE = 1 + 2 * kNP;   // energy + dE/dtheta + d(v.g)/dtheta
 constexpr int kTile = 64;
 
+constexpr int kErfcTerms = 40;
+
 struct FFDev {
     int natoms, nbonds, nangles;
     double box, rc, za, zp;
+    // erfcx on [0, exmax] as a polynomial in t = x ecx - 1 (monomial
+    // coefficients of its Chebyshev series): FP64 and binary32 copies
+    double ecf[kErfcTerms], ecx, exmax;
+    float ecf32[kErfcTerms];
     double sr42[2][2], r0bj[2][2], zrep[2][2], arep[2][2];
     const int *type, *mol, *bonds, *angles, *csr_ptr, *csr_term;
 };
@@ -75,157 +81,502 @@ __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 
-// The j-range of split s of the nonbonded pair loop: every kernel that walks
-// pairs by split uses the same ranges.
-__host__ __device__ __forceinline__ int split_chunk(int n, int nsplit)
+// ---------------------------------------------------------------------------
+// Pair structure of the box.  The atoms are kept in a spatial order: cells of
+// the bead centroids, the atoms of a cell in index order.  It is rebuilt only
+// when some bead has moved more than half the skin since the last build, or
+// the bead count changed.  A row per sorted atom lists, in ascending sorted
+// index, every atom j of another molecule whose bead centroid lies within
+// rc + skin + margin + dev_i + dev_j of atom i's (dev: the largest distance of
+// a bead from its centroid) at the build, so it holds every pair closer than
+// rc + skin + margin in some bead; while no bead has moved more than skin/2,
+// no pair left out comes closer than rc + margin in any bead.  The pair
+// kernel runs a brick of 256 sorted atoms per block against LDS chunks of up
+// to 512 sorted atoms (only the 128-atom tiles that some row of the brick
+// reaches); in all-pairs mode every atom of every tile.  Both meet the pairs
+// inside the cutoff in the same order, so both give the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kBrick = 256;
+constexpr int kPTile = 128;
+constexpr int kChunkTiles = 4;              // tiles staged together
+constexpr int kMaxCellsDim = 20;            // cells per dimension (8000 cells: 32 KB of LDS in the sort)
+constexpr int kMaxSplit = 32;
+constexpr double kSkin = 1.2;               // bohr
+constexpr double kMargin = 1e-3;            // bohr: binary32 rounding of the mixed kernel's distances
+enum { kCtlGo = 0, kCtlBuiltNb = 1, kCtlNc = 2, kCtlWords = 4 };
+
+struct PairDev {
+    int n, nb, nsplit, ntiles, rowcap, maskw, nbricks, cg;
+    double box, rlist;
+    const int *type, *mol;                  // original order
+    int *perm;                              // sorted -> original index
+    int *scell, *smt, *tcell;               // per sorted atom: cell, mol << 1 | type; per original atom: cell
+    double *scen;                           // per sorted atom: centroid (3) and spread
+    int *cstart, *ccount;                   // per cell: first sorted atom and count
+    int *row;                               // [sorted atom][rowcap]: sorted j, ascending
+    int *sst;                               // [sorted atom][nsplit + 1]: row offsets of the splits; -1: overflow
+    unsigned *mask;                         // [brick][maskw]: tiles its rows reach
+    double *xref;                           // [bead][atom][3]: positions at the last build
+    double *cen;                            // per atom: wrapped bead centroid (3) and spread
+    double *bdev;                           // per check block: largest spread
+    int *bover;                             // per check block: some bead moved more than skin/2
+    int *ctl;                               // go, bead count of the last build, cells per dimension
+};
+
+__device__ __forceinline__ int cell_of(double c, double L, int ncd)
 {
-    return ((n + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
+    return min(max((int)(c / L * ncd), 0), ncd - 1);
 }
 
-// Per atom: the centroid of its beads (the mean of the minimum-image offsets
-// from bead 0) and the largest distance of one of its beads from it.
-__global__ void centroid_dev_kernel(int n, int nbeads, double box, const double *__restrict__ x,
-                                    double *__restrict__ cen, double *__restrict__ dev)
+// Every call: per atom whether some bead moved more than skin/2 since the
+// last build, the bead centroid (the mean of the minimum-image offsets from
+// bead 0, wrapped into the box) and the spread (largest bead distance from
+// it); per block the flag and the largest spread.  A group of G lanes per
+// atom (G a power of two dividing 64), lane sub taking beads sub, sub + G, ...
+constexpr int kCheckBeads = 4;                       // beads per lane kept in registers
+
+__global__ __launch_bounds__(256) void pl_check_kernel(PairDev p, const double *__restrict__ x)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x0[3], s[3] = { 0.0, 0.0, 0.0 };
-    for (int c = 0; c < 3; ++c) x0[c] = x[3 * i + c];
-    for (int b = 0; b < nbeads; ++b)
-        for (int c = 0; c < 3; ++c) s[c] += min_image(x[((size_t)b * n + i) * 3 + c] - x0[c], box);
+    __shared__ int s_ow[4];
+    __shared__ double s_dw[4];
+    const int n = p.n, nb = p.nb, tid = threadIdx.x;
+    const int G = p.cg, sub = tid & (G - 1);
+    const int i = blockIdx.x * (256 / G) + tid / G;
+    const double L = p.box;
+    const bool act = i < n;
+    const bool held = nb <= kCheckBeads * G;         // every bead of the lane in registers
+    double x0[3] = { 0.0, 0.0, 0.0 }, sm[3] = { 0.0, 0.0, 0.0 }, v[kCheckBeads][3] = {};
+    int over = 0;
+    auto first = [&](int b, double *xv) {
+        double q2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const size_t k = ((size_t)b * n + i) * 3 + c;
+            xv[c] = x[k];
+            const double e = min_image(xv[c] - p.xref[k], L);
+            q2 += e * e;
+            sm[c] += min_image(xv[c] - x0[c], L);
+        }
+        over |= !(q2 <= 0.25 * kSkin * kSkin);       // also NaN
+    };
+    if (act) {
+        for (int c = 0; c < 3; ++c) x0[c] = x[3 * i + c];
+        if (held) {
+#pragma unroll
+            for (int u = 0; u < kCheckBeads; ++u)
+                if (sub + u * G < nb) first(sub + u * G, v[u]);
+        } else {
+            double xv[3];
+            for (int b = sub; b < nb; b += G) first(b, xv);
+        }
+    }
+    for (int o = G >> 1; o > 0; o >>= 1)
+        for (int c = 0; c < 3; ++c) sm[c] += __shfl_xor(sm[c], o);
     double cc[3];
-    for (int c = 0; c < 3; ++c) cc[c] = x0[c] + s[c] / nbeads;
-    double m = 0.0;
-    for (int b = 0; b < nbeads; ++b) {
+    for (int c = 0; c < 3; ++c) cc[c] = x0[c] + sm[c] / nb;
+    double m2 = 0.0;
+    auto second = [&](const double *xv) {
         double r2 = 0.0;
         for (int c = 0; c < 3; ++c) {
-            const double d = min_image(x[((size_t)b * n + i) * 3 + c] - cc[c], box);
+            const double d = min_image(xv[c] - cc[c], L);
             r2 += d * d;
         }
-        m = fmax(m, r2);
-    }
-    for (int c = 0; c < 3; ++c) cen[3 * i + c] = cc[c];
-    dev[i] = sqrt(m);
-}
-
-// Neighbour list shared by all beads: atom i keeps every atom j of another
-// molecule whose centroid lies within rc + dev_i + dev_j + margin of its own.
-// For any bead, |d_ij| >= |c_i - c_j| - dev_i - dev_j, so no pair inside the
-// cutoff in any bead is left out; margin covers the binary32 rounding of the
-// mixed kernel's distances.  Row i lists j in increasing order; nofs[i][s] is
-// where split s's j-range starts in it, so the pair loop meets the same pairs
-// in the same order as the all-pairs loop.  One wave per atom.
-__global__ __launch_bounds__(256) void nlist_kernel(FFDev f, int nsplit, double margin, const double *__restrict__ cen,
-                                                    const double *__restrict__ dev, int *__restrict__ nbr,
-                                                    int *__restrict__ nofs)
-{
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int n = f.natoms;
-    if (i >= n) return;                    // the whole wave
-    const double c0 = cen[3 * i], c1 = cen[3 * i + 1], c2 = cen[3 * i + 2], di = dev[i];
-    const int mi = f.mol[i];
-    const int chunk = split_chunk(n, nsplit);
-    int *row = nbr + (size_t)i * n;
-    int *ofs = nofs + (size_t)i * (nsplit + 1);
-    int cnt = 0;
-    for (int sp = 0; sp < nsplit; ++sp) {
-        if (lane == 0) ofs[sp] = cnt;
-        const int j0 = sp * chunk, j1 = min(n, j0 + chunk);
-        for (int jb = j0; jb < j1; jb += 64) {
-            const int j = jb + lane;
-            bool keep = false;
-            if (j < j1 && f.mol[j] != mi) {
-                const double d0 = min_image(c0 - cen[3 * j], f.box), d1 = min_image(c1 - cen[3 * j + 1], f.box),
-                             d2 = min_image(c2 - cen[3 * j + 2], f.box);
-                const double lim = f.rc + di + dev[j] + margin;
-                keep = d0 * d0 + d1 * d1 + d2 * d2 < lim * lim;
+        m2 = fmax(m2, r2);
+    };
+    if (act) {
+        if (held) {
+#pragma unroll
+            for (int u = 0; u < kCheckBeads; ++u)
+                if (sub + u * G < nb) second(v[u]);
+        } else {
+            for (int b = sub; b < nb; b += G) {
+                double xv[3];
+                for (int c = 0; c < 3; ++c) xv[c] = x[((size_t)b * n + i) * 3 + c];
+                second(xv);
             }
-            const unsigned long long m = __ballot(keep);
-            if (keep) row[cnt + __popcll(m & ((1ull << lane) - 1ull))] = j;
-            cnt += __popcll(m);
         }
     }
-    if (lane == 0) ofs[nsplit] = cnt;
+    for (int o = G >> 1; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o));
+    double devi = act ? sqrt(m2) : 0.0;
+    if (act && sub == 0) {
+        for (int c = 0; c < 3; ++c) p.cen[4 * i + c] = cc[c] - L * floor(cc[c] / L);
+        p.cen[4 * i + 3] = devi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        over |= __shfl_xor(over, o);
+        devi = fmax(devi, __shfl_xor(devi, o));
+    }
+    if ((tid & 63) == 0) {
+        s_ow[tid >> 6] = over;
+        s_dw[tid >> 6] = devi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        p.bover[blockIdx.x] = s_ow[0] | s_ow[1] | s_ow[2] | s_ow[3];
+        p.bdev[blockIdx.x] = fmax(fmax(s_dw[0], s_dw[1]), fmax(s_dw[2], s_dw[3]));
+    }
 }
 
-// Nonbonded pairs of i against a j-range; GRAD: gradient of i, PAR: parameter
-// derivatives (and with v, the force-matching contraction).  LIST: the pairs
-// come from the neighbour list (nbr, nofs) instead of all atoms of the range;
-// the pairs left out are beyond the cutoff in every bead, so both forms add
-// the same terms in the same order.
-template <typename T, bool GRAD, bool PAR, bool LIST>
-__global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, const double *__restrict__ x,
-                                                          const double *__restrict__ v, int nsplit,
-                                                          double *__restrict__ gpart, double *__restrict__ epart,
-                                                          const int *__restrict__ nbr, const int *__restrict__ nofs)
+// One block: decides whether to rebuild (a flag of the check, or a new bead
+// count) and, if so, orders the atoms by cell of their centroids (the atoms
+// of a cell in index order).
+constexpr int kSortThreads = 1024;
+
+__global__ __launch_bounds__(kSortThreads) void pl_sort_kernel(PairDev p, int nchk)
 {
-    __shared__ T sx[kTile][3];
-    __shared__ T sv[kTile][3];
-    __shared__ int st[kTile], sm[kTile];
-    __shared__ double red[kTile];
+    __shared__ int s_cnt[kMaxCellsDim * kMaxCellsDim * kMaxCellsDim];
+    __shared__ int s_c[kSortThreads];
+    __shared__ int s_ow[kSortThreads / 64];
+    __shared__ double s_dw[kSortThreads / 64];
+    const int n = p.n, nb = p.nb, tid = threadIdx.x;
+    const double L = p.box;
+    int go = 0;
+    double dmax = 0.0;
+    for (int k = tid; k < nchk; k += kSortThreads) {
+        go |= p.bover[k];
+        dmax = fmax(dmax, p.bdev[k]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        go |= __shfl_xor(go, o);
+        dmax = fmax(dmax, __shfl_xor(dmax, o));
+    }
+    if ((tid & 63) == 0) {
+        s_ow[tid >> 6] = go;
+        s_dw[tid >> 6] = dmax;
+    }
+    __syncthreads();
+    go = p.ctl[kCtlBuiltNb] != nb;
+    dmax = 0.0;
+    for (int w = 0; w < kSortThreads / 64; ++w) {
+        go |= s_ow[w];
+        dmax = fmax(dmax, s_dw[w]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        p.ctl[kCtlGo] = go;
+        if (go) p.ctl[kCtlBuiltNb] = nb;
+    }
+    if (!go) return;
+    // cells at least rc + skin + margin + twice the largest spread wide: the
+    // atoms a row can hold lie in the 27 cells around the atom's own
+    int ncd = (int)floor(L / (p.rlist + 2.0 * dmax));
+    ncd = ncd < 3 ? 1 : min(ncd, kMaxCellsDim);
+    const int nc3 = ncd * ncd * ncd;
+    if (tid == 0) p.ctl[kCtlNc] = ncd;
+    for (int c = tid; c < nc3; c += kSortThreads) s_cnt[c] = 0;
+    for (int k = tid; k < p.nbricks * p.maskw; k += kSortThreads) p.mask[k] = 0u;
+    __syncthreads();
+    for (int a = tid; a < n; a += kSortThreads) {
+        const double *cc = p.cen + 4 * a;
+        const int c = (cell_of(cc[0], L, ncd) * ncd + cell_of(cc[1], L, ncd)) * ncd + cell_of(cc[2], L, ncd);
+        p.tcell[a] = c;
+        atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {                                   // exclusive scan: first sorted atom of each cell
+        int acc = 0;
+        for (int c = 0; c < nc3; ++c) {
+            const int k = s_cnt[c];
+            p.cstart[c] = acc;
+            p.ccount[c] = k;
+            s_cnt[c] = acc;
+            acc += k;
+        }
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += kSortThreads) {   // atoms in index order, a block at a time
+        const int a = base + tid;
+        const int c = a < n ? p.tcell[a] : -1;
+        s_c[tid] = c;
+        __syncthreads();
+        if (a < n) {
+            int r = 0;
+            for (int t = 0; t < tid; ++t) r += s_c[t] == c;
+            p.perm[s_cnt[c] + r] = a;
+        }
+        __syncthreads();
+        if (a < n) atomicAdd(&s_cnt[c], 1);
+        __syncthreads();
+    }
+    for (int s = tid; s < n; s += kSortThreads) {
+        const int a = p.perm[s];
+        p.smt[s] = p.mol[a] << 1 | p.type[a];
+        p.scell[s] = p.tcell[a];
+        for (int c = 0; c < 4; ++c) p.scen[4 * s + c] = p.cen[4 * a + c];
+    }
+}
+
+// On a rebuild, one wave per sorted atom s (pl_rows_kernel's waves take the
+// atoms in turn).  Copies its bead positions as the
+// new reference, walks the 27 cells around its own in ascending order (so the
+// row comes out in ascending sorted index), keeps every atom of another
+// molecule within the list radius, marks the tiles the row reaches for its
+// brick and records where each split's part of the row starts.
+__device__ void pl_row(const PairDev &p, const double *__restrict__ x, int s, int lane)
+{
+    const int n = p.n, nb = p.nb;
+    const int i = p.perm[s], ms = p.smt[s] >> 1;
+    for (int k = lane; k < 3 * nb; k += 64) {
+        const size_t o = ((size_t)(k / 3) * n + i) * 3 + k % 3;
+        p.xref[o] = x[o];
+    }
+    const double L = p.box, rl = p.rlist;
+    const double ci0 = p.scen[4 * s], ci1 = p.scen[4 * s + 1], ci2 = p.scen[4 * s + 2], di = p.scen[4 * s + 3];
+    const int ncd = p.ctl[kCtlNc];
+    int cl[3][3], ncl[3];
+    {
+        const int cs = p.scell[s];
+        const int own[3] = { cs / (ncd * ncd), (cs / ncd) % ncd, cs % ncd };
+        for (int d = 0; d < 3; ++d) {
+            if (ncd == 1) {
+                ncl[d] = 1;
+                cl[d][0] = 0;
+            } else {                                  // ncd >= 3: three distinct cells, ascending
+                int v0 = (own[d] + ncd - 1) % ncd, v1 = own[d], v2 = (own[d] + 1) % ncd, t;
+                if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
+                if (v1 > v2) { t = v1; v1 = v2; v2 = t; }
+                if (v0 > v1) { t = v0; v0 = v1; v1 = t; }
+                ncl[d] = 3;
+                cl[d][0] = v0;
+                cl[d][1] = v1;
+                cl[d][2] = v2;
+            }
+        }
+    }
+    const int S = p.nsplit, T = p.ntiles;
+    int *row = p.row + (size_t)s * p.rowcap;
+    int *sst = p.sst + (size_t)s * (S + 1);
+    unsigned *mrow = p.mask + (size_t)(s / kBrick) * p.maskw;
+    int cnt = 0, nsp = 0;
+    int mw = -1;                                      // mask word being gathered (tiles ascend with j)
+    unsigned mbits = 0u;
+    auto mark = [&](int t) {
+        if ((t >> 5) != mw) {
+            if (mw >= 0 && lane == 0) atomicOr(&mrow[mw], mbits);
+            mw = t >> 5;
+            mbits = 0u;
+        }
+        mbits |= 1u << (t & 31);
+    };
+    auto split_begin = [&](int sp) { return (int)((long long)sp * T / S) * kPTile; };
+    for (int ax = 0; ax < ncl[0]; ++ax)
+        for (int ay = 0; ay < ncl[1]; ++ay)
+            for (int az = 0; az < ncl[2]; ++az) {
+                const int c = (cl[0][ax] * ncd + cl[1][ay]) * ncd + cl[2][az];
+                const int a0 = p.cstart[c], a1 = a0 + p.ccount[c];
+                // candidate data of the next batch loaded ahead of its use
+                int jq = a0 + lane;
+                double q0 = 0.0, q1 = 0.0, q2 = 0.0, qd = 0.0;
+                int qm = ms;
+                if (jq < a1) {
+                    q0 = p.scen[4 * jq];
+                    q1 = p.scen[4 * jq + 1];
+                    q2 = p.scen[4 * jq + 2];
+                    qd = p.scen[4 * jq + 3];
+                    qm = p.smt[jq] >> 1;
+                }
+                for (int j0 = a0; j0 < a1; j0 += 64) {
+                    const int j = jq;
+                    const double c0 = q0, c1 = q1, c2 = q2, cd = qd;
+                    const int mj = qm;
+                    jq = j0 + 64 + lane;
+                    if (jq < a1) {
+                        q0 = p.scen[4 * jq];
+                        q1 = p.scen[4 * jq + 1];
+                        q2 = p.scen[4 * jq + 2];
+                        qd = p.scen[4 * jq + 3];
+                        qm = p.smt[jq] >> 1;
+                    }
+                    bool keep = false;
+                    if (j < a1 && j != s && mj != ms) {
+                        const double d0 = min_image(ci0 - c0, L), d1 = min_image(ci1 - c1, L),
+                                     d2 = min_image(ci2 - c2, L);
+                        const double lim = rl + di + cd;
+                        keep = d0 * d0 + d1 * d1 + d2 * d2 < lim * lim;
+                    }
+                    const unsigned long long m = __ballot(keep);
+                    const int len = min(64, a1 - j0);
+                    // split starts at or before this batch, then inside it
+                    while (nsp <= S && split_begin(nsp) <= j0) {
+                        if (lane == 0) sst[nsp] = cnt;
+                        ++nsp;
+                    }
+                    while (nsp <= S && split_begin(nsp) < j0 + len) {
+                        const int below = split_begin(nsp) - j0;
+                        if (lane == 0) sst[nsp] = cnt + __popcll(m & ((1ull << below) - 1ull));
+                        ++nsp;
+                    }
+                    if (keep) {
+                        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+                        if (pos < p.rowcap) row[pos] = j;
+                    }
+                    if (m) {
+                        mark((j0 + __builtin_ctzll(m)) / kPTile);
+                        mark((j0 + 63 - __builtin_clzll(m)) / kPTile);
+                    }
+                    cnt += __popcll(m);
+                }
+            }
+    while (nsp <= S) {
+        if (lane == 0) sst[nsp] = cnt;
+        ++nsp;
+    }
+    if (mw >= 0 && lane == 0) atomicOr(&mrow[mw], mbits);
+    if (cnt > p.rowcap) {                             // row too long: this atom walks whole tiles
+        for (int k = lane; k < p.maskw; k += 64) atomicOr(&mrow[k], 0xFFFFFFFFu);
+        if (lane == 0) sst[0] = -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void pl_rows_kernel(PairDev p, const double *__restrict__ x)
+{
+    if (p.ctl[kCtlGo] == 0) return;
+    const int lane = threadIdx.x & 63;
+    for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < p.n; s += gridDim.x * 4)
+        pl_row(p, x, s, lane);
+}
+
+// erfc(x) for x >= 0 from the Chebyshev series of erfcx(x) = exp(x^2) erfc(x)
+// on [0, X] (Clenshaw) and ex2 = exp(-x^2); beyond X (X = min(a rc, 6)) the
+// series is held at X, where erfc is below 2.2e-17.
+// NT terms (a compile-time count, so the coefficients come from the kernel
+// arguments at fixed offsets): 24 FP64 / 16 binary32 terms for X <= 2.5,
+// 32 / 24 beyond.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double rint_t(double a) { return __builtin_rint(a); }
+__device__ __forceinline__ float rint_t(float a) { return __builtin_rintf(a); }
+
+template <typename T, int NT>
+__device__ __forceinline__ T erfc_series(const FFDev &f, T x, T ex2)
+{
+#pragma clang fp contract(off)
+    const T t = fma_t(min(x, (T)f.exmax), (T)f.ecx, (T)-1);
+    T h = sizeof(T) == 8 ? (T)f.ecf[NT - 1] : (T)f.ecf32[NT - 1];
+#pragma unroll
+    for (int k = NT - 2; k >= 0; --k) h = fma_t(h, t, sizeof(T) == 8 ? (T)f.ecf[k] : (T)f.ecf32[k]);
+    return ex2 * h;
+}
+
+// Pair-loop arithmetic.  FP64: reciprocal and reciprocal square root from
+// the hardware estimates with two Newton steps each (positive normal
+// arguments only); binary32: the hardware instructions as they are (1 ulp).
+__device__ __forceinline__ double rcp_pair(double a)
+{
+#pragma clang fp contract(off)
+    double y = __builtin_amdgcn_rcp(a);
+    y = y * fma(-a, y, 2.0);
+    return y * fma(-a, y, 2.0);
+}
+__device__ __forceinline__ float rcp_pair(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double rsq_pair(double a)
+{
+#pragma clang fp contract(off)
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    y = y * fma(-h * y, y, 1.5);
+    return y * fma(-h * y, y, 1.5);
+}
+__device__ __forceinline__ float rsq_pair(float a) { return __builtin_amdgcn_rsqf(a); }
+__device__ __forceinline__ double exp_pair(double a) { return exp(a); }
+__device__ __forceinline__ float exp_pair(float a) { return __expf(a); }
+
+// Nonbonded pairs of a brick of sorted atoms (one lane each) against the
+// tiles of split blockIdx.y; GRAD: gradient of the lane's atom, PAR:
+// parameter derivatives (and with v, the force-matching contraction).
+// LIST: the pairs of the rows; otherwise every atom of every tile.
+template <typename T, bool GRAD, bool PAR, bool LIST, bool WIDE>
+__device__ __forceinline__ void nonbonded_body(const FFDev &f, const PairDev &pl, const Theta &th,
+                                               const double *__restrict__ x, const double *__restrict__ v,
+                                               double *__restrict__ gpart, double *__restrict__ epart)
+{
+    // The arithmetic is written out operation by operation (fused only
+    // where fma_t says so): the row walk and the all-pairs walk, compiled
+    // apart, round every pair the same way.
+#pragma clang fp contract(off)
+    constexpr int kC = kChunkTiles * kPTile;
+    __shared__ T sx[kC][3];
+    __shared__ T sv[PAR ? kC : 1][3];
+    __shared__ int smt[kC];
+    __shared__ double red[kBrick / 64][11];
     const int n = f.natoms;
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * kTile + tid;
-    const int split = blockIdx.y, b = blockIdx.z;
+    const int brick = blockIdx.x, split = blockIdx.y, b = blockIdx.z, S = gridDim.y;
+    const int s = brick * kBrick + tid;
+    const bool act = s < n;
+    const int i = act ? pl.perm[s] : 0;
     const double *xb = x + (size_t)b * n * 3;
     const double *vb = v ? v + (size_t)b * n * 3 : nullptr;
-    const bool act = i < n;
-    const int ii = act ? i : 0;
-    const T xi0 = (T)xb[3 * ii], xi1 = (T)xb[3 * ii + 1], xi2 = (T)xb[3 * ii + 2];
+    const T xi0 = (T)xb[3 * i], xi1 = (T)xb[3 * i + 1], xi2 = (T)xb[3 * i + 2];
     T vi0 = 0, vi1 = 0, vi2 = 0;
     if (PAR && vb) {
-        vi0 = (T)vb[3 * ii];
-        vi1 = (T)vb[3 * ii + 1];
-        vi2 = (T)vb[3 * ii + 2];
+        vi0 = (T)vb[3 * i];
+        vi1 = (T)vb[3 * i + 1];
+        vi2 = (T)vb[3 * i + 2];
     }
-    const int ti = f.type[ii], mi = f.mol[ii];
-    const T box = (T)f.box, rc2 = (T)(f.rc * f.rc), za = (T)f.za, zp = (T)f.zp, rc = (T)f.rc;
+    const int mti = act ? pl.smt[s] : 0;
+    const int ti = mti & 1, mi = mti >> 1;
+    const T box = (T)f.box, ibox = (T)(1.0 / f.box), rc2 = (T)(f.rc * f.rc), za = (T)f.za, zp = (T)f.zp,
+            rc = (T)f.rc;
     const T two_over_spi = (T)1.1283791670955126;
     const T qi = (T)(ti == 0 ? th.p[5] : th.p[6]);
     // per-partner-type constants, selected by the partner's type below (no
     // run-time indexing of parameter arrays)
-    const T qt[2] = { (T)th.p[5], (T)th.p[6] };
-    const T c6t[2] = { (T)(ti == 0 ? th.p[7] : th.p[8]), (T)(ti == 0 ? th.p[8] : th.p[9]) };
-    const T r0t[2] = { (T)(ti == 0 ? f.r0bj[0][0] : f.r0bj[1][0]), (T)(ti == 0 ? f.r0bj[0][1] : f.r0bj[1][1]) };
-    const T srt[2] = { (T)(ti == 0 ? f.sr42[0][0] : f.sr42[1][0]), (T)(ti == 0 ? f.sr42[0][1] : f.sr42[1][1]) };
-    const T zrt[2] = { (T)(ti == 0 ? f.zrep[0][0] : f.zrep[1][0]), (T)(ti == 0 ? f.zrep[0][1] : f.zrep[1][1]) };
-    const T alt[2] = { (T)(ti == 0 ? f.arep[0][0] : f.arep[1][0]), (T)(ti == 0 ? f.arep[0][1] : f.arep[1][1]) };
+    // per-partner-type constants (partner type 0 / 1), selected per pair
+    const T qt0 = (T)th.p[5], qt1 = (T)th.p[6];
+    const T c6t0 = (T)(ti == 0 ? th.p[7] : th.p[8]), c6t1 = (T)(ti == 0 ? th.p[8] : th.p[9]);
+    const T r0t0 = (T)(ti == 0 ? f.r0bj[0][0] : f.r0bj[1][0]), r0t1 = (T)(ti == 0 ? f.r0bj[0][1] : f.r0bj[1][1]);
+    const T srt0 = (T)(ti == 0 ? f.sr42[0][0] : f.sr42[1][0]), srt1 = (T)(ti == 0 ? f.sr42[0][1] : f.sr42[1][1]);
+    const T zrt0 = (T)(ti == 0 ? f.zrep[0][0] : f.zrep[1][0]), zrt1 = (T)(ti == 0 ? f.zrep[0][1] : f.zrep[1][1]);
+    const T alt0 = (T)(ti == 0 ? f.arep[0][0] : f.arep[1][0]), alt1 = (T)(ti == 0 ? f.arep[0][1] : f.arep[1][1]);
     Acc<T> g0, g1, g2;
     double e = 0.0;
-    // parameter derivatives: [0] q_O, [1] q_H, [2..4] C6 of (ti, partner)
-    // pairs OO/OH/HH; dv* the same for the force-matching contraction
-    double dq[2] = { 0.0, 0.0 }, dc[3] = { 0.0, 0.0, 0.0 }, dvq[2] = { 0.0, 0.0 }, dvc[3] = { 0.0, 0.0, 0.0 };
+    // parameter derivatives: q_O, q_H, C6 of (ti, partner) pairs OO/OH/HH;
+    // dv* the same for the force-matching contraction (named scalars, no
+    // run-time indexed arrays)
+    double dq0 = 0.0, dq1 = 0.0, dc0 = 0.0, dc1 = 0.0, dc2 = 0.0;
+    double dvq0 = 0.0, dvq1 = 0.0, dvc0 = 0.0, dvc1 = 0.0, dvc2 = 0.0;
 
-    // one pair (i, j): j's position, type and direction vector
-    auto pair = [&](T xj0, T xj1, T xj2, int tj, T vj0, T vj1, T vj2) {
-            const T d0 = min_image(xi0 - xj0, box);
-            const T d1 = min_image(xi1 - xj1, box);
-            const T d2 = min_image(xi2 - xj2, box);
-            const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (r2 > rc2) return;
+    // minimum-image separation of i from the atom at LDS slot jj
+    auto sep = [&](int jj, T &d0, T &d1, T &d2) -> T {
+        d0 = xi0 - sx[jj][0];
+        d1 = xi1 - sx[jj][1];
+        d2 = xi2 - sx[jj][2];
+        d0 = fma_t(-box, rint_t(d0 * ibox), d0);
+        d1 = fma_t(-box, rint_t(d1 * ibox), d1);
+        d2 = fma_t(-box, rint_t(d2 * ibox), d2);
+        return fma_t(d0, d0, fma_t(d1, d1, d2 * d2));
+    };
+    // one pair inside the cutoff: j at LDS slot jj, separation d, r2 = |d|^2
+    auto pair = [&](int jj, T d0, T d1, T d2, T r2) {
+            const int tj = smt[jj] & 1;
             const bool jo = tj == 0;
-            const T r = sqrt_t(r2);
-            const T rinv = (T)1 / r;
-            const T qj = jo ? qt[0] : qt[1];
-            const T c6 = jo ? c6t[0] : c6t[1];
-            const T ev = erfc_t(za * r);
-            const T phi = ev * rinv - zp * (r - rc);
-            const T R0 = jo ? r0t[0] : r0t[1], sr = jo ? srt[0] : srt[1];
+            const T rinv = rsq_pair(r2);
+            const T r = r2 * rinv;
+            const T qj = jo ? qt0 : qt1;
+            const T c6 = jo ? c6t0 : c6t1;
+            const T xa = za * r;
+            const T ex2 = exp_pair(-(xa * xa));
+            constexpr int NT = sizeof(T) == 8 ? (WIDE ? 32 : 24) : (WIDE ? 24 : 16);
+            const T ev = erfc_series<T, NT>(f, xa, ex2);
+            const T R0 = jo ? r0t0 : r0t1, sr = jo ? srt0 : srt1;
             const T R02 = R0 * R0, R06 = R02 * R02 * R02;
             const T r4 = r2 * r2, r6 = r4 * r2;
-            const T t6 = r6 + R06, t8 = r6 * r2 + R06 * R02;
-            const T it6 = (T)1 / t6, it8 = (T)1 / t8;
-            const T dispu = -(it6 + sr * it8);
-            const T zr = jo ? zrt[0] : zrt[1], al = jo ? alt[0] : alt[1];
-            const T rep = zr * exp_t(-al * r) * rinv;
-            e += 0.5 * (double)(qi * qj * phi + c6 * dispu + rep);
+            const T t6 = r6 + R06, t8 = fma_t(r6, r2, R06 * R02);
+            // one reciprocal for 1/t6 and 1/t8
+            const T ip = rcp_pair(t6 * t8);
+            const T it6 = t8 * ip, it8 = t6 * ip;
+            const T phi = fma_t(ev, rinv, -(zp * (r - rc)));
+            const T dispu = -fma_t(sr, it8, it6);
+            const T zr = jo ? zrt0 : zrt1, al = jo ? alt0 : alt1;
+            const T rep = zr * exp_pair(-(al * r)) * rinv;
+            const T qq = qi * qj;
+            e += 0.5 * (double)fma_t(qq, phi, fma_t(c6, dispu, rep));
             if (GRAD || PAR) {
-                const T dphi = -ev * rinv * rinv - two_over_spi * za * exp_t(-za * za * r2) * rinv - zp;
-                const T ddisp = (T)6 * r4 * r * it6 * it6 + sr * (T)8 * r6 * r * it8 * it8;
-                const T drep = -rep * (al + rinv);
+                const T dphi = fma_t(-(ev * rinv), rinv, -fma_t(two_over_spi * za * ex2, rinv, zp));
+                const T ddisp = fma_t(sr * (T)8 * r6 * r * it8, it8, (T)6 * r4 * r * it6 * it6);
+                const T drep = -(rep * (al + rinv));
                 if (GRAD) {
-                    const T fp = (qi * qj * dphi + c6 * ddisp + drep) * rinv;
+                    const T fp = fma_t(qq, dphi, fma_t(c6, ddisp, drep)) * rinv;
                     g0.add(fp * d0);
                     g1.add(fp * d1);
                     g2.add(fp * d2);
@@ -233,67 +584,79 @@ __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, con
                 if (PAR) {
                     // d/dq_type of q_i q_j: own charge gets q_j, partner's charge gets q_i
                     const double hj = 0.5 * (double)(qj * phi), hi = 0.5 * (double)(qi * phi);
-                    if (ti == 0) dq[0] += hj; else dq[1] += hj;
-                    if (jo) dq[0] += hi; else dq[1] += hi;
+                    if (ti == 0) dq0 += hj; else dq1 += hj;
+                    if (jo) dq0 += hi; else dq1 += hi;
                     const int cp = ti + tj;
                     const double hd = 0.5 * (double)dispu;
-                    if (cp == 0) dc[0] += hd; else if (cp == 1) dc[1] += hd; else dc[2] += hd;
+                    if (cp == 0) dc0 += hd; else if (cp == 1) dc1 += hd; else dc2 += hd;
                     if (vb) {
-                        const T proj = ((vi0 - vj0) * d0 + (vi1 - vj1) * d1 + (vi2 - vj2) * d2) * rinv;
+                        const T proj =
+                            fma_t(vi0 - sv[jj][0], d0, fma_t(vi1 - sv[jj][1], d1, (vi2 - sv[jj][2]) * d2)) * rinv;
                         const double vj_ = 0.5 * (double)(qj * dphi * proj), vi_ = 0.5 * (double)(qi * dphi * proj);
-                        if (ti == 0) dvq[0] += vj_; else dvq[1] += vj_;
-                        if (jo) dvq[0] += vi_; else dvq[1] += vi_;
+                        if (ti == 0) dvq0 += vj_; else dvq1 += vj_;
+                        if (jo) dvq0 += vi_; else dvq1 += vi_;
                         const double vd = 0.5 * (double)(ddisp * proj);
-                        if (cp == 0) dvc[0] += vd; else if (cp == 1) dvc[1] += vd; else dvc[2] += vd;
+                        if (cp == 0) dvc0 += vd; else if (cp == 1) dvc1 += vd; else dvc2 += vd;
                     }
                 }
             }
     };
-    const int chunk = split_chunk(n, nsplit);
-    const int j0 = split * chunk, j1 = min(n, j0 + chunk);
-    if constexpr (LIST) {
-        if (act) {
-            const int *row = nbr + (size_t)i * n;
-            const int k0 = nofs[(size_t)i * (nsplit + 1) + split], k1 = nofs[(size_t)i * (nsplit + 1) + split + 1];
-            for (int k = k0; k < k1; ++k) {
-                const int j = row[k];
-                T vj0 = 0, vj1 = 0, vj2 = 0;
-                if (PAR && vb) {
-                    vj0 = (T)vb[3 * j];
-                    vj1 = (T)vb[3 * j + 1];
-                    vj2 = (T)vb[3 * j + 2];
-                }
-                pair((T)xb[3 * j], (T)xb[3 * j + 1], (T)xb[3 * j + 2], f.type[j], vj0, vj1, vj2);
+    // this lane's part of its row for the split (LIST), or whole tiles
+    bool scan = !LIST;
+    int k = 0, kend = 0;
+    const int *row = pl.row + (size_t)s * pl.rowcap;
+    if (LIST && act) {
+        const int *ss = pl.sst + (size_t)s * (S + 1);
+        if (ss[0] < 0) scan = true;
+        else {
+            k = ss[split];
+            kend = ss[split + 1];
+        }
+    }
+    int jn = k < kend ? row[k] : 0x7FFFFFFF;
+    const unsigned *mrow = pl.mask + (size_t)brick * pl.maskw;
+    const int t0 = (int)((long long)split * pl.ntiles / S), t1 = (int)((long long)(split + 1) * pl.ntiles / S);
+    for (int c0 = t0; c0 < t1; c0 += kChunkTiles) {
+        const int c1 = min(t1, c0 + kChunkTiles);
+        unsigned used = (1u << (c1 - c0)) - 1u;        // the same for the whole block
+        if (LIST) {
+            used = 0u;
+            for (int t = c0; t < c1; ++t) used |= ((mrow[t >> 5] >> (t & 31)) & 1u) << (t - c0);
+        }
+        if (!used) continue;
+        const int jb = c0 * kPTile, je = min(n, c1 * kPTile);
+        __syncthreads();
+        for (int q = tid; q < je - jb; q += kBrick) {
+            if (!((used >> (q / kPTile)) & 1u)) continue;
+            const int j = jb + q;
+            const int jo = pl.perm[j];
+            sx[q][0] = (T)xb[3 * jo];
+            sx[q][1] = (T)xb[3 * jo + 1];
+            sx[q][2] = (T)xb[3 * jo + 2];
+            smt[q] = pl.smt[j];
+            if (PAR) {
+                sv[q][0] = vb ? (T)vb[3 * jo] : T(0);
+                sv[q][1] = vb ? (T)vb[3 * jo + 1] : T(0);
+                sv[q][2] = vb ? (T)vb[3 * jo + 2] : T(0);
             }
         }
-    } else {
-        for (int jt = j0; jt < j1; jt += kTile) {
-            __syncthreads();
-            const int j = jt + tid;
-            if (j < j1) {
-                sx[tid][0] = (T)xb[3 * j];
-                sx[tid][1] = (T)xb[3 * j + 1];
-                sx[tid][2] = (T)xb[3 * j + 2];
-                st[tid] = f.type[j];
-                sm[tid] = f.mol[j];
-                if (PAR && vb) {
-                    sv[tid][0] = (T)vb[3 * j];
-                    sv[tid][1] = (T)vb[3 * j + 1];
-                    sv[tid][2] = (T)vb[3 * j + 2];
-                }
-            } else {
-                sm[tid] = mi;            // padding: excluded as intramolecular
-                st[tid] = 0;
-                sx[tid][0] = sx[tid][1] = sx[tid][2] = 0;
-                if (PAR) sv[tid][0] = sv[tid][1] = sv[tid][2] = 0;
+        __syncthreads();
+        if (!act) continue;
+        if (scan) {
+            for (int jj = 0; jj < je - jb; ++jj) {
+                if ((smt[jj] >> 1) == mi) continue;     // same molecule (and the atom itself)
+                T d0, d1, d2;
+                const T r2 = sep(jj, d0, d1, d2);
+                if (r2 <= rc2) pair(jj, d0, d1, d2, r2);
             }
-            __syncthreads();
-            if (!act) continue;
-            const int nt = min(kTile, j1 - jt);
-            for (int jj = 0; jj < nt; ++jj) {
-                if (sm[jj] == mi) continue;
-                pair(sx[jj][0], sx[jj][1], sx[jj][2], st[jj], PAR ? sv[jj][0] : T(0), PAR ? sv[jj][1] : T(0),
-                     PAR ? sv[jj][2] : T(0));
+        } else {
+            while (jn < je) {                           // the next entry is already requested
+                const int jj = jn - jb;
+                ++k;
+                jn = k < kend ? row[k] : 0x7FFFFFFF;
+                T d0, d1, d2;
+                const T r2 = sep(jj, d0, d1, d2);
+                if (r2 <= rc2) pair(jj, d0, d1, d2, r2);
             }
         }
     }
@@ -303,37 +666,60 @@ __global__ __launch_bounds__(kTile) void nonbonded_kernel(FFDev f, Theta th, con
         gp[1] = g1.get();
         gp[2] = g2.get();
     }
-    // block sums of energy and parameter terms, fixed order
+    // block sums of energy and parameter terms: butterflies within each wave,
+    // then the four waves in order
     const int nblk = gridDim.x * gridDim.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
     double *ep = epart + ((size_t)b * (nblk + 1) + blk) * kNE;
-    auto block_sum = [&](double val, int slot) {
-        __syncthreads();
-        red[tid] = act ? val : 0.0;
-        __syncthreads();
-        for (int w = kTile / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) ep[slot] = red[0];
-    };
-    block_sum(e, 0);
-    if (PAR) {
-        for (int p = 0; p < 5; ++p) {
-            block_sum(0.0, 1 + p);
-            block_sum(0.0, 1 + kNP + p);
-        }
-        block_sum(dq[0], 6);
-        block_sum(dq[1], 7);
-        block_sum(dc[0], 8);
-        block_sum(dc[1], 9);
-        block_sum(dc[2], 10);
-        block_sum(dvq[0], 1 + kNP + 5);
-        block_sum(dvq[1], 1 + kNP + 6);
-        block_sum(dvc[0], 1 + kNP + 7);
-        block_sum(dvc[1], 1 + kNP + 8);
-        block_sum(dvc[2], 1 + kNP + 9);
+    constexpr int nv = PAR ? 11 : 1;
+    const double vals[11] = { e, dq0, dq1, dc0, dc1, dc2, dvq0, dvq1, dvc0, dvc1, dvc2 };
+    const int wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < nv; ++q) {
+        double w = act ? vals[q] : 0.0;
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+        if ((tid & 63) == 0) red[wave][q] = w;
     }
+    __syncthreads();
+    if (tid < kNE) {
+        // slot -> value: 0 energy, 6..10 dq/dc, 16..20 dvq/dvc, the rest zero
+        int q = -1;
+        if (tid == 0) q = 0;
+        else if (PAR && tid >= 6 && tid <= 10) q = tid - 5;
+        else if (PAR && tid >= 1 + kNP + 5) q = tid - (1 + kNP + 5) + 6;
+        if (q >= 0) ep[tid] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+        else if (PAR) ep[tid] = 0.0;
+    }
+}
+
+// FP64 at the register count the compiler picks (four waves per SIMD);
+// binary32 forces and energies held to six waves per SIMD.
+template <typename T, bool GRAD, bool PAR, bool LIST, bool WIDE>
+__global__ __launch_bounds__(kBrick) void nonbonded_kernel(FFDev f, PairDev pl, Theta th, const double *__restrict__ x,
+                                                           const double *__restrict__ v, double *__restrict__ gpart,
+                                                           double *__restrict__ epart)
+{
+    nonbonded_body<T, GRAD, PAR, LIST, WIDE>(f, pl, th, x, v, gpart, epart);
+}
+
+template <bool GRAD, bool LIST, bool WIDE>
+__global__ __launch_bounds__(kBrick) __attribute__((amdgpu_waves_per_eu(6, 8))) void nonbonded_kernel_f32(
+    FFDev f, PairDev pl, Theta th, const double *__restrict__ x, const double *__restrict__ v,
+    double *__restrict__ gpart, double *__restrict__ epart)
+{
+    nonbonded_body<float, GRAD, false, LIST, WIDE>(f, pl, th, x, v, gpart, epart);
+}
+
+template <typename T, bool GRAD, bool PAR, bool LIST, bool WIDE>
+void launch_nonbonded(dim3 grid, hipStream_t s, const FFDev &f, const PairDev &pl, const Theta &th, const double *x,
+                      const double *v, double *gpart, double *epart)
+{
+    if constexpr (sizeof(T) == 4 && !PAR)
+        hipLaunchKernelGGL((nonbonded_kernel_f32<GRAD, LIST, WIDE>), grid, dim3(kBrick), 0, s, f, pl, th, x, v, gpart,
+                           epart);
+    else
+        hipLaunchKernelGGL((nonbonded_kernel<T, GRAD, PAR, LIST, WIDE>), grid, dim3(kBrick), 0, s, f, pl, th, x, v,
+                           gpart, epart);
 }
 
 // Bonded terms, one thread per (atom, bead): the thread evaluates every bond
@@ -346,7 +732,6 @@ __global__ __launch_bounds__(kTile) void bonded_kernel(FFDev f, Theta th, const 
                                                        const double *__restrict__ gpart, double *__restrict__ g,
                                                        double *__restrict__ epart, int nblk_nb)
 {
-    __shared__ double red[kTile];
     const int n = f.natoms;
     const int tid = threadIdx.x;
     const int a = blockIdx.x * kTile + tid;
@@ -447,45 +832,43 @@ __global__ __launch_bounds__(kTile) void bonded_kernel(FFDev f, Theta th, const 
             go[2] = gg[2];
         }
     }
+    // block sums (the block is one wave: butterflies, a fixed order) into the
+    // bonded rows, which follow every bead's nonbonded rows
     const int blk = blockIdx.x;
-    double *ep = epart + ((size_t)b * (nblk_nb + 1)) * kNE;   // bonded rows follow the nonbonded ones
-    auto block_sum = [&](double val) -> double {
-        __syncthreads();
-        red[tid] = act ? val : 0.0;
-        __syncthreads();
-        for (int w = kTile / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        return red[0];
-    };
-    // bonded block sums go into a separate per-bead array slot region
     double *bp = epart + ((size_t)gridDim.y * (nblk_nb + 1)) * kNE + ((size_t)b * gridDim.x + blk) * kNE;
-    (void)ep;
-    double s0 = block_sum(e);
+    auto wave_sum = [&](double val) -> double {
+        double w = act ? val : 0.0;
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+        return w;
+    };
+    const double s0 = wave_sum(e);
     if (tid == 0) bp[0] = s0;
-    for (int p = 0; p < kNP; ++p) {
-        const double sp = block_sum(dth[p]);
-        if (tid == 0) bp[1 + p] = PAR ? sp : 0.0;
-    }
-    for (int p = 0; p < kNP; ++p) {
-        const double sp = block_sum(dvth[p]);
-        if (tid == 0) bp[1 + kNP + p] = PAR ? sp : 0.0;
+    if (PAR) {
+#pragma unroll
+        for (int p = 0; p < kNP; ++p) {
+            const double a = wave_sum(dth[p]), c = wave_sum(dvth[p]);
+            if (tid == 0) {
+                bp[1 + p] = a;
+                bp[1 + kNP + p] = c;
+            }
+        }
     }
 }
 
-// Per-bead totals: nonbonded block rows then bonded block rows, fixed order.
-__global__ void ff_reduce_kernel(const double *epart, int nblk_nb, int nblk_b, int nbeads, double *e, double *dth,
-                                 double *dvth)
+// Per-bead totals, one wave per (slot, bead): the nonbonded block rows then
+// the bonded block rows, lane k taking rows k, k + 64, ..., then butterflies
+// (a fixed order).
+__global__ __launch_bounds__(64) void ff_reduce_kernel(const double *epart, int nblk_nb, int nblk_b, int nbeads,
+                                                       double *e, double *dth, double *dvth)
 {
-    const int b = blockIdx.x;
-    const int slot = threadIdx.x;
-    if (slot >= kNE) return;
-    double s = 0.0;
+    const int slot = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
     const double *nb = epart + (size_t)b * (nblk_nb + 1) * kNE;
-    for (int k = 0; k < nblk_nb; ++k) s += nb[(size_t)k * kNE + slot];
     const double *bd = epart + ((size_t)nbeads * (nblk_nb + 1)) * kNE + (size_t)b * nblk_b * kNE;
-    for (int k = 0; k < nblk_b; ++k) s += bd[(size_t)k * kNE + slot];
+    double s = 0.0;
+    for (int k = lane; k < nblk_nb + nblk_b; k += 64)
+        s += k < nblk_nb ? nb[(size_t)k * kNE + slot] : bd[(size_t)(k - nblk_nb) * kNE + slot];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane != 0) return;
     if (slot == 0) {
         if (e) e[b] = s;
     } else if (slot <= kNP) {
@@ -687,43 +1070,147 @@ struct cam_ff {
     int natoms = 0;
     int *type = nullptr, *mol = nullptr, *bonds = nullptr, *angles = nullptr, *csr_ptr = nullptr, *csr_term = nullptr;
     double *gpart = nullptr, *epart = nullptr;
-    int cap_beads = 0, nsplit = 4;
+    int cap_beads = 0, nsplit = 4;    // the water model's pair splits
+    size_t gpart_bytes = 0, epart_bytes = 0;
     int pair_mode = CAM_FF_PAIRS_LIST;
-    int *nbr = nullptr, *nofs = nullptr;   // neighbour list [atom][atom], split offsets [atom][nsplit + 1]
-    double *cen = nullptr, *dev = nullptr; // bead centroids and spreads of the last call
+    PairDev pl{};                     // kind 0: sorted order, rows and their reference positions
+    int xref_beads = 0;
+    std::vector<void *> pl_allocs;
 };
 
 namespace {
 
-// Largest box for which the neighbour list is kept: its rows hold up to
-// natoms entries each (natoms^2 ints, 1 GiB at this size).
-constexpr int kListMaxAtoms = 16384;
-
-bool ff_uses_list(const cam_ff *ff)
+// erfcx(x) = exp(x^2) erfc(x) on [0, X], X = min(a rc, 6) (beyond 6, erfc <
+// 2.2e-17), as a polynomial in t = 2 x / X - 1: the Chebyshev series from
+// erfcx at 96 Chebyshev nodes in long double, c_k = (2 - [k = 0]) / N sum_j
+// f(x_j) cos(k theta_j), truncated after 24 terms (X <= 2.5) or 32, then
+// written in powers of t (T_{k+1} = 2 t T_k - T_{k-1}).  The kernels take 24
+// (FP64) / 16 (binary32) terms for X <= 2.5, 32 / 24 beyond; the relative
+// error of the FP64 form is below 2e-14 on [0, 6].
+void erfc_fit(FFDev &d)
 {
-    return ff->kind == 0 && ff->pair_mode == CAM_FF_PAIRS_LIST && ff->natoms <= kListMaxAtoms;
+    const int N = 96;
+    const long double X = std::min(6.0, std::max(1e-3, d.za * d.rc)) * (1.0L + 1e-12L);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    long double fv[N], th[N];
+    for (int j = 0; j < N; ++j) {
+        th[j] = pi * (j + 0.5L) / N;
+        const long double xx = 0.5L * X * (std::cos(th[j]) + 1.0L);
+        fv[j] = std::exp(xx * xx) * std::erfc(xx);
+    }
+    const int nt64 = X <= 2.5L ? 24 : 32;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nt = pass == 0 ? nt64 : (X <= 2.5L ? 16 : 24);
+        long double cheb[kErfcTerms] = {}, mono[kErfcTerms] = {}, tk[kErfcTerms] = {}, tkm[kErfcTerms] = {};
+        for (int k = 0; k < nt; ++k) {
+            long double sum = 0.0L;
+            for (int j = 0; j < N; ++j) sum += fv[j] * std::cos(k * th[j]);
+            cheb[k] = (k == 0 ? 1.0L : 2.0L) * sum / N;
+        }
+        tkm[0] = 1.0L;                               // T_0
+        tk[1] = 1.0L;                                // T_1
+        mono[0] += cheb[0];
+        for (int q = 0; q < kErfcTerms; ++q) mono[q] += cheb[1] * tk[q];
+        for (int k = 2; k < nt; ++k) {               // T_k = 2 t T_{k-1} - T_{k-2}
+            long double tn[kErfcTerms] = {};
+            for (int q = 0; q + 1 < kErfcTerms; ++q) tn[q + 1] += 2.0L * tk[q];
+            for (int q = 0; q < kErfcTerms; ++q) tn[q] -= tkm[q];
+            for (int q = 0; q < kErfcTerms; ++q) {
+                tkm[q] = tk[q];
+                tk[q] = tn[q];
+                mono[q] += cheb[k] * tn[q];
+            }
+        }
+        for (int q = 0; q < kErfcTerms; ++q) {
+            if (pass == 0) d.ecf[q] = (double)mono[q];
+            else d.ecf32[q] = (float)mono[q];
+        }
+    }
+    d.exmax = (double)X;
+    d.ecx = (double)(2.0L / X);
+}
+
+// Splits of the pair kernel's tiles for nbeads beads: enough blocks to fill
+// the GPU (about 2048 of 256 lanes), at most one per tile.
+int pair_splits(const PairDev &p, int nbeads)
+{
+    const long long want = (2048 + (long long)p.nbricks * nbeads - 1) / ((long long)p.nbricks * nbeads);
+    return (int)std::max(1LL, std::min<long long>(want, std::min(kMaxSplit, p.ntiles)));
+}
+
+// The sorted order, rows and cell tables of a box of the QMDFF form, sized
+// once for its atom count (the rows for its density with a bead spread of up
+// to 1.5 bohr; a longer row makes its atom walk whole tiles).
+int pairs_create(cam_ff *ff)
+{
+    PairDev &p = ff->pl;
+    const int n = ff->natoms;
+    p.n = n;
+    p.box = ff->d.box;
+    p.rlist = ff->d.rc + kSkin + kMargin;
+    p.ntiles = (n + kPTile - 1) / kPTile;
+    p.nbricks = (n + kBrick - 1) / kBrick;
+    p.maskw = (p.ntiles + 31) / 32;
+    const double dens = n / (p.box * p.box * p.box), rl = p.rlist + 3.0;
+    const double est = 1.3 * dens * 4.18879020478639 * rl * rl * rl + 64.0;
+    p.rowcap = (int)std::min<double>(n, est);
+    p.rowcap = std::max(4, (p.rowcap + 3) / 4 * 4);
+    p.type = ff->type;
+    p.mol = ff->mol;
+    const int nchk = (n + 3) / 4, nc3 = kMaxCellsDim * kMaxCellsDim * kMaxCellsDim;
+    auto alloc = [&](void **ptr, size_t bytes) -> int {
+        CAM_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 16)));
+        ff->pl_allocs.push_back(*ptr);
+        return CAM_OK;
+    };
+    int rc = CAM_OK;
+#define CAM_PA(field, count, type) \
+    if (rc == CAM_OK) rc = alloc((void **)&p.field, (size_t)(count) * sizeof(type));
+    CAM_PA(perm, n, int) CAM_PA(scell, n, int) CAM_PA(smt, n, int) CAM_PA(tcell, n, int)
+    CAM_PA(scen, 4 * (size_t)n, double) CAM_PA(cstart, nc3, int) CAM_PA(ccount, nc3, int)
+    CAM_PA(row, (size_t)n * p.rowcap, int) CAM_PA(sst, (size_t)n * (kMaxSplit + 1), int)
+    CAM_PA(mask, (size_t)p.nbricks * p.maskw, unsigned) CAM_PA(cen, 4 * (size_t)n, double)
+    CAM_PA(bdev, nchk, double) CAM_PA(bover, nchk, int)
+    CAM_PA(ctl, kCtlWords, int)
+#undef CAM_PA
+    if (rc != CAM_OK) return rc;
+    CAM_HIP(hipMemset(p.ctl, 0, kCtlWords * sizeof(int)));
+    return CAM_OK;
 }
 
 int ff_ensure(cam_ff *ff, int nbeads)
 {
     if (ff->kind == 2) return CAM_OK;             // the QMDFF tables hold their own buffers
-    if (ff_uses_list(ff) && !ff->nbr) {
-        const size_t n = ff->natoms;
-        CAM_HIP(hipMalloc(&ff->nbr, n * n * sizeof(int)));
-        CAM_HIP(hipMalloc(&ff->nofs, n * (ff->nsplit + 1) * sizeof(int)));
-        CAM_HIP(hipMalloc(&ff->cen, n * 3 * sizeof(double)));
-        CAM_HIP(hipMalloc(&ff->dev, n * sizeof(double)));
-    }
-    if (nbeads <= ff->cap_beads) return CAM_OK;
-    if (ff->gpart) CAM_HIP(hipFree(ff->gpart));
-    if (ff->epart) CAM_HIP(hipFree(ff->epart));
-    ff->gpart = ff->epart = nullptr;
     const int n = ff->natoms;
     const int nbx = (n + kTile - 1) / kTile;
-    CAM_HIP(hipMalloc(&ff->gpart, (size_t)ff->nsplit * nbeads * n * 3 * sizeof(double)));
-    const size_t rows = (size_t)nbeads * (nbx * ff->nsplit + 1) + (size_t)nbeads * nbx;
-    CAM_HIP(hipMalloc(&ff->epart, rows * kNE * sizeof(double)));
-    ff->cap_beads = nbeads;
+    size_t gbytes, ebytes;
+    if (ff->kind == 0) {
+        const int S = pair_splits(ff->pl, nbeads);
+        gbytes = (size_t)S * nbeads * n * 3 * sizeof(double);
+        ebytes = ((size_t)nbeads * (ff->pl.nbricks * S + 1) + (size_t)nbeads * nbx) * kNE * sizeof(double);
+        if (nbeads > ff->xref_beads) {            // reference positions of the rows
+            if (ff->pl.xref) CAM_HIP(hipFree(ff->pl.xref));
+            ff->pl.xref = nullptr;
+            CAM_HIP(hipMalloc(&ff->pl.xref, (size_t)nbeads * n * 3 * sizeof(double)));
+            CAM_HIP(hipMemset(ff->pl.xref, 0, (size_t)nbeads * n * 3 * sizeof(double)));
+            ff->xref_beads = nbeads;
+        }
+    } else {
+        gbytes = (size_t)ff->nsplit * nbeads * n * 3 * sizeof(double);
+        ebytes = ((size_t)nbeads * (nbx * ff->nsplit + 1) + (size_t)nbeads * nbx) * kNE * sizeof(double);
+    }
+    if (gbytes > ff->gpart_bytes) {
+        if (ff->gpart) CAM_HIP(hipFree(ff->gpart));
+        ff->gpart = nullptr;
+        CAM_HIP(hipMalloc(&ff->gpart, gbytes));
+        ff->gpart_bytes = gbytes;
+    }
+    if (ebytes > ff->epart_bytes) {
+        if (ff->epart) CAM_HIP(hipFree(ff->epart));
+        ff->epart = nullptr;
+        CAM_HIP(hipMalloc(&ff->epart, ebytes));
+        ff->epart_bytes = ebytes;
+    }
     return CAM_OK;
 }
 
@@ -735,12 +1222,12 @@ int ff_launch(cam_ff *ff, const Theta &th, const double *x, int nbeads, double *
     if (rc != CAM_OK) return rc;
     const int n = ff->natoms;
     const int nbx = (n + kTile - 1) / kTile;
-    const int nblk_nb = nbx * ff->nsplit;
     const bool grad = g != nullptr;
     const bool par = dth != nullptr || dvth != nullptr;
-    dim3 gnb(nbx, ff->nsplit, nbeads);
     if (ff->kind == 2) return qmdff_launch<T>(ff->qm, x, nbeads, e, g, s);
     if (ff->kind == 1) {
+        const int nblk_nb = nbx * ff->nsplit;
+        dim3 gnb(nbx, ff->nsplit, nbeads);
         if (grad)
             hipLaunchKernelGGL((water_nb_kernel<T, true>), gnb, dim3(kTile), 0, s, ff->w, x, ff->nsplit, ff->gpart,
                                ff->epart);
@@ -755,62 +1242,63 @@ int ff_launch(cam_ff *ff, const Theta &th, const double *x, int nbeads, double *
             hipLaunchKernelGGL((water_intra_kernel<false>), dim3(nbx, nbeads), dim3(kTile), 0, s, ff->w, x,
                                ff->nsplit, ff->gpart, g, ff->epart, nblk_nb);
         CAM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(ff_reduce_kernel, dim3(nbeads), dim3(32), 0, s, ff->epart, nblk_nb, nbx, nbeads, e, nullptr,
+        hipLaunchKernelGGL(ff_reduce_kernel, dim3(1, nbeads), dim3(64), 0, s, ff->epart, nblk_nb, nbx, nbeads, e, nullptr,
                            nullptr);
         CAM_LAUNCH_CHECK();
         return CAM_OK;
     }
-    if (ff_uses_list(ff)) {
-        // the list of this call's beads: centroids and spreads, then one wave per atom
-        hipLaunchKernelGGL(centroid_dev_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, nbeads, ff->d.box, x,
-                           ff->cen, ff->dev);
-        CAM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(nlist_kernel, dim3((n + 3) / 4), dim3(256), 0, s, ff->d, ff->nsplit, 1e-3, ff->cen,
-                           ff->dev, ff->nbr, ff->nofs);
-        CAM_LAUNCH_CHECK();
-        if (grad && par)
-            hipLaunchKernelGGL((nonbonded_kernel<T, true, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
-                               ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
-        else if (grad)
-            hipLaunchKernelGGL((nonbonded_kernel<T, true, false, true>), gnb, dim3(kTile), 0, s, ff->d, th, x,
-                               nullptr, ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
-        else if (par)
-            hipLaunchKernelGGL((nonbonded_kernel<T, false, true, true>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
-                               ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
-        else
-            hipLaunchKernelGGL((nonbonded_kernel<T, false, false, true>), gnb, dim3(kTile), 0, s, ff->d, th, x,
-                               nullptr, ff->nsplit, ff->gpart, ff->epart, ff->nbr, ff->nofs);
+    // the sorted order and rows (rebuilt on the device when a bead moved more
+    // than skin/2 or the bead count changed), then the pairs
+    PairDev pl = ff->pl;
+    pl.nb = nbeads;
+    pl.nsplit = pair_splits(pl, nbeads);
+    const int nsplit = pl.nsplit, nblk_nb = pl.nbricks * nsplit;
+    pl.cg = 1;
+    while (pl.cg * 2 <= std::min(nbeads, 64)) pl.cg *= 2;
+    const int nchk = (n + 256 / pl.cg - 1) / (256 / pl.cg);
+    hipLaunchKernelGGL(pl_check_kernel, dim3(nchk), dim3(256), 0, s, pl, x);
+    CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pl_sort_kernel, dim3(1), dim3(kSortThreads), 0, s, pl, nchk);
+    CAM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pl_rows_kernel, dim3(std::min((n + 3) / 4, 512)), dim3(256), 0, s, pl, x);
+    CAM_LAUNCH_CHECK();
+    dim3 gnb(pl.nbricks, nsplit, nbeads);
+    const bool list = ff->pair_mode == CAM_FF_PAIRS_LIST;
+    const bool wide = ff->d.exmax > 2.5;
+#define CAM_NB(G, P, L)                                                                                        \
+    do {                                                                                                       \
+        if (wide) launch_nonbonded<T, G, P, L, true>(gnb, s, ff->d, pl, th, x, P ? v : nullptr, ff->gpart, ff->epart); \
+        else launch_nonbonded<T, G, P, L, false>(gnb, s, ff->d, pl, th, x, P ? v : nullptr, ff->gpart, ff->epart); \
+    } while (0)
+    if (list) {
+        if (grad && par) CAM_NB(true, true, true);
+        else if (grad) CAM_NB(true, false, true);
+        else if (par) CAM_NB(false, true, true);
+        else CAM_NB(false, false, true);
     } else {
-        const int *nl = nullptr;
-        if (grad && par)
-            hipLaunchKernelGGL((nonbonded_kernel<T, true, true, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
-                               ff->nsplit, ff->gpart, ff->epart, nl, nl);
-        else if (grad)
-            hipLaunchKernelGGL((nonbonded_kernel<T, true, false, false>), gnb, dim3(kTile), 0, s, ff->d, th, x,
-                               nullptr, ff->nsplit, ff->gpart, ff->epart, nl, nl);
-        else if (par)
-            hipLaunchKernelGGL((nonbonded_kernel<T, false, true, false>), gnb, dim3(kTile), 0, s, ff->d, th, x, v,
-                               ff->nsplit, ff->gpart, ff->epart, nl, nl);
-        else
-            hipLaunchKernelGGL((nonbonded_kernel<T, false, false, false>), gnb, dim3(kTile), 0, s, ff->d, th, x,
-                               nullptr, ff->nsplit, ff->gpart, ff->epart, nl, nl);
+        if (grad && par) CAM_NB(true, true, false);
+        else if (grad) CAM_NB(true, false, false);
+        else if (par) CAM_NB(false, true, false);
+        else CAM_NB(false, false, false);
     }
+#undef CAM_NB
     CAM_LAUNCH_CHECK();
     dim3 gb(nbx, nbeads);
     if (grad && par)
-        hipLaunchKernelGGL((bonded_kernel<true, true>), gb, dim3(kTile), 0, s, ff->d, th, x, v, ff->nsplit, ff->gpart, g,
+        hipLaunchKernelGGL((bonded_kernel<true, true>), gb, dim3(kTile), 0, s, ff->d, th, x, v, nsplit, ff->gpart, g,
                            ff->epart, nblk_nb);
     else if (grad)
-        hipLaunchKernelGGL((bonded_kernel<true, false>), gb, dim3(kTile), 0, s, ff->d, th, x, nullptr, ff->nsplit,
+        hipLaunchKernelGGL((bonded_kernel<true, false>), gb, dim3(kTile), 0, s, ff->d, th, x, nullptr, nsplit,
                            ff->gpart, g, ff->epart, nblk_nb);
     else if (par)
-        hipLaunchKernelGGL((bonded_kernel<false, true>), gb, dim3(kTile), 0, s, ff->d, th, x, v, ff->nsplit, ff->gpart,
+        hipLaunchKernelGGL((bonded_kernel<false, true>), gb, dim3(kTile), 0, s, ff->d, th, x, v, nsplit, ff->gpart,
                            g, ff->epart, nblk_nb);
     else
-        hipLaunchKernelGGL((bonded_kernel<false, false>), gb, dim3(kTile), 0, s, ff->d, th, x, nullptr, ff->nsplit,
+        hipLaunchKernelGGL((bonded_kernel<false, false>), gb, dim3(kTile), 0, s, ff->d, th, x, nullptr, nsplit,
                            ff->gpart, g, ff->epart, nblk_nb);
     CAM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ff_reduce_kernel, dim3(nbeads), dim3(32), 0, s, ff->epart, nblk_nb, nbx, nbeads, e, dth, dvth);
+    hipLaunchKernelGGL(ff_reduce_kernel, dim3(par ? kNE : 1, nbeads), dim3(64), 0, s, ff->epart, nblk_nb, nbx, nbeads, e,
+                       dth, dvth);
     CAM_LAUNCH_CHECK();
     return CAM_OK;
 }
@@ -872,6 +1360,7 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out)
     d.rc = desc->rcut;
     d.za = desc->zahn_a;
     d.zp = desc->zahn_par;
+    erfc_fit(d);
     std::memcpy(d.sr42, desc->sr42, sizeof d.sr42);
     std::memcpy(d.r0bj, desc->r0bj, sizeof d.r0bj);
     std::memcpy(d.zrep, desc->zrep, sizeof d.zrep);
@@ -882,6 +1371,11 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out)
     d.angles = ff->angles;
     d.csr_ptr = ff->csr_ptr;
     d.csr_term = ff->csr_term;
+    const int prc = pairs_create(ff.get());
+    if (prc != CAM_OK) {
+        cam_ff_destroy(ff.release());
+        return prc;
+    }
     *out = ff.release();
     return CAM_OK;
 }
@@ -999,11 +1493,8 @@ int cam_ff_destroy(cam_ff *ff)
         if (p) (void)hipFree(p);
     if (ff->gpart) (void)hipFree(ff->gpart);
     if (ff->epart) (void)hipFree(ff->epart);
-    int *lb[] = { ff->nbr, ff->nofs };
-    for (int *p : lb)
-        if (p) (void)hipFree(p);
-    if (ff->cen) (void)hipFree(ff->cen);
-    if (ff->dev) (void)hipFree(ff->dev);
+    for (void *p : ff->pl_allocs) (void)hipFree(p);
+    if (ff->pl.xref) (void)hipFree(ff->pl.xref);
     delete ff;
     return CAM_OK;
 }

@@ -265,6 +265,64 @@ def test_neighbour_list_equals_all_pairs_bitwise(prec, sigma):
     ff.close()
 
 
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+def test_cell_sorted_rows_equal_all_pairs_bitwise_large_box(prec):
+    """A box wide enough for the cell search (4000 molecules, 12000 atoms,
+    49.2 angstrom, four cells per side): rows and all pairs give the same
+    bits, and the FP64 energy matches the oracle on one bead."""
+    import torch
+    s, ff, fo = _box(4000, 49.2, 8, 10.0)
+    th = ffbox.theta_ref()
+    xb = _beads(s, 4, 3, sigma=0.1)
+    x = torch.tensor(xb, device="cuda")
+    v = torch.tensor(np.random.default_rng(4).normal(size=tuple(x.shape)), device="cuda")
+    out = {}
+    for mode in (cam._abi.CAM_FF_PAIRS_ALL, cam._abi.CAM_FF_PAIRS_LIST):
+        ff.set_pair_mode(mode)
+        e, g, dth, dvth = ff.forces(th, x, v=v, want_params=True, precision=prec)
+        out[mode] = [t.cpu().numpy() for t in (e, g, dth, dvth)]
+    for a, b in zip(out[cam._abi.CAM_FF_PAIRS_ALL], out[cam._abi.CAM_FF_PAIRS_LIST]):
+        assert np.array_equal(a, b)
+    if prec == cam.CAM_PREC_FP64:
+        er, gr, _, _ = fo.eval(th, xb[1])
+        assert abs(out[0][0][1] - er) < 1e-10 * abs(er)
+        np.testing.assert_allclose(out[0][1][1], gr, rtol=0, atol=1e-10 * np.abs(gr).max())
+    ff.close()
+
+
+def test_rows_rebuilt_only_past_half_skin_and_stay_exact():
+    """A sequence of calls on one force field: small moves (every bead by
+    0.35 bohr, under half the 0.8 bohr skin: the rows of the first call are
+    kept), then some atoms by 0.6 bohr (past it: rebuilt), then a new bead
+    count.  Every call equals the all-pairs loop bit for bit and the oracle
+    to 1e-10."""
+    import torch
+    s, ff, fo = _box(1000, 31.0, 4, 10.0)
+    th = ffbox.theta_ref()
+    rng = np.random.default_rng(12)
+    x0 = _beads(s, 4, 5, sigma=0.1)
+
+    def moved(x, frac, length):
+        d = rng.normal(size=x.shape)
+        d *= length / np.linalg.norm(d, axis=-1, keepdims=True)
+        pick = rng.random(x.shape[:2]) < frac
+        return x + d * pick[..., None]
+
+    seq = [x0, moved(x0, 1.0, 0.35), moved(x0, 0.02, 0.6), _beads(s, 2, 9, sigma=0.1)]
+    for k, xb in enumerate(seq):
+        x = torch.tensor(xb, device="cuda")
+        res = []
+        for mode in (cam._abi.CAM_FF_PAIRS_LIST, cam._abi.CAM_FF_PAIRS_ALL):
+            ff.set_pair_mode(mode)
+            e, g, _, _ = ff.forces(th, x)
+            res.append((e.cpu().numpy(), g.cpu().numpy()))
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), k
+        er, gr, _, _ = fo.eval(th, xb[-1])
+        assert abs(res[0][0][-1] - er) < 1e-10 * abs(er), k
+        np.testing.assert_allclose(res[0][1][-1], gr, rtol=0, atol=1e-10 * np.abs(gr).max())
+    ff.close()
+
+
 def test_box_engine_nhc_matches_oracle():
     """The box with the Nose-Hoover chain (thermostat 2): 125 molecules,
     8 beads, momenta 1.5 times too hot; 30 steps against the oracle's ring
