@@ -6,6 +6,7 @@ import pytest
 
 import caracal_amd as cam
 from caracal_amd import workloads
+import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
 
@@ -82,4 +83,48 @@ def test_ragged_batch_with_three_surfaces_in_fp64():
         qb, pb = eb.get_state()
         alone.close()
         assert np.array_equal(qa, qb) and np.array_equal(pa, pb), (surf, nb)
+    rb.close()
+
+
+def test_ragged_8192_mixed_forces_and_fp64_steps_match_oracle():
+    """The ragged batch at its size (8192 trajectories, H3 and CH5 at 4-64
+    beads, 200-1000 K).  Mixed precision: every bucket's start forces on 16
+    sampled trajectories (all their beads) within 1e-5 of the oracle's
+    surfaces relative to the largest |F| of the bucket; per bead, relative to
+    max(|F| of the bead, 1e-3 hartree/bohr), within 1e-4 (binary32 surface
+    arithmetic on beads with small forces reaches ~2e-5).  FP64: 50 steps of
+    the whole batch, two trajectories per bucket against the oracle's verlet
+    to 1e-10."""
+    buckets = workloads.ragged(ntraj=8192, seed=1)
+    assert sum(len(d["beta"]) for _, _, d in buckets) == 8192
+    rng = np.random.default_rng(5)
+    rb = cam.RaggedBatch(buckets, precision=cam.CAM_PREC_MIXED)
+    rb.set_states()
+    worst = {}
+    for (surf, nb, d), eng in zip(buckets, rb.engines):
+        _, g = eng.get_forces()
+        egrad = ol.egrad_h3 if surf == cam.CAM_SURFACE_H3 else ol.egrad_ch4h
+        pick = rng.choice(len(d["beta"]), min(16, len(d["beta"])), replace=False)
+        ref = np.stack([egrad(d["q"][t])[1] for t in pick])
+        dev = np.abs(g[pick] - ref)
+        whole = dev.max() / np.abs(ref).max()
+        per_bead = (dev.reshape(len(pick), nb, -1).max(axis=2)
+                    / np.maximum(np.abs(ref).reshape(len(pick), nb, -1).max(axis=2), 1e-3)).max()
+        worst[(surf, nb)] = (whole, per_bead)
+        assert whole <= 1e-5 and per_bead <= 1e-4, (surf, nb, whole, per_bead)
+    rb.close()
+    print("mixed vs oracle force error (bucket, per bead):",
+          {f"{'H3' if s == cam.CAM_SURFACE_H3 else 'CH4H'}/{n}": f"{a:.1e}/{b:.1e}" for (s, n), (a, b) in worst.items()})
+    rb = cam.RaggedBatch(buckets, precision=cam.CAM_PREC_FP64, steps_per_launch=25)
+    rb.set_states()
+    rb.run(50)
+    for (surf, nb, d), eng in zip(buckets, rb.engines):
+        q, p = eng.get_state()
+        s = ol.make_sys(surf, nb, d["dt"], d["masses"])
+        for t in (0, len(d["beta"]) - 1):
+            tr = ol.OracleTraj(s, d["q"][t], d["p"][t], float(d["beta"][t]), ensemble=0)
+            for _ in range(50):
+                tr.step()
+            assert np.abs(q[t] - tr.q).max() <= 1e-10 * np.abs(tr.q).max(), (surf, nb, t)
+            assert np.abs(p[t] - tr.p).max() <= 1e-8 * np.abs(tr.p).max(), (surf, nb, t)
     rb.close()
