@@ -1694,12 +1694,16 @@ __global__ __launch_bounds__(256) void box_ke_kernel(const double *p, const doub
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void box_nhc_kernel(const double *part, int nparts, double *st, double beta, double nose_q, int natoms,
-                               int nb, double dt)
+__global__ __launch_bounds__(64) void box_nhc_kernel(const double *part, int nparts, double *st, double beta,
+                                                     double nose_q, int natoms, int nb, double dt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // the kinetic-energy parts: lane k takes parts k, k + 64, ..., then
+    // butterflies (a fixed order); lane 0 runs the chain
+    const int lane = threadIdx.x;
     double ek = 0.0;
-    for (int i = 0; i < nparts; ++i) ek += part[i];
+    for (int i = lane; i < nparts; i += 64) ek += part[i];
+    for (int o = 32; o > 0; o >>= 1) ek += __shfl_xor(ek, o);
+    if (lane != 0 || blockIdx.x != 0) return;
     const double kelvin = 1.0 / (beta * 0.316679e-5);
     const double qterm = 0.316679e-5 * kelvin * nose_q * nose_q;    // mdinit (src/mdinit.f90:139-146)
     const int nfree = 3 * natoms;

@@ -824,30 +824,44 @@ __global__ void qm_wrap_kernel(QDev f, double *x, int nbeads)
     if (t >= 3 * f.n) return;
     const int i = t / 3, c = t % 3;
     const double L = f.box[c];
+    // the shifts so far, in registers (every index below is a constant)
     double sh[kWrapMaxShift];
+#pragma unroll
+    for (int s = 0; s < kWrapMaxShift; ++s) sh[s] = 0.0;
     int ns = 0;
     bool over = false;
-    for (int k0 = 0; k0 < nbeads; k0 += kWrapChunk) {
+    auto push = [&](double d) {
+        if (ns == kWrapMaxShift) {
+            over = true;
+            return;
+        }
+#pragma unroll
+        for (int s = 0; s < kWrapMaxShift; ++s)
+            if (s == ns) sh[s] = d;
+        ++ns;
+    };
+    auto shifted = [&](double w) {
+#pragma unroll
+        for (int s = 0; s < kWrapMaxShift; ++s)
+            if (s < ns) w = w + sh[s];
+        return w;
+    };
+    for (int k0 = 0; k0 < nbeads && !over; k0 += kWrapChunk) {
         double v[kWrapChunk];
 #pragma unroll
         for (int u = 0; u < kWrapChunk; ++u)
             v[u] = k0 + u < nbeads ? x[((size_t)(k0 + u) * f.n + i) * 3 + c] : 0.5 * L;
-        for (int u = 0; u < kWrapChunk && k0 + u < nbeads; ++u) {
-            double w = v[u];
-            for (int s = 0; s < ns; ++s) w = w + sh[s];
+#pragma unroll
+        for (int u = 0; u < kWrapChunk; ++u) {
+            if (k0 + u >= nbeads || over) continue;
+            double w = shifted(v[u]);
             while (w < 0.0 && !over) {
-                if (ns == kWrapMaxShift) over = true;
-                else {
-                    sh[ns++] = L;
-                    w = w + L;
-                }
+                push(L);
+                if (!over) w = w + L;
             }
             while (w > L && !over) {
-                if (ns == kWrapMaxShift) over = true;
-                else {
-                    sh[ns++] = -L;
-                    w = w - L;
-                }
+                push(-L);
+                if (!over) w = w - L;
             }
         }
     }
@@ -868,9 +882,7 @@ __global__ void qm_wrap_kernel(QDev f, double *x, int nbeads)
     if (ns == 0) return;
     for (int k = 0; k < nbeads; ++k) {
         double *p = x + ((size_t)k * f.n + i) * 3 + c;
-        double w = *p;
-        for (int s = 0; s < ns; ++s) w = w + sh[s];
-        *p = w;
+        *p = shifted(*p);
     }
 }
 
