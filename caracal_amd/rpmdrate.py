@@ -149,6 +149,32 @@ def h_h2_example(scale=1.0, kelvin=300.0, seed=0):
     return r.run(ts, float(egrad_h3(ts[None])[0][0]))
 
 
+def h_h2_kappa(seed, nm_mode, scale=1.0, kelvin=300.0, xi_barrier=1.0):
+    """The recrossing part of examples/calc_rate/h+h2/rate.key alone: parents
+    started at the BKMP2 saddle, constrained to xi_barrier, equilibrated for
+    50,000 steps, then 100 cycles of 100 children x 500 steps every 1000
+    parent steps (step counts times scale).  nm_mode: the normal-mode basis
+    of every engine (CAM_NM_REFERENCE, the reference's cosine transform, or
+    CAM_NM_ORTHOGONAL).  Returns kappa and kappa(t)."""
+    import functools
+
+    from . import CAM_SURFACE_H3, RPMDEngine, workloads
+    from .units import FS_TO_AU, beta_of, mass_au
+
+    def n(x):
+        return max(1, int(round(x * scale)))
+
+    nb = 8
+    factory = functools.partial(RPMDEngine, nm_mode=nm_mode)
+    rec = Recrossing(CAM_SURFACE_H3, nb, 0.1 * FS_TO_AU, mass_au([1.008] * 3),
+                     _with_xi_ideal(workloads.h3_reaction(), xi_barrier), float(beta_of(kelvin)), child_point=100,
+                     child_evol=n(500), child_interv=n(1000), child_times=max(1, n(100)), recr_equi=n(50000),
+                     seed=seed, engine_factory=factory)
+    q0 = np.broadcast_to(workloads.H3_TS, (1, nb) + workloads.H3_TS.shape).copy()
+    res = rec.run(q0)
+    return dict(kappa=float(res["kappa"]), kappa_t=res["kappa_t"], errors=res["errors"])
+
+
 if __name__ == "__main__":
     import argparse
     import json
@@ -157,7 +183,23 @@ if __name__ == "__main__":
     ap.add_argument("--scale", type=float, default=1.0, help="multiply every step count (1 = the key file's)")
     ap.add_argument("--kelvin", type=float, default=300.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--kappa-seeds", type=int, default=0,
+                    help="run only the recrossing at xi = 1 for this many seeds in both normal-mode bases")
     args = ap.parse_args()
+    if args.kappa_seeds:
+        from ._abi import CAM_NM_ORTHOGONAL, CAM_NM_REFERENCE
+        out = {}
+        for name, mode in (("reference", CAM_NM_REFERENCE), ("orthogonal", CAM_NM_ORTHOGONAL)):
+            ks = []
+            for sd in range(args.kappa_seeds):
+                r = h_h2_kappa(args.seed + sd, mode, args.scale, args.kelvin)
+                ks.append(r["kappa"])
+                print(json.dumps(dict(nm=name, seed=args.seed + sd, kappa=r["kappa"], errors=r["errors"])), flush=True)
+            ks = np.array(ks)
+            out[name] = dict(kappa_mean=float(ks.mean()), kappa_sem=float(ks.std(ddof=1) / np.sqrt(len(ks))),
+                             kappas=[float(k) for k in ks])
+        print(json.dumps(out))
+        raise SystemExit(0)
     out = h_h2_example(args.scale, args.kelvin, args.seed)
     print(json.dumps(dict(kelvin=args.kelvin, scale=args.scale, xi_barrier=out["xi_barrier"],
                           dW_kcal_mol=(out["pmf_max"] - out["pmf_min"]) * 627.5094743, kappa=out["kappa"],
