@@ -162,8 +162,8 @@ def energy_drift(cam, workloads, nsteps=20000):
 
 def ragged_batch(cam, workloads, steps=200):
     """8192 trajectories, half H3 and half CH4H, nbeads uniform on {4..64},
-    T uniform on [200, 1000] K, NVE, as one concurrent batch (one engine per
-    bucket).  Throughput in FP64 and mixed; mixed-vs-FP64 force error per
+    T uniform on [200, 1000] K, NVE, as one batch: one engine per bucket, a
+    run one captured graph with a branch per bucket.  Throughput in FP64 and mixed; mixed-vs-FP64 force error per
     (surface, nbeads) bucket on the start state."""
     import torch
     buckets = workloads.ragged(ntraj=8192, seed=1)

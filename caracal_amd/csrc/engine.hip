@@ -1643,6 +1643,7 @@ struct cam_engine {
     Kernels kern;                     // kernels of the current schedule
     bool split = false;               // split schedule: force phase in its own kernel
     std::map<int, hipGraphExec_t> graphs;
+    unsigned long graph_gen = 0;      // counts graph drops: graphs of a batch that hold this engine's launches
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;            // step launches of the last cam_engine_run
@@ -1854,6 +1855,7 @@ int run_forces(cam_engine *e)
 // first: no executable graph is destroyed while the GPU runs it.
 void drop_graphs(cam_engine *e)
 {
+    ++e->graph_gen;
     if (e->graphs.empty()) return;
     (void)hipStreamSynchronize(e->stream);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -2750,23 +2752,39 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
 }
 
 // ---------------------------------------------------------------------------
-// Ragged batches: one engine per (surface, nbeads) bucket, each on its own
-// stream, launched longest-first and joined on a batch stream.
+// Ragged batches: one engine per (surface, nbeads) bucket.  A run of the
+// batch is one captured graph: forked from the batch stream, every bucket's
+// steps on its own stream (branches in decreasing work estimate), joined
+// back; the graph is replayed like an engine's and captured again when an
+// engine dropped its own graphs (buffers or launch arguments changed).
 // ---------------------------------------------------------------------------
 struct cam_batch {
     std::vector<cam_engine *> eng;
     std::vector<int> order;            // launch order, decreasing work estimate
     std::vector<hipEvent_t> done;
-    hipStream_t stream = nullptr;      // join stream
-    hipEvent_t start = nullptr, stop = nullptr;
+    hipStream_t stream = nullptr;      // fork and join stream
+    hipEvent_t start = nullptr, stop = nullptr, fork = nullptr;
+    std::map<int, hipGraphExec_t> graphs;   // by steps per replay
+    std::vector<unsigned long> gens;        // the engines' graph generations the graphs were captured at
     bool pending = false;
     double last_ms = 0.0;
 };
+
+static void batch_drop_graphs(cam_batch *b)
+{
+    if (b->graphs.empty()) return;
+    (void)hipStreamSynchronize(b->stream);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
 
 int cam_batch_destroy(cam_batch *b)
 {
     if (!b) return CAM_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (cam_engine *e : b->eng) cam_engine_destroy(e);
     for (hipEvent_t ev : b->done)
         if (ev) (void)hipEventDestroy(ev);
@@ -2805,6 +2823,7 @@ int cam_batch_create(const cam_config *cfgs, int nbuckets, cam_batch **out)
     hipError_t err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipEventCreate(&b->start);
     if (err == hipSuccess) err = hipEventCreate(&b->stop);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     for (int i = 0; err == hipSuccess && i < nbuckets; ++i) err = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming);
     if (err != hipSuccess) {
         set_error("HIP error: %s", hipGetErrorString(err));
@@ -2833,21 +2852,89 @@ int cam_batch_autotune(cam_batch *b, int trial_steps)
     return CAM_OK;
 }
 
+// The graph of one replay of nsteps steps of every bucket.
+static int batch_graph(cam_batch *b, int nsteps, hipGraphExec_t *out)
+{
+    bool stale = b->gens.size() != b->eng.size();
+    for (size_t i = 0; !stale && i < b->eng.size(); ++i) stale = b->gens[i] != b->eng[i]->graph_gen;
+    if (stale) {
+        batch_drop_graphs(b);
+        b->gens.resize(b->eng.size());
+        for (size_t i = 0; i < b->eng.size(); ++i) b->gens[i] = b->eng[i]->graph_gen;
+    }
+    auto it = b->graphs.find(nsteps);
+    if (it == b->graphs.end()) {
+        CAM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = CAM_OK;
+        hipError_t err = hipEventRecord(b->fork, b->stream);
+        for (int i : b->order) {
+            if (err != hipSuccess || rc != CAM_OK) break;
+            cam_engine *e = b->eng[i];
+            err = hipStreamWaitEvent(e->stream, b->fork, 0);
+            if (err != hipSuccess) break;
+            rc = enqueue_run(e, nsteps);
+            // the branch is joined even after a failure, so the capture ends cleanly
+            hipError_t je = hipEventRecord(b->done[i], e->stream);
+            if (je == hipSuccess) je = hipStreamWaitEvent(b->stream, b->done[i], 0);
+            if (err == hipSuccess) err = je;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+        if (rc != CAM_OK || err != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc != CAM_OK) return rc;
+            CAM_HIP(err != hipSuccess ? err : ce);
+        }
+        hipGraphExec_t exec;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CAM_HIP(ie);
+        it = b->graphs.emplace(nsteps, exec).first;
+    }
+    *out = it->second;
+    return CAM_OK;
+}
+
 int cam_batch_run(cam_batch *b, int nsteps)
 {
     CAM_CHECK_ARG(b, "null batch");
     CAM_CHECK_ARG(nsteps >= 0, "nsteps must be >= 0");
-    clear_error();
-    CAM_HIP(hipEventRecord(b->start, b->stream));
-    for (int i : b->order) {
-        cam_engine *e = b->eng[i];
-        CAM_HIP(hipStreamWaitEvent(e->stream, b->start, 0));
-        const int rc = cam_engine_run(e, nsteps);
-        if (rc != CAM_OK) return rc;
-        CAM_HIP(hipEventRecord(b->done[i], e->stream));
-        CAM_HIP(hipStreamWaitEvent(b->stream, b->done[i], 0));
+    for (cam_engine *e : b->eng) {
+        CAM_CHECK_ARG(e->have_state, "no state uploaded");
+        CAM_CHECK_ARG(e->cfg.ensemble != CAM_ENS_PARENT && e->cfg.ensemble != CAM_ENS_UMBRELLA,
+                      "batches run NVE, Andersen, Nose-Hoover and child engines");
     }
+    clear_error();
+    if (nsteps == 0) return CAM_OK;
+    // replays of at most kGraphLaunches step launches of every bucket
+    int chunk = nsteps;
+    for (cam_engine *e : b->eng) chunk = std::min(chunk, (e->split ? 1 : e->spl) * kGraphLaunches);
+    const int reps = nsteps / chunk, tail = nsteps % chunk;
+    for (cam_engine *e : b->eng)
+        if (e->cfg.ensemble == CAM_ENS_CHILD && (chunk > e->flag_cap || e->child_steps + nsteps > e->num_cap)) {
+            // the buffers grow: nothing may still use them
+            CAM_HIP(hipStreamSynchronize(b->stream));
+            const int rc = ensure_child_buffers(e, chunk, e->child_steps + nsteps);
+            if (rc != CAM_OK) return rc;
+        }
+    hipGraphExec_t g = nullptr, gt = nullptr;
+    int rc = batch_graph(b, chunk, &g);
+    if (rc == CAM_OK && tail) rc = batch_graph(b, tail, &gt);
+    if (rc != CAM_OK) return rc;
+    CAM_HIP(hipEventRecord(b->start, b->stream));
+    for (cam_engine *e : b->eng) CAM_HIP(hipEventRecord(e->ev0, b->stream));
+    for (int r = 0; r < reps; ++r) CAM_HIP(hipGraphLaunch(g, b->stream));
+    if (tail) CAM_HIP(hipGraphLaunch(gt, b->stream));
     CAM_HIP(hipEventRecord(b->stop, b->stream));
+    for (cam_engine *e : b->eng) {
+        CAM_HIP(hipEventRecord(e->ev1, b->stream));
+        e->child_steps += nsteps;
+        e->ustat_steps += nsteps;
+        const int per = e->split ? 1 : e->spl;
+        e->last_launches = reps * ((chunk + per - 1) / per) + (tail ? (tail + per - 1) / per : 0);
+        e->last_steps = nsteps;
+        e->last_ms = -1.0;
+    }
     b->pending = true;
     return CAM_OK;
 }

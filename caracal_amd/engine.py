@@ -230,7 +230,7 @@ class RPMDEngine:
 
 class RaggedBatch:
     """Trajectories of different surfaces and bead counts as one batch: one
-    engine per bucket, run concurrently by the native batch layer.
+    engine per bucket; a run is one captured graph with a branch per bucket.
     buckets: list of (surface, nbeads, dict(q, p, beta, masses, dt)) as
     workloads.ragged returns; every bucket gets the same ensemble options."""
 

@@ -335,11 +335,15 @@ int cam_engine_autotune(cam_engine *eng, int trial_steps, cam_tuning *out);
 int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_per_launch, int *launches);
 
 /* Ragged batches (trajectories of different surfaces and bead counts): one
- * engine per bucket of equal (surface, nbeads), each on its own stream.
- * cam_batch_run launches every bucket's graph, longest bucket first by a
- * work estimate (trajectories x beads x surface flops), and joins them; the
- * buckets run concurrently on the GPU.  Engines are reached with
- * cam_batch_engine to set state and read results; the batch owns them.  No
+ * engine per bucket of equal (surface, nbeads).  cam_batch_run replays one
+ * captured graph per run length: forked from the batch stream, every
+ * bucket's steps as a branch on its own stream (longest bucket first by a
+ * work estimate, trajectories x beads x surface flops), joined back; the
+ * buckets run concurrently on the GPU.  The graph is captured again when a
+ * bucket's engine changed schedule or buffers.  Engines are reached with
+ * cam_batch_engine to set state and read results; the batch owns them; a
+ * bucket's engine must not be run on its own while a batch run is in
+ * flight.  Batches hold NVE, Andersen, Nose-Hoover and child engines.  No
  * counterpart in the reference, which runs one trajectory per MPI rank
  * (src/recross.f90:334-415). */
 typedef struct cam_batch cam_batch;

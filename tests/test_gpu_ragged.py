@@ -128,3 +128,58 @@ def test_ragged_8192_mixed_forces_and_fp64_steps_match_oracle():
             assert np.abs(q[t] - tr.q).max() <= 1e-10 * np.abs(tr.q).max(), (surf, nb, t)
             assert np.abs(p[t] - tr.p).max() <= 1e-8 * np.abs(tr.p).max(), (surf, nb, t)
     rb.close()
+
+
+def test_ragged_batch_graph_replays_and_recaptures():
+    """The batch runs as one captured graph per replay length.  Runs of 10,
+    25 and again 10 steps (two graphs, the first replayed) and a new state in
+    between give bit for bit what each bucket gives run alone.  After the
+    autotuner changed the buckets' schedules (each engine drops its graphs,
+    so the batch captures again), a batch run equals each bucket's engine run
+    alone from the same state."""
+    buckets = workloads.ragged(ntraj=300, seed=2)
+    rb = cam.RaggedBatch(buckets, precision=cam.CAM_PREC_FP64, steps_per_launch=5)
+    rb.set_states()
+    alone = []
+    for surf, nb, d in buckets:
+        e = cam.RPMDEngine(surf, nb, len(d["beta"]), d["dt"], d["masses"], precision=cam.CAM_PREC_FP64,
+                           steps_per_launch=5)
+        e.set_state(d["q"], d["p"], d["beta"])
+        alone.append(e)
+
+    def run(n):
+        rb.run(n)
+        for e in alone:
+            e.run(n)
+
+    def same():
+        for (surf, nb, _), eb, ea in zip(buckets, rb.engines, alone):
+            qa, pa = ea.get_state()
+            qb, pb = eb.get_state()
+            assert np.array_equal(qa, qb) and np.array_equal(pa, pb), (surf, nb)
+
+    run(10)
+    run(25)
+    run(10)
+    same()
+    for engs in (rb.engines, alone):                 # momenta reversed
+        for e, (_, _, d) in zip(engs, buckets):
+            q, p = e.get_state()
+            e.set_state(q, -p, d["beta"])
+    run(10)
+    same()
+    for e in alone:
+        e.close()
+    rb.autotune(10)
+    start = [e.get_state() for e in rb.engines]
+    # the state includes the forces: both runs start from set_state's
+    for (_, _, d), e, (q0, p0) in zip(buckets, rb.engines, start):
+        e.set_state(q0, p0, d["beta"])
+    rb.run(20)
+    after = [e.get_state() for e in rb.engines]
+    for (surf, nb, d), e, (q0, p0), (q1, p1) in zip(buckets, rb.engines, start, after):
+        e.set_state(q0, p0, d["beta"])
+        e.run(20)
+        q, p = e.get_state()
+        assert np.array_equal(q, q1) and np.array_equal(p, p1), (surf, nb)
+    rb.close()
