@@ -51,6 +51,24 @@ module caracal_amd_mod
       real(c_double) :: nose_q = 0.0_c_double
    end type cam_config
 
+   ! cam_qmdff_desc: what prepare and set_periodic leave in the qmdff,
+   ! evb_mod and pbc_mod modules (include/caracal_amd.h); the pointers are
+   ! c_loc of those arrays, which must stay allocated while cam_qmdff_create
+   ! runs (it copies them)
+   type, bind(C), public :: cam_qmdff_desc
+      integer(c_int32_t) :: natoms = 0, nbond = 0, nangl = 0, ntors = 0, nhb = 0, nnci = 0
+      type(c_ptr) :: at = c_null_ptr, molnum = c_null_ptr, q = c_null_ptr
+      type(c_ptr) :: bond = c_null_ptr, vbond = c_null_ptr, angl = c_null_ptr, vangl = c_null_ptr
+      type(c_ptr) :: tors = c_null_ptr, vtors = c_null_ptr, hb = c_null_ptr, vhb = c_null_ptr
+      type(c_ptr) :: nci = c_null_ptr, c6xy = c_null_ptr
+      type(c_ptr) :: r094 = c_null_ptr, sr42 = c_null_ptr, zab = c_null_ptr, r0ab = c_null_ptr
+      real(c_double) :: eps1(6) = 0.d0, eps2(6) = 0.d0
+      type(c_ptr) :: rad = c_null_ptr, scalehb = c_null_ptr
+      integer(c_int32_t) :: periodic = 0, zahn = 0
+      real(c_double) :: box(3) = 0.d0
+      real(c_double) :: coul_cut = 0.d0, vdw_cut = 0.d0, cut_low = 0.d0, zahn_a = 0.d0, zahn_par = 0.d0
+   end type cam_qmdff_desc
+
    integer, parameter, public :: CAM_MAX_TRIALS = 32
    type, bind(C), public :: cam_tuning
       integer(c_int32_t) :: n = 0, chosen = 0
@@ -268,6 +286,72 @@ module caracal_amd_mod
          integer(c_int), value :: mode
          integer(c_int) :: rc
       end function
+      ! the reference's QMDFF (ff_eg + ff_nonb + ff_hb) as a force field
+      function cam_qmdff_create(desc, ff) bind(C, name="cam_qmdff_create") result(rc)
+         import :: c_int, c_ptr, cam_qmdff_desc
+         type(cam_qmdff_desc), intent(in) :: desc
+         type(c_ptr), intent(out) :: ff
+         integer(c_int) :: rc
+      end function
+      ! ring-polymer dynamics of one box; q, p as q(3,natoms,nbeads)
+      function cam_box_create(ff, nbeads, dt, mass, beta, nm_mode, precision, andersen_step, seed, box) &
+            bind(C, name="cam_box_create") result(rc)
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: ff
+         integer(c_int), value :: nbeads, nm_mode, precision, andersen_step
+         real(c_double), value :: dt, beta
+         real(c_double), intent(in) :: mass(*)
+         integer(c_int64_t), value :: seed
+         type(c_ptr), intent(out) :: box
+         integer(c_int) :: rc
+      end function
+      function cam_box_destroy(box) bind(C, name="cam_box_destroy") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: box
+         integer(c_int) :: rc
+      end function
+      function cam_box_set_state(box, q, p) bind(C, name="cam_box_set_state") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: box
+         real(c_double), intent(in) :: q(*), p(*)
+         integer(c_int) :: rc
+      end function
+      function cam_box_get_state(box, q, p) bind(C, name="cam_box_get_state") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: box
+         real(c_double), intent(out) :: q(*), p(*)
+         integer(c_int) :: rc
+      end function
+      function cam_box_set_nhc(box, nose_q) bind(C, name="cam_box_set_nhc") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: box
+         real(c_double), value :: nose_q
+         integer(c_int) :: rc
+      end function
+      function cam_box_set_transrot(box, on) bind(C, name="cam_box_set_transrot") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: box
+         integer(c_int), value :: on
+         integer(c_int) :: rc
+      end function
+      function cam_box_run(box, nsteps) bind(C, name="cam_box_run") result(rc)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: box
+         integer(c_int), value :: nsteps
+         integer(c_int) :: rc
+      end function
+      function cam_box_sync(box, ms) bind(C, name="cam_box_sync") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: box
+         real(c_double), intent(out) :: ms
+         integer(c_int) :: rc
+      end function
+      function cam_box_energies(box, epot_beads, ekin, espring) bind(C, name="cam_box_energies") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: box
+         real(c_double), intent(out) :: epot_beads(*), ekin, espring
+         integer(c_int) :: rc
+      end function
    end interface
 
    public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h, cam_egrad_oh3
@@ -278,6 +362,8 @@ module caracal_amd_mod
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode
+   public :: cam_qmdff_create, cam_box_create, cam_box_destroy, cam_box_set_state, cam_box_get_state
+   public :: cam_box_set_nhc, cam_box_set_transrot, cam_box_run, cam_box_sync, cam_box_energies
    public :: egrad_h3_gpu, egrad_ch4h_gpu, egrad_oh3_gpu, cam_error_message
 
 contains

@@ -392,17 +392,19 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out);
  * parameter derivatives are not available. */
 int cam_water_create(int nwater, int periodic, double box, int zahn, double rcut, cam_ff **out);
 int cam_ff_destroy(cam_ff *ff);
-/* How the QMDFF-form nonbonded pairs are found.  LIST (default): a
- * neighbour list rebuilt on every call from the bead centroids, holding every
- * pair of different molecules whose centroids are within rc + d_i + d_j
- * (d = the largest bead distance from its atom's centroid), so that no pair
- * inside the cutoff in any bead is missed; the pairs are then evaluated in
- * the all-pairs loop's order and the results are the same bit for bit.  Up
- * to 16384 atoms (the list takes natoms^2 ints); larger boxes use ALL.  ALL:
- * every pair tested, as the reference's loop (src/ff_nonb.f90:198-332).  The
- * mode applies to work enqueued afterwards (a box engine's captured graphs
- * keep the mode they were captured with).  No counterpart in the reference,
- * which has no neighbour list. */
+/* How the QMDFF-form nonbonded pairs are found.  The atoms are kept in a
+ * spatial order (cells of their bead centroids) that is rebuilt on the
+ * device when some bead has moved more than half a 1.2 bohr skin since the
+ * last build, or the bead count changed.  LIST (default): per atom a row of
+ * every atom of another molecule whose centroid was within rc + skin + d_i +
+ * d_j (d = the largest bead distance from its atom's centroid) at the
+ * build, so that no pair inside the cutoff in any bead is missed.  ALL:
+ * every pair tested, as the reference's loop (src/ff_nonb.f90:198-332).  Both
+ * meet the pairs inside the cutoff in the same order and give the same
+ * results bit for bit.  The mode applies to work enqueued afterwards (a box
+ * engine's captured graphs keep the mode they were captured with).  Calls on
+ * one force field from several streams at once race on its pair structure.
+ * No counterpart in the reference, which has no neighbour list. */
 enum { CAM_FF_PAIRS_LIST = 0, CAM_FF_PAIRS_ALL = 1 };
 int cam_ff_set_pair_mode(cam_ff *ff, int mode);
 /* Energy of nbeads replicas: x device [nbeads][natoms][3] bohr, e device

@@ -71,3 +71,31 @@ def test_fortran_module_binds_declared_names_only():
     for n in ("cam_engine_create", "cam_engine_run", "cam_engine_kappa", "cam_engine_autotune",
               "cam_batch_create", "cam_batch_run", "cam_batch_sync", "cam_water_create"):
         assert n in bound, n
+
+
+def test_fortran_qmdff_desc_layout_matches_c(tmp_path):
+    """The Fortran bind(C) cam_qmdff_desc has the C struct's size and field
+    offsets (checked through the ctypes mirror of the header's struct)."""
+    fc = "amdflang"
+    mod_dir = os.path.join(ROOT, "caracal_amd", "fortran")
+    if subprocess.run(["which", fc], capture_output=True).returncode != 0:
+        pytest.skip("amdflang is not installed")
+    if not os.path.exists(os.path.join(mod_dir, "caracal_amd_mod.mod")):
+        pytest.fail("caracal_amd/fortran is not built")
+    fields = [f[0] for f in _abi.QmdffDesc._fields_]
+    lines = ["program layout", "use iso_c_binding", "use caracal_amd_mod",
+             "type(cam_qmdff_desc), target :: d", "integer(c_intptr_t) :: b",
+             "b = transfer(c_loc(d), b)", "print *, 'size', c_sizeof(d)"]
+    for f in fields:
+        lines.append(f"print *, '{f}', transfer(c_loc(d%{f}), b) - b")
+    lines.append("end program layout")
+    src = tmp_path / "layout.f90"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    # only the type is used: the module's .mod file, no object code
+    subprocess.run([fc, "-I", mod_dir, str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = dict(zip(out[0::2], (int(v) for v in out[1::2])))
+    assert got["size"] == ctypes.sizeof(_abi.QmdffDesc)
+    for f in fields:
+        assert got[f] == getattr(_abi.QmdffDesc, f).offset, f
