@@ -83,11 +83,11 @@ def step_flops_parts(natoms, nbeads, folded=False):
 
 
 def prepare_children(cam, workloads, ntraj, nbeads, precision, steps_per_launch=0, seed=0,
-                     nm_kernel=None, schedule=0):
+                     nm_kernel=None, schedule=0, waves_per_eu=0):
     w = workloads.ch4h_children(ntraj=ntraj, nbeads=nbeads, seed=seed)
     e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nbeads, ntraj, w["dt"], w["masses"], precision=precision,
                        ensemble=cam.CAM_ENS_CHILD, reaction=w["reaction"], seed=seed + 17,
-                       steps_per_launch=steps_per_launch, schedule=schedule,
+                       steps_per_launch=steps_per_launch, schedule=schedule, waves_per_eu=waves_per_eu,
                        nm_kernel=cam.CAM_NMK_AUTO if nm_kernel is None else nm_kernel)
     e.set_state(w["q"], w["p"], w["beta"])
     st = e.constrain(project_p=False)
@@ -480,8 +480,9 @@ def main():
                     help="child steps per host core of the CPU baseline (0: skip it)")
     ap.add_argument("--skip-extras", action="store_true", help="only the main FP64 batched measurement")
     ap.add_argument("--schedule", default=None,
-                    help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH' (fused) or 'valu|mfma,split' instead of "
-                         "the autotuner (for profiling: every launch then has the same shape)")
+                    help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH[,WAVES_PER_EU]' (fused) or "
+                         "'valu|mfma,split' instead of the autotuner (for profiling: every launch then has the same "
+                         "shape)")
     ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "force_matching"],
                     help="run only this extra measurement and print its result")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -519,12 +520,16 @@ def main():
 
     ntraj, nb = args.ntraj, args.nbeads
     if args.schedule:
-        unit, spl_fixed = args.schedule.split(",")
+        parts = args.schedule.split(",")
+        unit, spl_fixed = parts[0], parts[1]
+        wpe = int(parts[2]) if len(parts) > 2 else 1
         nmk = {"mfma": cam.CAM_NMK_MFMA, "folded": cam.CAM_NMK_FOLDED}.get(unit, cam.CAM_NMK_VALU)
         split = spl_fixed == "split"
         e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64, steps_per_launch=0 if split else int(spl_fixed),
-                                nm_kernel=nmk, schedule=cam.CAM_SCHED_SPLIT if split else cam.CAM_SCHED_FUSED)
-        trials, best = [], {"nm_kernel": nmk, "split": int(split), "steps_per_launch": 1 if split else int(spl_fixed)}
+                                nm_kernel=nmk, schedule=cam.CAM_SCHED_SPLIT if split else cam.CAM_SCHED_FUSED,
+                                waves_per_eu=wpe)
+        trials, best = [], {"nm_kernel": nmk, "split": int(split), "waves_per_eu": wpe,
+                            "steps_per_launch": 1 if split else int(spl_fixed)}
     else:
         e, w = prepare_children(cam, workloads, ntraj, nb, cam.CAM_PREC_FP64)
         trials, best = e.autotune(100)
@@ -546,7 +551,7 @@ def main():
         kernel = (f"split step: step_kernel<SurfCH4H, {nb}, double, CHILD, {nmk_name.upper()} normal modes, "
                   f"no forces> + force_kernel<SurfCH4H, {nb}, double>")
     elif folded:
-        kernel = f"fold_step_kernel<SurfCH4H, {nb}, double, CHILD>"
+        kernel = f"fold_step_kernel<SurfCH4H, {nb}, double, CHILD, {best.get('waves_per_eu', 1)}>"
     else:
         kernel = f"step_kernel<SurfCH4H, {nb}, double, CHILD, {nmk_name.upper()} normal modes>"
     distinct = nb // 2 + 1 if folded else nb

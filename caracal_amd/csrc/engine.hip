@@ -1674,8 +1674,8 @@ step_fn pick_step_w(int ens, bool mfma)
 }
 
 // wpe: minimum waves per SIMD the compiler is asked to fit (1 or 2).  Only
-// the FP64 CH5 step offers 2: it needs more than 256 registers at 1, and at 2
-// it spills; which is faster is measured (cam_engine_autotune).
+// the FP64 CH5 step offers 2 (here and in the folded kernel): it needs more
+// than 256 registers at 1; which is faster is measured (cam_engine_autotune).
 template <class S, int NB, typename T>
 step_fn pick_step(int ens, int wpe, bool mfma)
 {
@@ -1789,7 +1789,7 @@ bool fold_available(const cam_config *cfg)
 {
     int tpb = 0;
     return fold_eligible(cfg) &&
-           fold_step_kernel_for(cfg->surface, cfg->precision, cfg->nbeads, cfg->ensemble, &tpb) != nullptr;
+           fold_step_kernel_for(cfg->surface, cfg->precision, cfg->nbeads, cfg->ensemble, 1, &tpb) != nullptr;
 }
 
 bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
@@ -1817,7 +1817,7 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
     k.step_tpb = e->tpb;
     if (ok && nmk == CAM_NMK_FOLDED) {
         int tpb = 0;
-        const step_fn f = fold_eligible(cfg) ? fold_step_kernel_for(cfg->surface, cfg->precision, nb, ens, &tpb) : nullptr;
+        const step_fn f = fold_eligible(cfg) ? fold_step_kernel_for(cfg->surface, cfg->precision, nb, ens, wpe, &tpb) : nullptr;
         if (!f) return false;
         k.step = f;
         k.step_tpb = tpb;
@@ -2155,8 +2155,10 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     if (cfg->remove_com) e->spl = 1;
     if (cfg->nm_kernel != CAM_NMK_AUTO) e->nmk = cfg->nm_kernel;
     else e->nmk = fold_available(cfg) ? CAM_NMK_FOLDED : CAM_NMK_VALU;
-    e->wpe = cfg->waves_per_eu == 2 && surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64 &&
-                     e->nmk != CAM_NMK_FOLDED ? 2 : 1;
+    // the FP64 CH5 folded kernel is faster at two waves per SIMD, the unfolded
+    // step kernels at one (they spill at two)
+    const bool ch4h64 = surf == CAM_SURFACE_CH4H && cfg->precision == CAM_PREC_FP64;
+    e->wpe = ch4h64 && (cfg->waves_per_eu == 2 || (cfg->waves_per_eu == 0 && e->nmk == CAM_NMK_FOLDED)) ? 2 : 1;
     // windows run fused unless split is asked for or MFMA normal modes are
     // (their fused step has the vector-ALU transform only)
     e->split = cfg->schedule == CAM_SCHED_SPLIT || cfg->ensemble == CAM_ENS_PARENT ||
@@ -2650,8 +2652,9 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
     const bool try_fused = c.schedule != CAM_SCHED_SPLIT && !parent;
     const bool try_split = c.schedule != CAM_SCHED_FUSED && !c.remove_com && !nhc;
     for (int nmk : nmks) {
-        if (nmk == CAM_NMK_FOLDED) {       // fused only, one budget
-            for (int spl : spls) cand.push_back({ nmk, 1, spl, 0, 0.0 });
+        if (nmk == CAM_NMK_FOLDED) {       // fused only
+            for (int wpe : wpes)
+                for (int spl : spls) cand.push_back({ nmk, wpe, spl, 0, 0.0 });
             continue;
         }
         if (try_fused && !(vfused && nmk != CAM_NMK_VALU))

@@ -119,7 +119,7 @@ typedef void (*step_fn)(StepArgs);
 
 // The folded step kernel (fold.hip) of a configuration, or nullptr if it has
 // none; *tpb receives its trajectories per block.
-step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int *tpb);
+step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int *tpb);
 
 // Folded reference transform M[r][c] = w_c cos(2 pi r c / nbeads) / sqrt(nbeads),
 // r, c < nbeads/2, w_0 = 1 and w_c = 2 otherwise, row-major.

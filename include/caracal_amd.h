@@ -181,7 +181,8 @@ typedef struct {
     int32_t steps_per_launch; /* steps fused into one kernel launch; 0 = pick automatically */
     int32_t nm_kernel;        /* CAM_NMK_* */
     int32_t waves_per_eu;     /* register budget of the step kernel: 1 = up to 512 registers per lane,
-                                 2 = 256 (two waves per SIMD; FP64 CH4H only); 0 = 1 unless tuned */
+                                 2 = 256 (two waves per SIMD; FP64 CH4H only); 0 = 2 for the FP64 CH4H
+                                 folded kernel, 1 otherwise, unless tuned */
     int32_t schedule;         /* CAM_SCHED_* */
     int32_t reserved;         /* must be 0 */
     double dt;                /* time step, atomic time units */

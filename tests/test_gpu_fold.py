@@ -187,6 +187,37 @@ def test_every_folded_kernel_is_deterministic_and_tracks_the_unfolded_one(surfac
         np.testing.assert_array_equal(ka[0][0], kb[0][0])
 
 
+@pytest.mark.parametrize("nb", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("ens", [cam.CAM_ENS_NVE, cam.CAM_ENS_CHILD])
+def test_folded_ch4h_fp64_register_budgets_give_the_same_bits(nb, ens):
+    """The FP64 CH5 folded kernel at both register budgets (one and two
+    waves per SIMD) over 30 steps in 10-step launches, on three blocks: the
+    same arithmetic, so the same bits, and within rounding of the unfolded
+    vector-ALU kernel."""
+    tpb = 64 // (nb // 2 + 1)
+    ntraj = 2 * tpb + 1
+    w, q, p, beta, rx = _batch(cam.CAM_SURFACE_CH4H, nb, ntraj, seed=3 * nb + ens)
+    kw = dict(precision=cam.CAM_PREC_FP64, ensemble=ens, reaction=rx, seed=5, steps_per_launch=10)
+    out = []
+    for nmk, wpe in ((cam.CAM_NMK_FOLDED, 1), (cam.CAM_NMK_FOLDED, 2), (cam.CAM_NMK_VALU, 1)):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nb, ntraj, w["dt"], w["masses"], nm_kernel=nmk,
+                           waves_per_eu=wpe, **kw)
+        e.set_state(q, p, beta)
+        if ens == cam.CAM_ENS_CHILD:
+            e.begin_children()
+        e.run(30)
+        out.append(e.get_state() + ((e.kappa(30),) if ens == cam.CAM_ENS_CHILD else ()))
+        e.close()
+    (q1, p1, *k1), (q2, p2, *k2), (qv, pv, *kv) = out
+    assert np.array_equal(q1, q2) and np.array_equal(p1, p2)
+    err = np.abs(q2 - qv).max() / np.abs(qv).max()
+    print(f"nb={nb} ens={ens}: two-wave folded vs unfolded rel dq {err:.1e}")
+    assert err <= 1e-12
+    if ens == cam.CAM_ENS_CHILD:
+        np.testing.assert_array_equal(k1[0][0], k2[0][0])
+        assert k1[0][1] == k2[0][1] == kv[0][1]
+
+
 def test_folded_rejects_what_it_does_not_cover():
     w = workloads.h3_nve(nbeads=8)
     for kw in (dict(ensemble=cam.CAM_ENS_ANDERSEN, andersen_step=5), dict(remove_com=True),
