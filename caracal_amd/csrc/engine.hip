@@ -33,6 +33,9 @@ namespace {
 constexpr int kParentWaves = 4;
 constexpr int kGraphLaunches = 1000;
 constexpr int kPfConstraint = 1, kPfCheck = 2, kPfThisStep = 4;
+// Folded launches of at least 2 x kSliceMinBlocks blocks (the CU count) are
+// issued as two trajectory slices on two streams (launch_steps).
+constexpr int kSliceMinBlocks = 256;
 
 // xi of the centroid without its gradient, for the per-step child flags
 // (calc_xi BIMOLEC mode 2, src/calc_xi.f90:63-213).  One lane evaluates it
@@ -941,11 +944,12 @@ __global__ __launch_bounds__(256) void xi_flag_kernel(StepArgs a)
     }
     __syncthreads();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)a.nsteps * a.ntraj) return;
-    const int it = (int)(i / a.ntraj), traj = (int)(i % a.ntraj);
+    const int n = a.tend - a.tbegin;
+    if (i >= (size_t)a.nsteps * n) return;
+    const int it = (int)(i / n), traj = a.tbegin + (int)(i % n);
     double c[3 * NA];
 #pragma unroll
-    for (int j = 0; j < 3 * NA; ++j) c[j] = a.cbuf[i * 3 * NA + j];
+    for (int j = 0; j < 3 * NA; ++j) c[j] = a.cbuf[((size_t)it * a.ntraj + traj) * 3 * NA + j];
     const double xi = calc_xi_value<NA>(a.rx, wcom, c);
     a.xi_flag[(size_t)(a.step_off + it) * a.ntraj + traj] = xi > 0.0 ? 1 : 0;
 }
@@ -1609,12 +1613,15 @@ struct Kernels {
     step_fn step = nullptr, nm = nullptr;
     force_fn force = nullptr, fsplit = nullptr;
     int step_tpb = 1;                 // trajectories per block of step
+    bool folded = false;              // step is a folded kernel (takes a trajectory range)
 };
 
 struct cam_engine {
     cam_config cfg{};
     Rx rx{};
     hipStream_t stream = nullptr;
+    hipStream_t slice_stream = nullptr;   // second trajectory slice of large folded launches
+    hipEvent_t slice_fork = nullptr, slice_join = nullptr;
     double *q = nullptr, *p = nullptr, *g = nullptr, *vbead = nullptr, *beta = nullptr;
     double *tfwd = nullptr, *tinv = nullptr;
     double *tfold = nullptr;          // folded reference transform (nm_mode REFERENCE, nbeads >= 4)
@@ -1821,6 +1828,7 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
         if (!f) return false;
         k.step = f;
         k.step_tpb = tpb;
+        k.folded = true;
     }
     return ok;
 }
@@ -1920,6 +1928,8 @@ StepArgs step_args(cam_engine *e)
     a.check_after = e->pcheck_after;
     a.step_base = e->step_base;
     a.ntraj = e->cfg.ntraj;
+    a.tbegin = 0;
+    a.tend = e->cfg.ntraj;
     // ANDERSEN draws every andersen_step >= 1 steps; a window's 0 means never
     // (the fused window step tests andersen_step > 0)
     a.andersen_step = e->cfg.ensemble == CAM_ENS_UMBRELLA ? e->cfg.andersen_step
@@ -1935,7 +1945,7 @@ StepArgs step_args(cam_engine *e)
 // steps per launch, transrot after each launch when requested) or split (per
 // step the normal-mode kernel, then the force kernel; one last half kick).
 hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, StepArgs a, int nsteps,
-                        const TrajArgs *transrot)
+                        const TrajArgs *transrot, bool slices = true)
 {
     const int grid = (e->cfg.ntraj + e->tpb - 1) / e->tpb;
     hipError_t err = hipSuccess;
@@ -2020,30 +2030,58 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
     }
     const int tgrid = (e->cfg.ntraj + 127) / 128;
     const int sgrid = (e->cfg.ntraj + kn.step_tpb - 1) / kn.step_tpb;
-    for (int done = 0; done < nsteps && err == hipSuccess; done += a.nsteps) {
-        a.nsteps = std::min(spl, nsteps - done);
-        a.step_off = done;
-        hipLaunchKernelGGL(kn.step, dim3(sgrid), dim3(kBlock), 0, e->stream, a);
-        err = hipGetLastError();
-        if (err == hipSuccess && e->cfg.ensemble == CAM_ENS_CHILD) {
-            const unsigned xgrid = (unsigned)(((size_t)a.nsteps * a.ntraj + 255) / 256);
-            with_natoms(e->cfg.natoms, [&](auto na) {
-                hipLaunchKernelGGL(xi_flag_kernel<decltype(na)::value>, dim3(xgrid), dim3(256), 0, e->stream, a);
-            });
-            err = hipGetLastError();
+    // The launches of one trajectory slice [a.tbegin, a.tend) on stream st.
+    const auto slice = [&](StepArgs b, hipStream_t st) {
+        const int grid = (b.tend - b.tbegin + kn.step_tpb - 1) / kn.step_tpb;
+        hipError_t rc = hipSuccess;
+        for (int done = 0; done < nsteps && rc == hipSuccess; done += b.nsteps) {
+            b.nsteps = std::min(spl, nsteps - done);
+            b.step_off = done;
+            hipLaunchKernelGGL(kn.step, dim3(grid), dim3(kBlock), 0, st, b);
+            rc = hipGetLastError();
+            if (rc == hipSuccess && e->cfg.ensemble == CAM_ENS_CHILD) {
+                const unsigned xgrid = (unsigned)(((size_t)b.nsteps * (b.tend - b.tbegin) + 255) / 256);
+                with_natoms(e->cfg.natoms, [&](auto na) {
+                    hipLaunchKernelGGL(xi_flag_kernel<decltype(na)::value>, dim3(xgrid), dim3(256), 0, st, b);
+                });
+                rc = hipGetLastError();
+            }
+            if (rc == hipSuccess && transrot) {
+                hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, st, *transrot);
+                rc = hipGetLastError();
+            }
         }
-        if (err == hipSuccess && transrot) {
-            hipLaunchKernelGGL(transrot_kernel, dim3(tgrid), dim3(128), 0, e->stream, *transrot);
-            err = hipGetLastError();
-        }
-    }
-    return err;
+        return rc;
+    };
+    // A folded launch runs every block for all its steps, so the blocks of
+    // its last round (when the grid is not a whole number of rounds) leave
+    // part of the chip idle until they end.  With at least two blocks per CU
+    // the trajectories are issued as two slices on two streams: the slices
+    // are independent, and one slice's blocks fill the other's last round.
+    // Not when the caller's stream is itself a branch forked from another
+    // stream (a ragged batch's bucket, whose neighbours fill the chip): the
+    // slices fork only from a stream that starts its own work.
+    if (!slices || !kn.folded || transrot || sgrid < 2 * kSliceMinBlocks || !e->slice_stream)
+        return slice(a, e->stream);
+    StepArgs lo = a, hi = a;
+    lo.tend = hi.tbegin = (sgrid / 2) * kn.step_tpb;
+    err = hipEventRecord(e->slice_fork, e->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(e->slice_stream, e->slice_fork, 0);
+    if (err == hipSuccess) err = slice(lo, e->stream);
+    if (err == hipSuccess) err = slice(hi, e->slice_stream);
+    // joined in every case, so that a capture is never left forked
+    hipError_t jn = hipEventRecord(e->slice_join, e->slice_stream);
+    if (jn == hipSuccess) jn = hipStreamWaitEvent(e->stream, e->slice_join, 0);
+    return err != hipSuccess ? err : jn;
 }
 
-int enqueue_run(cam_engine *e, int nsteps)
+// slices: the folded launches may fork into two trajectory slices
+// (launch_steps); false for a stream that is itself a forked branch.
+int enqueue_run(cam_engine *e, int nsteps, bool slices)
 {
     const TrajArgs ta = traj_args(e);
-    CAM_HIP(launch_steps(e, e->kern, e->split, e->spl, step_args(e), nsteps, e->cfg.remove_com ? &ta : nullptr));
+    CAM_HIP(launch_steps(e, e->kern, e->split, e->spl, step_args(e), nsteps, e->cfg.remove_com ? &ta : nullptr,
+                         slices));
     if (e->cfg.ensemble == CAM_ENS_CHILD) {
         hipLaunchKernelGGL(kappa_reduce_kernel, dim3(nsteps), dim3(256), 0, e->stream, e->xi_flag, e->weight,
                            e->cfg.ntraj, e->num, e->step_base);
@@ -2169,6 +2207,9 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
         return CAM_ERR_UNSUPPORTED;
     }
     CAM_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    CAM_HIP(hipStreamCreateWithFlags(&e->slice_stream, hipStreamNonBlocking));
+    CAM_HIP(hipEventCreateWithFlags(&e->slice_fork, hipEventDisableTiming));
+    CAM_HIP(hipEventCreateWithFlags(&e->slice_join, hipEventDisableTiming));
     const size_t nstate = (size_t)cfg->ntraj * 3 * na * nb;
     CAM_HIP(hipMalloc(&e->q, nstate * sizeof(double)));
     CAM_HIP(hipMalloc(&e->p, nstate * sizeof(double)));
@@ -2241,6 +2282,9 @@ int cam_engine_destroy(cam_engine *e)
     if (e->step_base) (void)hipFree(e->step_base);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->slice_fork) (void)hipEventDestroy(e->slice_fork);
+    if (e->slice_join) (void)hipEventDestroy(e->slice_join);
+    if (e->slice_stream) (void)hipStreamDestroy(e->slice_stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return CAM_OK;
@@ -2478,7 +2522,7 @@ static int get_graph(cam_engine *e, int nsteps, hipGraphExec_t *out)
     auto it = e->graphs.find(nsteps);
     if (it == e->graphs.end()) {
         CAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_run(e, nsteps);
+        int rc = enqueue_run(e, nsteps, true);
         hipGraph_t graph = nullptr;
         hipError_t ce = hipStreamEndCapture(e->stream, &graph);
         if (rc != CAM_OK) {
@@ -2611,7 +2655,16 @@ int cam_engine_device_state(cam_engine *e, double **q, double **p, double **dvdq
     return CAM_OK;
 }
 
+// slices: the trials launch as the engine's own runs do (true) or as a
+// ragged batch's bucket does (false: no trajectory slices).
+static int autotune(cam_engine *e, int trial_steps, cam_tuning *out, bool slices);
+
 int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
+{
+    return autotune(e, trial_steps, out, true);
+}
+
+static int autotune(cam_engine *e, int trial_steps, cam_tuning *out, bool slices)
 {
     CAM_CHECK_ARG(e, "null engine");
     CAM_CHECK_ARG(e->have_state, "no state uploaded");
@@ -2706,9 +2759,10 @@ int cam_engine_autotune(cam_engine *e, int trial_steps, cam_tuning *out)
         if (err == hipSuccess) err = hipMemcpyAsync(a.p, e->p, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         if (err == hipSuccess) err = hipMemcpyAsync(a.g, e->g, nstate * sizeof(double), hipMemcpyDeviceToDevice, e->stream);
         // one untimed launch sequence, then the timed trial
-        if (err == hipSuccess) err = launch_steps(e, kn, tr.split != 0, tr.spl, a, std::min(tr.spl, trial_steps), nullptr);
+        if (err == hipSuccess)
+            err = launch_steps(e, kn, tr.split != 0, tr.spl, a, std::min(tr.spl, trial_steps), nullptr, slices);
         if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
-        if (err == hipSuccess) err = launch_steps(e, kn, tr.split != 0, tr.spl, a, trial_steps, nullptr);
+        if (err == hipSuccess) err = launch_steps(e, kn, tr.split != 0, tr.spl, a, trial_steps, nullptr, slices);
         if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
         if (err == hipSuccess) err = hipEventSynchronize(t1);
         float ms = 0.f;
@@ -2849,7 +2903,7 @@ int cam_batch_autotune(cam_batch *b, int trial_steps)
 {
     CAM_CHECK_ARG(b, "null batch");
     for (cam_engine *e : b->eng) {
-        const int rc = cam_engine_autotune(e, trial_steps, nullptr);
+        const int rc = autotune(e, trial_steps, nullptr, false);
         if (rc != CAM_OK) return rc;
     }
     return CAM_OK;
@@ -2875,7 +2929,7 @@ static int batch_graph(cam_batch *b, int nsteps, hipGraphExec_t *out)
             cam_engine *e = b->eng[i];
             err = hipStreamWaitEvent(e->stream, b->fork, 0);
             if (err != hipSuccess) break;
-            rc = enqueue_run(e, nsteps);
+            rc = enqueue_run(e, nsteps, false);       // the buckets fill the chip together
             // the branch is joined even after a failure, so the capture ends cleanly
             hipError_t je = hipEventRecord(b->done[i], e->stream);
             if (je == hipSuccess) je = hipStreamWaitEvent(b->stream, b->done[i], 0);

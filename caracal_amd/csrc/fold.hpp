@@ -215,7 +215,8 @@ __device__ __forceinline__ void nm_fold(double *xs, const double *prop, const do
 // per SIMD the compiler is asked to fit (its register budget, 512 / WPE).
 // Between two steps a lane holds q, p and g of its slots; during the forces
 // only the gradient is live, p and q are read back from the column buffer
-// after them.
+// after them.  The launch covers trajectories a.tbegin .. a.tend - 1 (state
+// arrays and the centroid buffer keep their whole-batch strides).
 template <class S, int NB, typename T, int ENS, int WPE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void fold_step_kernel(StepArgs a)
 {
@@ -246,9 +247,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         w.tl = w.live ? slot / NH : TPB - 1;
         w.j = w.live ? slot % NH : H;
         w.jm = w.j == 0 ? 0 : NB - w.j;                // mirror bead (H is its own)
-        const int traj_raw = blockIdx.x * TPB + w.tl;
-        w.act = w.live && traj_raw < a.ntraj;
-        w.traj = traj_raw < a.ntraj ? traj_raw : a.ntraj - 1;
+        const int traj_raw = a.tbegin + blockIdx.x * TPB + w.tl;
+        w.act = w.live && traj_raw < a.tend;
+        w.traj = traj_raw < a.tend ? traj_raw : a.tend - 1;
 #pragma unroll
         for (int s = 0; s < SL; ++s) w.cc[s] = S::coord(s, w.r);
         return w;
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // per-trajectory mode table of modes 0..H (src/verlet.f90:413-432), as PropTab lays it out
     for (int i = tid; i < TPB * NH; i += kBlock) {
         const int t = i / NH, mode = i % NH;
-        const int tr = min(blockIdx.x * TPB + t, a.ntraj - 1);
+        const int tr = min(a.tbegin + (int)blockIdx.x * TPB + t, a.tend - 1);
         const double bt = a.beta[tr];
         const double wk = (2.0 / (bt / NB)) * sin(mode * (3.141592653589793 / NB));
         const double wt = wk * a.dt;
@@ -320,8 +321,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             // centroids of this child step (src/get_centroid.f90:66-85); their xi
             // is evaluated after the launch by xi_flag_kernel
             for (int i = ti; i < TPB * NC; i += kBlock) {
-                const int tr = blockIdx.x * TPB + i / NC;
-                if (tr < a.ntraj) a.cbuf[((size_t)it * a.ntraj + tr) * NC + i % NC] = cent[i];
+                const int tr = a.tbegin + blockIdx.x * TPB + i / NC;
+                if (tr < a.tend) a.cbuf[((size_t)it * a.ntraj + tr) * NC + i % NC] = cent[i];
             }
         }
         // forces of the distinct beads (src/verlet.f90:772-777)

@@ -86,6 +86,8 @@ struct StepArgs {
     double nose_q;               // NHC: damping time (atomic time units)
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run
+    int tbegin, tend;            // trajectories of this launch (folded kernels and xi_flag_kernel;
+                                 // the other kernels take all ntraj)
     int andersen_step;
     uint64_t seed;
     double dt;

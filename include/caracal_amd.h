@@ -344,7 +344,10 @@ int cam_engine_last_timing(cam_engine *eng, double *ms_per_launch, int *steps_pe
  * bucket's engine changed schedule or buffers.  Engines are reached with
  * cam_batch_engine to set state and read results; the batch owns them; a
  * bucket's engine must not be run on its own while a batch run is in
- * flight.  Batches hold NVE, Andersen, Nose-Hoover and child engines.  No
+ * flight.  Batches hold NVE, Andersen, Nose-Hoover and child engines.  An
+ * engine run on its own issues folded launches of at least 512 blocks as two
+ * trajectory slices on two streams (the same bits); a bucket does not, and
+ * cam_batch_autotune times each bucket's schedules that way.  No
  * counterpart in the reference, which runs one trajectory per MPI rank
  * (src/recross.f90:334-415). */
 typedef struct cam_batch cam_batch;

@@ -110,6 +110,43 @@ def test_folded_and_unfolded_agree_on_a_ch4h_batch(prec):
         assert err <= 1e-6
 
 
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+def test_folded_trajectory_slices_give_the_bits_of_small_batches(prec):
+    """1600 CH4H children x 32 beads (534 folded blocks: the launches are
+    issued as two trajectory slices on two streams): every trajectory ends
+    with the bits it has when run in a batch of 40 that is not sliced, on
+    both sides of the slice boundary; the recrossing sums follow the unfolded
+    kernel to rounding."""
+    n, nb, steps = 1600, 32, 120
+    w, q0, p0, beta, rx = _batch(cam.CAM_SURFACE_CH4H, nb, n, seed=11)
+    kw = dict(precision=prec, ensemble=cam.CAM_ENS_CHILD, reaction=rx, seed=3, steps_per_launch=50)
+
+    def run(sel, nmk):
+        e = cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nb, len(sel), w["dt"], w["masses"], nm_kernel=nmk, **kw)
+        e.set_state(q0[sel], p0[sel], beta[sel])
+        e.begin_children()
+        e.run(steps)
+        q, p = e.get_state()
+        k = e.kappa(steps)
+        e.close()
+        return q, p, k
+
+    everything = np.arange(n)
+    qa, pa, (na, da) = run(everything, cam.CAM_NMK_FOLDED)
+    half = (((n + 2) // 3) // 2) * 3               # first trajectory of the second slice
+    for lo in (0, half - 20, n - 40):
+        sel = np.arange(lo, lo + 40)
+        qs, ps, _ = run(sel, cam.CAM_NMK_FOLDED)
+        assert np.array_equal(qa[sel], qs) and np.array_equal(pa[sel], ps), lo
+    qv, pv, (nv, dv) = run(everything, cam.CAM_NMK_VALU)
+    err = np.abs(qa - qv).max() / np.abs(qv).max()
+    print(f"prec={prec}: sliced folded vs unfolded rel dq {err:.1e}")
+    assert err <= (1e-11 if prec == cam.CAM_PREC_FP64 else 1e-6)
+    assert da == dv
+    if prec == cam.CAM_PREC_FP64:
+        np.testing.assert_allclose(na, nv, rtol=0, atol=1e-12 * max(1.0, np.abs(nv).max()))
+
+
 @pytest.mark.parametrize("surface,nb,ntraj", [(cam.CAM_SURFACE_H3, 4, 200), (cam.CAM_SURFACE_H3, 64, 20),
                                               (cam.CAM_SURFACE_CH4H, 16, 30), (cam.CAM_SURFACE_CH4H, 64, 3)])
 @pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
