@@ -118,6 +118,8 @@ PROTOTYPES = {
     "cam_engine_set_windows": (ctypes.c_int, [_vp, _dp, _dp]),
     "cam_engine_begin_umbrella": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "cam_engine_umbrella_stats": (ctypes.c_int, [_vp, _dp, _dp]),
+    "cam_engine_set_umbrella_histogram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "cam_engine_umbrella_histogram": (ctypes.c_int, [_vp, _dp]),
     "cam_engine_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_engine_prepare": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cam_engine_sync": (ctypes.c_int, [_vp]),

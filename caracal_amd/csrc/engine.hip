@@ -833,6 +833,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 }
                 usum += xi1;
                 usq += xi1 * xi1;
+                // slots past the last trajectory evaluate a copy of it: not counted
+                if (act) count_xi(a.uhist, a.hbins, a.hmin, a.hinv, traj, xi1);
                 const uint64_t istep = a.step_base[1] + (uint64_t)(a.step_off + it) + 1;
                 if (a.check_after >= 0 && istep > (uint64_t)a.check_after) {
                     double ep = 0.0;
@@ -1419,7 +1421,9 @@ __global__ __launch_bounds__(64 * kParentWaves) void umbrella_end_kernel(TrajArg
                                                                          double *ustat, int *pfail,
                                                                          const uint64_t *step_base, int step_off,
                                                                          int andersen_step, uint64_t seed,
-                                                                         double epot_max, double xi_tol, int check_after)
+                                                                         double epot_max, double xi_tol, int check_after,
+                                                                         unsigned *uhist, int hbins, double hmin,
+                                                                         double hinv)
 {
     constexpr int NC = 3 * NA;
     const int lane = threadIdx.x % 64, t = blockIdx.x * kParentWaves + threadIdx.x / 64;
@@ -1447,6 +1451,7 @@ __global__ __launch_bounds__(64 * kParentWaves) void umbrella_end_kernel(TrajArg
     if (lane == 0) {
         ustat[2 * t] = ustat[2 * t] + xi1;
         ustat[2 * t + 1] = ustat[2 * t + 1] + xi1 * xi1;
+        count_xi(uhist, hbins, hmin, hinv, t, xi1);
     }
     if (check_after >= 0 && istep > (uint64_t)check_after) {
         const bool bad = trajectory_fails(a, t, lane, 0.0, epot_max) || fabs(xi1 - xi_ideal) > xi_tol;
@@ -1634,6 +1639,9 @@ struct cam_engine {
     double *ustat = nullptr;          // UMBRELLA: running sums of xi, xi^2 per trajectory
     double *nhc = nullptr;            // NHC: thermostat velocities [traj][4]
     long ustat_steps = 0;             // UMBRELLA: steps in those sums
+    unsigned *uhist = nullptr;        // UMBRELLA: histograms of xi [traj][hbins] (cam_engine_set_umbrella_histogram)
+    int hbins = 0;
+    double hmin = 0.0, hinv = 0.0;
     bool parent_ready = false;        // PARENT, UMBRELLA: begun since the last set_state
     double pcheck_max = 0.0;          // PARENT, UMBRELLA: trajectory check
     double pcheck_xi_tol = 0.0;
@@ -1922,6 +1930,10 @@ StepArgs step_args(cam_engine *e)
     a.ustat = e->ustat;
     a.uwin = e->uwin;
     a.nhc = e->nhc;
+    a.uhist = e->uhist;
+    a.hbins = e->hbins;
+    a.hmin = e->hmin;
+    a.hinv = e->hinv;
     a.nose_q = e->cfg.nose_q;
     a.epot_max = e->pcheck_max;
     a.xi_tol = e->pcheck_xi_tol;
@@ -1997,7 +2009,7 @@ hipError_t launch_steps(cam_engine *e, const Kernels &kn, bool split, int spl, S
             err = hipGetLastError();
             if (err != hipSuccess) break;
             with_natoms(e->cfg.natoms, [&](auto na) {
-                hipLaunchKernelGGL(umbrella_end_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g, e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_xi_tol, e->pcheck_after);
+                hipLaunchKernelGGL(umbrella_end_kernel<decltype(na)::value>, dim3(pgrid), dim3(64 * kParentWaves), 0, e->stream, ta, a.g, e->uwin, a.ustat, a.pfail, a.step_base, j, as, a.seed, e->pcheck_max, e->pcheck_xi_tol, e->pcheck_after, a.uhist, a.hbins, a.hmin, a.hinv);
             });
             err = hipGetLastError();
             if (err == hipSuccess && transrot) {
@@ -2278,6 +2290,7 @@ int cam_engine_destroy(cam_engine *e)
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (e->pfail) (void)hipFree(e->pfail);
+    if (e->uhist) (void)hipFree(e->uhist);
     if (e->xi_flag) (void)hipFree(e->xi_flag);
     if (e->step_base) (void)hipFree(e->step_base);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -2488,9 +2501,48 @@ int cam_engine_begin_umbrella(cam_engine *e, int bias, double epot_max, double x
     CAM_LAUNCH_CHECK();
     hipLaunchKernelGGL(reset_child_kernel, dim3(1), dim3(1), 0, e->stream, e->step_base);   // steps since the start
     CAM_LAUNCH_CHECK();
+    if (e->uhist)
+        CAM_HIP(hipMemsetAsync(e->uhist, 0, (size_t)e->cfg.ntraj * e->hbins * sizeof(unsigned), e->stream));
     CAM_HIP(hipStreamSynchronize(e->stream));
     e->ustat_steps = 0;
     e->parent_ready = true;
+    return CAM_OK;
+}
+
+int cam_engine_set_umbrella_histogram(cam_engine *e, int nbins, double xi_min, double bin_size)
+{
+    CAM_CHECK_ARG(e, "null engine");
+    CAM_CHECK_ARG(e->cfg.ensemble == CAM_ENS_UMBRELLA, "umbrella histograms need ensemble UMBRELLA");
+    CAM_CHECK_ARG(nbins >= 0 && nbins <= (1 << 20), "nbins must be in 0..2^20");
+    CAM_CHECK_ARG(nbins == 0 || (std::isfinite(xi_min) && std::isfinite(bin_size) && bin_size > 0.0),
+                  "xi_min must be finite and bin_size positive");
+    clear_error();
+    CAM_HIP(hipStreamSynchronize(e->stream));         // no launch may still count into the old histograms
+    drop_graphs(e);                                   // the histogram is an argument of the captured launches
+    if (e->uhist) CAM_HIP(hipFree(e->uhist));
+    e->uhist = nullptr;
+    e->hbins = 0;
+    if (nbins > 0) {
+        const size_t bytes = (size_t)e->cfg.ntraj * nbins * sizeof(unsigned);
+        CAM_HIP(hipMalloc(&e->uhist, bytes));
+        CAM_HIP(hipMemset(e->uhist, 0, bytes));
+        e->hbins = nbins;
+        e->hmin = xi_min;
+        e->hinv = 1.0 / bin_size;
+    }
+    return CAM_OK;
+}
+
+int cam_engine_umbrella_histogram(cam_engine *e, double *counts)
+{
+    CAM_CHECK_ARG(e && counts, "null argument");
+    CAM_CHECK_ARG(e->uhist, "no umbrella histogram (cam_engine_set_umbrella_histogram)");
+    const size_t n = (size_t)e->cfg.ntraj * e->hbins;
+    std::vector<unsigned> h(n);
+    clear_error();
+    CAM_HIP(hipMemcpyAsync(h.data(), e->uhist, n * sizeof(unsigned), hipMemcpyDeviceToHost, e->stream));
+    CAM_HIP(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < n; ++i) counts[i] = (double)h[i];
     return CAM_OK;
 }
 
@@ -2747,6 +2799,7 @@ static int autotune(cam_engine *e, int trial_steps, cam_tuning *out, bool slices
         }
         if (window) {
             a.ustat = buf + 3 * nstate + (size_t)c.ntraj * nb;
+            a.uhist = nullptr;                      // trials count nothing
             a.pfail = reinterpret_cast<int *>(a.ustat + npd);
         }
         if (nhc) a.nhc = buf + 3 * nstate + (size_t)c.ntraj * nb;

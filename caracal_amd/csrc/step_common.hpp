@@ -69,6 +69,19 @@ struct Rx {                                  // reaction coordinate, 0-based ind
     double inv_nform, inv_nbreak, inv_terms;
 };
 
+// UMBRELLA: one sample xi of trajectory traj into its histogram h (hbins bins
+// per trajectory): bin j = nint((xi - hmin) hinv), 1 <= j <= hbins, the bin
+// the coordinate hmin + j / hinv stands for (src/wham.f90:104-106 gives bin j
+// that coordinate); other samples are not counted.  Only the lane that owns
+// the trajectory's xi counts it, but the add is atomic so that nothing about
+// the order of lanes matters.
+__device__ inline void count_xi(unsigned *h, int hbins, double hmin, double hinv, int traj, double xi)
+{
+    if (!h) return;
+    const double j = rint((xi - hmin) * hinv);
+    if (j >= 1.0 && j <= (double)hbins) atomicAdd(h + (size_t)traj * hbins + ((int)j - 1), 1u);
+}
+
 struct StepArgs {
     double *q, *p, *g, *vbead;
     const double *beta;          // [ntraj]
@@ -83,6 +96,9 @@ struct StepArgs {
     double epot_max, xi_tol;     // PARENT, UMBRELLA: trajectory check
     int check_after;             //   armed after this many steps (never if < 0)
     double *nhc;                 // NHC: thermostat velocities [traj][4]
+    unsigned *uhist;             // UMBRELLA: histograms of xi [traj][hbins], or null
+    int hbins;
+    double hmin, hinv;           //   bin j (1-based) holds the xi with nint((xi - hmin) hinv) = j
     double nose_q;               // NHC: damping time (atomic time units)
     const uint64_t *step_base;   // device counter of completed steps (Andersen draws)
     int ntraj, nsteps, step_off; // steps in this launch, offset of its first step in the run

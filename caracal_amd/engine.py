@@ -92,6 +92,7 @@ class RPMDEngine:
         self.natoms, self.nbeads, self.ntraj = cfg.natoms, cfg.nbeads, cfg.ntraj
         self._h = handle
         self._owned = owned
+        self._hbins = 0
 
     @classmethod
     def _borrowed(cls, cfg, handle):
@@ -183,6 +184,20 @@ class RPMDEngine:
         var = np.empty(self.ntraj)
         check("cam_engine_umbrella_stats", lib().cam_engine_umbrella_stats(self._h, _ptr(avg), _ptr(var)))
         return avg, var
+
+    def set_umbrella_histogram(self, nbins, xi_min, bin_size):
+        """Ensemble UMBRELLA: count every step's xi into bin
+        j = nint((xi - xi_min) / bin_size), 1 <= j <= nbins, per trajectory
+        (nbins 0: off); cleared by begin_umbrella."""
+        self._hbins = int(nbins)
+        check("cam_engine_set_umbrella_histogram", lib().cam_engine_set_umbrella_histogram(
+            self._h, int(nbins), float(xi_min), float(bin_size)))
+
+    def umbrella_histogram(self):
+        """(ntraj, nbins) float64 counts since begin_umbrella; column j - 1 is bin j."""
+        h = np.empty((self.ntraj, max(self._hbins, 1)))
+        check("cam_engine_umbrella_histogram", lib().cam_engine_umbrella_histogram(self._h, _ptr(h)))
+        return h
 
     def run(self, nsteps, sync=True):
         check("cam_engine_run", lib().cam_engine_run(self._h, nsteps))

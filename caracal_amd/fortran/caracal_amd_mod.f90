@@ -182,6 +182,21 @@ module caracal_amd_mod
          real(c_double), intent(out) :: average(*), variance(*)
          integer(c_int) :: rc
       end function
+      ! per-window histograms of xi for wham (counts(nbins, ntraj))
+      function cam_engine_set_umbrella_histogram(eng, nbins, xi_min, bin_size) &
+            bind(C, name="cam_engine_set_umbrella_histogram") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int), value :: nbins
+         real(c_double), value :: xi_min, bin_size
+         integer(c_int) :: rc
+      end function
+      function cam_engine_umbrella_histogram(eng, counts) bind(C, name="cam_engine_umbrella_histogram") result(rc)
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: counts(*)
+         integer(c_int) :: rc
+      end function
       function cam_engine_run(eng, nsteps) bind(C, name="cam_engine_run") result(rc)
          import :: c_int, c_ptr
          type(c_ptr), value :: eng
@@ -359,6 +374,7 @@ module caracal_amd_mod
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_begin_parent, cam_engine_run, cam_engine_sync, cam_engine_kappa
    public :: cam_engine_set_windows, cam_engine_begin_umbrella, cam_engine_umbrella_stats
+   public :: cam_engine_set_umbrella_histogram, cam_engine_umbrella_histogram
    public :: cam_engine_observables, cam_engine_prepare, cam_engine_autotune, cam_engine_last_timing
    public :: cam_batch_create, cam_batch_destroy, cam_batch_engine, cam_batch_autotune, cam_batch_run
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode

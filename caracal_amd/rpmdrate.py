@@ -9,7 +9,10 @@ The parameters keep the names and meaning of calc_rate's key-file sections
   umbrella: bias (hartree per kelvin: the force constant is bias * kelvin,
             src/calc_rate.f90:698-699), bonds (lo, hi), dist, gen_steps,
             equi_steps, sample_steps, sample_trajs
-  pmf:      xi_range (xi_min, xi_max), bins
+  pmf:      xi_range (xi_min, xi_max), bins, method (pmf_method: INTEGRATION,
+            WHAM or ALL; WHAM takes the windows' histograms of xi, collected
+            on the device; ALL computes both and keeps the integration's, as
+            src/calc_rate.f90:2018-2021 does)
   recross:  equi_steps, child_total, child_interval, child_perpoint,
             child_steps, nparents (independent parents; 1 in the reference)
 """
@@ -42,6 +45,7 @@ class UmbrellaKeys:
 class PmfKeys:
     xi_range: tuple
     bins: int
+    method: str = "INTEGRATION"
 
 
 @dataclass
@@ -85,11 +89,18 @@ class RPMDRate:
         there (hartree), the reference of rpmd_check's energy limit."""
         check = None if self.check is None else dict(self.check, ts_energy=ts_energy)
         um = self.um
+        method = self.pmf.method.upper()
+        if method not in ("INTEGRATION", "WHAM", "ALL"):
+            raise ValueError("pmf method must be INTEGRATION, WHAM or ALL")
+        xi_min, xi_max = self.pmf.xi_range
+        bin_size = (xi_max - xi_min) / (self.pmf.bins - 1)         # src/calc_rate.f90:1803, :1949
+        hist = None if method == "INTEGRATION" else (self.pmf.bins, xi_min, bin_size)
         us = UmbrellaSampling(self.surface, self.nbeads, self.dt, self.masses, self.reaction, self.beta,
                               umbr_lo=um.bonds[0], umbr_hi=um.bonds[1], umbr_dist=um.dist, k_force=self.k_force,
                               gen_step=um.gen_steps, equi_step=um.equi_steps, umbr_step=um.sample_steps,
                               umbr_traj=um.sample_trajs, andersen_step=self.andersen_step, check=check,
-                              precision=self.precision, seed=self.seed, parallel_trajs=self.parallel_trajs)
+                              precision=self.precision, seed=self.seed, parallel_trajs=self.parallel_trajs,
+                              histogram=hist)
         wall = {}
         t0 = time.perf_counter()
         try:
@@ -99,10 +110,23 @@ class RPMDRate:
             wall["windows"] = time.perf_counter() - t0 - wall["structures"]
         finally:
             us.close()
-        xi_min, xi_max = self.pmf.xi_range
         dA, bin_coord, pmf = rate.umbrella_integration(xi_wins, average, variance, np.full(len(xi_wins), self.k_force),
                                                        self.beta, xi_min, xi_max, self.pmf.bins,
                                                        um.sample_trajs, um.sample_steps)
+        extra = {}
+        if hist is not None:
+            # WHAM over the windows that hold samples (all of them here; the
+            # reference's last window never has a histogram)
+            hw = us.histograms
+            keep = hw.sum(axis=1) > 0
+            coord_w, pmf_w, _, it = rate.wham(hw[keep], xi_wins[keep], np.full(int(keep.sum()), self.k_force),
+                                              xi_min, xi_max, self.pmf.bins, self.kelvin)
+            extra = dict(wham_coord=coord_w, pmf_wham=pmf_w, wham_iterations=it, histograms=hw)
+            if method == "WHAM":
+                # the WHAM PMF in hartree (the reference stores it times
+                # 627.509 x 0.52917721092, src/wham.f90:189, and then uses it
+                # as hartree in the rate); its bins are centred at coord_w
+                dA, bin_coord, pmf = None, coord_w, pmf_w
         imax, imin = rate.pmf_extrema(pmf, xi_min, xi_max, self.pmf_minloc)
         xi_barrier = float(bin_coord[imax])
         # the parent starts from the start structure of the window nearest the barrier, every bead alike
@@ -123,14 +147,16 @@ class RPMDRate:
         k_mol = rate.rate_bimolecular(kappa, self.beta, pmf[imax], pmf[imin], self._mass_reac(), self.reaction.r_inf,
                                       self.n_paths)
         return dict(xi_wins=xi_wins, average=average, variance=variance, start_structures=struc, dA=dA,
-                    bin_coord=bin_coord, pmf=pmf[:-1], xi_barrier=xi_barrier, pmf_max=float(pmf[imax]),
+                    bin_coord=bin_coord, pmf=pmf if method == "WHAM" else pmf[:-1], xi_barrier=xi_barrier,
+                    pmf_max=float(pmf[imax]), **extra,
                     pmf_min=float(pmf[imin]), kappa=kappa, kappa_t=kres["kappa_t"], k_cm3_mol_s=k_mol,
                     k_cm3_molecule_s=k_mol / rate.AVOGADRO, errors=us.errors + kres["errors"], wall_s=wall)
 
 
-def h_h2_example(scale=1.0, kelvin=300.0, seed=0):
+def h_h2_example(scale=1.0, kelvin=300.0, seed=0, pmf_method="INTEGRATION"):
     """examples/calc_rate/h+h2/rate.key: H + H2 on BKMP2 at 300 K with 8
-    beads, dt 0.1 fs; every step count multiplied by scale."""
+    beads, dt 0.1 fs; every step count multiplied by scale; pmf_method as
+    the key file's pmf section names it (the example's is INTEGRATION)."""
     from . import CAM_SURFACE_H3, egrad_h3, workloads
     from .units import FS_TO_AU, mass_au
 
@@ -141,7 +167,7 @@ def h_h2_example(scale=1.0, kelvin=300.0, seed=0):
     r = RPMDRate(CAM_SURFACE_H3, 8, 0.1 * FS_TO_AU, mass_au([1.008] * 3), workloads.h3_reaction(), kelvin,
                  umbrella=UmbrellaKeys(bias=0.05, bonds=(-0.05, 1.05), dist=0.01, gen_steps=n(10000),
                                        equi_steps=n(10000), sample_steps=n(20000), sample_trajs=10),
-                 pmf=PmfKeys(xi_range=(-0.05, 1.05), bins=5000),
+                 pmf=PmfKeys(xi_range=(-0.05, 1.05), bins=5000, method=pmf_method),
                  recross=RecrossKeys(equi_steps=n(50000), child_total=perpoint * max(1, n(100)),
                                      child_interval=n(1000), child_perpoint=perpoint, child_steps=n(500)),
                  andersen_step=80, n_paths=2, seed=seed)
@@ -183,6 +209,7 @@ if __name__ == "__main__":
     ap.add_argument("--scale", type=float, default=1.0, help="multiply every step count (1 = the key file's)")
     ap.add_argument("--kelvin", type=float, default=300.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--pmf-method", default="INTEGRATION", choices=["INTEGRATION", "WHAM", "ALL"])
     ap.add_argument("--kappa-seeds", type=int, default=0,
                     help="run only the recrossing at xi = 1 for this many seeds in both normal-mode bases")
     args = ap.parse_args()
@@ -200,7 +227,12 @@ if __name__ == "__main__":
                              kappas=[float(k) for k in ks])
         print(json.dumps(out))
         raise SystemExit(0)
-    out = h_h2_example(args.scale, args.kelvin, args.seed)
-    print(json.dumps(dict(kelvin=args.kelvin, scale=args.scale, xi_barrier=out["xi_barrier"],
-                          dW_kcal_mol=(out["pmf_max"] - out["pmf_min"]) * 627.5094743, kappa=out["kappa"],
-                          k_cm3_molecule_s=out["k_cm3_molecule_s"], errors=out["errors"], wall_s=out["wall_s"])))
+    out = h_h2_example(args.scale, args.kelvin, args.seed, args.pmf_method)
+    res = dict(kelvin=args.kelvin, scale=args.scale, pmf_method=args.pmf_method, xi_barrier=out["xi_barrier"],
+               dW_kcal_mol=(out["pmf_max"] - out["pmf_min"]) * 627.5094743, kappa=out["kappa"],
+               k_cm3_molecule_s=out["k_cm3_molecule_s"], errors=out["errors"], wall_s=out["wall_s"])
+    if "pmf_wham" in out:
+        w = out["pmf_wham"]
+        res["wham"] = dict(iterations=out["wham_iterations"], xi_barrier=float(out["wham_coord"][int(np.argmax(w))]),
+                           max_kcal_mol=float(w.max() * 627.5094743))
+    print(json.dumps(res))

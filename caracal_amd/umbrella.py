@@ -29,6 +29,13 @@ neighbour instead, one to the right, one to the left, two to the right, two
 to the left (src/rpmd_check.f90:176-250, clamped to the windows that exist).
 One whose variance exceeds 1e-2 is run again from where it ended, as the
 reference does.
+
+With histogram=(nbins, xi_min, bin_size) the sampling steps of every
+trajectory are also counted into bins of xi on the device (bin j at
+xi_min + j bin_size, the coordinate src/wham.f90 gives bin j), summed per
+window over its umbr_traj accepted trajectories: the input of WHAM
+(caracal_amd.rate.wham), which the reference reads from files that its own
+sampling loop does not write (src/calc_rate.f90:1712-1724, :1808-1825).
 """
 import math
 
@@ -58,7 +65,7 @@ class UmbrellaSampling:
 
     def __init__(self, surface, nbeads, dt, masses, reaction, beta, *, umbr_lo, umbr_hi, umbr_dist, k_force,
                  gen_step, equi_step, umbr_step, umbr_traj, andersen_step, check=None, precision=_abi.CAM_PREC_FP64,
-                 seed=0, parallel_trajs=False, engine_factory=RPMDEngine):
+                 seed=0, parallel_trajs=False, engine_factory=RPMDEngine, histogram=None):
         self.surface, self.nbeads, self.dt = surface, nbeads, dt
         self.masses = np.asarray(masses, dtype=np.float64)
         self.reaction, self.beta = reaction, float(beta)
@@ -67,6 +74,9 @@ class UmbrellaSampling:
         self.andersen_step, self.precision, self.seed = andersen_step, precision, seed
         self.parallel_trajs, self.check = parallel_trajs, check
         self._factory = engine_factory
+        self.histogram = None if histogram is None else (int(histogram[0]), float(histogram[1]),
+                                                          float(histogram[2]))
+        self.histograms = None            # (windows, nbins) after sample() when histogram is set
         self._engines = {}
         self._created = 0
         self._draw = _DRAW_BASE
@@ -86,6 +96,8 @@ class UmbrellaSampling:
                 self.surface, self.nbeads, ntraj, self.dt, self.masses, precision=self.precision,
                 ensemble=_abi.CAM_ENS_UMBRELLA, reaction=self.reaction, andersen_step=self.andersen_step,
                 seed=self.seed + self._created, remove_com=True)
+            if self.histogram is not None:
+                self._engines[ntraj].set_umbrella_histogram(*self.histogram)
             self._created += 1
         return self._engines[ntraj]
 
@@ -158,9 +170,10 @@ class UmbrellaSampling:
         """One trajectory per entry (window win[i] at xi[i]): mdinit,
         equilibration, unbiased start, sampling; failed ones again (see module
         doc).  Returns the final positions and the mean and variance of xi of
-        each."""
+        each, and their histograms of xi (None without histogram)."""
         n = len(q_start)
         q_out, avg, var = np.empty_like(q_start), np.empty(n), np.empty(n)
+        hist = None if self.histogram is None else np.zeros((n, self.histogram[0]))
         q_run = q_start.copy()
         pending = np.arange(n)
         while len(pending):
@@ -179,6 +192,8 @@ class UmbrellaSampling:
             noisy = ~bad & (v > 1e-2)
             ok = ~bad & ~noisy
             q_out[pending[ok]], avg[pending[ok]], var[pending[ok]] = q_end[ok], a[ok], v[ok]
+            if hist is not None:
+                hist[pending[ok]] = e.umbrella_histogram()[ok]
             for i in pending[ok]:                            # a finished trajectory clears its window's record
                 self._err_act[win[i]] = self._resets[win[i]] = 0
             for i in pending[bad]:                           # check failed: a start structure again
@@ -186,24 +201,32 @@ class UmbrellaSampling:
             q_run[pending[noisy]] = q_end[noisy]             # variance too high: again from where it ended
             self._count_errors(int(bad.sum() + noisy.sum()))
             pending = pending[~ok]
-        return q_out, avg, var
+        return q_out, avg, var, hist
 
     def sample(self, struc_equi):
         """Mean and variance of xi per window, averaged over its umbr_traj
-        trajectories (src/calc_rate.f90:1692-1703)."""
+        trajectories (src/calc_rate.f90:1692-1703); with histogram, the
+        windows' histograms summed over those trajectories in
+        self.histograms."""
         xi_wins, _ = self.windows()
         nw, nb = len(xi_wins), self.nbeads
         struc = np.repeat(np.asarray(struc_equi, dtype=np.float64)[:, None], nb, axis=1)
         self._err_act, self._resets = np.zeros(nw, dtype=int), np.zeros(nw, dtype=int)
         if self.parallel_trajs:
             rep = np.repeat(np.arange(nw), self.umbr_traj)
-            _, a, v = self._trajectories(struc[rep], xi_wins[rep], rep, struc)
+            _, a, v, h = self._trajectories(struc[rep], xi_wins[rep], rep, struc)
+            if h is not None:
+                self.histograms = h.reshape(nw, self.umbr_traj, -1).sum(axis=1)
             return xi_wins, a.reshape(nw, -1).mean(axis=1), v.reshape(nw, -1).mean(axis=1)
         average, variance = np.zeros(nw), np.zeros(nw)
+        hsum = None if self.histogram is None else np.zeros((nw, self.histogram[0]))
         q = struc.copy()
         wins = np.arange(nw)
         for _ in range(self.umbr_traj):
-            q, a, v = self._trajectories(q, xi_wins, wins, struc)
+            q, a, v, h = self._trajectories(q, xi_wins, wins, struc)
             average += a
             variance += v
+            if h is not None:
+                hsum += h
+        self.histograms = hsum
         return xi_wins, average / self.umbr_traj, variance / self.umbr_traj

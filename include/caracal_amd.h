@@ -272,6 +272,20 @@ int cam_engine_begin_umbrella(cam_engine *eng, int bias, double epot_max, double
  * forms them (sum/n, sum of squares/n - average^2).  Host arrays of ntraj. */
 int cam_engine_umbrella_stats(cam_engine *eng, double *average, double *variance);
 
+/* UMBRELLA: histograms of the same samples, for WHAM (src/wham.f90, which
+ * calc_rate feeds with per-window histograms of xi, src/calc_rate.f90:1793-1840;
+ * the reference's writer of those histograms is commented out,
+ * src/calc_rate.f90:1712-1724, so they are collected here on the device).
+ * Every step's xi counts into bin j = nint((xi - xi_min) / bin_size),
+ * 1 <= j <= nbins, the bin src/wham.f90:104-106 gives the coordinate
+ * xi_min + j bin_size; samples outside are not counted.  nbins = 0 turns the
+ * histograms off.  cam_engine_begin_umbrella clears them.  Captured graphs
+ * are dropped.  No counterpart in the reference's engine. */
+int cam_engine_set_umbrella_histogram(cam_engine *eng, int nbins, double xi_min, double bin_size);
+/* counts[traj * nbins + j - 1]: samples of trajectory traj in bin j since
+ * cam_engine_begin_umbrella (host array of ntraj * nbins). */
+int cam_engine_umbrella_histogram(cam_engine *eng, double *counts);
+
 /* Advance every trajectory by nsteps velocity-Verlet RPMD steps.  Runs on the
  * engine's stream; all steps are captured in one HIP graph on first use and
  * replayed afterwards.  Asynchronous: call cam_engine_sync to wait.  The

@@ -110,3 +110,59 @@ def test_umbrella_window_check_and_errors():
     with pytest.raises(cam.CamError, match="remove_com"):
         cam.RPMDEngine(cam.CAM_SURFACE_H3, 8, 1, w["dt"], w["masses"], ensemble=cam.CAM_ENS_PARENT, reaction=rx,
                        remove_com=True)
+
+
+@pytest.mark.parametrize("surface,nb,schedule", [
+    (cam.CAM_SURFACE_CH4H, 8, cam.CAM_SCHED_AUTO),     # fused window step
+    (cam.CAM_SURFACE_H3, 16, cam.CAM_SCHED_SPLIT),     # split: umbrella_end_kernel
+])
+def test_umbrella_histograms_count_the_oracle_samples(surface, nb, schedule):
+    """Device histograms of the sampling steps (for WHAM): every window's
+    counts equal the oracle's xi of the same steps binned as
+    nint((xi - xi_min) / bin_size) on 1..nbins, leaving out samples within
+    1e-6 of a bin edge (rounding of near-equal xi may put them either side);
+    samples outside the range are not counted; begin_umbrella clears them."""
+    nwin, ast, seed, n_equi, n_samp = 6, 5, 33, 15, 40
+    w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb + 1)
+    xi_min, xi_max, nbins = 0.86, 1.10, 241
+    bs = (xi_max - xi_min) / (nbins - 1)
+    e = cam.RPMDEngine(surface, nb, nwin, w["dt"], w["masses"], ensemble=cam.CAM_ENS_UMBRELLA, reaction=rx,
+                       andersen_step=ast, seed=seed, schedule=schedule, steps_per_launch=9)
+    e.set_state(q, p, beta)
+    e.set_windows(xi_w, k_w)
+    e.set_umbrella_histogram(nbins, xi_min, bs)
+    e.begin_umbrella(bias=True)
+    e.run(n_equi)
+    assert e.umbrella_histogram().sum() > 0
+    e.begin_umbrella(bias=False)
+    assert e.umbrella_histogram().sum() == 0
+    e.run(n_samp)
+    hist = e.umbrella_histogram()
+    avg, _ = e.umbrella_stats()
+    s = ol.make_sys(surface, nb, w["dt"], w["masses"])
+    hinv = 1.0 / bs
+    for t in range(nwin):
+        orx = ol.make_rx(rx)
+        orx.xi_ideal = xi_w[t]
+        tr = ol.OracleTraj(s, q[t], p[t], float(beta[t]), rx=orx, ensemble=4, andersen_step=ast, seed=seed, traj=t)
+        tr.umbrella_begin(k_w[t], bias=True)
+        for _ in range(n_equi):
+            tr.step()
+        tr.umbrella_begin(k_w[t], bias=False)
+        xs = np.array([tr.step() for _ in range(n_samp)])
+        assert abs(avg[t] - xs.mean()) <= 1e-12
+        u = (xs - xi_min) * hinv
+        j = np.rint(u)
+        clear = np.abs(np.abs(u - np.floor(u)) - 0.5) > 1e-6
+        inside = (j >= 1) & (j <= nbins)
+        ref = np.zeros(nbins)
+        np.add.at(ref, (j[inside & clear] - 1).astype(int), 1.0)
+        near = np.zeros(nbins, dtype=bool)
+        for jj in np.floor(u[~clear]):
+            near[[int(k) for k in (jj - 1, jj) if 0 <= k < nbins]] = True
+        assert np.array_equal(hist[t][~near], ref[~near]), t
+        assert hist[t].sum() == (inside).sum() or (~clear).any()
+        print(f"surface={surface} window {xi_w[t]:.3f}: {int(hist[t].sum())} of {n_samp} samples counted, "
+              f"{int(inside.sum())} inside the range")
+    e.set_umbrella_histogram(0, 0.0, 1.0)
+    e.close()

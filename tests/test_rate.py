@@ -75,3 +75,57 @@ def test_rate_matches_oracle_and_the_atomic_unit_of_a_rate_coefficient():
     assert abs(rate.rate_bimolecular(1.0, 1000.0, 0.01, 0.0, (4.0, 4.0), 16.0) / k1 - 2) < 1e-14
     assert abs(rate.rate_bimolecular(0.5, 1000.0, 0.011, 0.0, (4.0, 4.0), 16.0) / k1 - np.exp(-1.0)) < 1e-12
     assert abs(rate.rate_bimolecular(0.5, 1000.0, 0.01, 0.0, (16.0, 16.0), 16.0) / k1 - 0.5) < 1e-14
+
+
+def _wham_golden():
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "wham_reference.npz"))
+    for n in range(int(z["ncases"])):
+        yield {k[len(f"c{n}_"):]: z[k] for k in z.files if k.startswith(f"c{n}_")}
+
+
+def test_wham_matches_the_compiled_reference():
+    """rate.wham against src/wham.f90 compiled from the reference
+    (tests/golden/make_wham_golden.py): multinomial window histograms of a
+    double well, and a range twice the windows' span whose empty bins take
+    the routine's 1e-15 floor.  The PMF in the reference's stored units."""
+    for c in _wham_golden():
+        coord, pmf, pref, it = rate.wham(c["hist"], c["xi_wins"], c["k_force"], float(c["xi_min"]),
+                                         float(c["xi_max"]), int(c["nbins"]), float(c["kelvin"]))
+        ref = c["pmf_ref"]
+        assert np.all(np.isfinite(ref))
+        np.testing.assert_allclose(pref, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        np.testing.assert_allclose(pmf * rate.KCAL_PER_HARTREE * rate.BOHR_ANGSTROM, pref, rtol=1e-15)
+        assert pmf.min() == 0.0
+        bs = (float(c["xi_max"]) - float(c["xi_min"])) / (int(c["nbins"]) - 1)
+        np.testing.assert_allclose(np.diff(coord), bs, rtol=1e-12)
+
+
+def test_wham_recovers_the_pmf_from_exact_histograms():
+    """Histograms proportional to exp(-beta (A + V_i)) on the bin
+    coordinates are WHAM's fixed point: converged tightly, the PMF is A up to
+    a constant (a wrong bias coordinate, sign or normalisation moves it)."""
+    kelvin = 300.0
+    beta = 1.0 / rate.KB_HARTREE / kelvin
+    xi_min, xi_max, nbins = -0.2, 1.2, 281
+    bs = (xi_max - xi_min) / (nbins - 1)
+    x = xi_min + bs * np.arange(1, nbins + 1)
+    A = 0.004 * np.sin(4.0 * x) + 0.002 * x
+    xw, k = np.linspace(-0.1, 1.1, 13), np.linspace(0.04, 0.07, 13)
+    h = np.array([np.exp(-beta * (A + ki * (x - xi) ** 2)) for xi, ki in zip(xw, k)])
+    h = 1e6 * h / h.sum(axis=1)[:, None]
+    _, pmf, _, _ = rate.wham(h, xw, k, xi_min, xi_max, nbins, kelvin, tol=1e-20)
+    np.testing.assert_allclose(pmf, A - A.min(), rtol=0, atol=1e-10)
+    # the default tolerance stops early but close
+    _, pmf5, _, it = rate.wham(h, xw, k, xi_min, xi_max, nbins, kelvin)
+    assert it < 100 and np.abs(pmf5 - (A - A.min())).max() < 1e-4
+
+
+def test_wham_needs_samples_in_every_window():
+    h = np.ones((3, 11))
+    h[1] = 0.0
+    try:
+        rate.wham(h, [0.0, 0.5, 1.0], 0.05, 0.0, 1.0, 11, 300.0)
+    except ValueError:
+        return
+    raise AssertionError("an empty window must be rejected")
