@@ -1,7 +1,7 @@
-// Folded step kernels of the H3 and OH3 surfaces, the dispatch over all
-// folded kernels and the folded transform matrix.  The kernels themselves are
-// in fold.hpp; the CH5 ones are built in fold_ch4h.hip, with other code
-// generation options.
+// Folded step kernels of the OH3 surface, the dispatch over all folded
+// kernels and the folded transform matrix.  The kernels themselves are in
+// fold.hpp; the CH5 and H3 ones are built in fold_ch4h.hip and fold_h3.hip,
+// with other code generation options.
 #include <cmath>
 
 #include "fold.hpp"
@@ -9,14 +9,13 @@
 namespace cam {
 
 step_fn fold_kernel_ch4h(int precision, int nbeads, int ensemble, int wpe, int *tpb);
+step_fn fold_kernel_h3(int precision, int nbeads, int ensemble, int *tpb);
 
 step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int *tpb)
 {
     if (ensemble != CAM_ENS_NVE && ensemble != CAM_ENS_CHILD) return nullptr;
     const bool f64 = precision == CAM_PREC_FP64;
-    if (surface == CAM_SURFACE_H3)
-        return f64 ? fold::pick_fold_w<SurfH3, double, 1>(nbeads, ensemble, tpb)
-                   : fold::pick_fold_w<SurfH3, float, 1>(nbeads, ensemble, tpb);
+    if (surface == CAM_SURFACE_H3) return fold_kernel_h3(precision, nbeads, ensemble, tpb);
     if (surface == CAM_SURFACE_OH3) return f64 ? fold::pick_fold_w<SurfOH3, double, 1>(nbeads, ensemble, tpb) : nullptr;
     return fold_kernel_ch4h(precision, nbeads, ensemble, wpe, tpb);
 }

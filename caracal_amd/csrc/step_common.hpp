@@ -18,20 +18,19 @@ constexpr int kBlock = 256;
 // ---------------------------------------------------------------------------
 // Surface policies: how the coordinates of one bead are spread over its L lanes.
 // ---------------------------------------------------------------------------
-// The step kernels evaluate the H3 surface in FP64 for either precision:
-// its binary32 correction terms, inlined into a kernel that evaluates forces
-// more than once (several steps per launch), returned wrong forces from the
-// second evaluation on (positions right, forces off by orders of magnitude),
-// in 12 of the 26 mixed-precision H3 step kernels, while the same source on
-// the host (under AddressSanitizer and UBSan), in one-step launches and in
-// every FP64 kernel was right.  On H3 those terms are a small part of the
-// work; the FP64 path is the one every H3 kernel is tested with.
+// H3: one lane per bead; T = float evaluates the surface's asymmetric and
+// compact correction terms in binary32 (mixed precision).  Every translation
+// unit that instantiates a step kernel with it is compiled without
+// machine-level loop-invariant code motion: with that pass, those kernels
+// (inlined surface in a multi-step loop, hundreds of hoisted 64-bit constants
+// spilled from SGPRs to VGPR lanes) returned NaN forces in 12 of 26
+// configurations, and without it all 26 are within 2e-7 of the oracle.
 struct SurfH3 {
     static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
     __device__ static int coord(int slot, int) { return slot; }
     __device__ static bool valid(int, int) { return true; }
     template <typename T>
-    __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<double>(own, v, g); }
+    __device__ static void egrad(const double *own, double &v, double *g) { h3::egrad_bead<T>(own, v, g); }
 };
 
 // OH + H2: one lane per bead, FP64 only (the engine rejects mixed precision).

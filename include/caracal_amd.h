@@ -81,11 +81,11 @@ int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, doub
 
 /* ---- (2) trajectory-batch RPMD engine ---- */
 enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3 };
-/* Precision of the engine.  MIXED: CH4H forces in FP32 arithmetic with FP64
- * positions, momenta, accumulation and normal modes.  The H3 surface is
- * evaluated in FP64 by the engine under either setting (its FP32 correction
- * terms gave wrong forces in multi-step kernels); cam_egrad_h3_dev keeps them.
- * The OH3 surface runs in FP64 only (CAM_PREC_MIXED is rejected). */
+/* Precision of the engine.  MIXED: forces in FP32 arithmetic with FP64
+ * positions, momenta, accumulation and normal modes (CH4H: the whole
+ * surface; H3: its asymmetric and compact correction terms, as
+ * cam_egrad_h3_dev).  The OH3 surface runs in FP64 only (CAM_PREC_MIXED is
+ * rejected). */
 enum { CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1 };
 /* Normal-mode transform.  REFERENCE reproduces the reference's rfft/irfft pair
  * (src/rfft.f90:36-62, src/irfft.f90:36-63): both directions are the
