@@ -498,10 +498,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const int tid = threadIdx.x;
     const int r = tid % L;
     const int bib = tid / L;                 // bead in block
-    const int tl = bib / NB, k = bib % NB;
+    // When NB does not divide the block's bead slots, the slots past the last
+    // whole ring polymer repeat beads 0.. of the block's last one: they
+    // compute the same values as those lanes, write the same values to LDS,
+    // and nothing to memory.
+    const int tl = min(bib / NB, TPB - 1), k = bib % NB;
+    const bool live = bib < TPB * NB;        // not such a repeat: does the per-trajectory work
     const int traj_raw = blockIdx.x * TPB + tl;
-    const bool act = traj_raw < a.ntraj;
-    const int traj = act ? traj_raw : a.ntraj - 1;
+    const bool act = live && traj_raw < a.ntraj;
+    const int traj = traj_raw < a.ntraj ? traj_raw : a.ntraj - 1;
     double *xs = X + tl * NC * NB;
 
     int cc[SL];
@@ -566,7 +571,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (S::valid(s, r)) part += p[s] * p[s] / (2.0 * (TAB ? smass[cc[s] / 3] : m[s]));
         nred[tid] = part;
         __syncthreads();
-        if (k == 0 && r == 0) {
+        if (live && k == 0 && r == 0) {
             double ek = 0.0;
             for (int i = 0; i < NB * L; ++i) ek += nred[tl * NB * L + i];
             nscale[tl] = nhc_chain(vnh, qnh, ek / NB / NB, ekt_chain, NC, a.dt);
@@ -721,7 +726,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             }
             if constexpr (CENT) {
                 // centroid of the propagated beads = mode 0 of q / sqrt(NB)
-                if (k == 0) {
+                if (live && k == 0) {
 #pragma unroll
                     for (int s = 0; s < SL; ++s)
                         if (S::valid(s, r)) cent[tl * NC + cc[s]] = yq[s] * (1.0 / kSqrt<NB>());
@@ -819,10 +824,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             bool qbad = false;
 #pragma unroll
             for (int s = 0; s < SL; ++s) qbad = qbad || (S::valid(s, r) && (q[s] != q[s] || q[s] > DBL_MAX));
-            if (r == 0) uvb[tl * NB + k] = vb;
-            if (qbad) ubad[tl] = 1;
+            if (live && r == 0) uvb[tl * NB + k] = vb;
+            if (live && qbad) ubad[tl] = 1;
             __syncthreads();
-            if (k == 0 && r == 0) {
+            if (live && k == 0 && r == 0) {
                 double dxi[NC], gk[NC], gh[NC], xi1;
                 const double xw = a.uwin[2 * traj], kw = a.uwin[2 * traj + 1];
                 (void)umbrella_bias<S::NA>(a.rx, cent + tl * NC, beta, kw, xw, xi1, dxi, gk, gh);
@@ -964,10 +969,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const double *__restrict__ qa, double *__restrict__ ga, double *__restrict__ vbead, int ntraj)
 {
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS, BPB = kBlock / L, TPB = BPB / NB;
-    const int tid = threadIdx.x, r = tid % L, bib = tid / L, tl = bib / NB, k = bib % NB;
+    // slots past the last whole ring polymer repeat beads of the last one and write nothing
+    const int tid = threadIdx.x, r = tid % L, bib = tid / L, tl = min(bib / NB, TPB - 1), k = bib % NB;
     const int traj_raw = blockIdx.x * TPB + tl;
-    const bool act = traj_raw < ntraj;
-    const int traj = act ? traj_raw : ntraj - 1;
+    const bool act = bib < TPB * NB && traj_raw < ntraj;
+    const int traj = traj_raw < ntraj ? traj_raw : ntraj - 1;
     const size_t base = (size_t)traj * NC * NB;
     double x[SL], g[SL], vb;
 #pragma unroll
@@ -1671,7 +1677,7 @@ namespace {
 template <class S, int NB, typename T, int WPE, bool SPLIT = false>
 step_fn pick_step_w(int ens, bool mfma)
 {
-    constexpr bool MF = NB >= 16 && NB % 16 == 0;
+    constexpr bool MF = mfma_bead_count(NB);
     // umbrella windows and the Nose-Hoover chain: the fused step only, with
     // the vector-ALU transform
     if constexpr (!SPLIT && WPE == 1) {
@@ -1734,9 +1740,13 @@ bool pick_nb(int nb, int ens, int ens_nm, int wpe, bool mfma, step_fn &st, force
         CAM_NB_CASE(1)
         CAM_NB_CASE(2)
         CAM_NB_CASE(4)
+        CAM_NB_CASE(6)
         CAM_NB_CASE(8)
+        CAM_NB_CASE(12)
         CAM_NB_CASE(16)
+        CAM_NB_CASE(24)
         CAM_NB_CASE(32)
+        CAM_NB_CASE(48)
         CAM_NB_CASE(64)
 #undef CAM_NB_CASE
     default:
@@ -2118,7 +2128,7 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     const int na = surf == CAM_SURFACE_H3 ? 3 : surf == CAM_SURFACE_OH3 ? 4 : 6;
     CAM_CHECK_ARG(cfg->natoms == na, "surface %d needs natoms = %d (got %d)", surf, na, cfg->natoms);
     const int nb = cfg->nbeads;
-    CAM_CHECK_ARG(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, "nbeads must be a power of two in 1..64 (got %d)", nb);
+    CAM_CHECK_ARG(bead_count_supported(nb), "nbeads must be 1, 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64 (got %d)", nb);
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
     CAM_CHECK_ARG(surf != CAM_SURFACE_OH3 || cfg->precision == CAM_PREC_FP64, "the OH3 surface runs in FP64 only");
@@ -2130,8 +2140,8 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
                   "ensemble NHC runs on the fused schedule with the vector-ALU transform");
     const bool steered = cfg->ensemble == CAM_ENS_PARENT || cfg->ensemble == CAM_ENS_UMBRELLA;
     CAM_CHECK_ARG(cfg->nm_kernel >= CAM_NMK_AUTO && cfg->nm_kernel <= CAM_NMK_FOLDED, "bad nm_kernel");
-    CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || (nb >= 16 && nb % 16 == 0),
-                  "nm_kernel MFMA needs nbeads a multiple of 16 (got %d)", nb);
+    CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_MFMA || mfma_bead_count(nb),
+                  "nm_kernel MFMA needs nbeads 16, 32 or 64 (got %d)", nb);
     CAM_CHECK_ARG(cfg->nm_kernel != CAM_NMK_FOLDED || fold_eligible(cfg),
                   "nm_kernel FOLDED needs nm_mode REFERENCE, nbeads >= 4, ensemble NVE or CHILD, no remove_com "
                   "and the fused schedule");
@@ -2734,7 +2744,7 @@ static int autotune(cam_engine *e, int trial_steps, cam_tuning *out, bool slices
     if (c.nm_kernel != CAM_NMK_AUTO) nmks.push_back(c.nm_kernel);
     else {
         nmks.push_back(CAM_NMK_VALU);
-        if (nb >= 16 && nb % 16 == 0) nmks.push_back(CAM_NMK_MFMA);
+        if (mfma_bead_count(nb)) nmks.push_back(CAM_NMK_MFMA);
         if (fold_available(&c)) nmks.push_back(CAM_NMK_FOLDED);
     }
     if (c.remove_com) spls.push_back(1);

@@ -373,9 +373,13 @@ step_fn pick_fold_w(int nb, int ens, int *tpb)
         return ens == CAM_ENS_NVE ? fold_step_kernel<S, N, T, CAM_ENS_NVE, WPE>               \
                                   : fold_step_kernel<S, N, T, CAM_ENS_CHILD, WPE>;
         CAM_FOLD_CASE(4)
+        CAM_FOLD_CASE(6)
         CAM_FOLD_CASE(8)
+        CAM_FOLD_CASE(12)
         CAM_FOLD_CASE(16)
+        CAM_FOLD_CASE(24)
         CAM_FOLD_CASE(32)
+        CAM_FOLD_CASE(48)
         CAM_FOLD_CASE(64)
 #undef CAM_FOLD_CASE
     default:

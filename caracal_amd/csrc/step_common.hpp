@@ -125,12 +125,28 @@ struct PropTab {
 // FP64 MFMA accumulator of v_mfma_f64_16x16x4f64
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// sqrt(NB), correctly rounded, for the bead counts the step kernels are built for
 template <int NB>
 constexpr double kSqrt()
 {
-    return NB == 1 ? 1.0 : NB == 2 ? 1.4142135623730951 : NB == 4 ? 2.0 : NB == 8 ? 2.8284271247461903
-         : NB == 16 ? 4.0 : NB == 32 ? 5.656854249492381 : 8.0;
+    static_assert(NB == 1 || NB == 2 || NB == 4 || NB == 6 || NB == 8 || NB == 12 || NB == 16 || NB == 24 ||
+                      NB == 32 || NB == 48 || NB == 64,
+                  "no sqrt for this bead count");
+    return NB == 1 ? 1.0 : NB == 2 ? 1.4142135623730951 : NB == 4 ? 2.0 : NB == 6 ? 2.449489742783178
+         : NB == 8 ? 2.8284271247461903 : NB == 12 ? 3.4641016151377544 : NB == 16 ? 4.0
+         : NB == 24 ? 4.898979485566356 : NB == 32 ? 5.656854249492381 : NB == 48 ? 6.928203230275509 : 8.0;
 }
+
+// Bead counts with step kernels: the powers of two to 64 and 6, 12, 24, 48
+// (even, so that the vector-ALU transform reads bead pairs).  The MFMA
+// transform needs NB a multiple of 16 whose 16-row bands map onto the four
+// waves of a block (16, 32, 64).
+constexpr bool bead_count_supported(int nb)
+{
+    return nb == 1 || nb == 2 || nb == 4 || nb == 6 || nb == 8 || nb == 12 || nb == 16 || nb == 24 || nb == 32 ||
+           nb == 48 || nb == 64;
+}
+constexpr bool mfma_bead_count(int nb) { return nb >= 16 && nb % 16 == 0 && 4 % (nb / 16) == 0; }
 
 typedef void (*step_fn)(StepArgs);
 

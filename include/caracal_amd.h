@@ -129,7 +129,7 @@ enum { CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, CAM_ENS_PARENT 
        CAM_ENS_NHC = 5 };
 /* Which unit computes the normal-mode transform inside the step kernel:
  * VALU (per-lane dot products over the beads) or MFMA (FP64 matrix-core tiles,
- * nbeads a multiple of 16).  FOLDED: the reference transform (rfft/irfft, the
+ * nbeads 16, 32 or 64).  FOLDED: the reference transform (rfft/irfft, the
  * real part of the DFT both ways) maps every ring polymer onto one that is
  * symmetric under the bead reflection j -> nbeads - j, so a step only has
  * nbeads/2 + 1 distinct beads; this kernel carries those, evaluates their
@@ -170,7 +170,7 @@ typedef struct {
 typedef struct {
     int32_t surface;          /* CAM_SURFACE_* */
     int32_t natoms;           /* 3 (H3), 6 (CH4H) or 4 (OH3) */
-    int32_t nbeads;           /* 1, 2, 4, 8, 16, 32 or 64 */
+    int32_t nbeads;           /* 1, 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64 */
     int32_t ntraj;            /* trajectories in the batch */
     int32_t precision;        /* CAM_PREC_* */
     int32_t nm_mode;          /* CAM_NM_* */

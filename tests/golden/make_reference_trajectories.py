@@ -19,10 +19,14 @@ stores inputs and frames as small .npz fixtures beside this script:
   oh3_child_xi05_reference.npz  an OH + H2 recrossing child with the dividing
                               surface at xi_ideal = 0.5 (so xi = (s0 + s1)/2
                               depends on the mass-weighted fragment centres)
+  h3_nve_6beads_reference.npz   C1's system with 6 beads (a bead count that is
+                              not a power of two), NVE with transrot
+  ch4h_child_12beads_reference.npz  a CH4 + H recrossing child with 12 beads
 
 Frames are kept at steps 1, 2, 10, 50 and 100 (q and p), with epot and xi
 at every step.  Run in the container where /root/reference is mounted:
-    python tests/golden/make_reference_trajectories.py
+    python tests/golden/make_reference_trajectories.py [fixture names]
+(no names: all of them).
 """
 import dataclasses
 import os
@@ -50,32 +54,49 @@ def _save(name, surface, q, p, masses, dt, beta, constrain, reaction=None, **kw)
              q=fr["q"][KEEP - 1], p=fr["p"][KEEP - 1], epot=fr["epot"], xi=fr["xi"], **extra)
 
 
-def child_start():
+def child_start(nb=32):
     """A CH4 + H child start: the seeded saddle perturbation of the C2
     workload (first trajectory) with momenta at 500 K from seed 7."""
-    w = workloads.ch4h_children(ntraj=1, nbeads=32)
+    w = workloads.ch4h_children(ntraj=1, nbeads=nb)
     rng = np.random.default_rng(7)
-    p = workloads.mb_momenta(rng, w["masses"], 32, w["beta"][0], (32, 6, 3))
+    p = workloads.mb_momenta(rng, w["masses"], nb, w["beta"][0], (nb, 6, 3))
     return w, w["q"][0], p
 
 
-def main():
+def main(names):
     if not refstep.available():
         raise SystemExit("oracle/_ref/ref_step_O1 is not built (make -C oracle -f Makefile.ref)")
     h = workloads.h3_nve(nbeads=16)
-    _save("c1_h3_nve_reference.npz", "h3", h["q"], h["p"], h["masses"], h["dt"], h["beta"], -1)
     w, q, p = child_start()
-    _save("ch4h_child_reference.npz", "ch4h", q, p, w["masses"], w["dt"], w["beta"][0], 2, w["reaction"])
     o = workloads.oh3_nve(nbeads=8)
-    _save("oh3_nve_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], -1)
     h8 = workloads.h3_nve(nbeads=8)
-    _save("h3_window_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], 0,
-          dataclasses.replace(workloads.h3_reaction(), xi_ideal=0.9), k_force=0.5)
-    _save("h3_nhc_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], -1,
-          thermostat=2, kelvin=1.0 / (0.316679e-5 * h8["beta"]), nose_q=20 * h8["dt"])
-    _save("oh3_child_xi05_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], 2,
-          dataclasses.replace(workloads.oh3_reaction(), xi_ideal=0.5))
+    h6 = workloads.h3_nve(nbeads=6)
+    w12, q12, p12 = child_start(12)
+    jobs = {
+        "c1_h3_nve_reference.npz": lambda: _save("c1_h3_nve_reference.npz", "h3", h["q"], h["p"], h["masses"],
+                                                 h["dt"], h["beta"], -1),
+        "ch4h_child_reference.npz": lambda: _save("ch4h_child_reference.npz", "ch4h", q, p, w["masses"], w["dt"],
+                                                  w["beta"][0], 2, w["reaction"]),
+        "oh3_nve_reference.npz": lambda: _save("oh3_nve_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"],
+                                               o["beta"], -1),
+        "h3_window_reference.npz": lambda: _save(
+            "h3_window_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], 0,
+            dataclasses.replace(workloads.h3_reaction(), xi_ideal=0.9), k_force=0.5),
+        "h3_nhc_reference.npz": lambda: _save(
+            "h3_nhc_reference.npz", "h3", h8["q"], h8["p"], h8["masses"], h8["dt"], h8["beta"], -1,
+            thermostat=2, kelvin=1.0 / (0.316679e-5 * h8["beta"]), nose_q=20 * h8["dt"]),
+        "oh3_child_xi05_reference.npz": lambda: _save(
+            "oh3_child_xi05_reference.npz", "oh3", o["q"], o["p"], o["masses"], o["dt"], o["beta"], 2,
+            dataclasses.replace(workloads.oh3_reaction(), xi_ideal=0.5)),
+        "h3_nve_6beads_reference.npz": lambda: _save("h3_nve_6beads_reference.npz", "h3", h6["q"], h6["p"],
+                                                     h6["masses"], h6["dt"], h6["beta"], -1),
+        "ch4h_child_12beads_reference.npz": lambda: _save("ch4h_child_12beads_reference.npz", "ch4h", q12, p12,
+                                                          w12["masses"], w12["dt"], w12["beta"][0], 2,
+                                                          w12["reaction"]),
+    }
+    for name in names or list(jobs):
+        jobs[name]()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -57,7 +57,7 @@ def test_argument_errors_are_reported_not_aborted():
     h = ctypes.c_void_p()
     assert lib.cam_engine_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
     assert b"surface" in lib.cam_last_error()
-    cfg.surface, cfg.natoms, cfg.nbeads, cfg.ntraj, cfg.dt = 1, 3, 12, 1, 1.0
+    cfg.surface, cfg.natoms, cfg.nbeads, cfg.ntraj, cfg.dt = 1, 3, 10, 1, 1.0
     assert lib.cam_engine_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
     assert b"nbeads" in lib.cam_last_error()
 

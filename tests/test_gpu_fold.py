@@ -19,7 +19,7 @@ def _mirror_symmetric(x):
     return all(np.array_equal(x[..., j, :, :], x[..., (nb - j) % nb, :, :]) for j in range(nb))
 
 
-@pytest.mark.parametrize("nb", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("nb", [4, 6, 8, 12, 16, 24, 32, 48, 64])
 def test_folded_h3_nve_trajectory_matches_oracle_100_steps(nb):
     w = workloads.h3_nve(nbeads=nb)
     rng = np.random.default_rng(nb)
@@ -190,7 +190,7 @@ def _batch(surface, nb, ntraj, seed):
 
 
 @pytest.mark.parametrize("surface", [cam.CAM_SURFACE_H3, cam.CAM_SURFACE_CH4H])
-@pytest.mark.parametrize("nb", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("nb", [4, 6, 8, 12, 16, 24, 32, 48, 64])
 @pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
 @pytest.mark.parametrize("ens", [cam.CAM_ENS_NVE, cam.CAM_ENS_CHILD])
 def test_every_folded_kernel_is_deterministic_and_tracks_the_unfolded_one(surface, nb, prec, ens):

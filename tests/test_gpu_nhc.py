@@ -28,9 +28,12 @@ def _batch(surface, nb, ntraj, seed):
 
 
 @pytest.mark.parametrize("surface,nb", [(cam.CAM_SURFACE_H3, 8), (cam.CAM_SURFACE_CH4H, 4),
-                                        (cam.CAM_SURFACE_CH4H, 16), (cam.CAM_SURFACE_H3, 1)])
+                                        (cam.CAM_SURFACE_CH4H, 16), (cam.CAM_SURFACE_H3, 1),
+                                        (cam.CAM_SURFACE_H3, 6), (cam.CAM_SURFACE_CH4H, 24)])
 def test_nhc_matches_oracle(surface, nb):
-    ntraj, nsteps = 5, 50
+    # 6 and 24 beads leave slots of a block past its last ring polymer: enough
+    # trajectories that a block is full and the next one holds the rest
+    ntraj, nsteps = (50 if nb == 6 else 5 if nb != 24 else 5), 50
     w, q, p, beta = _batch(surface, nb, ntraj, seed=nb + 3)
     nose_q = 20.0 * w["dt"]
     e = cam.RPMDEngine(surface, nb, ntraj, w["dt"], w["masses"], ensemble=cam.CAM_ENS_NHC, nose_q=nose_q,

@@ -60,6 +60,39 @@ def test_ch4h_child_matches_reference(nmk):
     assert abs(xi - f["xi"][-1]) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [cam.CAM_PREC_FP64, cam.CAM_PREC_MIXED])
+def test_h3_nve_6_beads_matches_reference(prec):
+    """A bead count that is not a power of two (the reference's FFTW
+    transforms take any): 6 beads, NVE with transrot; mixed precision to its
+    binary32 corrections' level."""
+    f = dict(np.load(os.path.join(GOLDEN, "h3_nve_6beads_reference.npz")))
+    nb = f["q0"].shape[0]
+    assert nb == 6
+    with cam.RPMDEngine(cam.CAM_SURFACE_H3, nb, 1, float(f["dt"]), f["masses"], remove_com=True,
+                        precision=prec) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        q, p = _frames(e, f["steps"])
+    if prec == cam.CAM_PREC_FP64:
+        _check("H3 6 beads", q, p, f)
+    else:
+        assert np.abs(q - f["q"]).max() <= 1e-6 * np.abs(f["q"]).max()
+
+
+@pytest.mark.parametrize("nmk", [cam.CAM_NMK_VALU, cam.CAM_NMK_FOLDED])
+def test_ch4h_child_12_beads_matches_reference(nmk):
+    f = dict(np.load(os.path.join(GOLDEN, "ch4h_child_12beads_reference.npz")))
+    nb = f["q0"].shape[0]
+    assert nb == 12
+    with cam.RPMDEngine(cam.CAM_SURFACE_CH4H, nb, 1, float(f["dt"]), f["masses"], ensemble=cam.CAM_ENS_CHILD,
+                        reaction=workloads.ch4h_reaction(), nm_kernel=nmk) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        e.begin_children()
+        q, p = _frames(e, f["steps"])
+        xi = e.observables()["xi"][0]
+    _check(f"CH4H child 12 beads nm_kernel={nmk}", q, p, f)
+    assert abs(xi - f["xi"][-1]) < 1e-10
+
+
 def test_oh3_nve_matches_reference():
     f = dict(np.load(os.path.join(GOLDEN, "oh3_nve_reference.npz")))
     nb = f["q0"].shape[0]

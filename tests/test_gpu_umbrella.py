@@ -38,7 +38,9 @@ def _windows(surface, nb, nwin, seed):
     (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT, False),
     (cam.CAM_SURFACE_H3, 16, cam.CAM_NMK_VALU, cam.CAM_SCHED_SPLIT, True),
     (cam.CAM_SURFACE_H3, 1, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),     # one bead: its own centroid
-    (cam.CAM_SURFACE_CH4H, 4, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False)])
+    (cam.CAM_SURFACE_CH4H, 4, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),
+    (cam.CAM_SURFACE_H3, 6, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, False),      # bead counts that leave
+    (cam.CAM_SURFACE_CH4H, 12, cam.CAM_NMK_VALU, cam.CAM_SCHED_AUTO, True)])   # slots of a block unused
 def test_umbrella_windows_match_oracle(surface, nb, nmk, schedule, remove_com):
     nwin, ast, seed, n_equi, n_samp = 6, 5, 21, 20, 30
     w, q, p, beta, rx, xi_w, k_w = _windows(surface, nb, nwin, seed=nb)

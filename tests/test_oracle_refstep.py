@@ -25,7 +25,9 @@ CASES = [("c1_h3_nve_reference.npz", 1, -1, 0, None),
          ("oh3_nve_reference.npz", 3, -1, 0, None),
          ("h3_window_reference.npz", 1, 0, 0, workloads.h3_reaction),
          ("h3_nhc_reference.npz", 1, -1, 2, None),
-         ("oh3_child_xi05_reference.npz", 3, 2, 0, workloads.oh3_reaction)]
+         ("oh3_child_xi05_reference.npz", 3, 2, 0, workloads.oh3_reaction),
+         ("h3_nve_6beads_reference.npz", 1, -1, 0, None),
+         ("ch4h_child_12beads_reference.npz", 2, 2, 0, workloads.ch4h_reaction)]
 SURF = {1: "h3", 2: "ch4h", 3: "oh3"}
 
 
