@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "erfc_tables.hpp"
 #include "common.hpp"
 #include "nhc.hpp"
 #include "philox.hpp"
@@ -34,15 +35,10 @@ constexpr int kNP = 10;       // parameters
 constexpr int kNE = 1 + 2 * kNP;   // energy + dE/dtheta + d(v.g)/dtheta
 constexpr int kTile = 64;
 
-constexpr int kErfcTerms = 40;
-
 struct FFDev {
     int natoms, nbonds, nangles;
     double box, rc, za, zp;
-    // erfcx on [0, exmax] as a polynomial in t = x ecx - 1 (monomial
-    // coefficients of its Chebyshev series): FP64 and binary32 copies
-    double ecf[kErfcTerms], ecx, exmax;
-    float ecf32[kErfcTerms];
+    double exmax;                  // a rc, bounded to [1e-3, 6]: erfc's argument range
     double sr42[2][2], r0bj[2][2], zrep[2][2], arep[2][2];
     const int *type, *mol, *bonds, *angles, *csr_ptr, *csr_term;
 };
@@ -435,26 +431,55 @@ __global__ __launch_bounds__(256) void pl_rows_kernel(PairDev p, const double *_
         pl_row(p, x, s, lane);
 }
 
-// erfc(x) for x >= 0 from the Chebyshev series of erfcx(x) = exp(x^2) erfc(x)
-// on [0, X] (Clenshaw) and ex2 = exp(-x^2); beyond X (X = min(a rc, 6)) the
-// series is held at X, where erfc is below 2.2e-17.
-// NT terms (a compile-time count, so the coefficients come from the kernel
-// arguments at fixed offsets): 24 FP64 / 16 binary32 terms for X <= 2.5,
-// 32 / 24 beyond.
+// erfc(x) for x >= 0 as ex2 = exp(-x^2) times the polynomial of erfcx(x) =
+// exp(x^2) erfc(x) on [0, X] (erfc_tables.hpp; Horner in t = 2x/X - 1), X =
+// 2.5 where a rc <= 2.5 and 6 beyond (WIDE); past X the polynomial is held at
+// X, where erfc is below 2.2e-17.  The coefficients are compile-time
+// constants: loaded from the kernel arguments, they were held in SGPRs across
+// the pair loop and, spilled to VGPR lanes, read back with a VALU
+// instruction each.
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double rint_t(double a) { return __builtin_rint(a); }
 __device__ __forceinline__ float rint_t(float a) { return __builtin_rintf(a); }
 
-template <typename T, int NT>
-__device__ __forceinline__ T erfc_series(const FFDev &f, T x, T ex2)
+// The FP64 coefficients are read from LDS (sc: a block's copy of the table)
+// through an address the compiler must treat as new at every use: one LDS
+// read per coefficient next to the polynomial, into a register the FMA then
+// accumulates in.  As constants they were held across the pair loop (in
+// SGPRs spilled to VGPR lanes, or in VGPRs) or copied into a VGPR one 32-bit
+// half at a time for every use (a 64-bit constant has no inline form in a
+// VOP3 instruction).
+typedef __attribute__((address_space(3))) const double lds_double;
+
+__device__ __forceinline__ lds_double *opaque_lds(lds_double *p)
+{
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+template <typename T, bool WIDE>
+__device__ __forceinline__ T erfc_series(T x, T ex2, lds_double *sc)
 {
 #pragma clang fp contract(off)
-    const T t = fma_t(min(x, (T)f.exmax), (T)f.ecx, (T)-1);
-    T h = sizeof(T) == 8 ? (T)f.ecf[NT - 1] : (T)f.ecf32[NT - 1];
+    namespace E = erfc_tab;
+    constexpr double X = WIDE ? E::kWideX : E::kNarrowX;
+    const T t = fma_t(min(x, (T)X), (T)(2.0 / X), (T)-1);
+    if constexpr (sizeof(T) == 8) {
+        constexpr int NT = WIDE ? 36 : 24;
+        lds_double *c = opaque_lds(sc);
+        T h = c[NT - 1];
 #pragma unroll
-    for (int k = NT - 2; k >= 0; --k) h = fma_t(h, t, sizeof(T) == 8 ? (T)f.ecf[k] : (T)f.ecf32[k]);
-    return ex2 * h;
+        for (int k = NT - 2; k >= 0; --k) h = fma_t(h, t, (T)c[k]);
+        return ex2 * h;
+    } else {
+        constexpr int NT = WIDE ? 24 : 16;
+        const float *c = WIDE ? E::kWide32 : E::kNarrow32;
+        T h = c[NT - 1];
+#pragma unroll
+        for (int k = NT - 2; k >= 0; --k) h = fma_t(h, t, (T)c[k]);
+        return ex2 * h;
+    }
 }
 
 // Pair-loop arithmetic.  FP64: reciprocal and reciprocal square root from
@@ -498,6 +523,11 @@ __device__ __forceinline__ void nonbonded_body(const FFDev &f, const PairDev &pl
     __shared__ T sv[PAR ? kC : 1][3];
     __shared__ int smt[kC];
     __shared__ double red[kBrick / 64][11];
+    __shared__ double scoef[36];                  // FP64 erfcx coefficients (erfc_series)
+    if constexpr (sizeof(T) == 8) {
+        const double *tab = WIDE ? erfc_tab::kWide64 : erfc_tab::kNarrow64;
+        if (threadIdx.x < (WIDE ? 36 : 24)) scoef[threadIdx.x] = tab[threadIdx.x];
+    }
     const int n = f.natoms;
     const int tid = threadIdx.x;
     const int brick = blockIdx.x, split = blockIdx.y, b = blockIdx.z, S = gridDim.y;
@@ -556,8 +586,7 @@ __device__ __forceinline__ void nonbonded_body(const FFDev &f, const PairDev &pl
             const T c6 = jo ? c6t0 : c6t1;
             const T xa = za * r;
             const T ex2 = exp_pair(-(xa * xa));
-            constexpr int NT = sizeof(T) == 8 ? (WIDE ? 32 : 24) : (WIDE ? 24 : 16);
-            const T ev = erfc_series<T, NT>(f, xa, ex2);
+            const T ev = erfc_series<T, WIDE>(xa, ex2, (lds_double *)scoef);
             const T R0 = jo ? r0t0 : r0t1, sr = jo ? srt0 : srt1;
             const T R02 = R0 * R0, R06 = R02 * R02 * R02;
             const T r4 = r2 * r2, r6 = r4 * r2;
@@ -1080,54 +1109,12 @@ struct cam_ff {
 
 namespace {
 
-// erfcx(x) = exp(x^2) erfc(x) on [0, X], X = min(a rc, 6) (beyond 6, erfc <
-// 2.2e-17), as a polynomial in t = 2 x / X - 1: the Chebyshev series from
-// erfcx at 96 Chebyshev nodes in long double, c_k = (2 - [k = 0]) / N sum_j
-// f(x_j) cos(k theta_j), truncated after 24 terms (X <= 2.5) or 32, then
-// written in powers of t (T_{k+1} = 2 t T_k - T_{k-1}).  The kernels take 24
-// (FP64) / 16 (binary32) terms for X <= 2.5, 32 / 24 beyond; the relative
-// error of the FP64 form is below 2e-14 on [0, 6].
-void erfc_fit(FFDev &d)
+// The range of erfc's argument a r over the pairs inside the cutoff, a rc
+// (held at 6 beyond, where erfc < 2.2e-17): it selects the pair kernel's
+// erfcx polynomial, on [0, 2.5] or on [0, 6] (erfc_tables.hpp).
+void erfc_range(FFDev &d)
 {
-    const int N = 96;
-    const long double X = std::min(6.0, std::max(1e-3, d.za * d.rc)) * (1.0L + 1e-12L);
-    const long double pi = 3.14159265358979323846264338327950288L;
-    long double fv[N], th[N];
-    for (int j = 0; j < N; ++j) {
-        th[j] = pi * (j + 0.5L) / N;
-        const long double xx = 0.5L * X * (std::cos(th[j]) + 1.0L);
-        fv[j] = std::exp(xx * xx) * std::erfc(xx);
-    }
-    const int nt64 = X <= 2.5L ? 24 : 32;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int nt = pass == 0 ? nt64 : (X <= 2.5L ? 16 : 24);
-        long double cheb[kErfcTerms] = {}, mono[kErfcTerms] = {}, tk[kErfcTerms] = {}, tkm[kErfcTerms] = {};
-        for (int k = 0; k < nt; ++k) {
-            long double sum = 0.0L;
-            for (int j = 0; j < N; ++j) sum += fv[j] * std::cos(k * th[j]);
-            cheb[k] = (k == 0 ? 1.0L : 2.0L) * sum / N;
-        }
-        tkm[0] = 1.0L;                               // T_0
-        tk[1] = 1.0L;                                // T_1
-        mono[0] += cheb[0];
-        for (int q = 0; q < kErfcTerms; ++q) mono[q] += cheb[1] * tk[q];
-        for (int k = 2; k < nt; ++k) {               // T_k = 2 t T_{k-1} - T_{k-2}
-            long double tn[kErfcTerms] = {};
-            for (int q = 0; q + 1 < kErfcTerms; ++q) tn[q + 1] += 2.0L * tk[q];
-            for (int q = 0; q < kErfcTerms; ++q) tn[q] -= tkm[q];
-            for (int q = 0; q < kErfcTerms; ++q) {
-                tkm[q] = tk[q];
-                tk[q] = tn[q];
-                mono[q] += cheb[k] * tn[q];
-            }
-        }
-        for (int q = 0; q < kErfcTerms; ++q) {
-            if (pass == 0) d.ecf[q] = (double)mono[q];
-            else d.ecf32[q] = (float)mono[q];
-        }
-    }
-    d.exmax = (double)X;
-    d.ecx = (double)(2.0L / X);
+    d.exmax = std::min(6.0, std::max(1e-3, d.za * d.rc));
 }
 
 // Splits of the pair kernel's tiles for nbeads beads: enough blocks to fill
@@ -1360,7 +1347,7 @@ int cam_ff_create(const cam_ff_desc *desc, cam_ff **out)
     d.rc = desc->rcut;
     d.za = desc->zahn_a;
     d.zp = desc->zahn_par;
-    erfc_fit(d);
+    erfc_range(d);
     std::memcpy(d.sr42, desc->sr42, sizeof d.sr42);
     std::memcpy(d.r0bj, desc->r0bj, sizeof d.r0bj);
     std::memcpy(d.zrep, desc->zrep, sizeof d.zrep);
