@@ -466,12 +466,21 @@ __device__ __forceinline__ T erfc_series(T x, T ex2, lds_double *sc)
     constexpr double X = WIDE ? E::kWideX : E::kNarrowX;
     const T t = fma_t(min(x, (T)X), (T)(2.0 / X), (T)-1);
     if constexpr (sizeof(T) == 8) {
+        // four interleaved Horner chains in v = t^4 (a quarter of the
+        // dependent FMAs of one chain): p = (a + t b) + t^2 (c + t d)
         constexpr int NT = WIDE ? 36 : 24;
+        static_assert(NT % 4 == 0, "four chains");
         lds_double *c = opaque_lds(sc);
-        T h = c[NT - 1];
+        const T u = t * t, v = u * u;
+        T h0 = c[NT - 4], h1 = c[NT - 3], h2 = c[NT - 2], h3 = c[NT - 1];
 #pragma unroll
-        for (int k = NT - 2; k >= 0; --k) h = fma_t(h, t, (T)c[k]);
-        return ex2 * h;
+        for (int k = NT - 8; k >= 0; k -= 4) {
+            h0 = fma_t(h0, v, (T)c[k]);
+            h1 = fma_t(h1, v, (T)c[k + 1]);
+            h2 = fma_t(h2, v, (T)c[k + 2]);
+            h3 = fma_t(h3, v, (T)c[k + 3]);
+        }
+        return ex2 * fma_t(fma_t(h3, t, h2), u, fma_t(h1, t, h0));
     } else {
         constexpr int NT = WIDE ? 24 : 16;
         const float *c = WIDE ? E::kWide32 : E::kNarrow32;
