@@ -435,15 +435,15 @@ def _cpu_worker(args):
     return n, time.perf_counter() - t0
 
 
-def oh3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
-    """OH + H2 (src/egrad_oh3.f): 4096 NVE trajectories of 32 beads at 300 K
-    around the saddle, FP64, autotuned schedule; bead-force-evals/s."""
+def _surface_batch(cam, workloads, surface, w, ts, ntraj, nb, steps):
+    """NVE trajectories of one analytic surface around its saddle (positions
+    + N(0, 0.02^2) bohr, Maxwell-Boltzmann momenta), FP64, autotuned."""
     import torch
-    w = workloads.oh3_nve(nbeads=nb)
     rng = np.random.default_rng(2)
-    q = workloads.OH3_TS[None, None] + rng.normal(0.0, 0.02, (ntraj, nb, 4, 3))
-    p = workloads.mb_momenta(rng, w["masses"], nb, w["beta"], (ntraj, nb, 4, 3))
-    e = cam.RPMDEngine(cam.CAM_SURFACE_OH3, nb, ntraj, w["dt"], w["masses"])
+    na = ts.shape[0]
+    q = ts[None, None] + rng.normal(0.0, 0.02, (ntraj, nb, na, 3))
+    p = workloads.mb_momenta(rng, w["masses"], nb, w["beta"], (ntraj, nb, na, 3))
+    e = cam.RPMDEngine(surface, nb, ntraj, w["dt"], w["masses"])
     e.set_state(q, p, np.full(ntraj, w["beta"]))
     _, best = e.autotune(50)
     e.prepare(steps)
@@ -455,6 +455,21 @@ def oh3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
     e.close()
     return {"bead_force_evals_per_s": ntraj * nb * steps / wall, "us_per_step": wall / steps * 1e6,
             "ntraj": ntraj, "nbeads": nb, "steps": steps, "schedule": best}
+
+
+def oh3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
+    """OH + H2 (src/egrad_oh3.f): 4096 NVE trajectories of 32 beads at 300 K
+    around the saddle, FP64, autotuned schedule; bead-force-evals/s."""
+    return _surface_batch(cam, workloads, cam.CAM_SURFACE_OH3, workloads.oh3_nve(nbeads=nb), workloads.OH3_TS,
+                          ntraj, nb, steps)
+
+
+def brh2_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
+    """H + HBr on the BrH2 DIM-3C surface (src/egrad_brh2.f): 4096 NVE
+    trajectories of 32 beads at 300 K around the saddle, FP64, autotuned
+    schedule; bead-force-evals/s."""
+    return _surface_batch(cam, workloads, cam.CAM_SURFACE_BRH2, workloads.brh2_nve(nbeads=nb), workloads.BRH2_TS,
+                          ntraj, nb, steps)
 
 
 def rpmd_rate_h_h2(scale=0.1):
@@ -483,7 +498,7 @@ def main():
                     help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH[,WAVES_PER_EU]' (fused) or "
                          "'valu|mfma,split' instead of the autotuner (for profiling: every launch then has the same "
                          "shape)")
-    ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "force_matching"],
+    ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "brh2", "force_matching"],
                     help="run only this extra measurement and print its result")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
@@ -514,7 +529,7 @@ def main():
     if args.only:
         fn = {"qmdff_box": lambda: qmdff_ethanol_box(cam), "box": lambda: box_nvt(cam),
               "ragged": lambda: ragged_batch(cam, workloads), "oh3": lambda: oh3_batch(cam, workloads),
-              "force_matching": lambda: force_matching(cam)}[args.only]
+              "brh2": lambda: brh2_batch(cam, workloads), "force_matching": lambda: force_matching(cam)}[args.only]
         print(json.dumps({args.only: fn()}))
         return
 
@@ -608,6 +623,7 @@ def main():
         out["speedup_fp64_batched_vs_plain"] = value / plain
         out["ragged"] = ragged_batch(cam, workloads)
         out["oh3"] = oh3_batch(cam, workloads)
+        out["brh2"] = brh2_batch(cam, workloads)
         out["box"] = box_nvt(cam)
         out["qmdff_ethanol_box"] = qmdff_ethanol_box(cam)
         out["force_matching"] = force_matching(cam)

@@ -1,12 +1,13 @@
 // Drop-in analytic surfaces: the bead-batched plugin contract of the
 // reference (egrad_h3, src/egrad_h3.f:29-77; egrad_ch4h, src/egrad_ch4h.f:74-129;
-// egrad_oh3, src/egrad_oh3.f:33-190).
+// egrad_oh3, src/egrad_oh3.f:33-190; egrad_brh2, src/egrad_brh2.f:33-79).
 // One lane per bead.  A block stages its 256 beads through LDS so the global
 // reads and writes are coalesced although each bead's coordinates are one
 // contiguous 3*natoms-double record.
 #include <mutex>
 
 #include "common.hpp"
+#include "pes_brh2.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
 #include "pes_oh3.hpp"
@@ -26,6 +27,15 @@ struct LaneOH3 {
     static constexpr int NC = 12;
     __device__ static void egrad(const double *x, double &v, double *g) { oh3::egrad_bead(x, v, g); }
 };
+struct LaneBrH2 {
+    static constexpr int NC = 9;
+    __device__ static void egrad(const double *x, double &v, double *g) { brh2::egrad_bead(x, v, g); }
+};
+
+constexpr int surface_natoms(int surf)
+{
+    return surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_BRH2 ? 3 : surf == CAM_SURFACE_OH3 ? 4 : 6;
+}
 
 template <typename P>
 __global__ __launch_bounds__(kBlock) void egrad_lane_kernel(const double *__restrict__ q, int nbeads,
@@ -90,19 +100,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     for (int i = threadIdx.x; i < nb * NC; i += kBlock) g[b0 * NC + i] = tile[i];
 }
 
-template <int NA>
+template <int SURF>
 int launch_egrad(const double *q, int nbeads, double *v, double *g, int precision, hipStream_t s)
 {
     if (nbeads == 0) return CAM_OK;
-    if constexpr (NA == 3) {
+    if constexpr (SURF == CAM_SURFACE_H3) {
         const int grid = (nbeads + kBlock - 1) / kBlock;
         if (precision == CAM_PREC_MIXED)
             hipLaunchKernelGGL(egrad_lane_kernel<LaneH3<float>>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
         else
             hipLaunchKernelGGL(egrad_lane_kernel<LaneH3<double>>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
-    } else if constexpr (NA == 4) {
+    } else if constexpr (SURF == CAM_SURFACE_OH3) {
         const int grid = (nbeads + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(egrad_lane_kernel<LaneOH3>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+    } else if constexpr (SURF == CAM_SURFACE_BRH2) {
+        const int grid = (nbeads + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(egrad_lane_kernel<LaneBrH2>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
     } else {
         const int grid = (nbeads + kBlock / 4 - 1) / (kBlock / 4);
         if (precision == CAM_PREC_MIXED)
@@ -114,22 +127,25 @@ int launch_egrad(const double *q, int nbeads, double *v, double *g, int precisio
     return CAM_OK;
 }
 
-template <int NA>
+template <int SURF>
 int egrad_dev(const double *q, int natoms, int nbeads, double *v, double *g, int precision, void *stream)
 {
+    constexpr int NA = surface_natoms(SURF);
     CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
     CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
     CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null device pointer");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision %d", precision);
-    CAM_CHECK_ARG(NA != 4 || precision == CAM_PREC_FP64, "the OH3 surface is evaluated in FP64 only");
+    CAM_CHECK_ARG((SURF != CAM_SURFACE_OH3 && SURF != CAM_SURFACE_BRH2) || precision == CAM_PREC_FP64,
+                  "the OH3 and BrH2 surfaces are evaluated in FP64 only");
     clear_error();
-    return launch_egrad<NA>(q, nbeads, v, g, precision, (hipStream_t)stream);
+    return launch_egrad<SURF>(q, nbeads, v, g, precision, (hipStream_t)stream);
 }
 
 // Host-pointer form: stage through device buffers owned by this call.
-template <int NA>
+template <int SURF>
 int egrad_host(const double *q, int natoms, int nbeads, double *v, double *g)
 {
+    constexpr int NA = surface_natoms(SURF);
     CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
     CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
     CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null host pointer");
@@ -148,7 +164,7 @@ int egrad_host(const double *q, int natoms, int nbeads, double *v, double *g)
         CAM_HIP(hipMalloc(&dv, (size_t)nbeads * sizeof(double)));
         CAM_HIP(hipMalloc(&dg, nq * sizeof(double)));
         CAM_HIP(hipMemcpy(dq, q, nq * sizeof(double), hipMemcpyHostToDevice));
-        int r = launch_egrad<NA>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
+        int r = launch_egrad<SURF>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
         if (r != CAM_OK) return r;
         CAM_HIP(hipMemcpy(v, dv, (size_t)nbeads * sizeof(double), hipMemcpyDeviceToHost));
         CAM_HIP(hipMemcpy(g, dg, nq * sizeof(double), hipMemcpyDeviceToHost));
@@ -176,9 +192,10 @@ constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mutex[kMaxDevices];
 double *g_noalloc_ptr[kMaxDevices] = {};
 
-template <int NA>
+template <int SURF>
 int egrad_host_noalloc(const double *q, int natoms, int nbeads, double *v, double *g)
 {
+    constexpr int NA = surface_natoms(SURF);
     CAM_CHECK_ARG(natoms == NA, "natoms must be %d for this surface (got %d)", NA, natoms);
     CAM_CHECK_ARG(nbeads >= 0 && nbeads <= kNoAllocBeads,
                   "nbeads must be in 0..%d for the allocation-free entry point (got %d)", kNoAllocBeads, nbeads);
@@ -199,7 +216,7 @@ int egrad_host_noalloc(const double *q, int natoms, int nbeads, double *v, doubl
     double *dv = dq + (size_t)kNoAllocBeads * 18;
     double *dg = dv + kNoAllocBeads;
     CAM_HIP(hipMemcpy(dq, q, nq * sizeof(double), hipMemcpyHostToDevice));
-    const int r = launch_egrad<NA>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
+    const int r = launch_egrad<SURF>(dq, nbeads, dv, dg, CAM_PREC_FP64, nullptr);
     if (r != CAM_OK) return r;
     CAM_HIP(hipMemcpy(v, dv, (size_t)nbeads * sizeof(double), hipMemcpyDeviceToHost));
     CAM_HIP(hipMemcpy(g, dg, nq * sizeof(double), hipMemcpyDeviceToHost));
@@ -212,33 +229,42 @@ int egrad_host_noalloc(const double *q, int natoms, int nbeads, double *v, doubl
 extern "C" {
 int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
-    if (surface == CAM_SURFACE_H3) return cam::egrad_host_noalloc<3>(q, natoms, nbeads, v, dvdq);
-    if (surface == CAM_SURFACE_CH4H) return cam::egrad_host_noalloc<6>(q, natoms, nbeads, v, dvdq);
-    if (surface == CAM_SURFACE_OH3) return cam::egrad_host_noalloc<4>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_H3) return cam::egrad_host_noalloc<CAM_SURFACE_H3>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_CH4H) return cam::egrad_host_noalloc<CAM_SURFACE_CH4H>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_OH3) return cam::egrad_host_noalloc<CAM_SURFACE_OH3>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_BRH2) return cam::egrad_host_noalloc<CAM_SURFACE_BRH2>(q, natoms, nbeads, v, dvdq);
     CAM_CHECK_ARG(false, "unknown surface %d", surface);
 }
 int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
-    return cam::egrad_host<3>(q, natoms, nbeads, v, dvdq);
+    return cam::egrad_host<CAM_SURFACE_H3>(q, natoms, nbeads, v, dvdq);
 }
 int cam_egrad_ch4h(const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
-    return cam::egrad_host<6>(q, natoms, nbeads, v, dvdq);
+    return cam::egrad_host<CAM_SURFACE_CH4H>(q, natoms, nbeads, v, dvdq);
 }
 int cam_egrad_oh3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
 {
-    return cam::egrad_host<4>(q, natoms, nbeads, v, dvdq);
+    return cam::egrad_host<CAM_SURFACE_OH3>(q, natoms, nbeads, v, dvdq);
 }
 int cam_egrad_oh3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {
-    return cam::egrad_dev<4>(q, natoms, nbeads, v, dvdq, precision, stream);
+    return cam::egrad_dev<CAM_SURFACE_OH3>(q, natoms, nbeads, v, dvdq, precision, stream);
+}
+int cam_egrad_brh2(const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    return cam::egrad_host<CAM_SURFACE_BRH2>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_brh2_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
+{
+    return cam::egrad_dev<CAM_SURFACE_BRH2>(q, natoms, nbeads, v, dvdq, precision, stream);
 }
 int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {
-    return cam::egrad_dev<3>(q, natoms, nbeads, v, dvdq, precision, stream);
+    return cam::egrad_dev<CAM_SURFACE_H3>(q, natoms, nbeads, v, dvdq, precision, stream);
 }
 int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {
-    return cam::egrad_dev<6>(q, natoms, nbeads, v, dvdq, precision, stream);
+    return cam::egrad_dev<CAM_SURFACE_CH4H>(q, natoms, nbeads, v, dvdq, precision, stream);
 }
 }

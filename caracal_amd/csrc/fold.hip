@@ -1,4 +1,4 @@
-// Folded step kernels of the OH3 surface, the dispatch over all folded
+// Folded step kernels of the OH3 and BrH2 surfaces, the dispatch over all folded
 // kernels and the folded transform matrix.  The kernels themselves are in
 // fold.hpp; the CH5 and H3 ones are built in fold_ch4h.hip and fold_h3.hip,
 // with other code generation options.
@@ -17,6 +17,7 @@ step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensembl
     const bool f64 = precision == CAM_PREC_FP64;
     if (surface == CAM_SURFACE_H3) return fold_kernel_h3(precision, nbeads, ensemble, tpb);
     if (surface == CAM_SURFACE_OH3) return f64 ? fold::pick_fold_w<SurfOH3, double, 1>(nbeads, ensemble, tpb) : nullptr;
+    if (surface == CAM_SURFACE_BRH2) return f64 ? fold::pick_fold_w<SurfBrH2, double, 1>(nbeads, ensemble, tpb) : nullptr;
     return fold_kernel_ch4h(precision, nbeads, ensemble, wpe, tpb);
 }
 

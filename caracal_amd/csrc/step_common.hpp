@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "common.hpp"
+#include "pes_brh2.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
 #include "pes_oh3.hpp"
@@ -40,6 +41,15 @@ struct SurfOH3 {
     __device__ static bool valid(int, int) { return true; }
     template <typename T>
     __device__ static void egrad(const double *own, double &v, double *g) { oh3::egrad_bead(own, v, g); }
+};
+
+// BrH2 DIM-3C: one lane per bead, FP64 only (the engine rejects mixed precision).
+struct SurfBrH2 {
+    static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
+    __device__ static int coord(int slot, int) { return slot; }
+    __device__ static bool valid(int, int) { return true; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { brh2::egrad_bead(own, v, g); }
 };
 
 struct SurfCH4H {

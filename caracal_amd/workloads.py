@@ -8,6 +8,9 @@ ch4h_children CH4 + H recrossing children on the CH5 surface: saddle
               xi = 0 (dividing surface xi_ideal = 1), momenta at 500 K.
 oh3_nve       OH + H2 (O 15.999, H 1.008 amu) on the Schatz-Elgersma surface at
               its saddle, bead positions + N(0, 0.01^2) bohr, momenta at 300 K.
+brh2_nve      H + HBr (H 1.008, Br 79.904 amu) on the BrH2 DIM-3C surface at
+              its collinear saddle, bead positions + N(0, 0.01^2) bohr, momenta
+              at 300 K.
 ragged        H3 / CH4H half-and-half with nbeads in {4,8,16,32,64} and
               T in U[200, 1000] K, seed 1 (bucketed by surface and nbeads).
 """
@@ -41,6 +44,18 @@ OH3_TS = np.array([
 ])
 MASSES_OH3_AMU = [15.999, 1.008, 1.008, 1.008]
 
+# First-order saddle of H + HBr -> H2 + Br on the BrH2 DIM-3C surface of
+# src/egrad_brh2.f (bohr; atoms Ha, Br, Hc), collinear Hc-Ha-Br with
+# R(Br-Ha) = 2.7216, R(Ha-Hc) = 2.6406, V = 20.996 kcal/mol above Br + H2
+# (1.82 above H + HBr), one imaginary mode, |dV/dq| ~ 6e-17
+# (tests/golden/make_brh2_saddle.py).
+BRH2_TS = np.array([
+    [0.0, 0.0, -0.027009331805548],
+    [0.0, 0.0, 2.6945795463155617],
+    [0.0, 0.0, -2.6675702145100137],
+])
+MASSES_BRH2_AMU = [1.008, 79.904, 1.008]
+
 
 def h3_reaction():
     """H_A H_B + H_C -> H_A + H_B H_C (examples/calc_rate/h+h2/rate.key)."""
@@ -65,6 +80,14 @@ def oh3_reaction():
                     reactants=[[1, 2], [3, 4]], r_inf=16.0, xi_ideal=1.0)
 
 
+def brh2_reaction():
+    """H + HBr -> H2 + Br: Ha-Hc forms, Ha-Br breaks, at the saddle's lengths."""
+    t = BRH2_TS
+    return Reaction(bond_form=[(1, 3)], bond_break=[(1, 2)],
+                    form_ref=[float(np.linalg.norm(t[0] - t[2]))], break_ref=[float(np.linalg.norm(t[0] - t[1]))],
+                    reactants=[[1, 2], [3]], r_inf=16.0, xi_ideal=1.0)
+
+
 def mb_momenta(rng, masses, nbeads, beta, shape):
     """Maxwell-Boltzmann bead momenta: p ~ N(0, m nbeads / beta) (andersen.f90:50-56)."""
     sd = np.sqrt(np.asarray(masses)[:, None] * nbeads / np.asarray(beta))  # (natoms, 1)
@@ -86,6 +109,15 @@ def oh3_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
     beta = float(beta_of(kelvin))
     q = OH3_TS[None] + rng.normal(0.0, 0.01, (nbeads, 4, 3))
     p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 4, 3))
+    return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
+
+
+def brh2_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
+    rng = np.random.default_rng(seed)
+    masses = mass_au(MASSES_BRH2_AMU)
+    beta = float(beta_of(kelvin))
+    q = BRH2_TS[None] + rng.normal(0.0, 0.01, (nbeads, 3, 3))
+    p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 3, 3))
     return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
 
 

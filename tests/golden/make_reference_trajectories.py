@@ -22,6 +22,8 @@ stores inputs and frames as small .npz fixtures beside this script:
   h3_nve_6beads_reference.npz   C1's system with 6 beads (a bead count that is
                               not a power of two), NVE with transrot
   ch4h_child_12beads_reference.npz  a CH4 + H recrossing child with 12 beads
+  brh2_nve_reference.npz      H + HBr on the BrH2 DIM-3C surface at its
+                              saddle, 8 beads, NVE with transrot, 100 steps
 
 Frames are kept at steps 1, 2, 10, 50 and 100 (q and p), with epot and xi
 at every step.  Run in the container where /root/reference is mounted:
@@ -69,6 +71,7 @@ def main(names):
     h = workloads.h3_nve(nbeads=16)
     w, q, p = child_start()
     o = workloads.oh3_nve(nbeads=8)
+    br = workloads.brh2_nve(nbeads=8)
     h8 = workloads.h3_nve(nbeads=8)
     h6 = workloads.h3_nve(nbeads=6)
     w12, q12, p12 = child_start(12)
@@ -93,6 +96,8 @@ def main(names):
         "ch4h_child_12beads_reference.npz": lambda: _save("ch4h_child_12beads_reference.npz", "ch4h", q12, p12,
                                                           w12["masses"], w12["dt"], w12["beta"][0], 2,
                                                           w12["reaction"]),
+        "brh2_nve_reference.npz": lambda: _save("brh2_nve_reference.npz", "brh2", br["q"], br["p"], br["masses"],
+                                                br["dt"], br["beta"], -1),
     }
     for name in names or list(jobs):
         jobs[name]()

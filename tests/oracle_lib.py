@@ -38,6 +38,7 @@ def ref():
         _ref = ctypes.CDLL(REF_SO)
         _ref.initialize_ch4h_()
         _ref.initialize_oh3_()
+        _ref.initialize_brh2_()
     return _ref
 
 
@@ -60,6 +61,10 @@ def egrad_ch4h(q):
 
 def egrad_oh3(q):
     return _egrad(oracle().oracle_egrad_oh3, q)
+
+
+def egrad_brh2(q):
+    return _egrad(oracle().oracle_egrad_brh2, q)
 
 
 def _ref_egrad(fn, q):
@@ -86,6 +91,10 @@ def ref_egrad_oh3(q):
     q = np.ascontiguousarray(q, dtype=np.float64)
     out = [_ref_egrad(ref().egrad_oh3_, q[b:b + 1]) for b in range(q.shape[0])]
     return np.concatenate([o[0] for o in out]), np.concatenate([o[1] for o in out])
+
+
+def ref_egrad_brh2(q):
+    return _ref_egrad(ref().egrad_brh2_, q)
 
 
 # ---- RPMD step (oracle/rpmd.c) ----

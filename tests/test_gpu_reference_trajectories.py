@@ -102,6 +102,17 @@ def test_oh3_nve_matches_reference():
     _check("OH3", q, p, f)
 
 
+def test_brh2_nve_matches_reference():
+    """H + HBr at the BrH2 saddle, 8 beads, NVE with transrot: the engine's
+    Jacobi eigen-solve against the reference's EISPACK one, step by step."""
+    f = dict(np.load(os.path.join(GOLDEN, "brh2_nve_reference.npz")))
+    nb = f["q0"].shape[0]
+    with cam.RPMDEngine(cam.CAM_SURFACE_BRH2, nb, 1, float(f["dt"]), f["masses"], remove_com=True) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        q, p = _frames(e, f["steps"])
+    _check("BrH2", q, p, f)
+
+
 @pytest.mark.parametrize("schedule", [cam.CAM_SCHED_FUSED, cam.CAM_SCHED_SPLIT])
 def test_h3_umbrella_window_matches_reference(schedule):
     """An umbrella window as calc_rate runs it (verlet constrain = 0): mdinit's

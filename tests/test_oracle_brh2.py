@@ -1,0 +1,144 @@
+"""Pins the oracle's BrH2 DIM-3C surface (src/egrad_brh2.f): against the
+compiled reference (oracle/_ref), against central differences of its energy,
+against the Ha <-> Hc exchange symmetry and rigid motions, and against the
+closed forms of the separated fragments.  With Br far from H2 the DIM matrix's
+lowest eigenvalue is the H2 singlet curve V1HH(r), so V = V1HH(r) + DHH; with
+one H far from HBr it is the HBr 1-Sigma curve V1S(r), so V = V1S(r) + DHH
+(both a Morse curve times exp(-b (r - re)^3), src/egrad_brh2.f:279-289 and
+:442-463).
+
+The reference diagonalises the 4 x 4 DIM matrix with EISPACK; the oracle with
+Jacobi rotations.  Energies agree to rounding.  Gradients (u^T dH/dR u with
+u the lowest eigenvector) agree to 1e-13 at most geometries; where the lowest
+two eigenvalues lie close the eigenvector is ill-conditioned and the
+reference's gradient is off by up to ~6e-11 (an 80-bit build of the same
+Jacobi code agrees with the oracle's to 6e-15 there), hence the 2e-10 bound."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from caracal_amd import workloads
+
+needs_ref = pytest.mark.skipif(ol.ref() is None, reason="compiled reference not built")
+
+RHH, AHH, DHH, BHH = 1.4016, 1.0291, 0.17447, 0.018
+RHX, AHX, DHX, BHX = 2.673, 0.957, 0.1439, 0.012
+
+
+def _cloud(n, sigma, seed):
+    rng = np.random.default_rng(seed)
+    return workloads.BRH2_TS[None] + rng.normal(0.0, sigma, (n, 3, 3))
+
+
+def _line(xs):
+    """Ha, Br, Hc on the x axis at xs."""
+    return np.array([[xs[0], 0, 0], [xs[1], 0, 0], [xs[2], 0, 0]], dtype=np.float64)
+
+
+def _special():
+    out = []
+    # collinear Hc-Ha-Br (the reaction path) and Ha-Br-Hc, exact: the
+    # reference's collinear branch (sin^2 < 1e-6) and its triangle clamp
+    for r3 in (1.2, 1.6, 2.0, 2.64, 3.5, 6.0):
+        out.append(_line((0.0, 2.72, -r3)))
+    out.append(_line((0.0, 2.7, 5.4)))
+    out.append(_line((-1.0, 0.0, 1.0)))
+    # just off and just inside the collinear threshold
+    for y in (1e-4, 2e-3, 5e-2):
+        out.append(np.array([[0, 0, 0], [2.72, 0, 0], [-2.64, y, 0]], dtype=np.float64))
+    # separated fragments and a compact, bent geometry
+    out.append(np.array([[0, 0, 0], [0, 0, 40.0], [1.4, 0, 0]], dtype=np.float64))
+    out.append(np.array([[0, 0, 0], [2.67, 0, 0], [0, 40.0, 0]], dtype=np.float64))
+    out.append(np.array([[0, 0, 0], [1.9, 1.4, 0], [1.1, -0.3, 0.2]], dtype=np.float64))
+    return np.array(out)
+
+
+@needs_ref
+@pytest.mark.parametrize("q", [_cloud(400, 0.15, 0), _cloud(300, 0.5, 1), _cloud(300, 1.0, 2), _special()],
+                         ids=["near-saddle", "wide", "wider", "special"])
+def test_brh2_matches_compiled_reference(q):
+    v, g = ol.egrad_brh2(q)
+    vr, gr = ol.ref_egrad_brh2(q)
+    np.testing.assert_allclose(v, vr, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(g, gr, rtol=0, atol=2e-10)
+    assert np.median(np.abs(g - gr).max(axis=(1, 2))) <= 1e-13
+
+
+def _fd(q, h=1e-5):
+    g = np.zeros_like(q)
+    for b in range(q.shape[0]):
+        for i in range(3):
+            for d in range(3):
+                qp, qm = q.copy(), q.copy()
+                qp[b, i, d] += h
+                qm[b, i, d] -= h
+                g[b, i, d] = (ol.egrad_brh2(qp)[0][b] - ol.egrad_brh2(qm)[0][b]) / (2 * h)
+    return g
+
+
+def _sin2(q):
+    """sin^2 of the angles at Ha and Hc."""
+    def s2(a, b, c):
+        u, w = q[:, b] - q[:, a], q[:, c] - q[:, a]
+        cs = (u * w).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(w, axis=1)
+        return 1.0 - cs * cs
+    return np.minimum(s2(0, 1, 2), s2(2, 1, 0))
+
+
+def test_brh2_gradient_is_derivative_of_energy():
+    # away from the collinear threshold, where the off-diagonal angular
+    # terms switch off
+    q = _cloud(40, 0.4, 3)
+    q = q[_sin2(q) > 1e-3][:12]
+    _, g = ol.egrad_brh2(q)
+    np.testing.assert_allclose(_fd(q), g, atol=1e-8)
+
+
+def test_brh2_exchange_symmetry_and_rigid_motion():
+    q = _cloud(100, 0.4, 4)
+    v, g = ol.egrad_brh2(q)
+    vs, gs = ol.egrad_brh2(q[:, [2, 1, 0]])
+    np.testing.assert_allclose(vs, v, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(gs, g[:, [2, 1, 0]], rtol=0, atol=1e-12)
+    rng = np.random.default_rng(5)
+    a, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    vr, gr = ol.egrad_brh2(q @ a.T + rng.normal(size=3))
+    np.testing.assert_allclose(vr, v, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(gr, g @ a.T, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(g.sum(axis=1), 0.0, atol=1e-14)
+    np.testing.assert_allclose(np.cross(q, g).sum(axis=1), 0.0, atol=1e-12)
+
+
+def _curve(d, a, b, re, r):
+    """d e1 e2 (e1 - 2), e1 = exp(-a (r - re)), e2 = exp(-b (r - re)^3), and its r-derivative."""
+    x = r - re
+    e1, e2 = np.exp(-a * x), np.exp(-b * x ** 3)
+    v = d * e1 * e2 * (e1 - 2.0)
+    dv = d * e2 * (-2.0 * a * e1 * e1 + 2.0 * a * e1) - 3.0 * b * x * x * v
+    return v, dv
+
+
+@pytest.mark.parametrize("r", [1.0, 1.4016, 1.9, 2.6])
+def test_brh2_br_plus_h2_closed_form(r):
+    q = np.array([[0, 0, 0], [0, 0, 60.0], [r, 0, 0]], dtype=np.float64)
+    v, g = ol.egrad_brh2(q[None])
+    vc, dvc = _curve(DHH, AHH, BHH, RHH, r)
+    assert abs(v[0] - (vc + DHH)) <= 1e-15
+    np.testing.assert_allclose(g[0, 2], [dvc, 0, 0], atol=1e-14)
+    np.testing.assert_allclose(g[0, 0], [-dvc, 0, 0], atol=1e-14)
+
+
+@pytest.mark.parametrize("r", [2.0, 2.673, 3.3, 4.5])
+def test_brh2_h_plus_hbr_closed_form(r):
+    q = np.array([[0, 0, 0], [r, 0, 0], [0, 60.0, 0]], dtype=np.float64)
+    v, g = ol.egrad_brh2(q[None])
+    vc, dvc = _curve(DHX, AHX, BHX, RHX, r)
+    assert abs(v[0] - (vc + DHH)) <= 1e-15
+    np.testing.assert_allclose(g[0, 1], [dvc, 0, 0], atol=1e-14)
+    np.testing.assert_allclose(g[0, 2], 0.0, atol=1e-14)
+
+
+def test_brh2_saddle():
+    v, g = ol.egrad_brh2(workloads.BRH2_TS[None])
+    assert abs(v[0] * 627.5094743 - 20.9962) < 1e-3
+    assert np.abs(g).max() < 1e-12

@@ -27,8 +27,9 @@ CASES = [("c1_h3_nve_reference.npz", 1, -1, 0, None),
          ("h3_nhc_reference.npz", 1, -1, 2, None),
          ("oh3_child_xi05_reference.npz", 3, 2, 0, workloads.oh3_reaction),
          ("h3_nve_6beads_reference.npz", 1, -1, 0, None),
-         ("ch4h_child_12beads_reference.npz", 2, 2, 0, workloads.ch4h_reaction)]
-SURF = {1: "h3", 2: "ch4h", 3: "oh3"}
+         ("ch4h_child_12beads_reference.npz", 2, 2, 0, workloads.ch4h_reaction),
+         ("brh2_nve_reference.npz", 4, -1, 0, None)]
+SURF = {1: "h3", 2: "ch4h", 3: "oh3", 4: "brh2"}
 
 
 def _fixture(name):
