@@ -9,15 +9,22 @@
 namespace cam {
 
 step_fn fold_kernel_ch4h(int precision, int nbeads, int ensemble, int wpe, int *tpb);
-step_fn fold_kernel_h3(int precision, int nbeads, int ensemble, int *tpb);
+step_fn fold_kernel_h3(int precision, int nbeads, int ensemble, int nbuf, int *tpb);
 
-step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int *tpb)
+template <class S>
+step_fn fold_fp64(int nbeads, int ensemble, int nbuf, int *tpb)
+{
+    return nbuf == 2 ? fold::pick_fold_w<S, double, 1, 2>(nbeads, ensemble, tpb)
+                     : fold::pick_fold_w<S, double, 1, 1>(nbeads, ensemble, tpb);
+}
+
+step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int nbuf, int *tpb)
 {
     if (ensemble != CAM_ENS_NVE && ensemble != CAM_ENS_CHILD) return nullptr;
     const bool f64 = precision == CAM_PREC_FP64;
-    if (surface == CAM_SURFACE_H3) return fold_kernel_h3(precision, nbeads, ensemble, tpb);
-    if (surface == CAM_SURFACE_OH3) return f64 ? fold::pick_fold_w<SurfOH3, double, 1>(nbeads, ensemble, tpb) : nullptr;
-    if (surface == CAM_SURFACE_BRH2) return f64 ? fold::pick_fold_w<SurfBrH2, double, 1>(nbeads, ensemble, tpb) : nullptr;
+    if (surface == CAM_SURFACE_H3) return fold_kernel_h3(precision, nbeads, ensemble, nbuf, tpb);
+    if (surface == CAM_SURFACE_OH3) return f64 ? fold_fp64<SurfOH3>(nbeads, ensemble, nbuf, tpb) : nullptr;
+    if (surface == CAM_SURFACE_BRH2) return f64 ? fold_fp64<SurfBrH2>(nbeads, ensemble, nbuf, tpb) : nullptr;
     return fold_kernel_ch4h(precision, nbeads, ensemble, wpe, tpb);
 }
 

@@ -216,18 +216,26 @@ __device__ __forceinline__ void nm_fold(double *xs, const double *prop, const do
 // Between two steps a lane holds q, p and g of its slots; during the forces
 // only the gradient is live, p and q are read back from the column buffer
 // after them.  The launch covers trajectories a.tbegin .. a.tend - 1 (state
-// arrays and the centroid buffer keep their whole-batch strides).
-template <class S, int NB, typename T, int ENS, int WPE>
+// arrays and the centroid buffer keep their whole-batch strides).  NBUF: LDS
+// column buffers (1 or 2, below).
+template <class S, int NB, typename T, int ENS, int WPE, int NBUF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void fold_step_kernel(StepArgs a)
 {
     using F = Fold<S, NB>;
     constexpr int NC = S::NC, L = S::L, SL = S::SLOTS, H = F::H, NH = F::NH, TPB = F::TPB;
     constexpr int PW = PropTab<S::NA>::W;
     constexpr bool CHILD = ENS == CAM_ENS_CHILD;
-    // Two column buffers, used by alternate steps: a step's staging writes
-    // cannot reach the buffer the previous step's read-back still reads, so
-    // the step needs no barrier before its staging.
-    __shared__ __align__(16) double X[2][F::NCOL * NH];
+    // Column buffers.  A lane stages into and reads back from only its own
+    // positions (its slots' coordinates at its bead j), and the transform,
+    // which reads and writes every column, ends with a barrier; so one
+    // buffer needs no barrier before the next step's staging.  With one, the
+    // one-lane-per-bead surfaces (15 ring polymers a block at 32 beads) take
+    // 53-70 KB of LDS and two or three blocks share a CU; with two (90 KB),
+    // one block per CU leaves registers for other kernels' blocks, which a
+    // batch of several surfaces sharing the chip runs faster with.  The CH5
+    // kernels always take two, the layout their registers were tuned with.
+    static_assert(NBUF == 1 || NBUF == 2, "one or two column buffers");
+    __shared__ __align__(16) double X[NBUF][F::NCOL * NH];
     __shared__ double tfs[H * (H + 1)];       // M, row stride H + 1
     __shared__ __align__(16) double cent[CHILD ? TPB * NC : 1];
     __shared__ double prop[TPB * NH * PW];
@@ -297,7 +305,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     for (int it = 0; it < a.nsteps; ++it) {
         const int ti = opaque(tid);
         const Place w = place(ti);
-        double *xb = X[it & 1];
+        double *xb = X[NBUF == 1 ? 0 : it & 1];
         double *xs = xb + w.tl * 2 * NC * NH;      // this ring polymer's columns
         // half kick (src/verlet.f90:216-218)
 #pragma unroll
@@ -313,8 +321,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
         __syncthreads();
         nm_fold<S, NB, CHILD>(xb, prop, tfs, cent, ti);
-        // The column buffer keeps this step's p and q until the step after
-        // next stages into it, behind the next step's barrier.
+        // The column buffer keeps this lane's p and q until the lane itself
+        // stages the next step into it.
 #pragma unroll
         for (int s = 0; s < SL; ++s) q[s] = xs[(2 * w.cc[s] + 1) * NH + w.j];
         if constexpr (CHILD) {
@@ -363,15 +371,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
 }
 
-template <class S, typename T, int WPE>
+template <class S, typename T, int WPE, int NBUF>
 step_fn pick_fold_w(int nb, int ens, int *tpb)
 {
     switch (nb) {
 #define CAM_FOLD_CASE(N)                                                                      \
     case N:                                                                                   \
         *tpb = Fold<S, N>::TPB;                                                               \
-        return ens == CAM_ENS_NVE ? fold_step_kernel<S, N, T, CAM_ENS_NVE, WPE>               \
-                                  : fold_step_kernel<S, N, T, CAM_ENS_CHILD, WPE>;
+        return ens == CAM_ENS_NVE ? fold_step_kernel<S, N, T, CAM_ENS_NVE, WPE, NBUF>         \
+                                  : fold_step_kernel<S, N, T, CAM_ENS_CHILD, WPE, NBUF>;
         CAM_FOLD_CASE(4)
         CAM_FOLD_CASE(6)
         CAM_FOLD_CASE(8)

@@ -12,9 +12,9 @@ namespace cam {
 // binary32 kernels fit three waves within the first.
 step_fn fold_kernel_ch4h(int precision, int nbeads, int ensemble, int wpe, int *tpb)
 {
-    if (precision != CAM_PREC_FP64) return fold::pick_fold_w<SurfCH4H, float, 1>(nbeads, ensemble, tpb);
-    if (wpe == 2) return fold::pick_fold_w<SurfCH4H, double, 2>(nbeads, ensemble, tpb);
-    return fold::pick_fold_w<SurfCH4H, double, 1>(nbeads, ensemble, tpb);
+    if (precision != CAM_PREC_FP64) return fold::pick_fold_w<SurfCH4H, float, 1, 2>(nbeads, ensemble, tpb);
+    if (wpe == 2) return fold::pick_fold_w<SurfCH4H, double, 2, 2>(nbeads, ensemble, tpb);
+    return fold::pick_fold_w<SurfCH4H, double, 1, 2>(nbeads, ensemble, tpb);
 }
 
 }  // namespace cam

@@ -5,10 +5,13 @@
 
 namespace cam {
 
-step_fn fold_kernel_h3(int precision, int nbeads, int ensemble, int *tpb)
+step_fn fold_kernel_h3(int precision, int nbeads, int ensemble, int nbuf, int *tpb)
 {
-    if (precision == CAM_PREC_FP64) return fold::pick_fold_w<SurfH3, double, 1>(nbeads, ensemble, tpb);
-    return fold::pick_fold_w<SurfH3, float, 1>(nbeads, ensemble, tpb);
+    if (precision == CAM_PREC_FP64)
+        return nbuf == 2 ? fold::pick_fold_w<SurfH3, double, 1, 2>(nbeads, ensemble, tpb)
+                         : fold::pick_fold_w<SurfH3, double, 1, 1>(nbeads, ensemble, tpb);
+    return nbuf == 2 ? fold::pick_fold_w<SurfH3, float, 1, 2>(nbeads, ensemble, tpb)
+                     : fold::pick_fold_w<SurfH3, float, 1, 1>(nbeads, ensemble, tpb);
 }
 
 }  // namespace cam

@@ -161,8 +161,11 @@ constexpr bool mfma_bead_count(int nb) { return nb >= 16 && nb % 16 == 0 && 4 % 
 typedef void (*step_fn)(StepArgs);
 
 // The folded step kernel (fold.hip) of a configuration, or nullptr if it has
-// none; *tpb receives its trajectories per block.
-step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int *tpb);
+// none; *tpb receives its trajectories per block.  nbuf: LDS column buffers
+// of the one-lane-per-bead surfaces' kernels (1: more blocks per CU; 2: one
+// block per CU, for engines that share the chip with other engines); the CH5
+// kernels take 2 whatever is asked.
+step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensemble, int wpe, int nbuf, int *tpb);
 
 // Folded reference transform M[r][c] = w_c cos(2 pi r c / nbeads) / sqrt(nbeads),
 // r, c < nbeads/2, w_0 = 1 and w_c = 2 otherwise, row-major.
