@@ -472,6 +472,14 @@ def brh2_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
                           ntraj, nb, steps)
 
 
+def o3_batch(cam, workloads, ntraj=4096, nb=32, steps=500):
+    """O + O2 on the Varga-Paukku-Truhlar O3 surface (src/egrad_o3.f): 4096
+    NVE trajectories of 32 beads at 300 K around the exchange saddle, FP64,
+    autotuned schedule; bead-force-evals/s."""
+    return _surface_batch(cam, workloads, cam.CAM_SURFACE_O3, workloads.o3_nve(nbeads=nb), workloads.O3_TS,
+                          ntraj, nb, steps)
+
+
 def rpmd_rate_h_h2(scale=0.1):
     """The reference's H + H2 rate example (examples/calc_rate/h+h2/rate.key)
     end to end -- start structures, 110 umbrella windows, PMF, recrossing,
@@ -498,7 +506,7 @@ def main():
                     help="fixed schedule 'valu|mfma|folded,STEPS_PER_LAUNCH[,WAVES_PER_EU]' (fused) or "
                          "'valu|mfma,split' instead of the autotuner (for profiling: every launch then has the same "
                          "shape)")
-    ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "brh2", "force_matching"],
+    ap.add_argument("--only", default=None, choices=["qmdff_box", "box", "ragged", "oh3", "brh2", "o3", "force_matching"],
                     help="run only this extra measurement and print its result")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes five times (HBM counter calibration)")
@@ -529,7 +537,8 @@ def main():
     if args.only:
         fn = {"qmdff_box": lambda: qmdff_ethanol_box(cam), "box": lambda: box_nvt(cam),
               "ragged": lambda: ragged_batch(cam, workloads), "oh3": lambda: oh3_batch(cam, workloads),
-              "brh2": lambda: brh2_batch(cam, workloads), "force_matching": lambda: force_matching(cam)}[args.only]
+              "brh2": lambda: brh2_batch(cam, workloads), "o3": lambda: o3_batch(cam, workloads),
+              "force_matching": lambda: force_matching(cam)}[args.only]
         print(json.dumps({args.only: fn()}))
         return
 
@@ -624,6 +633,7 @@ def main():
         out["ragged"] = ragged_batch(cam, workloads)
         out["oh3"] = oh3_batch(cam, workloads)
         out["brh2"] = brh2_batch(cam, workloads)
+        out["o3"] = o3_batch(cam, workloads)
         out["box"] = box_nvt(cam)
         out["qmdff_ethanol_box"] = qmdff_ethanol_box(cam)
         out["force_matching"] = force_matching(cam)

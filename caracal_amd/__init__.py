@@ -22,7 +22,7 @@ from ._abi import (CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_NHC, CAM_ENS_NVE, CA
                    CAM_NM_ORTHOGONAL,
                    CAM_NM_REFERENCE, CAM_NMK_AUTO, CAM_NMK_FOLDED, CAM_NMK_MFMA, CAM_NMK_VALU, CAM_PREC_FP64,
                    CAM_SCHED_AUTO, CAM_SCHED_FUSED, CAM_SCHED_SPLIT, CAM_PREC_MIXED, CAM_SURFACE_CH4H, CAM_SURFACE_H3,
-                   CAM_SURFACE_OH3, CAM_SURFACE_BRH2,
+                   CAM_SURFACE_OH3, CAM_SURFACE_BRH2, CAM_SURFACE_O3,
                    CamError, check, lib)
 from .units import (AMU_TO_ME, FS_TO_AU, KB_HARTREE, beta_of, mass_au)  # noqa: F401
 
@@ -74,10 +74,17 @@ def egrad_brh2(q):
     return _egrad("cam_egrad_brh2", 3, q)
 
 
+def egrad_o3(q):
+    """O + O2 surface (Varga, Paukku, Truhlar) for a stack of beads: q
+    (nbeads, 3, 3) bohr, three O atoms -> (V, dV/dq)."""
+    return _egrad("cam_egrad_o3", 3, q)
+
+
 def egrad_dev(surface, q_ptr, nbeads, v_ptr, g_ptr, precision=CAM_PREC_FP64, stream=None):
     """Device-pointer form (pointers are integers, e.g. torch tensor data_ptr())."""
     name = {CAM_SURFACE_H3: "cam_egrad_h3_dev", CAM_SURFACE_CH4H: "cam_egrad_ch4h_dev",
-            CAM_SURFACE_OH3: "cam_egrad_oh3_dev", CAM_SURFACE_BRH2: "cam_egrad_brh2_dev"}[surface]
+            CAM_SURFACE_OH3: "cam_egrad_oh3_dev", CAM_SURFACE_BRH2: "cam_egrad_brh2_dev",
+            CAM_SURFACE_O3: "cam_egrad_o3_dev"}[surface]
     na = _abi.NATOMS[surface]
     check(name, getattr(lib(), name)(q_ptr, na, nbeads, v_ptr, g_ptr, precision, stream))
 

@@ -7,8 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcaracal_amd.so")
 
 CAM_OK = 0
-CAM_SURFACE_H3, CAM_SURFACE_CH4H, CAM_SURFACE_OH3, CAM_SURFACE_BRH2 = 1, 2, 3, 4
-NATOMS = {CAM_SURFACE_H3: 3, CAM_SURFACE_CH4H: 6, CAM_SURFACE_OH3: 4, CAM_SURFACE_BRH2: 3}
+CAM_SURFACE_H3, CAM_SURFACE_CH4H, CAM_SURFACE_OH3, CAM_SURFACE_BRH2, CAM_SURFACE_O3 = 1, 2, 3, 4, 5
+NATOMS = {CAM_SURFACE_H3: 3, CAM_SURFACE_CH4H: 6, CAM_SURFACE_OH3: 4, CAM_SURFACE_BRH2: 3, CAM_SURFACE_O3: 3}
 CAM_PREC_FP64, CAM_PREC_MIXED = 0, 1
 CAM_NM_REFERENCE, CAM_NM_ORTHOGONAL = 0, 1
 CAM_ENS_NVE, CAM_ENS_ANDERSEN, CAM_ENS_CHILD, CAM_ENS_PARENT, CAM_ENS_UMBRELLA, CAM_ENS_NHC = 0, 1, 2, 3, 4, 5
@@ -103,10 +103,12 @@ PROTOTYPES = {
     "cam_egrad_ch4h": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "cam_egrad_oh3": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "cam_egrad_brh2": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "cam_egrad_o3": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "cam_egrad_h3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_ch4h_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_oh3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_brh2_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
+    "cam_egrad_o3_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]),
     "cam_egrad_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "cam_engine_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
     "cam_engine_destroy": (ctypes.c_int, [_vp]),

@@ -1,6 +1,7 @@
 // Drop-in analytic surfaces: the bead-batched plugin contract of the
 // reference (egrad_h3, src/egrad_h3.f:29-77; egrad_ch4h, src/egrad_ch4h.f:74-129;
-// egrad_oh3, src/egrad_oh3.f:33-190; egrad_brh2, src/egrad_brh2.f:33-79).
+// egrad_oh3, src/egrad_oh3.f:33-190; egrad_brh2, src/egrad_brh2.f:33-79;
+// egrad_o3, src/egrad_o3.f:33-56).
 // One lane per bead.  A block stages its 256 beads through LDS so the global
 // reads and writes are coalesced although each bead's coordinates are one
 // contiguous 3*natoms-double record.
@@ -10,6 +11,7 @@
 #include "pes_brh2.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
+#include "pes_o3.hpp"
 #include "pes_oh3.hpp"
 
 namespace cam {
@@ -31,10 +33,16 @@ struct LaneBrH2 {
     static constexpr int NC = 9;
     __device__ static void egrad(const double *x, double &v, double *g) { brh2::egrad_bead(x, v, g); }
 };
+struct LaneO3 {
+    static constexpr int NC = 9;
+    __device__ static void egrad(const double *x, double &v, double *g) { o3::egrad_bead(x, v, g); }
+};
 
 constexpr int surface_natoms(int surf)
 {
-    return surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_BRH2 ? 3 : surf == CAM_SURFACE_OH3 ? 4 : 6;
+    return surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_BRH2 || surf == CAM_SURFACE_O3 ? 3
+           : surf == CAM_SURFACE_OH3                                                  ? 4
+                                                                                      : 6;
 }
 
 template <typename P>
@@ -116,6 +124,9 @@ int launch_egrad(const double *q, int nbeads, double *v, double *g, int precisio
     } else if constexpr (SURF == CAM_SURFACE_BRH2) {
         const int grid = (nbeads + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(egrad_lane_kernel<LaneBrH2>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
+    } else if constexpr (SURF == CAM_SURFACE_O3) {
+        const int grid = (nbeads + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(egrad_lane_kernel<LaneO3>, dim3(grid), dim3(kBlock), 0, s, q, nbeads, v, g);
     } else {
         const int grid = (nbeads + kBlock / 4 - 1) / (kBlock / 4);
         if (precision == CAM_PREC_MIXED)
@@ -135,8 +146,8 @@ int egrad_dev(const double *q, int natoms, int nbeads, double *v, double *g, int
     CAM_CHECK_ARG(nbeads >= 0, "nbeads must be >= 0 (got %d)", nbeads);
     CAM_CHECK_ARG(nbeads == 0 || (q && v && g), "null device pointer");
     CAM_CHECK_ARG(precision == CAM_PREC_FP64 || precision == CAM_PREC_MIXED, "bad precision %d", precision);
-    CAM_CHECK_ARG((SURF != CAM_SURFACE_OH3 && SURF != CAM_SURFACE_BRH2) || precision == CAM_PREC_FP64,
-                  "the OH3 and BrH2 surfaces are evaluated in FP64 only");
+    CAM_CHECK_ARG(SURF == CAM_SURFACE_H3 || SURF == CAM_SURFACE_CH4H || precision == CAM_PREC_FP64,
+                  "the OH3, BrH2 and O3 surfaces are evaluated in FP64 only");
     clear_error();
     return launch_egrad<SURF>(q, nbeads, v, g, precision, (hipStream_t)stream);
 }
@@ -233,6 +244,7 @@ int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, doub
     if (surface == CAM_SURFACE_CH4H) return cam::egrad_host_noalloc<CAM_SURFACE_CH4H>(q, natoms, nbeads, v, dvdq);
     if (surface == CAM_SURFACE_OH3) return cam::egrad_host_noalloc<CAM_SURFACE_OH3>(q, natoms, nbeads, v, dvdq);
     if (surface == CAM_SURFACE_BRH2) return cam::egrad_host_noalloc<CAM_SURFACE_BRH2>(q, natoms, nbeads, v, dvdq);
+    if (surface == CAM_SURFACE_O3) return cam::egrad_host_noalloc<CAM_SURFACE_O3>(q, natoms, nbeads, v, dvdq);
     CAM_CHECK_ARG(false, "unknown surface %d", surface);
 }
 int cam_egrad_h3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
@@ -258,6 +270,14 @@ int cam_egrad_brh2(const double *q, int natoms, int nbeads, double *v, double *d
 int cam_egrad_brh2_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {
     return cam::egrad_dev<CAM_SURFACE_BRH2>(q, natoms, nbeads, v, dvdq, precision, stream);
+}
+int cam_egrad_o3(const double *q, int natoms, int nbeads, double *v, double *dvdq)
+{
+    return cam::egrad_host<CAM_SURFACE_O3>(q, natoms, nbeads, v, dvdq);
+}
+int cam_egrad_o3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
+{
+    return cam::egrad_dev<CAM_SURFACE_O3>(q, natoms, nbeads, v, dvdq, precision, stream);
 }
 int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq, int precision, void *stream)
 {

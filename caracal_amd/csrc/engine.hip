@@ -1838,6 +1838,8 @@ bool select_kernels(cam_engine *e, int wpe, int nmk, Kernels &k)
         ok = pick_nb<SurfOH3, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
     else if (cfg->surface == CAM_SURFACE_BRH2)
         ok = pick_nb<SurfBrH2, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
+    else if (cfg->surface == CAM_SURFACE_O3)
+        ok = pick_nb<SurfO3, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit);
     else
         ok = cfg->precision == CAM_PREC_FP64
                  ? pick_nb<SurfCH4H, double>(nb, ens, ens_nm, wpe, mfma, k.step, k.force, k.nm, k.fsplit)
@@ -2129,16 +2131,16 @@ int cam_engine_create(const cam_config *cfg, cam_engine **out)
     *out = nullptr;
     const int surf = cfg->surface;
     CAM_CHECK_ARG(surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_CH4H || surf == CAM_SURFACE_OH3 ||
-                      surf == CAM_SURFACE_BRH2,
+                      surf == CAM_SURFACE_BRH2 || surf == CAM_SURFACE_O3,
                   "unknown surface %d", surf);
-    const int na = surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_BRH2 ? 3 : surf == CAM_SURFACE_OH3 ? 4 : 6;
+    const int na = surf == CAM_SURFACE_OH3 ? 4 : surf == CAM_SURFACE_CH4H ? 6 : 3;
     CAM_CHECK_ARG(cfg->natoms == na, "surface %d needs natoms = %d (got %d)", surf, na, cfg->natoms);
     const int nb = cfg->nbeads;
     CAM_CHECK_ARG(bead_count_supported(nb), "nbeads must be 1, 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64 (got %d)", nb);
     CAM_CHECK_ARG(cfg->ntraj >= 1, "ntraj must be >= 1 (got %d)", cfg->ntraj);
     CAM_CHECK_ARG(cfg->precision == CAM_PREC_FP64 || cfg->precision == CAM_PREC_MIXED, "bad precision");
-    CAM_CHECK_ARG((surf != CAM_SURFACE_OH3 && surf != CAM_SURFACE_BRH2) || cfg->precision == CAM_PREC_FP64,
-                  "the OH3 and BrH2 surfaces run in FP64 only");
+    CAM_CHECK_ARG(surf == CAM_SURFACE_H3 || surf == CAM_SURFACE_CH4H || cfg->precision == CAM_PREC_FP64,
+                  "the OH3, BrH2 and O3 surfaces run in FP64 only");
     CAM_CHECK_ARG(cfg->nm_mode == CAM_NM_REFERENCE || cfg->nm_mode == CAM_NM_ORTHOGONAL, "bad nm_mode");
     CAM_CHECK_ARG(cfg->ensemble >= CAM_ENS_NVE && cfg->ensemble <= CAM_ENS_NHC, "bad ensemble");
     CAM_CHECK_ARG(cfg->ensemble != CAM_ENS_NHC || cfg->nose_q > 0.0, "ensemble NHC needs nose_q > 0");
@@ -2948,6 +2950,7 @@ int cam_batch_create(const cam_config *cfgs, int nbuckets, cam_batch **out)
         const double bead = cfgs[i].surface == CAM_SURFACE_H3     ? 1438.0 + 20.0 * 22
                             : cfgs[i].surface == CAM_SURFACE_OH3  ? 420.0 + 20.0 * 26
                             : cfgs[i].surface == CAM_SURFACE_BRH2 ? 2200.0 + 20.0 * 100
+                            : cfgs[i].surface == CAM_SURFACE_O3   ? 2000.0 + 20.0 * 40
                                                                  : 12707.0 + 20.0 * 321;
         work[i] = (double)cfgs[i].ntraj * cfgs[i].nbeads * bead;
     }

@@ -1,4 +1,4 @@
-// Folded step kernels of the OH3 and BrH2 surfaces, the dispatch over all folded
+// Folded step kernels of the OH3, BrH2 and O3 surfaces, the dispatch over all folded
 // kernels and the folded transform matrix.  The kernels themselves are in
 // fold.hpp; the CH5 and H3 ones are built in fold_ch4h.hip and fold_h3.hip,
 // with other code generation options.
@@ -25,6 +25,7 @@ step_fn fold_step_kernel_for(int surface, int precision, int nbeads, int ensembl
     if (surface == CAM_SURFACE_H3) return fold_kernel_h3(precision, nbeads, ensemble, nbuf, tpb);
     if (surface == CAM_SURFACE_OH3) return f64 ? fold_fp64<SurfOH3>(nbeads, ensemble, nbuf, tpb) : nullptr;
     if (surface == CAM_SURFACE_BRH2) return f64 ? fold_fp64<SurfBrH2>(nbeads, ensemble, nbuf, tpb) : nullptr;
+    if (surface == CAM_SURFACE_O3) return f64 ? fold_fp64<SurfO3>(nbeads, ensemble, nbuf, tpb) : nullptr;
     return fold_kernel_ch4h(precision, nbeads, ensemble, wpe, tpb);
 }
 

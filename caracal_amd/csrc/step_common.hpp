@@ -9,6 +9,7 @@
 #include "pes_brh2.hpp"
 #include "pes_ch4h.hpp"
 #include "pes_h3.hpp"
+#include "pes_o3.hpp"
 #include "pes_oh3.hpp"
 #include "nhc.hpp"
 
@@ -50,6 +51,15 @@ struct SurfBrH2 {
     __device__ static bool valid(int, int) { return true; }
     template <typename T>
     __device__ static void egrad(const double *own, double &v, double *g) { brh2::egrad_bead(own, v, g); }
+};
+
+// O + O2 (Varga-Paukku-Truhlar): one lane per bead, FP64 only.
+struct SurfO3 {
+    static constexpr int NA = 3, NC = 9, L = 1, SLOTS = 9;
+    __device__ static int coord(int slot, int) { return slot; }
+    __device__ static bool valid(int, int) { return true; }
+    template <typename T>
+    __device__ static void egrad(const double *own, double &v, double *g) { o3::egrad_bead(own, v, g); }
 };
 
 struct SurfCH4H {

@@ -16,7 +16,7 @@ module caracal_amd_mod
 
    integer(c_int), parameter, public :: CAM_OK = 0
    integer(c_int32_t), parameter, public :: CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3, &
-                                            CAM_SURFACE_BRH2 = 4
+                                            CAM_SURFACE_BRH2 = 4, CAM_SURFACE_O3 = 5
    integer(c_int32_t), parameter, public :: CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1
    integer(c_int32_t), parameter, public :: CAM_NM_REFERENCE = 0, CAM_NM_ORTHOGONAL = 1
    integer(c_int32_t), parameter, public :: CAM_ENS_NVE = 0, CAM_ENS_ANDERSEN = 1, CAM_ENS_CHILD = 2, &
@@ -109,6 +109,14 @@ module caracal_amd_mod
       end function
       ! drop-in for egrad_brh2(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_brh2.f:33)
       function cam_egrad_brh2(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_brh2") result(rc)
+         import :: c_int, c_double
+         real(c_double), intent(in) :: q(*)
+         integer(c_int), value :: natoms, nbeads
+         real(c_double), intent(out) :: v(*), dvdq(*)
+         integer(c_int) :: rc
+      end function
+      ! drop-in for egrad_o3(q,Natoms,Nbeads,V,dVdq,info) (src/egrad_o3.f:33)
+      function cam_egrad_o3(q, natoms, nbeads, v, dvdq) bind(C, name="cam_egrad_o3") result(rc)
          import :: c_int, c_double
          real(c_double), intent(in) :: q(*)
          integer(c_int), value :: natoms, nbeads
@@ -378,7 +386,7 @@ module caracal_amd_mod
       end function
    end interface
 
-   public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h, cam_egrad_oh3, cam_egrad_brh2
+   public :: cam_last_error, cam_egrad_h3, cam_egrad_ch4h, cam_egrad_oh3, cam_egrad_brh2, cam_egrad_o3
    public :: cam_engine_create, cam_engine_destroy, cam_engine_set_state, cam_engine_get_state
    public :: cam_engine_get_forces, cam_engine_draw_momenta, cam_engine_constrain
    public :: cam_engine_begin_children, cam_engine_begin_parent, cam_engine_run, cam_engine_sync, cam_engine_kappa
@@ -389,7 +397,7 @@ module caracal_amd_mod
    public :: cam_batch_sync, cam_water_create, cam_ff_destroy, cam_ff_set_pair_mode
    public :: cam_qmdff_create, cam_box_create, cam_box_destroy, cam_box_set_state, cam_box_get_state
    public :: cam_box_set_nhc, cam_box_set_transrot, cam_box_run, cam_box_sync, cam_box_energies
-   public :: egrad_h3_gpu, egrad_ch4h_gpu, egrad_oh3_gpu, egrad_brh2_gpu, cam_error_message
+   public :: egrad_h3_gpu, egrad_ch4h_gpu, egrad_oh3_gpu, egrad_brh2_gpu, egrad_o3_gpu, cam_error_message
 
 contains
 
@@ -426,6 +434,14 @@ contains
       integer, intent(out) :: info
       info = cam_egrad_brh2(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
    end subroutine egrad_brh2_gpu
+
+   subroutine egrad_o3_gpu(q, natoms, nbeads, v, dvdq, info)
+      integer, intent(in) :: natoms, nbeads
+      real(c_double), intent(in) :: q(3, natoms, nbeads)
+      real(c_double), intent(out) :: v(nbeads), dvdq(3, natoms, nbeads)
+      integer, intent(out) :: info
+      info = cam_egrad_o3(q, int(natoms, c_int), int(nbeads, c_int), v, dvdq)
+   end subroutine egrad_o3_gpu
 
    function cam_error_message() result(msg)
       character(len=:), allocatable :: msg

@@ -11,6 +11,9 @@ oh3_nve       OH + H2 (O 15.999, H 1.008 amu) on the Schatz-Elgersma surface at
 brh2_nve      H + HBr (H 1.008, Br 79.904 amu) on the BrH2 DIM-3C surface at
               its collinear saddle, bead positions + N(0, 0.01^2) bohr, momenta
               at 300 K.
+o3_nve        O + O2 (O 15.999 amu) on the Varga-Paukku-Truhlar O3 surface at
+              the C2v saddle of the exchange, bead positions + N(0, 0.01^2)
+              bohr, momenta at 300 K.
 ragged        H3 / CH4H half-and-half with nbeads in {4,8,16,32,64} and
               T in U[200, 1000] K, seed 1 (bucketed by surface and nbeads).
 """
@@ -56,6 +59,17 @@ BRH2_TS = np.array([
 ])
 MASSES_BRH2_AMU = [1.008, 79.904, 1.008]
 
+# C2v saddle of the O + O2 exchange on the O3 surface of src/egrad_o3.f (bohr;
+# atoms O1 (apex), O2, O3): bonds 2.5645 bohr at 105.25 degrees, V = 10.165
+# kcal/mol above O + O2(re), one imaginary mode (the antisymmetric stretch),
+# |dV/dq| ~ 6e-15 (tests/golden/make_o3_saddle.py).
+O3_TS = np.array([
+    [-0.6300267594282657, -0.8247489761317799, 0.0],
+    [1.9344942909242242, -0.8247489761317799, 0.0],
+    [-1.3044675314959586, 1.6494979522635598, 0.0],
+])
+MASSES_O3_AMU = [15.999, 15.999, 15.999]
+
 
 def h3_reaction():
     """H_A H_B + H_C -> H_A + H_B H_C (examples/calc_rate/h+h2/rate.key)."""
@@ -83,6 +97,14 @@ def oh3_reaction():
 def brh2_reaction():
     """H + HBr -> H2 + Br: Ha-Hc forms, Ha-Br breaks, at the saddle's lengths."""
     t = BRH2_TS
+    return Reaction(bond_form=[(1, 3)], bond_break=[(1, 2)],
+                    form_ref=[float(np.linalg.norm(t[0] - t[2]))], break_ref=[float(np.linalg.norm(t[0] - t[1]))],
+                    reactants=[[1, 2], [3]], r_inf=16.0, xi_ideal=1.0)
+
+
+def o3_reaction():
+    """O3 + O1O2 -> O3O1 + O2: O1-O3 forms, O1-O2 breaks, at the saddle's (equal) lengths."""
+    t = O3_TS
     return Reaction(bond_form=[(1, 3)], bond_break=[(1, 2)],
                     form_ref=[float(np.linalg.norm(t[0] - t[2]))], break_ref=[float(np.linalg.norm(t[0] - t[1]))],
                     reactants=[[1, 2], [3]], r_inf=16.0, xi_ideal=1.0)
@@ -117,6 +139,15 @@ def brh2_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
     masses = mass_au(MASSES_BRH2_AMU)
     beta = float(beta_of(kelvin))
     q = BRH2_TS[None] + rng.normal(0.0, 0.01, (nbeads, 3, 3))
+    p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 3, 3))
+    return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
+
+
+def o3_nve(nbeads=16, seed=0, kelvin=300.0, dt_fs=0.1):
+    rng = np.random.default_rng(seed)
+    masses = mass_au(MASSES_O3_AMU)
+    beta = float(beta_of(kelvin))
+    q = O3_TS[None] + rng.normal(0.0, 0.01, (nbeads, 3, 3))
     p = mb_momenta(rng, masses, nbeads, beta, (nbeads, 3, 3))
     return dict(q=q, p=p, beta=beta, masses=masses, dt=dt_fs * FS_TO_AU, nbeads=nbeads)
 

@@ -64,6 +64,13 @@ int cam_egrad_oh3(const double *q, int natoms, int nbeads, double *v, double *dv
  * instead of the reference's EISPACK: energies agree to rounding, gradients
  * to the eigenvector's conditioning (within 1e-10 at the geometries checked). */
 int cam_egrad_brh2(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+/* Replaces: egrad_o3 (src/egrad_o3.f:33-56, the O + O2 surface of Varga,
+ * Paukku and Truhlar: permutationally invariant polynomials of
+ * mixed exponential-Gaussian variables, a diatomic curve and D3(BJ)
+ * dispersion), selected by ANA_O3 (src/read_pes.f90:456-473, no
+ * initialisation call) and called per bead from gradient
+ * (src/gradient.f90:184-185).  natoms must be 3 (three O atoms). */
+int cam_egrad_o3(const double *q, int natoms, int nbeads, double *v, double *dvdq);
 
 /* Device-resident variants: q, v, dvdq are device pointers, stream is a
  * hipStream_t (NULL = default stream); the call is asynchronous on that
@@ -73,13 +80,15 @@ int cam_egrad_h3_dev(const double *q, int natoms, int nbeads, double *v, double 
                      int precision, void *stream);
 int cam_egrad_ch4h_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
                        int precision, void *stream);
-/* OH3 and BrH2 have no binary32 form: precision must be CAM_PREC_FP64. */
+/* OH3, BrH2 and O3 have no binary32 form: precision must be CAM_PREC_FP64. */
 int cam_egrad_oh3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
                       int precision, void *stream);
 int cam_egrad_brh2_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
                        int precision, void *stream);
+int cam_egrad_o3_dev(const double *q, int natoms, int nbeads, double *v, double *dvdq,
+                     int precision, void *stream);
 
-/* Allocation-free form of cam_egrad_h3 / _ch4h / _oh3 / _brh2 (the
+/* Allocation-free form of cam_egrad_h3 / _ch4h / _oh3 / _brh2 / _o3 (the
  * surface by its CAM_SURFACE_* code; same arguments, host pointers,
  * synchronous).  No counterpart in the reference: it is for callers that
  * cannot allocate device memory per call (a tight per-step force loop, code
@@ -90,11 +99,11 @@ int cam_egrad_brh2_dev(const double *q, int natoms, int nbeads, double *v, doubl
 int cam_egrad_noalloc(int surface, const double *q, int natoms, int nbeads, double *v, double *dvdq);
 
 /* ---- (2) trajectory-batch RPMD engine ---- */
-enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3, CAM_SURFACE_BRH2 = 4 };
+enum { CAM_SURFACE_H3 = 1, CAM_SURFACE_CH4H = 2, CAM_SURFACE_OH3 = 3, CAM_SURFACE_BRH2 = 4, CAM_SURFACE_O3 = 5 };
 /* Precision of the engine.  MIXED: forces in FP32 arithmetic with FP64
  * positions, momenta, accumulation and normal modes (CH4H: the whole
  * surface; H3: its asymmetric and compact correction terms, as
- * cam_egrad_h3_dev).  The OH3 and BrH2 surfaces run in FP64 only
+ * cam_egrad_h3_dev).  The OH3, BrH2 and O3 surfaces run in FP64 only
  * (CAM_PREC_MIXED is rejected). */
 enum { CAM_PREC_FP64 = 0, CAM_PREC_MIXED = 1 };
 /* Normal-mode transform.  REFERENCE reproduces the reference's rfft/irfft pair
@@ -179,7 +188,7 @@ typedef struct {
 
 typedef struct {
     int32_t surface;          /* CAM_SURFACE_* */
-    int32_t natoms;           /* 3 (H3, BrH2), 6 (CH4H) or 4 (OH3) */
+    int32_t natoms;           /* 3 (H3, BrH2, O3), 6 (CH4H) or 4 (OH3) */
     int32_t nbeads;           /* 1, 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64 */
     int32_t ntraj;            /* trajectories in the batch */
     int32_t precision;        /* CAM_PREC_* */

@@ -17,7 +17,7 @@ extern "C" {
 
 #define ORACLE_MAX_ATOMS 16
 #define ORACLE_MAX_BEADS 256
-enum { ORACLE_SURF_H3 = 1, ORACLE_SURF_CH4H = 2, ORACLE_SURF_OH3 = 3, ORACLE_SURF_BRH2 = 4 };
+enum { ORACLE_SURF_H3 = 1, ORACLE_SURF_CH4H = 2, ORACLE_SURF_OH3 = 3, ORACLE_SURF_BRH2 = 4, ORACLE_SURF_O3 = 5 };
 
 typedef struct {
     int surface, natoms, nbeads;
@@ -94,6 +94,8 @@ void oracle_egrad_oh3(const double *q, int natoms, int nbeads, double *v, double
 /* src/egrad_brh2.f: atoms Ha, Br, Hc; r = (Br-Ha, Ha-Hc, Br-Hc), its R(1..3) */
 void oracle_brh2_pot(const double r[3], double *energy, double dedr[3]);
 void oracle_egrad_brh2(const double *q, int natoms, int nbeads, double *v, double *dvdq);
+/* src/egrad_o3.f: three O atoms, bohr and hartree */
+void oracle_egrad_o3(const double *q, int natoms, int nbeads, double *v, double *dvdq);
 
 /* RPMD step and helpers (src/verlet.f90 and callees) */
 void oracle_rfft(double *x, int n);

@@ -12,7 +12,7 @@
 ! Input (stream, native endianness):
 !   int32  natoms, nbeads, nsteps, constrain (-1 dynamic.x, 0 umbrella
 !          window, 2 recrossing child), surface (1 H3, 2 CH4H, 3 OH3, 4 QMDFF,
-!          5 BrH2),
+!          5 BrH2, 6 O3),
 !          every (write a frame after every 'every' steps; 0 = timing only),
 !          thermostat (0 none, 1 Andersen, 2 Nose-Hoover chain), andersen_step
 !   real64 dt, beta, mass(natoms), q(3,natoms,nbeads), p(3,natoms,nbeads)
@@ -127,6 +127,8 @@ program ref_step_harness
    case (5)
       pot_type = "brh2"
       call initialize_brh2()
+   case (6)
+      pot_type = "o3"
    case (4)
       ! one QMDFF (src/read_pes.f90:1745-1919, :2729-2736); gradient then
       ! takes its ff_eg / ff_nonb / ff_hb branch (src/gradient.f90:341-355)

@@ -48,6 +48,8 @@ void oracle_forces(const oracle_sys *s, const double *q, double *v, double *g)
         oracle_egrad_oh3(q, s->natoms, s->nbeads, v, g);
     else if (s->surface == ORACLE_SURF_BRH2)
         oracle_egrad_brh2(q, s->natoms, s->nbeads, v, g);
+    else if (s->surface == ORACLE_SURF_O3)
+        oracle_egrad_o3(q, s->natoms, s->nbeads, v, g);
     else
         oracle_egrad_ch4h(q, s->natoms, s->nbeads, v, g);
 }

@@ -24,6 +24,8 @@ stores inputs and frames as small .npz fixtures beside this script:
   ch4h_child_12beads_reference.npz  a CH4 + H recrossing child with 12 beads
   brh2_nve_reference.npz      H + HBr on the BrH2 DIM-3C surface at its
                               saddle, 8 beads, NVE with transrot, 100 steps
+  o3_nve_reference.npz        O + O2 on the O3 surface at the exchange
+                              saddle, 8 beads, NVE with transrot, 100 steps
 
 Frames are kept at steps 1, 2, 10, 50 and 100 (q and p), with epot and xi
 at every step.  Run in the container where /root/reference is mounted:
@@ -72,6 +74,7 @@ def main(names):
     w, q, p = child_start()
     o = workloads.oh3_nve(nbeads=8)
     br = workloads.brh2_nve(nbeads=8)
+    oz = workloads.o3_nve(nbeads=8)
     h8 = workloads.h3_nve(nbeads=8)
     h6 = workloads.h3_nve(nbeads=6)
     w12, q12, p12 = child_start(12)
@@ -98,6 +101,8 @@ def main(names):
                                                           w12["reaction"]),
         "brh2_nve_reference.npz": lambda: _save("brh2_nve_reference.npz", "brh2", br["q"], br["p"], br["masses"],
                                                 br["dt"], br["beta"], -1),
+        "o3_nve_reference.npz": lambda: _save("o3_nve_reference.npz", "o3", oz["q"], oz["p"], oz["masses"],
+                                              oz["dt"], oz["beta"], -1),
     }
     for name in names or list(jobs):
         jobs[name]()

@@ -67,6 +67,10 @@ def egrad_brh2(q):
     return _egrad(oracle().oracle_egrad_brh2, q)
 
 
+def egrad_o3(q):
+    return _egrad(oracle().oracle_egrad_o3, q)
+
+
 def _ref_egrad(fn, q):
     q = np.ascontiguousarray(q, dtype=np.float64)
     nb, na, _ = q.shape
@@ -95,6 +99,10 @@ def ref_egrad_oh3(q):
 
 def ref_egrad_brh2(q):
     return _ref_egrad(ref().egrad_brh2_, q)
+
+
+def ref_egrad_o3(q):
+    return _ref_egrad(ref().egrad_o3_, q)
 
 
 # ---- RPMD step (oracle/rpmd.c) ----

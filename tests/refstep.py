@@ -11,7 +11,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
-SURFACE_CODE = {"h3": 1, "ch4h": 2, "oh3": 3, "qmdff": 4, "brh2": 5}
+SURFACE_CODE = {"h3": 1, "ch4h": 2, "oh3": 3, "qmdff": 4, "brh2": 5, "o3": 6}
 
 
 def binary(opt="O1"):

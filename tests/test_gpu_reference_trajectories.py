@@ -113,6 +113,17 @@ def test_brh2_nve_matches_reference():
     _check("BrH2", q, p, f)
 
 
+def test_o3_nve_matches_reference():
+    """O + O2 at the exchange saddle of the O3 surface, 8 beads, NVE with
+    transrot."""
+    f = dict(np.load(os.path.join(GOLDEN, "o3_nve_reference.npz")))
+    nb = f["q0"].shape[0]
+    with cam.RPMDEngine(cam.CAM_SURFACE_O3, nb, 1, float(f["dt"]), f["masses"], remove_com=True) as e:
+        e.set_state(f["q0"][None], f["p0"][None], [float(f["beta"])])
+        q, p = _frames(e, f["steps"])
+    _check("O3", q, p, f)
+
+
 @pytest.mark.parametrize("schedule", [cam.CAM_SCHED_FUSED, cam.CAM_SCHED_SPLIT])
 def test_h3_umbrella_window_matches_reference(schedule):
     """An umbrella window as calc_rate runs it (verlet constrain = 0): mdinit's

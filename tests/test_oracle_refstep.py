@@ -28,8 +28,14 @@ CASES = [("c1_h3_nve_reference.npz", 1, -1, 0, None),
          ("oh3_child_xi05_reference.npz", 3, 2, 0, workloads.oh3_reaction),
          ("h3_nve_6beads_reference.npz", 1, -1, 0, None),
          ("ch4h_child_12beads_reference.npz", 2, 2, 0, workloads.ch4h_reaction),
-         ("brh2_nve_reference.npz", 4, -1, 0, None)]
-SURF = {1: "h3", 2: "ch4h", 3: "oh3", 4: "brh2"}
+         ("brh2_nve_reference.npz", 4, -1, 0, None),
+         ("o3_nve_reference.npz", 5, -1, 0, None)]
+SURF = {1: "h3", 2: "ch4h", 3: "oh3", 4: "brh2", 5: "o3"}
+# Absolute bound on the energy where the default (1e-12 of its scale) is below
+# the rounding of the surface: an O3 bead energy sums polynomial terms of up
+# to 1e5 kcal/mol (160 hartree) that cancel to about 0.02 hartree, so either
+# evaluation order moves it by ~1e-13; 1e-12 per bead.
+EPOT_ATOL = {"o3_nve_reference.npz": 8e-12}
 
 
 def _fixture(name):
@@ -82,7 +88,7 @@ def test_oracle_reproduces_reference_trajectory(name, surface, constrain, thermo
     epm = np.abs(p - f["p"]).max() / np.abs(f["p"]).max()
     print(f"{name}: max rel position error {eq:.2e}, momentum {epm:.2e}")
     assert eq < 1e-12 and epm < 1e-12
-    assert np.abs(ep - f["epot"]).max() <= 1e-12 * np.abs(f["epot"]).max()
+    assert np.abs(ep - f["epot"]).max() <= EPOT_ATOL.get(name, 1e-12 * np.abs(f["epot"]).max())
     if constrain >= 0:
         np.testing.assert_allclose(xi, f["xi"], rtol=0, atol=1e-12)
 

@@ -1,5 +1,5 @@
 """The allocation-free surface entry point (cam_egrad_noalloc) against the
-allocating one (cam_egrad_h3 / _ch4h / _oh3 / _brh2), through the C
+allocating one (cam_egrad_h3 / _ch4h / _oh3 / _brh2 / _o3), through the C
 ABI: two calls in a row per surface, the second with fewer beads, bit for
 bit; sizes beyond its limit are rejected."""
 import ctypes
@@ -16,7 +16,8 @@ LIMIT = 16384
 CASES = [(cam.CAM_SURFACE_H3, 3, workloads.H3_TS, "cam_egrad_h3"),
          (cam.CAM_SURFACE_CH4H, 6, workloads.CH4H_TS, "cam_egrad_ch4h"),
          (cam.CAM_SURFACE_OH3, 4, workloads.OH3_TS, "cam_egrad_oh3"),
-         (cam.CAM_SURFACE_BRH2, 3, workloads.BRH2_TS, "cam_egrad_brh2")]
+         (cam.CAM_SURFACE_BRH2, 3, workloads.BRH2_TS, "cam_egrad_brh2"),
+         (cam.CAM_SURFACE_O3, 3, workloads.O3_TS, "cam_egrad_o3")]
 
 
 def _call(fn, *args):
