@@ -31,7 +31,7 @@ program rpmd_driver
    open(newunit=u, file=trim(fin), access="stream", form="unformatted", status="old")
    read(u) hdr
    select case (hdr(1))
-   case (CAM_SURFACE_H3)
+   case (CAM_SURFACE_H3, CAM_SURFACE_BRH2, CAM_SURFACE_O3)
       natoms = 3
    case (CAM_SURFACE_OH3)
       natoms = 4
@@ -73,6 +73,10 @@ program rpmd_driver
       call egrad_h3_gpu(q0, natoms, nbeads*ntraj, v, g, info)
    else if (hdr(1) == CAM_SURFACE_OH3) then
       call egrad_oh3_gpu(q0, natoms, nbeads*ntraj, v, g, info)
+   else if (hdr(1) == CAM_SURFACE_BRH2) then
+      call egrad_brh2_gpu(q0, natoms, nbeads*ntraj, v, g, info)
+   else if (hdr(1) == CAM_SURFACE_O3) then
+      call egrad_o3_gpu(q0, natoms, nbeads*ntraj, v, g, info)
    else
       call egrad_ch4h_gpu(q0, natoms, nbeads*ntraj, v, g, info)
    end if

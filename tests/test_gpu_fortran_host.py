@@ -1,5 +1,5 @@
 """The Fortran host path: caracal_amd/fortran/rpmd_driver (iso_c_binding
-module caracal_amd_mod) advances an H3 or OH + H2 NVE ring polymer on the GPU
+module caracal_amd_mod) advances an H3, OH + H2, BrH2 or O3 NVE ring polymer on the GPU
 through the C ABI; its output must match the oracle's verlet."""
 import os
 import subprocess
@@ -15,11 +15,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "caracal_amd", "fortran", "rpmd_driver")
 
 
-@pytest.mark.parametrize("surface", [1, 3], ids=["H3", "OH3"])
+SYSTEMS = {1: (workloads.h3_nve, ol.egrad_h3), 3: (workloads.oh3_nve, ol.egrad_oh3),
+           4: (workloads.brh2_nve, ol.egrad_brh2), 5: (workloads.o3_nve, ol.egrad_o3)}
+
+
+@pytest.mark.parametrize("surface", [1, 3, 4, 5], ids=["H3", "OH3", "BrH2", "O3"])
 def test_fortran_driver_matches_oracle(tmp_path, surface):
     if not os.path.exists(DRIVER):
         pytest.fail("caracal_amd/fortran/rpmd_driver is not built")
-    w = workloads.h3_nve(nbeads=16) if surface == 1 else workloads.oh3_nve(nbeads=16)
+    w = SYSTEMS[surface][0](nbeads=16)
     na = len(w["masses"])
     nsteps = 100
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
@@ -40,5 +44,6 @@ def test_fortran_driver_matches_oracle(tmp_path, surface):
     for _ in range(nsteps):
         t.step()
     assert np.abs(q - t.q).max() / np.abs(t.q).max() < 1e-10
-    v_ref = (ol.egrad_h3 if surface == 1 else ol.egrad_oh3)(w["q"])[0]
-    np.testing.assert_allclose(v, v_ref, rtol=1e-12, atol=1e-15)
+    v_ref = SYSTEMS[surface][1](w["q"])[0]
+    # O3 energies cancel polynomial terms of up to 1e5 kcal/mol: 1e-12 absolute
+    np.testing.assert_allclose(v, v_ref, rtol=1e-12, atol=1e-12 if surface == 5 else 1e-15)

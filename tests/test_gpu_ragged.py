@@ -52,19 +52,22 @@ def test_ragged_mixed_forces_within_1e5_of_fp64_per_bucket():
     assert max(errs.values()) <= 1e-5
 
 
-def test_ragged_batch_with_three_surfaces_in_fp64():
-    """H3, CH5 and OH + H2 buckets (4-32 beads) in one FP64 batch, each equal
-    bit for bit to its bucket run alone."""
+def test_ragged_batch_with_five_surfaces_in_fp64():
+    """H3, CH5, OH + H2, BrH2 and O3 buckets (4-32 beads) in one FP64 batch,
+    each equal bit for bit to its bucket run alone (the batch's folded
+    kernels take two LDS column buffers, an engine alone one)."""
     rng = np.random.default_rng(4)
+    systems = {cam.CAM_SURFACE_OH3: (workloads.oh3_nve, workloads.OH3_TS),
+               cam.CAM_SURFACE_H3: (workloads.h3_nve, workloads.H3_TS),
+               cam.CAM_SURFACE_BRH2: (workloads.brh2_nve, workloads.BRH2_TS),
+               cam.CAM_SURFACE_O3: (workloads.o3_nve, workloads.O3_TS)}
     buckets = []
     for surf, nb, n in ((cam.CAM_SURFACE_OH3, 16, 40), (cam.CAM_SURFACE_H3, 8, 30), (cam.CAM_SURFACE_CH4H, 4, 20),
-                        (cam.CAM_SURFACE_OH3, 32, 10)):
-        if surf == cam.CAM_SURFACE_OH3:
-            w = workloads.oh3_nve(nbeads=nb)
-            ts = workloads.OH3_TS
-        elif surf == cam.CAM_SURFACE_H3:
-            w = workloads.h3_nve(nbeads=nb)
-            ts = workloads.H3_TS
+                        (cam.CAM_SURFACE_OH3, 32, 10), (cam.CAM_SURFACE_BRH2, 32, 20), (cam.CAM_SURFACE_O3, 16, 30),
+                        (cam.CAM_SURFACE_BRH2, 6, 25)):
+        if surf in systems:
+            w = systems[surf][0](nbeads=nb)
+            ts = systems[surf][1]
         else:
             w = workloads.ch4h_children(ntraj=1, nbeads=nb)
             ts = workloads.CH4H_TS
