@@ -108,3 +108,4 @@ def test_h_plus_hbr_rate_end_to_end():
     assert 0.1 < out["kappa"] <= 1.05
     assert np.isfinite(out["k_cm3_molecule_s"]) and out["k_cm3_molecule_s"] > 0
 
+
