@@ -169,6 +169,21 @@ constexpr bool bead_count_supported(int nb)
 constexpr bool mfma_bead_count(int nb) { return nb >= 16 && nb % 16 == 0 && 4 % (nb / 16) == 0; }
 
 typedef void (*step_fn)(StepArgs);
+typedef void (*force_fn)(const double *, double *, double *, int);
+
+// The engine's kernels for one surface, precision and bead count: the fused
+// step (ensemble ens), the initial-force kernel, and the split schedule's
+// normal-mode step (ensemble ens_nm) and force kernel; false if the bead count
+// has none.  wpe: waves per SIMD asked of the compiler (2 only for the FP64
+// CH5 step); mfma: the MFMA normal-mode transform where the bead count has
+// one.  Built in engine_h3.hip, engine_ch4h.hip and engine_fp64.hip (OH3,
+// BrH2, O3: FP64 only).
+bool pick_kernels_h3(int precision, int nb, int ens, int ens_nm, int wpe, bool mfma, step_fn &st, force_fn &fo,
+                     step_fn &nm, force_fn &fs);
+bool pick_kernels_ch4h(int precision, int nb, int ens, int ens_nm, int wpe, bool mfma, step_fn &st, force_fn &fo,
+                       step_fn &nm, force_fn &fs);
+bool pick_kernels_fp64(int surface, int nb, int ens, int ens_nm, int wpe, bool mfma, step_fn &st, force_fn &fo,
+                       step_fn &nm, force_fn &fs);
 
 // The folded step kernel (fold.hip) of a configuration, or nullptr if it has
 // none; *tpb receives its trajectories per block.  nbuf: LDS column buffers
