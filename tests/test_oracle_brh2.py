@@ -13,6 +13,8 @@ u the lowest eigenvector) agree to 1e-13 at most geometries; where the lowest
 two eigenvalues lie close the eigenvector is ill-conditioned and the
 reference's gradient is off by up to ~6e-11 (an 80-bit build of the same
 Jacobi code agrees with the oracle's to 6e-15 there), hence the 2e-10 bound."""
+import os
+
 import numpy as np
 import pytest
 
@@ -142,3 +144,50 @@ def test_brh2_saddle():
     v, g = ol.egrad_brh2(workloads.BRH2_TS[None])
     assert abs(v[0] * 627.5094743 - 20.9962) < 1e-3
     assert np.abs(g).max() < 1e-12
+
+
+def test_brh2_jacobi_gradient_against_extended_precision(tmp_path):
+    """The oracle's gradient against the same code in 80-bit long double
+    (built here from oracle/pes_brh2.c with every double, exp, sqrt and fabs
+    widened): within 1e-13 everywhere, also where the compiled reference's
+    EISPACK eigenvector leaves its gradient 1e-11 off."""
+    import ctypes
+    import re
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is not installed")
+    src = open(os.path.join(os.path.dirname(ol.ORACLE_SO), "pes_brh2.c")).read()
+    src = re.sub(r"\bdouble\b", "long double", src)
+    for f in ("exp", "sqrt", "fabs"):
+        src = re.sub(rf"\b{f}\(", f"{f}l(", src)
+    src = src.replace('#include "oracle.h"', "#include <stddef.h>").replace("0x1p-60", "0x1p-70L")
+    src = src.replace("oracle_", "ld_")
+    src += """
+void ld_egrad_brh2_d(const double *q, int n, double *v, double *g)
+{
+    for (int k = 0; k < n; ++k) {
+        long double ql[9], vl, gl[9];
+        for (int i = 0; i < 9; ++i) ql[i] = q[9 * k + i];
+        ld_egrad_brh2(ql, 3, 1, &vl, gl);
+        v[k] = (double)vl;
+        for (int i = 0; i < 9; ++i) g[9 * k + i] = (double)gl[i];
+    }
+}
+"""
+    c = tmp_path / "brh2_ld.c"
+    c.write_text(src)
+    so = tmp_path / "brh2_ld.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-o", str(so), str(c), "-lm"], check=True)
+    lib = ctypes.CDLL(str(so))
+    q = np.ascontiguousarray(_cloud(1500, 0.4, 1))
+    vl, gl = np.zeros(len(q)), np.zeros_like(q)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.ld_egrad_brh2_d(q.ctypes.data_as(dp), len(q), vl.ctypes.data_as(dp), gl.ctypes.data_as(dp))
+    v, g = ol.egrad_brh2(q)
+    assert np.abs(v - vl).max() <= 1e-14
+    assert np.abs(g - gl).max() <= 1e-13
+    if ol.ref() is not None:
+        # the compiled reference is the less accurate of the two there
+        _, gr = ol.ref_egrad_brh2(q)
+        assert np.abs(gr - gl).max() > 10 * np.abs(g - gl).max()
